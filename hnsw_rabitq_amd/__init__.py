@@ -1,0 +1,361 @@
+"""MI355X query engine for hnswlib indexes.
+
+Drop-in mirror of the reference Python API for the query path (python_bindings/bindings.cpp):
+  Index(space, dim)     init_index / add_items (CPU graph construction, restating addPoint)
+                        knn_query -> batched HNSW search on the GPU
+                        set_ef / ef, save_index / load_index (hnswlib on-disk format)
+  BFIndex(space, dim)   init_index / add_items / knn_query -> GPU brute force
+Results have the reference's shapes, dtypes and order: labels uint64 [rows, k], distances
+float32 [rows, k], closest first (bindings.cpp:612-697, 850-929).  Errors raise RuntimeError
+with the reference's messages where the condition is the same.
+
+The query path runs only on the GPU: importing works without one, but searching without the
+built library or a device raises.
+"""
+import ctypes
+import os
+
+import numpy as np
+
+from . import _lib
+from ._lib import HnswGpuError, PREC_BF16, PREC_FP32, PREC_FP8
+
+__all__ = ["Index", "BFIndex", "DeviceArray", "GpuHnsw", "HnswGpuError", "normalize_rows"]
+
+_SPACES = {"l2": (_lib.SPACE_L2, False), "ip": (_lib.SPACE_IP, False), "cosine": (_lib.SPACE_IP, True)}
+_PRECISIONS = {"fp32": PREC_FP32, "bf16": PREC_BF16, "fp8": PREC_FP8}
+
+
+def _space(name):
+    if name not in _SPACES:
+        raise RuntimeError("Space name must be one of l2, ip, or cosine.")
+    return _SPACES[name]
+
+
+def _as_rows(data, dim):
+    arr = np.ascontiguousarray(np.asarray(data, dtype=np.float32))
+    if arr.ndim == 1:
+        arr = arr.reshape(1, -1)
+    if arr.ndim != 2:
+        raise RuntimeError(
+            f"Input vector data wrong shape. Number of dimensions {arr.ndim}. Data must be a 1D or 2D array.")
+    if arr.shape[1] != dim:
+        raise RuntimeError("Wrong dimensionality of the vectors")
+    return arr
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data) if a is not None else None
+
+
+def normalize_rows(arr):
+    """Index.normalize_vector (bindings.cpp:241-248) row by row, in the reference's FP32 order."""
+    arr = np.ascontiguousarray(arr, dtype=np.float32)
+    out = np.empty_like(arr)
+    _lib.call("hg_normalize_rows", _ptr(arr), _ptr(out), arr.shape[0], arr.shape[1])
+    return out
+
+
+class DeviceArray:
+    """A device allocation owned by Python (hipMalloc through the C ABI)."""
+
+    def __init__(self, nbytes):
+        p = ctypes.c_void_p()
+        _lib.call("hg_malloc", ctypes.byref(p), max(int(nbytes), 1))
+        self.ptr = p
+        self.nbytes = int(nbytes)
+
+    @classmethod
+    def from_numpy(cls, a, stream=None):
+        a = np.ascontiguousarray(a)
+        d = cls(a.nbytes)
+        _lib.call("hg_memcpy_h2d", d.ptr, _ptr(a), a.nbytes, stream)
+        return d
+
+    def to_numpy(self, dtype, shape, stream=None):
+        out = np.empty(shape, dtype=dtype)
+        _lib.call("hg_memcpy_d2h", _ptr(out), self.ptr, out.nbytes, stream)
+        return out
+
+    def free(self):
+        if self.ptr is not None and self.ptr.value:
+            _lib.call("hg_free", self.ptr)
+        self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class GpuHnsw:
+    """Device copy of a CPU-built HNSW index (the exported GPU layout)."""
+
+    def __init__(self, cpu_handle, device=0):
+        h = ctypes.c_void_p()
+        _lib.call("hg_gpu_index_create", cpu_handle, device, ctypes.byref(h))
+        self.handle = h
+
+    def search(self, queries, k, ef, counters=False):
+        q = np.ascontiguousarray(queries, dtype=np.float32)
+        nq = q.shape[0]
+        labels = np.empty((nq, k), dtype=np.uint64)
+        dists = np.empty((nq, k), dtype=np.float32)
+        ctr = np.empty((nq, 4), dtype=np.int32) if counters else None
+        code = _lib.load().hg_gpu_search(self.handle, _ptr(q), nq, k, ef, _ptr(labels), _ptr(dists), _ptr(ctr))
+        if code not in (_lib.HG_OK, _lib.HG_ERR_SHORT):
+            _lib.check(code)
+        return labels, dists, ctr, code == _lib.HG_ERR_SHORT
+
+    def search_device(self, d_queries, nq, k, ef, d_labels, d_dists, d_counters=None, d_ef=None, d_k=None,
+                      k_stride=None, stop_model=None, stream=None):
+        _lib.call("hg_gpu_search_device", self.handle, d_queries.ptr, nq, d_ef.ptr if d_ef else None,
+                  d_k.ptr if d_k else None, ef, k, k_stride or k, d_labels.ptr, d_dists.ptr,
+                  d_counters.ptr if d_counters else None, stop_model, stream)
+
+    def plan(self, max_ef, max_k, nq):
+        out = np.zeros(5, dtype=np.int32)
+        _lib.call("hg_gpu_search_plan", self.handle, max_ef, max_k, nq, _ptr(out))
+        return dict(waves_per_group=int(out[0]), hash_slots=int(out[1]), beam_cap=int(out[2]),
+                    lds_bytes=int(out[3]), workgroups=int(out[4]))
+
+    def autotune(self, d_queries, nq, ef, k, stream=None):
+        _lib.call("hg_gpu_autotune", self.handle, d_queries.ptr, nq, ef, k, stream)
+
+    def nbytes(self):
+        b = ctypes.c_size_t()
+        _lib.call("hg_gpu_index_bytes", self.handle, ctypes.byref(b))
+        return b.value
+
+    def free(self):
+        if self.handle is not None and self.handle.value:
+            _lib.load().hg_gpu_index_free(self.handle)
+        self.handle = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class Index:
+    """hnswlib.Index mirror (bindings.cpp:143-722, 933-1010) with the GPU query path."""
+
+    def __init__(self, space, dim):
+        self.space_id, self.normalize = _space(space)
+        self.space = space
+        self.dim = int(dim)
+        self._h = None
+        self._gpu = None
+        self._dirty = True
+        self.default_ef = 10
+        self.num_threads = os.cpu_count() or 1
+        self.device = 0
+        self.ef_construction_ = None
+        self.M_ = None
+
+    # -- construction (CPU) --
+    def init_index(self, max_elements, M=16, ef_construction=200, random_seed=100, allow_replace_deleted=False):
+        if self._h is not None:
+            raise RuntimeError("The index is already initiated.")
+        if allow_replace_deleted:
+            raise RuntimeError("allow_replace_deleted: deletions are outside the GPU query path")
+        h = ctypes.c_void_p()
+        _lib.call("hg_hnsw_create", self.space_id, self.dim, int(max_elements), int(M), int(ef_construction),
+                  int(random_seed), ctypes.byref(h))
+        self._h = h
+        _lib.call("hg_hnsw_set_ef", self._h, self.default_ef)
+        self._dirty = True
+
+    def add_items(self, data, ids=None, num_threads=-1, replace_deleted=False):
+        if self._h is None:
+            raise RuntimeError("Index not initialized")
+        if replace_deleted:
+            raise RuntimeError("Replacement of deleted elements is disabled in constructor")
+        arr = _as_rows(data, self.dim)
+        if self.normalize:
+            arr = normalize_rows(arr)
+        lab = None
+        if ids is not None:
+            lab = np.ascontiguousarray(np.asarray(ids, dtype=np.uint64).reshape(-1))
+            if lab.shape[0] != arr.shape[0]:
+                raise RuntimeError("The input label shape %d does not match the input data vector shape %d"
+                                   % (lab.ndim, arr.shape[0]))
+        nt = self.num_threads if num_threads <= 0 else num_threads
+        _lib.call("hg_hnsw_add_items", self._h, _ptr(arr), arr.shape[0], _ptr(lab), int(nt))
+        self._dirty = True
+
+    # -- parameters --
+    def set_ef(self, ef):
+        self.default_ef = int(ef)
+        if self._h is not None:
+            _lib.call("hg_hnsw_set_ef", self._h, int(ef))
+
+    @property
+    def ef(self):
+        return self.default_ef
+
+    @ef.setter
+    def ef(self, v):
+        self.set_ef(v)
+
+    def set_num_threads(self, n):
+        self.num_threads = int(n)
+
+    def _info(self):
+        info = np.zeros(8, dtype=np.int64)
+        _lib.call("hg_hnsw_info", self._h, _ptr(info))
+        return info
+
+    def get_current_count(self):
+        return int(self._info()[0]) if self._h is not None else 0
+
+    @property
+    def element_count(self):
+        return self.get_current_count()
+
+    def get_max_elements(self):
+        return int(self._info()[1])
+
+    @property
+    def max_elements(self):
+        return self.get_max_elements() if self._h is not None else 0
+
+    @property
+    def M(self):
+        return int(self._info()[3])
+
+    @property
+    def ef_construction(self):
+        return int(self._info()[4])
+
+    # -- persistence (hnswlib on-disk format) --
+    def save_index(self, path):
+        _lib.call("hg_hnsw_save", self._h, str(path).encode())
+
+    def serialize(self):
+        size = ctypes.c_size_t()
+        _lib.call("hg_hnsw_serialize", self._h, None, 0, ctypes.byref(size))
+        buf = np.empty(size.value, dtype=np.uint8)
+        _lib.call("hg_hnsw_serialize", self._h, _ptr(buf), size.value, ctypes.byref(size))
+        return buf
+
+    def load_index(self, path, max_elements=0, allow_replace_deleted=False):
+        if self._h is not None:
+            _lib.load().hg_hnsw_free(self._h)
+            self._h = None
+        h = ctypes.c_void_p()
+        _lib.call("hg_hnsw_load", self.space_id, self.dim, str(path).encode(), int(max_elements), ctypes.byref(h))
+        self._h = h
+        _lib.call("hg_hnsw_set_ef", self._h, self.default_ef)
+        self._dirty = True
+
+    # -- query (GPU) --
+    def gpu(self):
+        """The device copy of the index, exported on first use after a change."""
+        if self._gpu is None or self._dirty:
+            if self._gpu is not None:
+                self._gpu.free()
+            self._gpu = GpuHnsw(self._h, self.device)
+            self._dirty = False
+        return self._gpu
+
+    def knn_query(self, data, k=1, num_threads=-1, filter=None):
+        if filter is not None:
+            raise RuntimeError("filter: the filtered (non-bare-bone) search is not implemented by the GPU engine")
+        if self._h is None:
+            raise RuntimeError("Index not initialized")
+        arr = _as_rows(data, self.dim)
+        if self.normalize:
+            arr = normalize_rows(arr)
+        k = int(k)
+        if arr.shape[0] == 0:
+            return np.empty((0, k), dtype=np.uint64), np.empty((0, k), dtype=np.float32)
+        labels, dists, _, short = self.gpu().search(arr, k, self.default_ef)
+        if short:
+            raise RuntimeError("Cannot return the results in a contiguous 2D array. Probably ef or M is too small")
+        return labels, dists
+
+    def cpu_handle(self):
+        return self._h
+
+    def __del__(self):
+        try:
+            if self._gpu is not None:
+                self._gpu.free()
+            if self._h is not None:
+                _lib.load().hg_hnsw_free(self._h)
+        except Exception:
+            pass
+
+    def __repr__(self):
+        return f"<hnsw_rabitq_amd.Index(space='{self.space}', dim={self.dim})>"
+
+
+class BFIndex:
+    """hnswlib.BFIndex mirror (bindings.cpp:725-929, 1011-1023) on the GPU."""
+
+    def __init__(self, space, dim):
+        self.space_id, self.normalize = _space(space)
+        self.space = space
+        self.dim = int(dim)
+        self._h = None
+        self.num_threads = os.cpu_count() or 1
+        self.device = 0
+
+    def init_index(self, max_elements):
+        if self._h is not None:
+            raise RuntimeError("The index is already initiated.")
+        h = ctypes.c_void_p()
+        _lib.call("hg_bf_create", self.space_id, self.dim, int(max_elements), self.device, ctypes.byref(h))
+        self._h = h
+
+    def add_items(self, data, ids=None):
+        arr = _as_rows(data, self.dim)
+        if self.normalize:
+            arr = normalize_rows(arr)
+        lab = None
+        if ids is not None:
+            lab = np.ascontiguousarray(np.asarray(ids, dtype=np.uint64).reshape(-1))
+            if lab.shape[0] != arr.shape[0]:
+                raise RuntimeError("The input label shape %d does not match the input data vector shape %d"
+                                   % (lab.ndim, arr.shape[0]))
+        _lib.call("hg_bf_add", self._h, _ptr(arr), arr.shape[0], _ptr(lab))
+
+    def get_current_count(self):
+        c = ctypes.c_size_t()
+        _lib.call("hg_bf_count", self._h, ctypes.byref(c))
+        return c.value
+
+    def set_num_threads(self, n):
+        self.num_threads = int(n)
+
+    def knn_query(self, data, k=1, num_threads=-1, filter=None):
+        if filter is not None:
+            raise RuntimeError("filter: filtered brute force is not implemented by the GPU engine")
+        arr = _as_rows(data, self.dim)
+        if self.normalize:
+            arr = normalize_rows(arr)
+        k = int(k)
+        nq = arr.shape[0]
+        labels = np.empty((nq, k), dtype=np.uint64)
+        dists = np.empty((nq, k), dtype=np.float32)
+        if nq == 0:
+            return labels, dists
+        code = _lib.load().hg_bf_search(self._h, _ptr(arr), nq, k, _ptr(labels), _ptr(dists))
+        if code == _lib.HG_ERR_SHORT:
+            raise RuntimeError("Cannot return the results in a contiguous 2D array. There are not enough elements.")
+        _lib.check(code)
+        return labels, dists
+
+    def __del__(self):
+        try:
+            if self._h is not None:
+                _lib.load().hg_bf_free(self._h)
+        except Exception:
+            pass
+
+    def __repr__(self):
+        return f"<hnsw_rabitq_amd.BFIndex(space='{self.space}', dim={self.dim})>"

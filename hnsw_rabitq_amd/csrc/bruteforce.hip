@@ -1,0 +1,879 @@
+// Brute-force k-NN on gfx950.
+//
+// Exact path (BruteforceSearch::searchKnn, bruteforce.h:106-125): the reference keeps the k
+// lexicographically smallest (distance, label) pairs of a full scan.  Here:
+//   1. distance tiles: exact FP32 distances in the reference's order for a chunk of queries
+//      against every row, written to a scratch matrix;
+//   2. selection: one workgroup per query streams its row of the matrix, keeps every entry
+//      below the running k-th (distance, label) key in an LDS buffer, compacts by sorting when
+//      the buffer fills, and writes the k smallest closest first.
+//
+// Reduced-precision path (BF16 / FP8 e4m3 MFMA with FP32 accumulation): the distance GEMM
+// produces approximate scores; a per-query threshold keeps about c candidates; their exact
+// FP32 distances (reference order) are recomputed and the k smallest (distance, label) pairs
+// are selected as above.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "common.h"
+#include "dist.h"
+#include "kernels.h"
+
+namespace hg {
+
+// ------------------------------------------------------------------------------------------
+// Exact distance tile, dim % 16 == 0.  32 queries x 32 rows per workgroup of 256 threads; each
+// thread owns 2 queries x 2 rows and keeps the 16 reference accumulator lanes per pair, so the
+// dimension can be streamed through LDS in chunks of 64 coordinates without changing the order
+// of any accumulation: acc[l] takes coordinates l, l+16, l+32, ... in increasing order.
+// ------------------------------------------------------------------------------------------
+constexpr int TQ = 32, TR = 32, TC = 64, TPAD = 4;
+
+template <int SPACE>
+__global__ __launch_bounds__(256) void bf_dist_tile_kernel(const float* __restrict__ Q, uint32_t nq,
+                                                           const float* __restrict__ X, uint32_t n, uint32_t dim,
+                                                           float* __restrict__ out, size_t ld_out) {
+    __shared__ __align__(16) float Qs[TQ][TC + TPAD];
+    __shared__ __align__(16) float Xs[TR][TC + TPAD];
+    const int tid = threadIdx.x;
+    const uint32_t q0 = blockIdx.y * TQ, r0 = blockIdx.x * TR;
+    const int tq = (tid >> 4) * 2, tr = (tid & 15) * 2;
+    float acc[4][16];
+#pragma unroll
+    for (int p = 0; p < 4; p++)
+#pragma unroll
+        for (int l = 0; l < 16; l++) acc[p][l] = 0.f;
+
+    for (uint32_t c0 = 0; c0 < dim; c0 += TC) {
+        const uint32_t cw = dim - c0 < (uint32_t)TC ? dim - c0 : (uint32_t)TC;  // multiple of 16
+        // load 32 x cw of queries and rows (float4 granules)
+        for (int i = tid; i < TQ * (TC / 4); i += 256) {
+            int rr = i / (TC / 4), cc = (i % (TC / 4)) * 4;
+            float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (q0 + rr < nq && (uint32_t)cc < cw)
+                v = *reinterpret_cast<const float4*>(Q + (size_t)(q0 + rr) * dim + c0 + cc);
+            *reinterpret_cast<float4*>(&Qs[rr][cc]) = v;
+            float4 w = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (r0 + rr < n && (uint32_t)cc < cw)
+                w = *reinterpret_cast<const float4*>(X + (size_t)(r0 + rr) * dim + c0 + cc);
+            *reinterpret_cast<float4*>(&Xs[rr][cc]) = w;
+        }
+        __syncthreads();
+        for (uint32_t c = 0; c < cw; c += 16) {
+#pragma unroll
+            for (int l4 = 0; l4 < 16; l4 += 4) {
+                float4 qa = *reinterpret_cast<const float4*>(&Qs[tq][c + l4]);
+                float4 qb = *reinterpret_cast<const float4*>(&Qs[tq + 1][c + l4]);
+                float4 xa = *reinterpret_cast<const float4*>(&Xs[tr][c + l4]);
+                float4 xb = *reinterpret_cast<const float4*>(&Xs[tr + 1][c + l4]);
+                const float qv[2][4] = {{qa.x, qa.y, qa.z, qa.w}, {qb.x, qb.y, qb.z, qb.w}};
+                const float xv[2][4] = {{xa.x, xa.y, xa.z, xa.w}, {xb.x, xb.y, xb.z, xb.w}};
+#pragma unroll
+                for (int a = 0; a < 2; a++)
+#pragma unroll
+                    for (int b = 0; b < 2; b++)
+#pragma unroll
+                        for (int j = 0; j < 4; j++) {
+                            if (SPACE == 0) {
+                                float d = qv[a][j] - xv[b][j];
+                                acc[a * 2 + b][l4 + j] = fma_(d, d, acc[a * 2 + b][l4 + j]);
+                            } else {
+                                acc[a * 2 + b][l4 + j] = fma_(qv[a][j], xv[b][j], acc[a * 2 + b][l4 + j]);
+                            }
+                        }
+            }
+        }
+        __syncthreads();
+    }
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+        for (int b = 0; b < 2; b++) {
+            const float* A = acc[a * 2 + b];
+            float r;
+            if (SPACE == 0) {
+                r = A[0];
+#pragma unroll
+                for (int l = 1; l < 16; l++) r = r + A[l];
+            } else {
+                float h8[8], h4[4];
+#pragma unroll
+                for (int i = 0; i < 8; i++) h8[i] = A[i] + A[i + 8];
+#pragma unroll
+                for (int i = 0; i < 4; i++) h4[i] = h8[i] + h8[i + 4];
+                float s0 = h4[0] + h4[2], s1 = h4[1] + h4[3];
+                r = 1.0f - (s0 + s1);
+            }
+            uint32_t qq = q0 + tq + a, rr = r0 + tr + b;
+            if (qq < nq && rr < n) out[(size_t)qq * ld_out + rr] = r;
+        }
+}
+
+// Any dimension: one thread per (query, row) pair, the reference kernel for the dimension class.
+__global__ __launch_bounds__(256) void bf_dist_generic_kernel(int space, const float* __restrict__ Q, uint32_t nq,
+                                                              const float* __restrict__ X, uint32_t n, uint32_t dim,
+                                                              uint32_t stride, float* __restrict__ out,
+                                                              size_t ld_out) {
+    uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    uint32_t q = blockIdx.y;
+    if (r >= n || q >= nq) return;
+    out[(size_t)q * ld_out + r] = dist_generic(space, Q + (size_t)q * dim, X + (size_t)r * stride, (int)dim);
+}
+
+// ------------------------------------------------------------------------------------------
+// Selection: k smallest (distance, label) of one row of `dists` (length n), closest first.
+// Optional `rows` maps position -> base row (candidate lists); labels come from base rows.
+// ------------------------------------------------------------------------------------------
+constexpr int SEL_THREADS = 256;
+constexpr int SEL_CAP = 4096;    // LDS candidate buffer
+constexpr int SEL_CHUNK = 2048;  // elements scanned between compactions
+
+struct SelEntry {
+    float d;
+    uint32_t r;
+};
+
+__device__ __forceinline__ bool sel_less(const SelEntry& a, const SelEntry& b, const uint64_t* labels) {
+    if (a.d < b.d) return true;
+    if (b.d < a.d) return false;
+    if (a.r == b.r) return false;
+    if (a.r == SENT) return false;
+    if (b.r == SENT) return true;
+    return labels[a.r] < labels[b.r];
+}
+
+// Bitonic sort of buf[0, cnt) (padded to a power of two with sentinels) by (distance, label).
+__device__ void sel_sort(SelEntry* buf, int cnt, const uint64_t* labels) {
+    int size = 1;
+    while (size < cnt) size <<= 1;
+    for (int i = cnt + (int)threadIdx.x; i < size; i += SEL_THREADS) buf[i] = SelEntry{INFINITY, SENT};
+    __syncthreads();
+    for (int k = 2; k <= size; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            for (int i = threadIdx.x; i < size; i += SEL_THREADS) {
+                int p = i ^ j;
+                if (p > i) {
+                    bool up = (i & k) == 0;
+                    SelEntry a = buf[i], b = buf[p];
+                    bool sw = up ? sel_less(b, a, labels) : sel_less(a, b, labels);
+                    if (sw) {
+                        buf[i] = b;
+                        buf[p] = a;
+                    }
+                }
+            }
+            __syncthreads();
+        }
+    }
+}
+
+__global__ __launch_bounds__(SEL_THREADS) void bf_select_kernel(const float* __restrict__ dists, size_t ld,
+                                                                 uint32_t n, const uint32_t* __restrict__ rows,
+                                                                 size_t ld_rows, const uint64_t* __restrict__ labels,
+                                                                 uint32_t k, uint32_t q_offset,
+                                                                 uint64_t* __restrict__ out_l, float* __restrict__ out_d,
+                                                                 size_t ld_outp) {
+    __shared__ SelEntry buf[SEL_CAP + SEL_CHUNK];
+    __shared__ int cnt;
+    __shared__ SelEntry thr;
+    const uint32_t q = blockIdx.x;
+    const float* row = dists + (size_t)q * ld;
+    const uint32_t* rmap = rows ? rows + (size_t)q * ld_rows : nullptr;
+    if (threadIdx.x == 0) {
+        cnt = 0;
+        thr = SelEntry{INFINITY, SENT};
+    }
+    __syncthreads();
+    for (uint32_t base = 0; base < n; base += SEL_CHUNK) {
+        SelEntry t = thr;
+        for (uint32_t i = base + threadIdx.x; i < base + SEL_CHUNK && i < n; i += SEL_THREADS) {
+            uint32_t r = rmap ? rmap[i] : i;
+            if (r == SENT) continue;
+            SelEntry e{row[i], r};
+            if (sel_less(e, t, labels) || t.r == SENT) {
+                int pos = atomicAdd(&cnt, 1);
+                buf[pos] = e;
+            }
+        }
+        __syncthreads();
+        if (cnt > SEL_CAP - SEL_CHUNK || base + SEL_CHUNK >= n) {
+            int c = cnt;
+            sel_sort(buf, c, labels);
+            int keep = c < (int)k ? c : (int)k;
+            if (threadIdx.x == 0) {
+                cnt = keep;
+                if (keep == (int)k) thr = buf[k - 1];
+            }
+            __syncthreads();
+        }
+    }
+    int c = cnt;
+    uint64_t* ol = out_l + (size_t)(q + q_offset) * ld_outp;
+    float* od = out_d + (size_t)(q + q_offset) * ld_outp;
+    for (uint32_t i = threadIdx.x; i < k; i += SEL_THREADS) {
+        if ((int)i < c) {
+            ol[i] = labels[buf[i].r];
+            od[i] = buf[i].d;
+        } else {
+            ol[i] = ~0ull;
+            od[i] = INFINITY;
+        }
+    }
+}
+
+size_t bf_exact_scratch_bytes(uint32_t nq, uint32_t k, uint32_t n) {
+    (void)k;
+    // a chunk of queries x n distances, at most 1 GiB (at least one query row)
+    size_t per_q = (size_t)n * 4;
+    size_t qc = std::max<size_t>(1, std::min<size_t>(nq, ((size_t)1 << 30) / std::max<size_t>(per_q, 1)));
+    qc = (qc + TQ - 1) / TQ * TQ;
+    return qc * per_q;
+}
+
+hipError_t launch_bf_exact(const BfData& d, const float* q, uint32_t nq, uint32_t k, uint64_t* labels, float* dists,
+                           void* scratch, hipStream_t s) {
+    size_t scratch_bytes = bf_exact_scratch_bytes(nq, k, d.n);
+    uint32_t qc = (uint32_t)(scratch_bytes / ((size_t)d.n * 4));
+    float* mat = (float*)scratch;
+    for (uint32_t q0 = 0; q0 < nq; q0 += qc) {
+        uint32_t cq = nq - q0 < qc ? nq - q0 : qc;
+        const float* qp = q + (size_t)q0 * d.dim;
+        if (d.dim % 16 == 0) {
+            dim3 grid((d.n + TR - 1) / TR, (cq + TQ - 1) / TQ);
+            if (d.space == 0)
+                hipLaunchKernelGGL(bf_dist_tile_kernel<0>, grid, dim3(256), 0, s, qp, cq, d.vec, d.n, d.dim, mat,
+                                   (size_t)d.n);
+            else
+                hipLaunchKernelGGL(bf_dist_tile_kernel<1>, grid, dim3(256), 0, s, qp, cq, d.vec, d.n, d.dim, mat,
+                                   (size_t)d.n);
+        } else {
+            dim3 grid((d.n + 255) / 256, cq);
+            hipLaunchKernelGGL(bf_dist_generic_kernel, grid, dim3(256), 0, s, (int)d.space, qp, cq, d.vec, d.n, d.dim,
+                               d.dim, mat, (size_t)d.n);
+        }
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(bf_select_kernel, dim3(cq), dim3(SEL_THREADS), 0, s, mat, (size_t)d.n, d.n,
+                           (const uint32_t*)nullptr, (size_t)0, d.labels, k, q0, labels, dists, (size_t)k);
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+    }
+    return hipSuccess;
+}
+
+// ==========================================================================================
+// Reduced precision: BF16 / FP8 (OCP e4m3, per-vector scale) MFMA distance GEMM.
+// ==========================================================================================
+typedef __bf16 bf16x8_t __attribute__((ext_vector_type(8)));
+typedef float f32x16_t __attribute__((ext_vector_type(16)));
+typedef int i32x8_t __attribute__((ext_vector_type(8)));
+
+__device__ __forceinline__ uint16_t f32_to_bf16_rne(float x) {
+    uint32_t u = __float_as_uint(x);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
+
+// float -> OCP FP8 E4M3 (bias 7, max 448, no infinities), round to nearest even, saturating.
+__device__ __forceinline__ uint8_t f32_to_e4m3(float x) {
+    uint8_t sign = x < 0.f ? 0x80 : 0x00;
+    float a = fabsf(x);
+    if (!(a > 0.f)) return sign;  // zero (and NaN mapped to zero)
+    if (a >= 448.f) return sign | 0x7E;
+    int e;
+    float m = frexpf(a, &e);  // a = m * 2^e, m in [0.5, 1)
+    int ex = e - 1;           // a = (2m) * 2^ex, 2m in [1, 2)
+    if (ex < -6) {
+        // subnormal: units of 2^-9
+        float q = rintf(ldexpf(a, 9));
+        return sign | (uint8_t)q;  // q == 8 encodes the smallest normal
+    }
+    float mant = rintf((2.f * m - 1.f) * 8.f);
+    if (mant >= 8.f) {
+        mant = 0.f;
+        ex += 1;
+    }
+    int code = ((ex + 7) << 3) | (int)mant;
+    if (code > 0x7E) code = 0x7E;
+    return sign | (uint8_t)code;
+}
+
+// Row conversion: bf16 copy, or e4m3 copy with scale amax/448 per row; optional squared norms.
+__global__ __launch_bounds__(256) void convert_rows_kernel(const float* __restrict__ X, uint32_t n, uint32_t dim,
+                                                           uint16_t* __restrict__ out_bf16,
+                                                           uint8_t* __restrict__ out_fp8, float* __restrict__ scale) {
+    const uint32_t row = blockIdx.x;
+    if (row >= n) return;
+    const float* x = X + (size_t)row * dim;
+    __shared__ float red[256];
+    float amax = 0.f;
+    for (uint32_t i = threadIdx.x; i < dim; i += 256) amax = fmaxf(amax, fabsf(x[i]));
+    red[threadIdx.x] = amax;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] = fmaxf(red[threadIdx.x], red[threadIdx.x + s]);
+        __syncthreads();
+    }
+    amax = red[0];
+    if (out_bf16) {
+        for (uint32_t i = threadIdx.x; i < dim; i += 256) out_bf16[(size_t)row * dim + i] = f32_to_bf16_rne(x[i]);
+        if (scale && threadIdx.x == 0) scale[row] = 1.0f;
+    } else {
+        float sc = amax > 0.f ? amax / 448.f : 1.0f;
+        float inv = 1.0f / sc;
+        for (uint32_t i = threadIdx.x; i < dim; i += 256) out_fp8[(size_t)row * dim + i] = f32_to_e4m3(x[i] * inv);
+        if (threadIdx.x == 0) scale[row] = sc;
+    }
+}
+
+// GEMM tile: 128 queries x 128 rows per workgroup of 4 waves (2 x 2 waves of 64 x 64), each
+// wave 2 x 2 MFMA 32x32 blocks.  LDS holds 128-byte k-slices of both operands (64 bf16 or 128
+// fp8 per row), double buffered.  MFMA operand layout (gfx950): lane l supplies row l % 32 and
+// the contiguous k run starting at (l / 32) * (k per lane); the 32x32 result holds
+// (row (i/4)*8 + (l/32)*4 + i%4, column l % 32) in register i of lane l.
+constexpr int GM = 128, GN = 128, GKB = 128, GPAD = 16;  // GKB = bytes of k per stage
+
+struct LowpGemmArgs {
+    const uint8_t* A;        // queries, [nq][dim] low precision
+    const float* a_scale;    // [nq]
+    const uint8_t* B;        // rows, [n][dim] low precision
+    const float* b_scale;    // [n]
+    const float* b_norm;     // [n] squared norms (L2), FP32
+    const float* a_norm;     // [nq] squared norms (L2)
+    uint32_t nq, n, dim;     // dim in elements
+    int32_t space;
+    // sample mode: write every approximate distance
+    float* out_mat;
+    size_t ld_out;
+    // filter mode: keep rows whose approximate distance is below tau[q]
+    const float* tau;
+    uint32_t* cand_rows;     // [nq][cap]
+    float* cand_d;           // [nq][cap]
+    uint32_t* cand_cnt;      // [nq]
+    uint32_t cap;
+    uint32_t tiles_m, tiles_n;
+};
+
+template <int PREC, bool FILTER>
+__global__ __launch_bounds__(256, 2) void lowp_gemm_kernel(LowpGemmArgs g) {
+    __shared__ __align__(16) uint8_t As[2][GM][GKB + GPAD];
+    __shared__ __align__(16) uint8_t Bs[2][GN][GKB + GPAD];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    // XCD-aware tile order: consecutive workgroups land on different XCDs, so each XCD takes a
+    // contiguous range of tiles; inside it the query tile varies fastest so the row tile is
+    // shared through that XCD's L2.
+    const uint32_t ntiles = g.tiles_m * g.tiles_n;
+    const uint32_t b = blockIdx.x;
+    const uint32_t per_xcd = (ntiles + 7) / 8;
+    uint32_t t = (b % 8) * per_xcd + b / 8;
+    if (t >= ntiles) return;
+    const uint32_t tm = t % g.tiles_m, tn = t / g.tiles_m;
+    const uint32_t m0 = tm * GM, n0 = tn * GN;
+    const size_t row_bytes = (size_t)g.dim * (PREC == 1 ? 2 : 1);
+
+    f32x16_t acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+#pragma unroll
+            for (int r = 0; r < 16; r++) acc[i][j][r] = 0.f;
+
+    // each thread moves 4 x 16 bytes of A and of B per stage
+    uint4 ra[4], rb[4];
+    auto gload = [&](size_t kb) {
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            int idx = tid + 256 * i;
+            int row = idx >> 3, ch = idx & 7;
+            uint32_t qa = m0 + row, qb = n0 + row;
+            ra[i] = qa < g.nq ? *reinterpret_cast<const uint4*>(g.A + qa * row_bytes + kb + ch * 16)
+                              : make_uint4(0, 0, 0, 0);
+            rb[i] = qb < g.n ? *reinterpret_cast<const uint4*>(g.B + qb * row_bytes + kb + ch * 16)
+                             : make_uint4(0, 0, 0, 0);
+        }
+    };
+    auto lstore = [&](int buf) {
+#pragma unroll
+        for (int i = 0; i < 4; i++) {
+            int idx = tid + 256 * i;
+            int row = idx >> 3, ch = idx & 7;
+            *reinterpret_cast<uint4*>(&As[buf][row][ch * 16]) = ra[i];
+            *reinterpret_cast<uint4*>(&Bs[buf][row][ch * 16]) = rb[i];
+        }
+    };
+    const size_t kbytes = row_bytes;
+    gload(0);
+    lstore(0);
+    __syncthreads();
+    int buf = 0;
+    for (size_t kb = 0; kb < kbytes; kb += GKB) {
+        const bool more = kb + GKB < kbytes;
+        if (more) gload(kb + GKB);
+        // MFMAs on the current stage
+        if (PREC == 1) {
+#pragma unroll
+            for (int ks = 0; ks < GKB; ks += 32) {  // 16 bf16 = 32 bytes per MFMA k-step
+                bf16x8_t fa[2], fb[2];
+#pragma unroll
+                for (int i = 0; i < 2; i++) {
+                    const uint8_t* pa = &As[buf][wm * 64 + i * 32 + (lane & 31)][ks + (lane >> 5) * 16];
+                    const uint8_t* pb = &Bs[buf][wn * 64 + i * 32 + (lane & 31)][ks + (lane >> 5) * 16];
+                    fa[i] = *reinterpret_cast<const bf16x8_t*>(pa);
+                    fb[i] = *reinterpret_cast<const bf16x8_t*>(pb);
+                }
+#pragma unroll
+                for (int i = 0; i < 2; i++)
+#pragma unroll
+                    for (int j = 0; j < 2; j++)
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[i], fb[j], acc[i][j], 0, 0, 0);
+            }
+        } else {
+#pragma unroll
+            for (int ks = 0; ks < GKB; ks += 64) {  // 64 fp8 per MFMA k-step
+                i32x8_t fa[2], fb[2];
+#pragma unroll
+                for (int i = 0; i < 2; i++) {
+                    const uint8_t* pa = &As[buf][wm * 64 + i * 32 + (lane & 31)][ks + (lane >> 5) * 32];
+                    const uint8_t* pb = &Bs[buf][wn * 64 + i * 32 + (lane & 31)][ks + (lane >> 5) * 32];
+                    fa[i] = *reinterpret_cast<const i32x8_t*>(pa);
+                    fb[i] = *reinterpret_cast<const i32x8_t*>(pb);
+                }
+#pragma unroll
+                for (int i = 0; i < 2; i++)
+#pragma unroll
+                    for (int j = 0; j < 2; j++)
+                        acc[i][j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(fa[i], fb[j], acc[i][j], 0, 0, 0,
+                                                                                    127, 0, 127);
+            }
+        }
+        if (more) {
+            lstore(buf ^ 1);
+            __syncthreads();
+            buf ^= 1;
+        }
+    }
+
+    // epilogue
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+            const uint32_t col = n0 + wn * 64 + j * 32 + (lane & 31);
+            const bool col_ok = col < g.n;
+            const float bsc = (PREC == 2 && col_ok) ? g.b_scale[col] : 1.0f;
+            const float bn = (g.space == 0 && col_ok) ? g.b_norm[col] : 0.f;
+#pragma unroll
+            for (int r = 0; r < 16; r++) {
+                const uint32_t row = m0 + wm * 64 + i * 32 + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3);
+                const bool ok = col_ok && row < g.nq;
+                float s = acc[i][j][r];
+                if (PREC == 2 && ok) s = s * g.a_scale[row] * bsc;
+                float d = g.space == 0 ? (ok ? g.a_norm[row] : 0.f) + bn - 2.0f * s : 1.0f - s;
+                if (!FILTER) {
+                    if (ok) g.out_mat[(size_t)row * g.ld_out + col] = d;
+                } else {
+                    bool pass = ok && d < g.tau[row < g.nq ? row : 0];
+                    uint64_t m = __ballot(pass);
+                    uint32_t half = lane >> 5;
+                    uint32_t hm = (uint32_t)(m >> (32 * half));
+                    if (hm) {
+                        uint32_t leader = __ffs(hm) - 1;
+                        uint32_t base = 0;
+                        if ((lane & 31) == leader) base = atomicAdd(&g.cand_cnt[row], (uint32_t)__popc(hm));
+                        base = __shfl(base, (int)(half * 32 + leader));
+                        if (pass) {
+                            uint32_t slot = base + __popc(hm & ((1u << (lane & 31)) - 1u));
+                            if (slot < g.cap) {
+                                g.cand_rows[(size_t)row * g.cap + slot] = col;
+                                g.cand_d[(size_t)row * g.cap + slot] = d;
+                            }
+                        }
+                    }
+                }
+            }
+        }
+    }
+}
+
+// Copy every `step`-th row of a low-precision matrix (and its scale / norm) to a compact sample.
+__global__ void gather_rows_kernel(const uint8_t* __restrict__ src, size_t row_bytes, uint32_t step, uint32_t s_rows,
+                                   uint8_t* __restrict__ dst, const float* sc, float* dsc, const float* nm,
+                                   float* dnm) {
+    uint32_t r = blockIdx.x;
+    if (r >= s_rows) return;
+    const uint8_t* a = src + (size_t)r * step * row_bytes;
+    uint8_t* b = dst + (size_t)r * row_bytes;
+    for (size_t i = threadIdx.x * 16; i < row_bytes; i += blockDim.x * 16)
+        *reinterpret_cast<uint4*>(b + i) = *reinterpret_cast<const uint4*>(a + i);
+    if (threadIdx.x == 0) {
+        if (sc) dsc[r] = sc[(size_t)r * step];
+        if (nm) dnm[r] = nm[(size_t)r * step];
+    }
+}
+
+// Squared norms in FP32 (L2 form of the approximate distance).
+__global__ void row_norms_kernel(const float* __restrict__ X, uint32_t n, uint32_t dim, float* __restrict__ out) {
+    uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= n) return;
+    const float* x = X + (size_t)r * dim;
+    float s = 0.f;
+    for (uint32_t i = 0; i < dim; i++) s = fma_(x[i], x[i], s);
+    out[r] = s;
+}
+
+__device__ __forceinline__ uint32_t f2key(float f) {
+    uint32_t u = __float_as_uint(f);
+    return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float key2f(uint32_t k) {
+    uint32_t u = (k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k;
+    return __uint_as_float(u);
+}
+
+// tau[q] = the rank-th smallest value of row q of the sample matrix (radix select, 4 x 8 bits),
+// nudged up by one ulp so that exactly the sample values below it and at it pass `d < tau`.
+__global__ __launch_bounds__(256) void kth_kernel(const float* __restrict__ mat, size_t ld, uint32_t s,
+                                                  uint32_t rank, float* __restrict__ tau) {
+    __shared__ uint32_t hist[256];
+    __shared__ uint32_t prefix, want;
+    const float* row = mat + (size_t)blockIdx.x * ld;
+    if (threadIdx.x == 0) {
+        prefix = 0;
+        want = rank;  // 1-based rank among the remaining candidates
+    }
+    __syncthreads();
+    for (int shift = 24; shift >= 0; shift -= 8) {
+        hist[threadIdx.x] = 0;
+        __syncthreads();
+        const uint32_t pre = prefix;
+        const uint32_t mask_hi = shift == 24 ? 0u : (0xFFFFFFFFu << (shift + 8));
+        for (uint32_t i = threadIdx.x; i < s; i += 256) {
+            uint32_t k = f2key(row[i]);
+            if ((k & mask_hi) == pre) atomicAdd(&hist[(k >> shift) & 255u], 1u);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t w = want, acc = 0, bsel = 255;
+            for (uint32_t bkt = 0; bkt < 256; bkt++) {
+                if (acc + hist[bkt] >= w) {
+                    bsel = bkt;
+                    break;
+                }
+                acc += hist[bkt];
+            }
+            want = w - acc;
+            prefix = pre | (bsel << shift);
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        float v = key2f(prefix);
+        tau[blockIdx.x] = nextafterf(v, INFINITY);
+    }
+}
+
+// Per query: take the candidates of the filter pass (at most cap), keep the c best by
+// approximate distance, recompute their exact FP32 distances in the reference order, and write
+// the k smallest (distance, label) closest first.  A query whose candidate count overflowed the
+// buffer is flagged for the exact path.
+constexpr int RR_THREADS = 256;
+constexpr int RR_CAP = 8192;
+
+template <int SPACE>
+__global__ __launch_bounds__(RR_THREADS) void rerank_kernel(const float* __restrict__ Q, const float* __restrict__ X,
+                                                            const uint64_t* __restrict__ labels, uint32_t dim,
+                                                            const uint32_t* __restrict__ cand_rows,
+                                                            const float* __restrict__ cand_d,
+                                                            const uint32_t* __restrict__ cand_cnt, uint32_t cap,
+                                                            uint32_t c, uint32_t k, uint64_t* __restrict__ out_l,
+                                                            float* __restrict__ out_d, size_t ld_out,
+                                                            uint32_t* __restrict__ overflow) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    SelEntry* buf = reinterpret_cast<SelEntry*>(smem);  // RR_CAP entries
+    float* qs = reinterpret_cast<float*>(buf + RR_CAP);  // dim floats
+    const uint32_t q = blockIdx.x;
+    const uint32_t cnt = cand_cnt[q];
+    uint64_t* ol = out_l + (size_t)q * ld_out;
+    float* od = out_d + (size_t)q * ld_out;
+    if (cnt > cap || cnt < k) {
+        if (threadIdx.x == 0) overflow[q] = 1;
+        return;
+    }
+    for (uint32_t i = threadIdx.x; i < dim; i += RR_THREADS) qs[i] = Q[(size_t)q * dim + i];
+    for (uint32_t i = threadIdx.x; i < cnt; i += RR_THREADS)
+        buf[i] = SelEntry{cand_d[(size_t)q * cap + i], cand_rows[(size_t)q * cap + i]};
+    __syncthreads();
+    uint32_t m = cnt;
+    if (cnt > c) {
+        // order by approximate distance, ties by row (labels are not needed for this cut)
+        int size = 1;
+        while (size < (int)cnt) size <<= 1;
+        for (int i = cnt + threadIdx.x; i < size; i += RR_THREADS) buf[i] = SelEntry{INFINITY, SENT};
+        __syncthreads();
+        for (int kk = 2; kk <= size; kk <<= 1)
+            for (int j = kk >> 1; j > 0; j >>= 1) {
+                for (int i = threadIdx.x; i < size; i += RR_THREADS) {
+                    int p = i ^ j;
+                    if (p > i) {
+                        SelEntry a = buf[i], b = buf[p];
+                        bool a_gt = a.d > b.d || (a.d == b.d && a.r > b.r);
+                        bool up = (i & kk) == 0;
+                        if (up ? a_gt : !a_gt) {
+                            buf[i] = b;
+                            buf[p] = a;
+                        }
+                    }
+                }
+                __syncthreads();
+            }
+        m = c;
+    }
+    // exact distances, four lanes per row (reference order)
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int grp = lane >> 2, sub = lane & 3;
+    const int nch = (int)(dim >> 4);
+    for (uint32_t base = wave * 16; base < m; base += 4 * 16) {
+        uint32_t idx = base + grp;
+        bool act = idx < m;
+        uint32_t r = act ? buf[idx].r : 0u;
+        float res;
+        dist_group16<SPACE, 0>(X + (size_t)r * dim, qs, nch, act, sub, res, lane);
+        if (act && sub == 0) buf[idx].d = res;
+    }
+    __syncthreads();
+    sel_sort(buf, (int)m, labels);
+    for (uint32_t i = threadIdx.x; i < k; i += RR_THREADS) {
+        ol[i] = labels[buf[i].r];
+        od[i] = buf[i].d;
+    }
+}
+
+size_t bf_lowp_scratch_bytes(uint32_t nq, uint32_t dim, uint32_t c, uint32_t n) {
+    (void)c;
+    const uint32_t step = n > 16384 ? (n + 16383) / 16384 : 1;
+    const uint32_t s = (n + step - 1) / step;
+    size_t b = 0;
+    b += (size_t)nq * dim;            // queries low precision (bf16 needs 2x: counted below)
+    b += (size_t)nq * dim;
+    b += (size_t)nq * 4 * 3;          // a_scale, a_norm, tau
+    b += (size_t)s * dim * 2 + (size_t)s * 8;  // sample rows + scale + norm
+    b += (size_t)nq * s * 4;          // sample matrix
+    b += (size_t)nq * RR_CAP * 8;     // candidates
+    b += (size_t)nq * 8;              // counts + overflow flags
+    b += (size_t)n * 4;               // row norms
+    return b + 16 * 256;
+}
+
+hipError_t launch_bf_convert(const BfData& d, uint16_t* vec_bf16, uint8_t* vec_fp8, float* fp8_scale, float* norms,
+                             hipStream_t s) {
+    (void)norms;
+    if (d.n == 0) return hipSuccess;
+    hipLaunchKernelGGL(convert_rows_kernel, dim3(d.n), dim3(256), 0, s, d.vec, d.n, d.dim, vec_bf16, vec_fp8,
+                       fp8_scale);
+    return hipGetLastError();
+}
+
+static inline size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
+
+hipError_t launch_bf_lowp(const BfData& d, int precision, const float* q, uint32_t nq, uint32_t k, uint32_t c,
+                          uint64_t* labels, float* dists, void* scratch, hipStream_t s, double* stats) {
+    const uint32_t n = d.n, dim = d.dim;
+    const size_t elem = precision == 1 ? 2 : 1;
+    const uint32_t step = n > 16384 ? (n + 16383) / 16384 : 1;
+    const uint32_t srows = (n + step - 1) / step;
+    char* p = (char*)scratch;
+    auto take = [&](size_t bytes) {
+        char* r = p;
+        p += align_up(bytes);
+        return r;
+    };
+    uint8_t* qlp = (uint8_t*)take((size_t)nq * dim * elem);
+    float* qsc = (float*)take((size_t)nq * 4);
+    float* qnm = (float*)take((size_t)nq * 4);
+    float* tau = (float*)take((size_t)nq * 4);
+    uint8_t* slp = (uint8_t*)take((size_t)srows * dim * elem);
+    float* ssc = (float*)take((size_t)srows * 4);
+    float* snm = (float*)take((size_t)srows * 4);
+    float* smat = (float*)take((size_t)nq * srows * 4);
+    uint32_t* crow = (uint32_t*)take((size_t)nq * RR_CAP * 4);
+    float* cd = (float*)take((size_t)nq * RR_CAP * 4);
+    uint32_t* ccnt = (uint32_t*)take((size_t)nq * 4);
+    uint32_t* ovf = (uint32_t*)take((size_t)nq * 4);
+    float* xnm = (float*)take((size_t)n * 4);
+    const uint8_t* xlp = precision == 1 ? (const uint8_t*)d.vec_bf16 : d.vec_fp8;
+    hipEvent_t ev[4];
+    for (auto& e : ev) (void)hipEventCreate(&e);
+    (void)hipEventRecord(ev[0], s);
+
+    // queries to low precision; norms for the L2 form
+    hipLaunchKernelGGL(convert_rows_kernel, dim3(nq), dim3(256), 0, s, q, nq, dim,
+                       precision == 1 ? (uint16_t*)qlp : nullptr, precision == 2 ? qlp : nullptr, qsc);
+    if (d.space == 0) {
+        hipLaunchKernelGGL(row_norms_kernel, dim3((nq + 255) / 256), dim3(256), 0, s, q, nq, dim, qnm);
+        hipLaunchKernelGGL(row_norms_kernel, dim3((n + 255) / 256), dim3(256), 0, s, d.vec, n, dim, xnm);
+    }
+    // sample pass -> per-query threshold
+    hipLaunchKernelGGL(gather_rows_kernel, dim3(srows), dim3(64), 0, s, xlp, (size_t)dim * elem, step, srows, slp,
+                       d.fp8_scale, ssc, d.space == 0 ? xnm : nullptr, snm);
+    LowpGemmArgs ga{};
+    ga.A = qlp;
+    ga.a_scale = qsc;
+    ga.a_norm = qnm;
+    ga.nq = nq;
+    ga.dim = dim;
+    ga.space = d.space;
+    ga.B = slp;
+    ga.b_scale = ssc;
+    ga.b_norm = snm;
+    ga.n = srows;
+    ga.out_mat = smat;
+    ga.ld_out = srows;
+    ga.tiles_m = (nq + GM - 1) / GM;
+    ga.tiles_n = (srows + GN - 1) / GN;
+    uint32_t nt = ga.tiles_m * ga.tiles_n;
+    uint32_t grid = (nt + 7) / 8 * 8;
+    if (precision == 1)
+        hipLaunchKernelGGL((lowp_gemm_kernel<1, false>), dim3(grid), dim3(256), 0, s, ga);
+    else
+        hipLaunchKernelGGL((lowp_gemm_kernel<2, false>), dim3(grid), dim3(256), 0, s, ga);
+    // ~1.5 c candidates expected; never below k + 1 sample ranks
+    double want = 1.5 * (double)c * (double)srows / (double)n;
+    uint32_t rank = (uint32_t)std::min<double>((double)srows, std::max<double>((double)k + 1, std::ceil(want)));
+    hipLaunchKernelGGL(kth_kernel, dim3(nq), dim3(256), 0, s, smat, (size_t)srows, srows, rank, tau);
+    (void)hipMemsetAsync(ccnt, 0, (size_t)nq * 4, s);
+    (void)hipMemsetAsync(ovf, 0, (size_t)nq * 4, s);
+    (void)hipEventRecord(ev[1], s);
+    // filter pass over every row
+    ga.B = xlp;
+    ga.b_scale = d.fp8_scale;
+    ga.b_norm = xnm;
+    ga.n = n;
+    ga.tau = tau;
+    ga.cand_rows = crow;
+    ga.cand_d = cd;
+    ga.cand_cnt = ccnt;
+    ga.cap = RR_CAP;
+    ga.tiles_n = (n + GN - 1) / GN;
+    nt = ga.tiles_m * ga.tiles_n;
+    grid = (nt + 7) / 8 * 8;
+    if (precision == 1)
+        hipLaunchKernelGGL((lowp_gemm_kernel<1, true>), dim3(grid), dim3(256), 0, s, ga);
+    else
+        hipLaunchKernelGGL((lowp_gemm_kernel<2, true>), dim3(grid), dim3(256), 0, s, ga);
+    (void)hipEventRecord(ev[2], s);
+    size_t lds = (size_t)RR_CAP * sizeof(SelEntry) + (size_t)dim * 4;
+    if (d.space == 0) {
+        (void)hipFuncSetAttribute((const void*)rerank_kernel<0>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        hipLaunchKernelGGL(rerank_kernel<0>, dim3(nq), dim3(RR_THREADS), lds, s, q, d.vec, d.labels, dim, crow, cd,
+                           ccnt, (uint32_t)RR_CAP, c, k, labels, dists, (size_t)k, ovf);
+    } else {
+        (void)hipFuncSetAttribute((const void*)rerank_kernel<1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        hipLaunchKernelGGL(rerank_kernel<1>, dim3(nq), dim3(RR_THREADS), lds, s, q, d.vec, d.labels, dim, crow, cd,
+                           ccnt, (uint32_t)RR_CAP, c, k, labels, dists, (size_t)k, ovf);
+    }
+    (void)hipEventRecord(ev[3], s);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    // queries whose candidate pass overflowed (or came up short) take the exact path
+    std::vector<uint32_t> flags(nq);
+    e = hipMemcpyAsync(flags.data(), ovf, (size_t)nq * 4, hipMemcpyDeviceToHost, s);
+    if (e != hipSuccess) return e;
+    e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return e;
+    uint32_t retried = 0;
+    for (uint32_t i = 0; i < nq; i++) {
+        if (!flags[i]) continue;
+        retried++;
+        // the sample matrix is no longer needed: reuse it as exact scratch for one query
+        size_t need = bf_exact_scratch_bytes(1, k, n);
+        if (need > (size_t)nq * srows * 4) return hipErrorInvalidValue;
+        e = launch_bf_exact(d, q + (size_t)i * dim, 1, k, labels + (size_t)i * k, dists + (size_t)i * k, smat, s);
+        if (e != hipSuccess) return e;
+    }
+    if (stats) {
+        float t0 = 0, t1 = 0, t2 = 0;
+        (void)hipEventSynchronize(ev[3]);
+        (void)hipEventElapsedTime(&t0, ev[0], ev[1]);
+        (void)hipEventElapsedTime(&t1, ev[1], ev[2]);
+        (void)hipEventElapsedTime(&t2, ev[2], ev[3]);
+        stats[0] = t1;  // filter GEMM over all rows
+        stats[1] = t0;  // conversion + sample GEMM + thresholds
+        stats[2] = t2;  // re-rank
+        stats[3] = retried;
+    }
+    for (auto& ev_ : ev) (void)hipEventDestroy(ev_);
+    return hipGetLastError();
+}
+
+// FP64 check of the final lists (see hg_bf_fp64_check).
+__global__ void fp64_check_kernel(int space, const float* __restrict__ Q, const float* __restrict__ X,
+                                  const uint64_t* __restrict__ row_of_label_is_label, uint32_t dim, uint32_t nq,
+                                  uint32_t k, const uint64_t* __restrict__ labels, const float* __restrict__ dists,
+                                  double rel_tol, unsigned long long* counters, double* maxes) {
+    uint32_t q = blockIdx.x;
+    if (q >= nq) return;
+    (void)row_of_label_is_label;
+    __shared__ double d64[2048];
+    for (uint32_t i = threadIdx.x; i < k; i += blockDim.x) {
+        uint64_t r = labels[(size_t)q * k + i];
+        double s = 0;
+        const float* a = Q + (size_t)q * dim;
+        const float* x = X + (size_t)r * dim;
+        if (space == 0) {
+            for (uint32_t j = 0; j < dim; j++) {
+                double t = (double)a[j] - (double)x[j];
+                s += t * t;
+            }
+        } else {
+            for (uint32_t j = 0; j < dim; j++) s += (double)a[j] * (double)x[j];
+            s = 1.0 - s;
+        }
+        d64[i] = s;
+        double err = fabs((double)dists[(size_t)q * k + i] - s);
+        double rel = err / fmax(fabs(s), 1e-30);
+        unsigned long long eb = (unsigned long long)__double_as_longlong(err);
+        unsigned long long rb = (unsigned long long)__double_as_longlong(rel);
+        atomicMax((unsigned long long*)&maxes[0], eb);
+        atomicMax((unsigned long long*)&maxes[1], rb);
+    }
+    __syncthreads();
+    for (uint32_t i = threadIdx.x; i + 1 < k; i += blockDim.x) {
+        float a = dists[(size_t)q * k + i], b = dists[(size_t)q * k + i + 1];
+        if ((double)b - (double)a <= rel_tol * fabs((double)a)) {
+            atomicAdd(&counters[0], 1ull);
+            if (d64[i] > d64[i + 1]) atomicAdd(&counters[1], 1ull);
+        }
+    }
+}
+
+hipError_t launch_fp64_check(const BfData& d, const float* q, uint32_t nq, uint32_t k, const uint64_t* labels,
+                             const float* dists, double rel_tol, double* out_host, void* scratch, hipStream_t s) {
+    if (k > 2048) return hipErrorInvalidValue;
+    // labels are row indices for this check (the caller's rows carry label == row)
+    unsigned long long* ctr = (unsigned long long*)scratch;
+    double* mx = (double*)((char*)scratch + 16);
+    hipError_t e = hipMemsetAsync(scratch, 0, 32, s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(fp64_check_kernel, dim3(nq), dim3(256), 0, s, (int)d.space, q, d.vec, d.labels, d.dim, nq, k,
+                       labels, dists, rel_tol, ctr, mx);
+    e = hipGetLastError();
+    if (e != hipSuccess) return e;
+    unsigned long long hc[2];
+    double hm[2];
+    e = hipMemcpyAsync(hc, ctr, 16, hipMemcpyDeviceToHost, s);
+    if (e != hipSuccess) return e;
+    e = hipMemcpyAsync(hm, mx, 16, hipMemcpyDeviceToHost, s);
+    if (e != hipSuccess) return e;
+    e = hipStreamSynchronize(s);
+    if (e != hipSuccess) return e;
+    out_host[0] = (double)hc[0];
+    out_host[1] = (double)hc[1];
+    out_host[2] = hm[0];
+    out_host[3] = hm[1];
+    return hipSuccess;
+}
+
+}  // namespace hg

@@ -1,0 +1,655 @@
+// C ABI of the engine (declarations and reference mapping in include/hnswgpu.h).
+#include "../../include/hnswgpu.h"
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <stdexcept>
+#include <string>
+#include <unordered_map>
+#include <vector>
+
+#include "host/hnsw_index.hpp"
+#include "kernels.h"
+
+#include "capi_common.h"
+
+using namespace hgapi;
+
+namespace {
+
+// ------------------------------------------------------------------------------------------
+// GPU copy of an HNSW graph.
+// ------------------------------------------------------------------------------------------
+struct GpuIndex {
+    int device = 0;
+    int cus = 0;
+    hg::DevGraph g{};
+    DevBuf vec, links0, upper_off, upper, labels;
+    DevBuf work;
+    // host-path staging
+    DevBuf q, ol, od, ctr;
+    hipStream_t stream = nullptr;
+    size_t bytes = 0;
+    std::mutex mu;
+    // tuning: beam capacity -> (hash_bits, blocks per CU)
+    std::map<uint32_t, std::pair<uint32_t, int>> tuned;
+    ~GpuIndex() {
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+struct Plan {
+    uint32_t cap, hash_bits;
+    int blocks;
+    size_t lds;
+};
+
+uint32_t ceil_log2(uint64_t x) {
+    uint32_t b = 0;
+    while ((1ull << b) < x) b++;
+    return b;
+}
+
+Plan make_plan(GpuIndex* gi, size_t max_ef, size_t max_k, size_t nq) {
+    size_t ef_eff = std::max(max_ef, max_k);
+    Plan p;
+    p.cap = (uint32_t)((ef_eff + 63) / 64 * 64);
+    auto it = gi->tuned.find(p.cap);
+    if (it != gi->tuned.end()) {
+        p.hash_bits = it->second.first;
+    } else {
+        // visited nodes grow roughly as 16-32 per beam entry at M=16; the hash may be lossy,
+        // so it is sized for occupancy rather than for the worst case
+        p.hash_bits = std::min<uint32_t>(13, std::max<uint32_t>(10, ceil_log2(16 * ef_eff)));
+    }
+    p.hash_bits = std::max(p.hash_bits, ceil_log2(2 * (uint64_t)p.cap));  // output staging reuse
+    hg::SearchParams sp{};
+    sp.g = gi->g;
+    sp.cap = p.cap;
+    sp.hash_bits = p.hash_bits;
+    p.lds = hg::search_lds_bytes(gi->g.dim, p.cap, p.hash_bits);
+    int per_cu = hg::search_blocks_per_cu(sp);
+    if (it != gi->tuned.end() && it->second.second > 0) per_cu = std::min(per_cu, it->second.second);
+    size_t blocks = (size_t)gi->cus * (size_t)per_cu;
+    p.blocks = (int)std::max<size_t>(1, std::min(blocks, nq));
+    return p;
+}
+
+hg::SearchParams base_params(GpuIndex* gi, const Plan& pl) {
+    hg::SearchParams sp{};
+    sp.g = gi->g;
+    sp.cap = pl.cap;
+    sp.hash_bits = pl.hash_bits;
+    sp.work = (uint32_t*)gi->work.p;
+    sp.trace_period = 1;
+    return sp;
+}
+
+hipError_t run_search(GpuIndex* gi, const float* dq, size_t nq, const int32_t* def, const int32_t* dk, size_t ef,
+                      size_t k, size_t k_stride, uint64_t* dl, float* dd, int32_t* dc, StopModel* stop,
+                      hipStream_t s) {
+    Plan pl = make_plan(gi, ef, k, nq);
+    hg::SearchParams sp = base_params(gi, pl);
+    sp.queries = dq;
+    sp.nq = (uint32_t)nq;
+    sp.ef_pq = def;
+    sp.k_pq = dk;
+    sp.ef = (uint32_t)ef;
+    sp.k = (uint32_t)k;
+    sp.k_stride = (uint32_t)k_stride;
+    sp.out_labels = dl;
+    sp.out_dists = dd;
+    sp.counters = dc;
+    if (stop) {
+        sp.use_stop = 1;
+        sp.stop = dev_stop(stop);
+    }
+    return hg::launch_search(sp, pl.blocks, s);
+}
+
+void check_search_args(GpuIndex* gi, size_t nq, size_t ef, size_t k, size_t k_stride) {
+    require(k >= 1, "k must be at least 1");
+    require(k_stride >= k, "k_stride must be >= k");
+    require(std::max(ef, k) <= 4096, "max(ef, k) above 4096 is not supported by the GPU search");
+    require(nq < (1ull << 31), "too many queries");
+    (void)gi;
+}
+
+}  // namespace
+
+// ============================================================================================
+HG_API const char* hg_last_error(void) { return g_err.c_str(); }
+
+// ---- CPU index ----
+HG_API int hg_hnsw_create(int space, size_t dim, size_t max_elements, size_t M, size_t ef_construction,
+                          size_t random_seed, void** out) {
+    return guarded([&] {
+        require(out != nullptr, "null output handle");
+        require(space == HG_SPACE_L2 || space == HG_SPACE_IP, "Space name must be one of l2, ip, or cosine.");
+        require(dim >= 1, "dim must be positive");
+        require(M >= 2, "M must be at least 2");
+        *out = new hnswgpu::HnswIndex(space, dim, max_elements, M, ef_construction, random_seed);
+        return HG_OK;
+    });
+}
+
+HG_API int hg_hnsw_load(int space, size_t dim, const char* path, size_t max_elements, void** out) {
+    return guarded([&] {
+        require(out && path, "null argument");
+        require(space == HG_SPACE_L2 || space == HG_SPACE_IP, "Space name must be one of l2, ip, or cosine.");
+        *out = new hnswgpu::HnswIndex(space, dim, std::string(path), max_elements);
+        return HG_OK;
+    });
+}
+
+HG_API int hg_hnsw_add_items(void* h, const float* data, size_t rows, const uint64_t* labels, int num_threads) {
+    return guarded([&] {
+        require(h != nullptr, "null index");
+        require(rows == 0 || data != nullptr, "null data");
+        static_cast<hnswgpu::HnswIndex*>(h)->add_items(data, rows, labels, num_threads);
+        return HG_OK;
+    });
+}
+
+HG_API int hg_hnsw_save(void* h, const char* path) {
+    return guarded([&] {
+        require(h && path, "null argument");
+        static_cast<hnswgpu::HnswIndex*>(h)->save(std::string(path));
+        return HG_OK;
+    });
+}
+
+HG_API int hg_hnsw_serialize(void* h, void* buffer, size_t capacity, size_t* size) {
+    return guarded([&] {
+        require(h && size, "null argument");
+        auto* idx = static_cast<hnswgpu::HnswIndex*>(h);
+        size_t need = idx->serialized_size();
+        *size = need;
+        if (buffer && capacity >= need) idx->serialize((char*)buffer);
+        else if (buffer) throw Error(HG_ERR_INVALID, "buffer too small");
+        return HG_OK;
+    });
+}
+
+HG_API int hg_hnsw_info(void* h, int64_t* info) {
+    return guarded([&] {
+        require(h && info, "null argument");
+        auto* idx = static_cast<hnswgpu::HnswIndex*>(h);
+        info[0] = (int64_t)idx->count();
+        info[1] = (int64_t)idx->max_elements();
+        info[2] = (int64_t)idx->dim();
+        info[3] = (int64_t)idx->M();
+        info[4] = (int64_t)idx->ef_construction();
+        info[5] = (int64_t)idx->maxlevel();
+        info[6] = (int64_t)idx->entry();
+        info[7] = (int64_t)idx->num_deleted();
+        return HG_OK;
+    });
+}
+
+HG_API int hg_hnsw_set_ef(void* h, size_t ef) {
+    return guarded([&] {
+        require(h != nullptr, "null index");
+        static_cast<hnswgpu::HnswIndex*>(h)->set_ef(ef);
+        return HG_OK;
+    });
+}
+
+HG_API int hg_hnsw_get_ef(void* h, size_t* ef) {
+    return guarded([&] {
+        require(h && ef, "null argument");
+        *ef = static_cast<hnswgpu::HnswIndex*>(h)->ef();
+        return HG_OK;
+    });
+}
+
+HG_API int hg_hnsw_free(void* h) {
+    delete static_cast<hnswgpu::HnswIndex*>(h);
+    return HG_OK;
+}
+
+HG_API int hg_normalize_rows(const float* in, float* out, size_t rows, size_t dim) {
+    return guarded([&] {
+        require((in && out) || rows == 0, "null argument");
+        for (size_t r = 0; r < rows; r++) hnswgpu::normalize_vector(in + r * dim, out + r * dim, dim);
+        return HG_OK;
+    });
+}
+
+// ---- GPU index ----
+HG_API int hg_gpu_index_create(void* hv, int device, void** out) {
+    return guarded([&] {
+        require(hv && out, "null argument");
+        auto* h = static_cast<hnswgpu::HnswIndex*>(hv);
+        size_t n = h->count();
+        require(n > 0, "the index is empty");
+        require(n < (1ull << 31), "more than 2^31 elements are not supported by the GPU search");
+        if (h->num_deleted() != 0)
+            throw Error(HG_ERR_INVALID,
+                        "indexes with deleted elements need the filtered (non-bare-bone) search, which the GPU "
+                        "engine does not implement");
+        int ndev = 0;
+        hip_check(hipGetDeviceCount(&ndev), "hipGetDeviceCount");
+        require(device >= 0 && device < ndev, "invalid device");
+        DeviceGuard dg(device);
+        std::unique_ptr<GpuIndex> gi(new GpuIndex());
+        gi->device = device;
+        hipDeviceProp_t prop;
+        hip_check(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
+        gi->cus = prop.multiProcessorCount;
+
+        const size_t dim = h->dim(), stride = (dim + 3) / 4 * 4, maxM = h->maxM(), maxM0 = h->maxM0();
+        // one wave holds a whole neighbour list: 2M <= 64 (M <= 32)
+        require(maxM0 <= 64 && maxM <= 64, "M above 32 is not supported by the GPU search");
+        require(dim <= 4096, "dim above 4096 is not supported by the GPU search");
+        // Export (integrity checked as in checkIntegrity, hnswalg.h:1380-1409: ids in range).
+        std::vector<float> vec(n * stride, 0.f);
+        std::vector<uint32_t> l0(n * maxM0, hg::SENT);
+        std::vector<uint32_t> uoff(n, hg::SENT);
+        std::vector<uint64_t> labels(n);
+        size_t urows = 0;
+        for (size_t i = 0; i < n; i++) {
+            if (h->element_level((uint32_t)i) > 0) {
+                uoff[i] = (uint32_t)urows;
+                urows += (size_t)h->element_level((uint32_t)i);
+            }
+        }
+        std::vector<uint32_t> up(std::max<size_t>(urows, 1) * maxM, hg::SENT);
+        for (size_t i = 0; i < n; i++) {
+            memcpy(&vec[i * stride], h->vector((uint32_t)i), dim * 4);
+            labels[i] = h->label((uint32_t)i);
+            const uint32_t* ll = h->linklist0((uint32_t)i);
+            size_t c = hnswgpu::HnswIndex::list_count(ll);
+            if (c > maxM0) throw Error(HG_ERR_RUNTIME, "Bad value of sz_link_list_other");
+            for (size_t j = 0; j < c; j++) {
+                if (ll[1 + j] >= n) throw Error(HG_ERR_RUNTIME, "cand error");
+                l0[i * maxM0 + j] = ll[1 + j];
+            }
+            for (int lv = 1; lv <= h->element_level((uint32_t)i); lv++) {
+                const uint32_t* lu = h->linklist((uint32_t)i, lv);
+                size_t cu = hnswgpu::HnswIndex::list_count(lu);
+                if (cu > maxM) throw Error(HG_ERR_RUNTIME, "Bad value of sz_link_list_other");
+                for (size_t j = 0; j < cu; j++) {
+                    if (lu[1 + j] >= n) throw Error(HG_ERR_RUNTIME, "cand error");
+                    if (h->element_level(lu[1 + j]) < lv)
+                        throw Error(HG_ERR_RUNTIME, "Trying to make a link on a non-existent level");
+                    up[(uoff[i] + (size_t)(lv - 1)) * maxM + j] = lu[1 + j];
+                }
+            }
+        }
+        require(h->element_level(h->entry()) >= h->maxlevel(), "entry point level mismatch");
+        auto upload = [&](DevBuf& b, const void* src, size_t bytes) {
+            b.reserve(std::max<size_t>(bytes, 16));
+            hip_check(hipMemcpy(b.p, src, bytes, hipMemcpyHostToDevice), "hipMemcpy H2D");
+            gi->bytes += bytes;
+        };
+        upload(gi->vec, vec.data(), vec.size() * 4);
+        upload(gi->links0, l0.data(), l0.size() * 4);
+        upload(gi->upper_off, uoff.data(), uoff.size() * 4);
+        upload(gi->upper, up.data(), up.size() * 4);
+        upload(gi->labels, labels.data(), labels.size() * 8);
+        gi->work.reserve(256);
+        hip_check(hipStreamCreateWithFlags(&gi->stream, hipStreamNonBlocking), "hipStreamCreate");
+        hg::DevGraph& g = gi->g;
+        g.vec = (const float*)gi->vec.p;
+        g.links0 = (const uint32_t*)gi->links0.p;
+        g.upper_off = (const uint32_t*)gi->upper_off.p;
+        g.upper = (const uint32_t*)gi->upper.p;
+        g.labels = (const uint64_t*)gi->labels.p;
+        g.n = (uint32_t)n;
+        g.dim = (uint32_t)dim;
+        g.stride = (uint32_t)stride;
+        g.maxM = (uint32_t)maxM;
+        g.maxM0 = (uint32_t)maxM0;
+        g.maxlevel = h->maxlevel();
+        g.entry = h->entry();
+        g.space = h->space();
+        *out = gi.release();
+        return HG_OK;
+    });
+}
+
+HG_API int hg_gpu_index_free(void* g) {
+    if (!g) return HG_OK;
+    auto* gi = static_cast<GpuIndex*>(g);
+    int prev = -1;
+    (void)hipGetDevice(&prev);
+    (void)hipSetDevice(gi->device);
+    delete gi;
+    if (prev >= 0) (void)hipSetDevice(prev);
+    return HG_OK;
+}
+
+HG_API int hg_gpu_index_bytes(void* g, size_t* bytes) {
+    return guarded([&] {
+        require(g && bytes, "null argument");
+        *bytes = static_cast<GpuIndex*>(g)->bytes;
+        return HG_OK;
+    });
+}
+
+HG_API int hg_gpu_search(void* gv, const float* queries, size_t nq, size_t k, size_t ef, uint64_t* labels,
+                         float* dists, int32_t* counters) {
+    return guarded([&] {
+        require(gv && labels && dists, "null argument");
+        auto* gi = static_cast<GpuIndex*>(gv);
+        check_search_args(gi, nq, ef, k, k);
+        if (nq == 0) return HG_OK;
+        require(queries != nullptr, "null queries");
+        std::lock_guard<std::mutex> lk(gi->mu);
+        DeviceGuard dg(gi->device);
+        const size_t dim = gi->g.dim;
+        gi->q.reserve(nq * dim * 4);
+        gi->ol.reserve(nq * k * 8);
+        gi->od.reserve(nq * k * 4);
+        if (counters) gi->ctr.reserve(nq * 16);
+        hipStream_t s = gi->stream;
+        hip_check(hipMemcpyAsync(gi->q.p, queries, nq * dim * 4, hipMemcpyHostToDevice, s), "hipMemcpy H2D");
+        hip_check(run_search(gi, (const float*)gi->q.p, nq, nullptr, nullptr, ef, k, k, (uint64_t*)gi->ol.p,
+                             (float*)gi->od.p, counters ? (int32_t*)gi->ctr.p : nullptr, nullptr, s),
+                  "search launch");
+        hip_check(hipMemcpyAsync(labels, gi->ol.p, nq * k * 8, hipMemcpyDeviceToHost, s), "hipMemcpy D2H");
+        hip_check(hipMemcpyAsync(dists, gi->od.p, nq * k * 4, hipMemcpyDeviceToHost, s), "hipMemcpy D2H");
+        if (counters)
+            hip_check(hipMemcpyAsync(counters, gi->ctr.p, nq * 16, hipMemcpyDeviceToHost, s), "hipMemcpy D2H");
+        hip_check(hipStreamSynchronize(s), "search");
+        for (size_t r = 0; r < nq; r++)
+            if (labels[r * k + k - 1] == ~0ull) {
+                g_err = "Cannot return the results in a contiguous 2D array. Probably ef or M is too small";
+                return HG_ERR_SHORT;
+            }
+        return HG_OK;
+    });
+}
+
+HG_API int hg_gpu_search_device(void* gv, const float* dq, size_t nq, const int32_t* def, const int32_t* dk, size_t ef,
+                                size_t k, size_t k_stride, uint64_t* dl, float* dd, int32_t* dc, void* stop,
+                                void* stream) {
+    return guarded([&] {
+        require(gv && dl && dd, "null argument");
+        auto* gi = static_cast<GpuIndex*>(gv);
+        check_search_args(gi, nq, ef, k, k_stride);
+        if (nq == 0) return HG_OK;
+        require(dq != nullptr, "null queries");
+        DeviceGuard dg(gi->device);
+        std::lock_guard<std::mutex> lk(gi->mu);
+        hip_check(run_search(gi, dq, nq, def, dk, ef, k, k_stride, dl, dd, dc, static_cast<StopModel*>(stop),
+                             (hipStream_t)stream),
+                  "search launch");
+        return HG_OK;
+    });
+}
+
+// Trace collection for the learned stop: the plain search, recording every `period`
+// expansions the 12 traversal features and the current top-k ids.
+HG_API int hg_gpu_collect_traces(void* gv, const float* dq, size_t nq, size_t ef, size_t k, int period,
+                                 int max_records, float* d_features, uint32_t* d_topk, int32_t* d_nrec, void* stream) {
+    return guarded([&] {
+        require(gv && dq && d_features && d_topk && d_nrec, "null argument");
+        require(period >= 1 && max_records >= 1, "period and max_records must be positive");
+        auto* gi = static_cast<GpuIndex*>(gv);
+        check_search_args(gi, nq, ef, k, k);
+        if (nq == 0) return HG_OK;
+        DeviceGuard dg(gi->device);
+        std::lock_guard<std::mutex> lk(gi->mu);
+        hipStream_t s = (hipStream_t)stream;
+        DevBuf ol, od;
+        ol.reserve(nq * k * 8);
+        od.reserve(nq * k * 4);
+        Plan pl = make_plan(gi, ef, k, nq);
+        hg::SearchParams sp = base_params(gi, pl);
+        sp.queries = dq;
+        sp.nq = (uint32_t)nq;
+        sp.ef = (uint32_t)ef;
+        sp.k = (uint32_t)k;
+        sp.k_stride = (uint32_t)k;
+        sp.out_labels = (uint64_t*)ol.p;
+        sp.out_dists = (float*)od.p;
+        sp.trace_feat = d_features;
+        sp.trace_topk = d_topk;
+        sp.trace_nrec = d_nrec;
+        sp.trace_period = period;
+        sp.trace_max_rec = max_records;
+        hip_check(hg::launch_search(sp, pl.blocks, s), "trace launch");
+        hip_check(hipStreamSynchronize(s), "trace search");
+        return HG_OK;
+    });
+}
+
+HG_API int hg_gpu_search_plan(void* gv, size_t max_ef, size_t max_k, size_t nq, int32_t* out) {
+    return guarded([&] {
+        require(gv && out, "null argument");
+        auto* gi = static_cast<GpuIndex*>(gv);
+        DeviceGuard dg(gi->device);
+        Plan p = make_plan(gi, max_ef, max_k, nq);
+        out[0] = 1;
+        out[1] = (int32_t)(1u << p.hash_bits);
+        out[2] = (int32_t)p.cap;
+        out[3] = (int32_t)p.lds;
+        out[4] = p.blocks;
+        return HG_OK;
+    });
+}
+
+// Autotuner: times (hash size, workgroups per CU) candidates on the caller's queries and keeps
+// the fastest for this beam capacity.
+HG_API int hg_gpu_autotune(void* gv, const float* dq, size_t nq, size_t ef, size_t k, void* stream) {
+    return guarded([&] {
+        require(gv && dq, "null argument");
+        auto* gi = static_cast<GpuIndex*>(gv);
+        DeviceGuard dg(gi->device);
+        std::lock_guard<std::mutex> lk(gi->mu);
+        hipStream_t s = (hipStream_t)stream;
+        DevBuf ol, od;
+        ol.reserve(nq * k * 8);
+        od.reserve(nq * k * 4);
+        size_t ef_eff = std::max(ef, k);
+        uint32_t cap = (uint32_t)((ef_eff + 63) / 64 * 64);
+        uint32_t min_bits = ceil_log2(2 * (uint64_t)cap);
+        hipEvent_t e0, e1;
+        hip_check(hipEventCreate(&e0), "hipEventCreate");
+        hip_check(hipEventCreate(&e1), "hipEventCreate");
+        float best = 1e30f;
+        std::pair<uint32_t, int> best_cfg(0, 0);
+        gi->tuned.erase(cap);
+        for (uint32_t bits = std::max<uint32_t>(min_bits, 9); bits <= 14; bits++) {
+            for (int per_cu : {4, 8, 12, 16, 24, 32}) {
+                gi->tuned[cap] = std::make_pair(bits, per_cu);
+                Plan pl = make_plan(gi, ef, k, nq);
+                if (pl.blocks < std::min<size_t>(nq, (size_t)gi->cus * per_cu) && per_cu > 4) continue;
+                float ms_sum = 0;
+                for (int rep = 0; rep < 2; rep++) {
+                    hip_check(hipEventRecord(e0, s), "hipEventRecord");
+                    hip_check(run_search(gi, dq, nq, nullptr, nullptr, ef, k, k, (uint64_t*)ol.p, (float*)od.p,
+                                         nullptr, nullptr, s),
+                              "search launch");
+                    hip_check(hipEventRecord(e1, s), "hipEventRecord");
+                    hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
+                    float ms = 0;
+                    hip_check(hipEventElapsedTime(&ms, e0, e1), "hipEventElapsedTime");
+                    if (rep > 0) ms_sum += ms;
+                }
+                if (ms_sum < best) {
+                    best = ms_sum;
+                    best_cfg = std::make_pair(bits, per_cu);
+                }
+            }
+        }
+        (void)hipEventDestroy(e0);
+        (void)hipEventDestroy(e1);
+        gi->tuned[cap] = best_cfg;
+        return HG_OK;
+    });
+}
+
+// ---- hipGraph of a device search ----
+struct GraphHandle {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    ~GraphHandle() {
+        if (exec) (void)hipGraphExecDestroy(exec);
+        if (graph) (void)hipGraphDestroy(graph);
+    }
+};
+
+HG_API int hg_graph_capture_search(void* gv, const float* dq, size_t nq, const int32_t* def, const int32_t* dk,
+                                   size_t ef, size_t k, size_t k_stride, uint64_t* dl, float* dd, int32_t* dc,
+                                   void* stop, void* stream, void** out_graph) {
+    return guarded([&] {
+        require(gv && dq && dl && dd && out_graph, "null argument");
+        require(stream != nullptr, "graph capture needs a non-default stream");
+        auto* gi = static_cast<GpuIndex*>(gv);
+        check_search_args(gi, nq, ef, k, k_stride);
+        require(nq > 0, "empty batch");
+        DeviceGuard dg(gi->device);
+        std::lock_guard<std::mutex> lk(gi->mu);
+        hipStream_t s = (hipStream_t)stream;
+        std::unique_ptr<GraphHandle> gh(new GraphHandle());
+        // resolve kernel attributes outside the capture
+        Plan pl = make_plan(gi, ef, k, nq);
+        (void)pl;
+        hip_check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
+        hipError_t le = run_search(gi, dq, nq, def, dk, ef, k, k_stride, dl, dd, dc, static_cast<StopModel*>(stop), s);
+        hipGraph_t graph = nullptr;
+        hipError_t ce = hipStreamEndCapture(s, &graph);
+        hip_check(le, "search launch (capture)");
+        hip_check(ce, "hipStreamEndCapture");
+        gh->graph = graph;
+        hip_check(hipGraphInstantiate(&gh->exec, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+        *out_graph = gh.release();
+        return HG_OK;
+    });
+}
+
+HG_API int hg_graph_launch(void* gh, void* stream) {
+    return guarded([&] {
+        require(gh != nullptr, "null graph");
+        hip_check(hipGraphLaunch(static_cast<GraphHandle*>(gh)->exec, (hipStream_t)stream), "hipGraphLaunch");
+        return HG_OK;
+    });
+}
+
+HG_API int hg_graph_free(void* gh) {
+    delete static_cast<GraphHandle*>(gh);
+    return HG_OK;
+}
+
+// ---- device plumbing ----
+HG_API int hg_device_count(int* count) {
+    return guarded([&] {
+        require(count != nullptr, "null argument");
+        hip_check(hipGetDeviceCount(count), "hipGetDeviceCount");
+        return HG_OK;
+    });
+}
+HG_API int hg_set_device(int device) {
+    return guarded([&] {
+        hip_check(hipSetDevice(device), "hipSetDevice");
+        return HG_OK;
+    });
+}
+HG_API int hg_malloc(void** ptr, size_t bytes) {
+    return guarded([&] {
+        require(ptr != nullptr, "null argument");
+        hip_check(hipMalloc(ptr, std::max<size_t>(bytes, 1)), "hipMalloc");
+        return HG_OK;
+    });
+}
+HG_API int hg_free(void* ptr) {
+    return guarded([&] {
+        if (ptr) hip_check(hipFree(ptr), "hipFree");
+        return HG_OK;
+    });
+}
+HG_API int hg_memcpy_h2d(void* dst, const void* src, size_t bytes, void* stream) {
+    return guarded([&] {
+        hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, (hipStream_t)stream), "hipMemcpy H2D");
+        hip_check(hipStreamSynchronize((hipStream_t)stream), "hipStreamSynchronize");
+        return HG_OK;
+    });
+}
+HG_API int hg_memcpy_d2h(void* dst, const void* src, size_t bytes, void* stream) {
+    return guarded([&] {
+        hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, (hipStream_t)stream), "hipMemcpy D2H");
+        hip_check(hipStreamSynchronize((hipStream_t)stream), "hipStreamSynchronize");
+        return HG_OK;
+    });
+}
+HG_API int hg_memset(void* dst, int value, size_t bytes, void* stream) {
+    return guarded([&] {
+        hip_check(hipMemsetAsync(dst, value, bytes, (hipStream_t)stream), "hipMemsetAsync");
+        return HG_OK;
+    });
+}
+HG_API int hg_stream_create(void** stream) {
+    return guarded([&] {
+        require(stream != nullptr, "null argument");
+        hipStream_t s;
+        hip_check(hipStreamCreateWithFlags(&s, hipStreamNonBlocking), "hipStreamCreate");
+        *stream = s;
+        return HG_OK;
+    });
+}
+HG_API int hg_stream_destroy(void* stream) {
+    return guarded([&] {
+        if (stream) hip_check(hipStreamDestroy((hipStream_t)stream), "hipStreamDestroy");
+        return HG_OK;
+    });
+}
+HG_API int hg_stream_synchronize(void* stream) {
+    return guarded([&] {
+        hip_check(hipStreamSynchronize((hipStream_t)stream), "hipStreamSynchronize");
+        return HG_OK;
+    });
+}
+HG_API int hg_device_synchronize(void) {
+    return guarded([&] {
+        hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
+        return HG_OK;
+    });
+}
+
+struct Timer {
+    hipEvent_t a = nullptr, b = nullptr;
+    ~Timer() {
+        if (a) (void)hipEventDestroy(a);
+        if (b) (void)hipEventDestroy(b);
+    }
+};
+HG_API int hg_timer_create(void** t) {
+    return guarded([&] {
+        require(t != nullptr, "null argument");
+        std::unique_ptr<Timer> tm(new Timer());
+        hip_check(hipEventCreate(&tm->a), "hipEventCreate");
+        hip_check(hipEventCreate(&tm->b), "hipEventCreate");
+        *t = tm.release();
+        return HG_OK;
+    });
+}
+HG_API int hg_timer_destroy(void* t) {
+    delete static_cast<Timer*>(t);
+    return HG_OK;
+}
+HG_API int hg_timer_start(void* t, void* stream) {
+    return guarded([&] {
+        require(t != nullptr, "null timer");
+        hip_check(hipEventRecord(static_cast<Timer*>(t)->a, (hipStream_t)stream), "hipEventRecord");
+        return HG_OK;
+    });
+}
+HG_API int hg_timer_stop(void* t, void* stream, float* ms) {
+    return guarded([&] {
+        require(t && ms, "null argument");
+        Timer* tm = static_cast<Timer*>(t);
+        hip_check(hipEventRecord(tm->b, (hipStream_t)stream), "hipEventRecord");
+        hip_check(hipEventSynchronize(tm->b), "hipEventSynchronize");
+        hip_check(hipEventElapsedTime(ms, tm->a, tm->b), "hipEventElapsedTime");
+        return HG_OK;
+    });
+}

@@ -1,0 +1,68 @@
+// Shared device-side definitions for the gfx950 kernels.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace hg {
+
+constexpr uint32_t SENT = 0xFFFFFFFFu;    // empty link slot / empty hash slot / no result
+constexpr uint32_t XFLAG = 0x80000000u;   // "expanded" mark carried in a beam id
+constexpr uint32_t ID_MASK = 0x7FFFFFFFu;
+
+// GPU layout of an HNSW graph (exported once from the CPU index):
+//   vec      [n][stride] f32, stride = dim rounded up to 4 (16-byte rows), zero padded
+//   links0   [n][maxM0] u32 level-0 neighbours in list order, SENT padded
+//   upper_off[n] u32: row of level 1 in `upper` (levels l>1 follow at +l-1), SENT if none
+//   upper    [rows][maxM] u32 upper-level neighbours, SENT padded
+//   labels   [n] u64 external labels
+struct DevGraph {
+    const float* vec;
+    const uint32_t* links0;
+    const uint32_t* upper_off;
+    const uint32_t* upper;
+    const uint64_t* labels;
+    uint32_t n, dim, stride, maxM, maxM0;
+    int32_t maxlevel;
+    uint32_t entry;
+    int32_t space;
+};
+
+// Learned early-termination network held in device memory (FP32 master copy plus BF16
+// weights used by the in-kernel inference).
+struct DevStopModel {
+    const float* params;        // W1[h][in] b1[h] W2[h][h] b2[h] w3[h] b3
+    const float* mean;          // [in]
+    const float* inv_std;       // [in]
+    int32_t in_dim, hidden;
+    int32_t period;             // evaluate every `period` level-0 expansions
+    int32_t min_hops;           // never stop before this many expansions
+    float logit_threshold;      // stop when logit >= threshold (= logit(confidence))
+};
+
+struct SearchParams {
+    DevGraph g;
+    const float* queries;       // [nq][dim]
+    const uint32_t* order;      // optional permutation: slot -> query index
+    uint32_t nq;
+    const int32_t* ef_pq;       // optional per-query ef
+    const int32_t* k_pq;        // optional per-query k
+    uint32_t ef, k, k_stride;
+    uint64_t* out_labels;       // [nq][k_stride]
+    float* out_dists;           // [nq][k_stride]
+    int32_t* counters;          // optional [nq][4]
+    uint32_t* work;             // persistent work counter (zeroed before each launch)
+    uint32_t cap;               // beam capacity per wave (multiple of 64, >= every max(ef,k))
+    uint32_t hash_bits;         // visited hash has 1 << hash_bits slots per wave
+    // learned stop (optional)
+    DevStopModel stop;
+    int32_t use_stop;
+    // trace collection (optional)
+    float* trace_feat;          // [nq][max_rec][in_dim]
+    uint32_t* trace_topk;       // [nq][max_rec][k]
+    int32_t* trace_nrec;        // [nq]
+    int32_t trace_period, trace_max_rec;
+};
+
+constexpr int kStopFeatures = 12;
+
+}  // namespace hg

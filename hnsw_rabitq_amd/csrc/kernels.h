@@ -1,0 +1,50 @@
+// Host-callable launchers of the gfx950 kernels (implemented in the .hip files).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stddef.h>
+#include <stdint.h>
+
+#include "common.h"
+
+namespace hg {
+
+// ---- graph search (search.hip) ----
+size_t search_lds_bytes(uint32_t dim, uint32_t cap, uint32_t hash_bits);
+int search_blocks_per_cu(const SearchParams& p);
+hipError_t launch_search(const SearchParams& p, int blocks, hipStream_t s);
+
+// ---- brute force (bruteforce.hip) ----
+struct BfData {
+    const float* vec;        // [n][dim] f32
+    const uint64_t* labels;  // [n]
+    const uint16_t* vec_bf16;  // [n][dim] (optional, built on demand)
+    const uint8_t* vec_fp8;    // [n][dim] e4m3 (optional)
+    const float* fp8_scale;    // [n] per-vector scale (optional)
+    const float* norms;        // [n] squared norms in FP32 (L2 GEMM form), optional
+    uint32_t n, dim;
+    int32_t space;
+};
+
+// Exact FP32 search: k smallest (distance, label) pairs per query, rows closest first.
+// Needs scratch of bf_exact_scratch_bytes().
+size_t bf_exact_scratch_bytes(uint32_t nq, uint32_t k, uint32_t n);
+hipError_t launch_bf_exact(const BfData& d, const float* q, uint32_t nq, uint32_t k, uint64_t* labels, float* dists,
+                           void* scratch, hipStream_t s);
+
+// Reduced-precision candidate pass + FP32 re-rank.
+size_t bf_lowp_scratch_bytes(uint32_t nq, uint32_t dim, uint32_t c, uint32_t n);
+hipError_t launch_bf_convert(const BfData& d, uint16_t* vec_bf16, uint8_t* vec_fp8, float* fp8_scale, float* norms,
+                             hipStream_t s);
+hipError_t launch_bf_lowp(const BfData& d, int precision, const float* q, uint32_t nq, uint32_t k, uint32_t c,
+                          uint64_t* labels, float* dists, void* scratch, hipStream_t s, double* stats);
+hipError_t launch_fp64_check(const BfData& d, const float* q, uint32_t nq, uint32_t k, const uint64_t* labels,
+                             const float* dists, double rel_tol, double* out_host, void* scratch, hipStream_t s);
+
+// ---- learned stop (mlp.hip) ----
+hipError_t launch_mlp_forward(const float* params, int in_dim, int hidden, const float* x, uint32_t n, float* logits,
+                              hipStream_t s);
+hipError_t launch_mlp_train_step(float* params, float* m, float* v, int step, int in_dim, int hidden, const float* x,
+                                 const float* y, uint32_t n, float lr, float b1, float b2, float eps, float* loss_dev,
+                                 float* grad_dev, hipStream_t s);
+
+}  // namespace hg

@@ -1,0 +1,587 @@
+// Batched HNSW query on gfx950: one wavefront per query, persistent workgroups.
+//
+// Restates HierarchicalNSW::searchKnn (hnswalg.h:1269-1323): entry distance, greedy descent
+// through the upper layers (hnswalg.h:1277-1302, strict '<', first minimum in list order), then
+// the level-0 best-first search searchBaseLayerST<bare_bone=true> (hnswalg.h:308-440) with
+// ef_eff = max(ef, k), truncation to k and closest-first rows (bindings.cpp:649-654).
+//
+// Level-0 search state lives in LDS, per wave:
+//   beam   : the reference's top_candidates as a list sorted by (distance, id), at most ef_eff
+//            long; bit 31 of an id marks "expanded".  Under the (distance, id) total order the
+//            reference's candidate_set is exactly the unexpanded part of this list (entries it
+//            pops after they left top_candidates are beyond lowerBound and end the search), and
+//            the next node it expands is the first unexpanded entry.
+//   hash   : visited set (open addressing over node ids).  A lookup never reports a node that
+//            was not inserted; when a probe window is full the slot is overwritten, so a node
+//            may be evaluated twice.  That cannot change results: a re-evaluated node is either
+//            still in the beam (dropped by the duplicate check of the merge) or was rejected or
+//            evicted under a lowerBound that has only decreased since.
+//   stage  : the neighbours of the expanded node that were not visited, then their distances,
+//            then the sorted batch that is merged into the beam.
+// A batch of neighbours is merged set-wise: under a total order, processing the neighbours
+// one by one with the admission test `size < ef || d < lowerBound` and evicting the maximum
+// leaves exactly the ef smallest keys of (beam U batch) -- what the merge computes.
+//
+// Distances follow the reference's FP32 order exactly (see dist_* below); all comparisons use
+// (distance, id), so ties between equal distances resolve by node id.
+#include "common.h"
+#include "dist.h"
+#include "kernels.h"
+
+namespace hg {
+
+__device__ __forceinline__ bool key_less(float da, uint32_t ia, float db, uint32_t ib) {
+    return da < db || (da == db && ia < ib);
+}
+
+// Distances of the c vectors whose ids are ids[0..c) to the query in LDS; results to out[].
+// MODE 0 = generic (one lane per vector), otherwise dim % 16 == 0 with NCH chunks (0 = runtime).
+template <int SPACE, int MODE, int NCH>
+__device__ __forceinline__ void dist_batch(const DevGraph& g, const float* qs, const uint32_t* ids, int c,
+                                           float* out, int lane) {
+    if (MODE == 0) {
+        for (int base = 0; base < c; base += 64) {
+            int idx = base + lane;
+            if (idx < c) {
+                const float* x = g.vec + (size_t)ids[idx] * g.stride;
+                out[idx] = dist_generic(SPACE, qs, x, (int)g.dim);
+            }
+        }
+        return;
+    }
+    const int grp = lane >> 2, sub = lane & 3;
+    const int nch = (int)(g.dim >> 4);
+    for (int base = 0; base < c; base += 32) {
+        // two vectors per group per round: slots grp and grp + 16
+        int i0 = base + grp, i1 = base + grp + 16;
+        bool a0 = i0 < c, a1 = i1 < c;
+        uint32_t id0 = a0 ? ids[i0] : 0u, id1 = a1 ? ids[i1] : 0u;
+        float r0, r1;
+        dist_group16<SPACE, NCH>(g.vec + (size_t)id0 * g.stride, qs, nch, a0, sub, r0, lane);
+        dist_group16<SPACE, NCH>(g.vec + (size_t)id1 * g.stride, qs, nch, a1, sub, r1, lane);
+        if (sub == 0) {
+            if (a0) out[i0] = r0;
+            if (a1) out[i1] = r1;
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// Wave helpers
+// ------------------------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
+__device__ __forceinline__ int lanes_below(uint64_t m, int lane) {
+    return __popcll(m & ((lane == 0) ? 0ull : (~0ull >> (64 - lane))));
+}
+
+// Visited hash: returns true when id was not present (and inserts it).
+__device__ __forceinline__ bool hash_insert(uint32_t* hash, uint32_t mask, uint32_t id) {
+    uint32_t h = (id * 2654435761u) & mask;
+#pragma unroll 1
+    for (int p = 0; p < 16; p++) {
+        uint32_t slot = (h + p) & mask;
+        uint32_t old = atomicCAS(&hash[slot], SENT, id);
+        if (old == SENT) return true;
+        if (old == id) return false;
+    }
+    hash[h] = id;  // window full: overwrite (may cause one extra evaluation later)
+    return true;
+}
+
+// Bitonic sort of one (d, id) pair per lane across the 64 lanes, ascending by (d, id).
+__device__ __forceinline__ void wave_sort64(float& d, uint32_t& id, int lane) {
+#pragma unroll
+    for (int k = 2; k <= 64; k <<= 1) {
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            float od = __shfl_xor(d, j);
+            uint32_t oi = __shfl_xor(id, j);
+            bool up = (lane & k) == 0;
+            bool lower = (lane & j) == 0;
+            bool o_less = key_less(od, oi, d, id);
+            // the lower lane keeps the smaller key in ascending blocks
+            bool take = (lower == up) ? o_less : !o_less && !(od == d && oi == id);
+            if (take) {
+                d = od;
+                id = oi;
+            }
+        }
+    }
+}
+
+// Number of beam entries with key < (d, id), over beam[0, n).
+__device__ __forceinline__ int beam_lower_bound(const float* bd, const uint32_t* bi, int n, float d, uint32_t id) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (key_less(bd[mid], bi[mid] & ID_MASK, d, id))
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// Number of staged entries with key < (d, id), over s[0, m).
+__device__ __forceinline__ int stage_lower_bound(const float* sd, const uint32_t* si, int m, float d, uint32_t id) {
+    int lo = 0, hi = m;
+    while (lo < hi) {
+        int mid = (lo + hi) >> 1;
+        if (key_less(sd[mid], si[mid], d, id))
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return lo;
+}
+
+// ------------------------------------------------------------------------------------------
+// Learned stop: 12 traversal features and an in-wave MLP evaluation.
+// ------------------------------------------------------------------------------------------
+struct StopState {
+    float qnorm, d_entry;
+    int improved_recent;  // bitmask of the last 16 expansions that changed the top-k
+};
+
+__device__ __forceinline__ float safe_div(float a, float b) { return a / (fabsf(b) + 1e-20f); }
+
+// Features (same order as the trace collector and the host-side training code):
+//  0 query norm            1 entry distance at level 0     2 d_1 / d_entry
+//  3 d_k / d_entry         4 d_last / d_entry              5 d_k / d_1
+//  6 d_last / d_k          7 hops / ef_eff                 8 log(1 + hops)
+//  9 fraction of the last 16 expansions that changed the top-k
+// 10 distance evaluations per hop                          11 next candidate distance / d_k
+__device__ __forceinline__ float stop_feature(int f, const StopState& st, float d1, float dk, float dlast,
+                                              float dnext, int hops, int evals, int ef_eff) {
+    switch (f) {
+        case 0: return st.qnorm;
+        case 1: return st.d_entry;
+        case 2: return safe_div(d1, st.d_entry);
+        case 3: return safe_div(dk, st.d_entry);
+        case 4: return safe_div(dlast, st.d_entry);
+        case 5: return safe_div(dk, d1);
+        case 6: return safe_div(dlast, dk);
+        case 7: return (float)hops / (float)ef_eff;
+        case 8: return logf(1.0f + (float)hops);
+        case 9: return (float)__popc((uint32_t)st.improved_recent & 0xFFFFu) / 16.0f;
+        case 10: return (float)evals / (float)(hops + 1);
+        default: return safe_div(dnext, dk);
+    }
+}
+
+__device__ __forceinline__ float gelu_tanh(float x) {
+    const float c = 0.7978845608028654f;
+    return 0.5f * x * (1.0f + tanhf(c * (x + 0.044715f * x * x * x)));
+}
+
+// MLP logit for one feature vector held in LDS (feat[in_dim]); hidden activations in LDS
+// scratch (hbuf[2*hidden]).  Weights are read as BF16-rounded values of the FP32 master
+// copy, accumulation in FP32.  Lanes own hidden units (hidden <= 64).
+__device__ __forceinline__ float bf16_round(float x) {
+    uint32_t u = __float_as_uint(x);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return __uint_as_float(u & 0xFFFF0000u);
+}
+
+__device__ float mlp_logit(const DevStopModel& m, const float* feat, float* hbuf, int lane) {
+    const int in = m.in_dim, H = m.hidden;
+    const float* W1 = m.params;
+    const float* b1 = W1 + H * in;
+    const float* W2 = b1 + H;
+    const float* b2 = W2 + H * H;
+    const float* w3 = b2 + H;
+    const float* b3 = w3 + H;
+    if (lane < H) {
+        float s = b1[lane];
+        for (int i = 0; i < in; i++) s = fma_(bf16_round(W1[lane * in + i]), feat[i], s);
+        hbuf[lane] = gelu_tanh(s);
+    }
+    __syncthreads();
+    float h2 = 0.f;
+    if (lane < H) {
+        float s = b2[lane];
+        for (int i = 0; i < H; i++) s = fma_(bf16_round(W2[lane * H + i]), hbuf[i], s);
+        h2 = gelu_tanh(s);
+    }
+    __syncthreads();
+    if (lane < H) hbuf[lane] = h2;
+    __syncthreads();
+    // output layer accumulated in unit order, as the training forward pass does
+    float o = 0.f;
+    for (int j = 0; j < H; j++) o = fma_(hbuf[j], bf16_round(w3[j]), o);
+    __syncthreads();
+    return o + b3[0];
+}
+
+// ------------------------------------------------------------------------------------------
+// The search kernel.  Block = one wave (64 lanes) = one query at a time; workgroups loop over
+// queries taken from a global counter until none are left.
+// ------------------------------------------------------------------------------------------
+template <int SPACE, int MODE, int NCH, bool STOP, bool TRACE>
+__global__ __launch_bounds__(64) void hnsw_search_kernel(SearchParams P) {
+    extern __shared__ __align__(16) unsigned char smem[];
+    const int lane = threadIdx.x;
+    const DevGraph& g = P.g;
+    const uint32_t dimpad = (g.dim + 3u) & ~3u;
+    const uint32_t cap = P.cap;
+    const uint32_t hsize = 1u << P.hash_bits;
+    const uint32_t hmask = hsize - 1u;
+
+    float* qs = reinterpret_cast<float*>(smem);
+    float* bd = qs + dimpad;
+    uint32_t* bi = reinterpret_cast<uint32_t*>(bd + cap);
+    uint32_t* hash = bi + cap;
+    float* sd = reinterpret_cast<float*>(hash + hsize);
+    uint32_t* si = reinterpret_cast<uint32_t*>(sd + 64);
+    float* feat = reinterpret_cast<float*>(si + 64);  // kStopFeatures + 2*64 floats (STOP/TRACE)
+    float* hbuf = feat + 16;
+
+    __shared__ uint32_t s_slot;
+
+    for (;;) {
+        if (lane == 0) s_slot = atomicAdd(P.work, 1u);
+        __syncthreads();
+        const uint32_t slot = s_slot;
+        __syncthreads();
+        if (slot >= P.nq) break;
+        const uint32_t qi = P.order ? P.order[slot] : slot;
+        const int kq = P.k_pq ? P.k_pq[qi] : (int)P.k;
+        const int efq = P.ef_pq ? P.ef_pq[qi] : (int)P.ef;
+        const int ef_eff = efq > kq ? efq : kq;
+
+        // query to LDS, hash cleared
+        const float* qg = P.queries + (size_t)qi * g.dim;
+        for (uint32_t i = lane; i < dimpad; i += 64) qs[i] = i < g.dim ? qg[i] : 0.f;
+        for (uint32_t i = lane; i < hsize; i += 64) hash[i] = SENT;
+        __syncthreads();
+
+        int Hu = 0, Eu = 0, H0 = 0, E0 = 0;
+        // ---- entry point and upper-layer greedy descent ----
+        uint32_t cur = g.entry;
+        if (lane == 0) si[0] = cur;
+        __syncthreads();
+        dist_batch<SPACE, MODE, NCH>(g, qs, si, 1, sd, lane);
+        __syncthreads();
+        float curdist = sd[0];
+        Eu = 1;
+        for (int level = g.maxlevel; level > 0; level--) {
+            bool changed = true;
+            while (changed) {
+                changed = false;
+                const uint32_t* row = g.upper + ((size_t)g.upper_off[cur] + (size_t)(level - 1)) * g.maxM;
+                Hu++;
+                int cnt = 0;
+                for (uint32_t b = 0; b < g.maxM; b += 64) {
+                    uint32_t j = b + lane;
+                    uint32_t id = j < g.maxM ? row[j] : SENT;
+                    uint64_t m = ballot(id != SENT);
+                    if (id != SENT) si[cnt + lanes_below(m, lane)] = id;
+                    cnt += __popcll(m);
+                }
+                __syncthreads();
+                if (cnt == 0) continue;
+                dist_batch<SPACE, MODE, NCH>(g, qs, si, cnt, sd, lane);
+                __syncthreads();
+                Eu += cnt;
+                // first minimum in list order
+                float bestd = INFINITY;
+                int bestj = 0x7FFFFFFF;
+                for (int j = lane; j < cnt; j += 64) {
+                    float d = sd[j];
+                    if (d < bestd) {
+                        bestd = d;
+                        bestj = j;
+                    }
+                }
+#pragma unroll
+                for (int o = 32; o > 0; o >>= 1) {
+                    float od = __shfl_xor(bestd, o);
+                    int oj = __shfl_xor(bestj, o);
+                    if (od < bestd || (od == bestd && oj < bestj)) {
+                        bestd = od;
+                        bestj = oj;
+                    }
+                }
+                if (bestd < curdist) {
+                    curdist = bestd;
+                    cur = si[bestj];
+                    changed = true;
+                }
+                __syncthreads();
+            }
+        }
+
+        // ---- level 0 ----
+        int n = 1;
+        if (lane == 0) {
+            bd[0] = curdist;
+            bi[0] = cur;
+            hash_insert(hash, hmask, cur);
+        }
+        E0 = 1;
+        __syncthreads();
+        int scan_lo = 0;
+        StopState st;
+        st.d_entry = curdist;
+        st.improved_recent = 0;
+        if (STOP || TRACE) {
+            float s = 0.f;
+            for (uint32_t i = lane; i < g.dim; i += 64) s = fma_(qs[i], qs[i], s);
+#pragma unroll
+            for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o);
+            st.qnorm = sqrtf(s);
+        }
+        int nrec = 0;
+        bool stopped = false;
+        for (;;) {
+            // first unexpanded entry at or after scan_lo
+            int p = -1;
+            for (int b = scan_lo; b < n; b += 64) {
+                int i = b + lane;
+                bool un = i < n && (bi[i] & XFLAG) == 0;
+                uint64_t m = ballot(un);
+                if (m) {
+                    p = b + __ffsll((unsigned long long)m) - 1;
+                    break;
+                }
+            }
+            if (p < 0) break;
+            const uint32_t node = bi[p] & ID_MASK;
+            __syncthreads();
+            if (lane == 0) bi[p] = node | XFLAG;
+            H0++;
+
+            // neighbours of node: keep those not visited, in list order
+            const uint32_t* row = g.links0 + (size_t)node * g.maxM0;
+            int cnt = 0;
+            for (uint32_t b = 0; b < g.maxM0; b += 64) {
+                uint32_t j = b + lane;
+                uint32_t id = j < g.maxM0 ? row[j] : SENT;
+                bool fresh = id != SENT && hash_insert(hash, hmask, id);
+                uint64_t m = ballot(fresh);
+                if (fresh) si[cnt + lanes_below(m, lane)] = id;
+                cnt += __popcll(m);
+            }
+            __syncthreads();
+            if (cnt > 0) {
+                dist_batch<SPACE, MODE, NCH>(g, qs, si, cnt, sd, lane);
+                __syncthreads();
+                E0 += cnt;
+                // admission against the current worst (when full) and sort of the batch
+                float d = INFINITY;
+                uint32_t id = SENT;
+                if (lane < cnt) {
+                    d = sd[lane];
+                    id = si[lane];
+                    if (n >= ef_eff && !key_less(d, id, bd[n - 1], bi[n - 1] & ID_MASK)) {
+                        d = INFINITY;
+                        id = SENT;
+                    }
+                }
+                uint64_t live = ballot(id != SENT);
+                if (live) {
+                    wave_sort64(d, id, lane);
+                    // drop nodes already in the beam (re-evaluated after a hash overwrite)
+                    int pos = 0;
+                    bool ok = id != SENT;
+                    if (ok) {
+                        pos = beam_lower_bound(bd, bi, n, d, id);
+                        if (pos < n && bd[pos] == d && (bi[pos] & ID_MASK) == id) ok = false;
+                    }
+                    uint64_t km = ballot(ok);
+                    int m = __popcll(km);
+                    __syncthreads();
+                    int j = lanes_below(km, lane);
+                    if (ok) {
+                        sd[j] = d;
+                        si[j] = id;
+                    }
+                    __syncthreads();
+                    if (m > 0) {
+                        // first staged position in the merged list
+                        int first_pos = __shfl(pos, __ffsll((unsigned long long)km) - 1);
+                        // move beam entries [first_pos, n) right, last chunk first
+                        int new_n = n + m < ef_eff ? n + m : ef_eff;
+                        int hi_chunk = (n - 1) & ~63;
+                        int lo_chunk = first_pos & ~63;
+                        for (int b = hi_chunk; b >= lo_chunk; b -= 64) {
+                            int i = b + lane;
+                            bool mv = i >= first_pos && i < n;
+                            float vd = 0.f;
+                            uint32_t vi = 0;
+                            int np = 0;
+                            if (mv) {
+                                vd = bd[i];
+                                vi = bi[i];
+                                np = i + stage_lower_bound(sd, si, m, vd, vi & ID_MASK);
+                            }
+                            __syncthreads();
+                            if (mv && np < ef_eff) {
+                                bd[np] = vd;
+                                bi[np] = vi;
+                            }
+                            __syncthreads();
+                        }
+                        int fp = ok ? pos + j : 0x7FFFFFFF;
+                        if (ok && fp < ef_eff) {
+                            bd[fp] = d;
+                            bi[fp] = id;
+                        }
+                        int min_fp = fp;
+#pragma unroll
+                        for (int o = 32; o > 0; o >>= 1) {
+                            int t = __shfl_xor(min_fp, o);
+                            min_fp = t < min_fp ? t : min_fp;
+                        }
+                        // Every position below min(first insert, p) holds an old entry whose old
+                        // index was below p, hence already expanded.
+                        scan_lo = min_fp < p ? min_fp : p;
+                        if (STOP || TRACE) {
+                            int kk = kq < new_n ? kq : new_n;
+                            if (min_fp < kk) st.improved_recent |= 1;
+                        }
+                        n = new_n;
+                        __syncthreads();
+                    } else {
+                        scan_lo = p;
+                    }
+                } else {
+                    scan_lo = p;
+                }
+            } else {
+                scan_lo = p;
+            }
+            if (STOP || TRACE) {
+                const int period = STOP ? P.stop.period : P.trace_period;
+                if (H0 % period == 0) {
+                    int kk = kq < n ? kq : n;
+                    float d1 = bd[0], dk = bd[kk - 1], dlast = bd[n - 1];
+                    // next candidate: first unexpanded from scan_lo (approximate: scan_lo entry)
+                    float dnext = scan_lo < n ? bd[scan_lo] : dlast;
+                    if (lane < kStopFeatures)
+                        feat[lane] = stop_feature(lane, st, d1, dk, dlast, dnext, H0, E0, ef_eff);
+                    __syncthreads();
+                    if (TRACE && nrec < P.trace_max_rec) {
+                        float* fo = P.trace_feat + ((size_t)qi * P.trace_max_rec + nrec) * kStopFeatures;
+                        if (lane < kStopFeatures) fo[lane] = feat[lane];
+                        uint32_t* to = P.trace_topk + ((size_t)qi * P.trace_max_rec + nrec) * P.k_stride;
+                        for (int i = lane; i < kq; i += 64) to[i] = i < n ? (bi[i] & ID_MASK) : SENT;
+                        nrec++;
+                    }
+                    if (STOP && H0 >= P.stop.min_hops) {
+                        if (lane < kStopFeatures)
+                            feat[lane] = (feat[lane] - P.stop.mean[lane]) * P.stop.inv_std[lane];
+                        __syncthreads();
+                        float logit = mlp_logit(P.stop, feat, hbuf, lane);
+                        if (logit >= P.stop.logit_threshold) stopped = true;
+                    }
+                    __syncthreads();
+                }
+                st.improved_recent <<= 1;
+                if (stopped) break;
+            }
+        }
+
+        // ---- output: first k entries, rows ordered by (distance, label) ----
+        const int kk = kq < n ? kq : n;
+        uint64_t* lab_s = reinterpret_cast<uint64_t*>(hash);  // hash no longer needed
+        __syncthreads();
+        for (int i = lane; i < kk; i += 64) lab_s[i] = g.labels[bi[i] & ID_MASK];
+        __syncthreads();
+        uint64_t* ol = P.out_labels + (size_t)qi * P.k_stride;
+        float* od = P.out_dists + (size_t)qi * P.k_stride;
+        for (int i = lane; i < kk; i += 64) {
+            float di = bd[i];
+            uint64_t li = lab_s[i];
+            int r = 0;
+            for (int j = 0; j < kk; j++) {
+                float dj = bd[j];
+                uint64_t lj = lab_s[j];
+                r += (dj < di || (dj == di && lj < li)) ? 1 : 0;
+            }
+            ol[r] = li;
+            od[r] = di;
+        }
+        for (int i = kk + lane; i < kq; i += 64) {
+            ol[i] = ~0ull;
+            od[i] = INFINITY;
+        }
+        if (P.counters && lane == 0) {
+            int32_t* c = P.counters + (size_t)qi * 4;
+            c[0] = H0;
+            c[1] = E0;
+            c[2] = Hu;
+            c[3] = Eu;
+        }
+        if (TRACE && lane == 0) P.trace_nrec[qi] = nrec;
+        __syncthreads();
+    }
+}
+
+__global__ void zero_u32_kernel(uint32_t* p, uint32_t n) {
+    uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0u;
+}
+
+// ------------------------------------------------------------------------------------------
+// Host-side launch
+// ------------------------------------------------------------------------------------------
+size_t search_lds_bytes(uint32_t dim, uint32_t cap, uint32_t hash_bits) {
+    uint32_t dimpad = (dim + 3u) & ~3u;
+    return (size_t)dimpad * 4 + (size_t)cap * 8 + ((size_t)1 << hash_bits) * 4 + 64 * 8 + (16 + 128) * 4;
+}
+
+template <int SPACE, int MODE, int NCH, bool STOP, bool TRACE>
+static hipError_t launch_t(const SearchParams& p, int blocks, size_t lds, hipStream_t s) {
+    auto k = hnsw_search_kernel<SPACE, MODE, NCH, STOP, TRACE>;
+    if (lds > 64 * 1024) {
+        hipError_t e = hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+        if (e != hipSuccess) return e;
+    }
+    hipLaunchKernelGGL(k, dim3(blocks), dim3(64), lds, s, p);
+    return hipGetLastError();
+}
+
+template <int SPACE, bool STOP, bool TRACE>
+static hipError_t launch_dim(const SearchParams& p, int blocks, size_t lds, hipStream_t s) {
+    if (p.g.dim % 16 != 0) return launch_t<SPACE, 0, 0, STOP, TRACE>(p, blocks, lds, s);
+    switch (p.g.dim / 16) {
+        case 1: return launch_t<SPACE, 1, 1, STOP, TRACE>(p, blocks, lds, s);
+        case 2: return launch_t<SPACE, 1, 2, STOP, TRACE>(p, blocks, lds, s);
+        case 4: return launch_t<SPACE, 1, 4, STOP, TRACE>(p, blocks, lds, s);
+        case 6: return launch_t<SPACE, 1, 6, STOP, TRACE>(p, blocks, lds, s);
+        case 8: return launch_t<SPACE, 1, 8, STOP, TRACE>(p, blocks, lds, s);
+        default: return launch_t<SPACE, 1, 0, STOP, TRACE>(p, blocks, lds, s);
+    }
+}
+
+hipError_t launch_search(const SearchParams& p, int blocks, hipStream_t s) {
+    size_t lds = search_lds_bytes(p.g.dim, p.cap, p.hash_bits);
+    uint32_t* w = p.work;
+    hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, s, w, 1u);
+    bool stop = p.use_stop != 0, trace = p.trace_feat != nullptr;
+    if (p.g.space == 0) {
+        if (trace) return launch_dim<0, false, true>(p, blocks, lds, s);
+        if (stop) return launch_dim<0, true, false>(p, blocks, lds, s);
+        return launch_dim<0, false, false>(p, blocks, lds, s);
+    }
+    if (trace) return launch_dim<1, false, true>(p, blocks, lds, s);
+    if (stop) return launch_dim<1, true, false>(p, blocks, lds, s);
+    return launch_dim<1, false, false>(p, blocks, lds, s);
+}
+
+int search_blocks_per_cu(const SearchParams& p) {
+    size_t lds = search_lds_bytes(p.g.dim, p.cap, p.hash_bits);
+    int nb = 0;
+    hipError_t e;
+    if (p.g.dim % 16 == 0 && p.g.dim / 16 == 8 && p.g.space == 0)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, hnsw_search_kernel<0, 1, 8, false, false>, 64, lds);
+    else if (p.g.space == 0)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, hnsw_search_kernel<0, 1, 0, false, false>, 64, lds);
+    else
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, hnsw_search_kernel<1, 1, 0, false, false>, 64, lds);
+    if (e != hipSuccess || nb <= 0) nb = 1;
+    return nb;
+}
+
+}  // namespace hg

@@ -1,0 +1,172 @@
+/* hnswgpu: MI355X (gfx950) query engine for hnswlib indexes -- C ABI.
+ *
+ * Plain pointers and sizes only.  Every call returns HG_OK (0) or an error code; the message
+ * of the last error on the calling thread is returned by hg_last_error().  Messages reuse the
+ * reference's texts where the condition is the same one.
+ *
+ * Which reference interface each entry point replaces (file:line in hnswlib v0.9.0):
+ *   hg_hnsw_create        HierarchicalNSW(space, max_elements, M, ef_construction, random_seed)
+ *                         hnswalg.h:89-144; Index.init_index bindings.cpp:192-207
+ *   hg_hnsw_add_items     Index.add_items -> ParallelFor -> addPoint  bindings.cpp:251-304,
+ *                         hnswalg.h:1152-1266 (CPU: the index stays "built by addPoint")
+ *   hg_hnsw_save / _load  saveIndex / loadIndex  hnswalg.h:685-821
+ *   hg_hnsw_serialize     the same byte stream as saveIndex, into memory
+ *   hg_gpu_index_create   (new) export of data_level0_memory_ / linkLists_ into the GPU layout
+ *   hg_gpu_search         Index.knn_query bindings.cpp:612-697 over
+ *                         HierarchicalNSW::searchKnn hnswalg.h:1269-1323 (bare-bone branch)
+ *   hg_gpu_search_device  the same with device-resident queries/results, per-query ef and k
+ *                         (extends the single ef_ of hnswalg.h:173-175)
+ *   hg_bf_create / hg_bf_add   BruteforceSearch(space, maxElements) / addPoint
+ *                         bruteforce.h:46-83; BFIndex.init_index/add_items bindings.cpp:785-819
+ *   hg_bf_search          BFIndex.knn_query bindings.cpp:850-929 over
+ *                         BruteforceSearch::searchKnn bruteforce.h:106-125
+ *   hg_bf_search_device   the same, device-resident, with reduced-precision distance GEMM
+ *                         (BF16 / FP8) and an exact FP32 re-rank of the top-c candidates
+ */
+#ifndef HNSWGPU_H
+#define HNSWGPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HG_OK 0
+#define HG_ERR_INVALID 1  /* bad argument (reference: std::runtime_error on the same checks) */
+#define HG_ERR_RUNTIME 2  /* index/algorithm error (capacity, corrupted file, cand error ...) */
+#define HG_ERR_HIP 3      /* HIP runtime failure (device OOM, launch failure ...) */
+#define HG_ERR_SHORT 4    /* some query found fewer than k results */
+
+#define HG_SPACE_L2 0
+#define HG_SPACE_IP 1
+
+#define HG_PREC_FP32 0 /* exact: reference distance order, no candidates dropped */
+#define HG_PREC_BF16 1 /* BF16 MFMA distance GEMM, FP32 accumulate, top-c FP32 re-rank */
+#define HG_PREC_FP8 2  /* FP8 e4m3 (per-vector scale) MFMA GEMM, FP32 accumulate, re-rank */
+
+const char* hg_last_error(void);
+
+/* ---- CPU-side HNSW index (construction prerequisite, persistence) ---- */
+int hg_hnsw_create(int space, size_t dim, size_t max_elements, size_t M, size_t ef_construction,
+                   size_t random_seed, void** out_index);
+int hg_hnsw_load(int space, size_t dim, const char* path, size_t max_elements, void** out_index);
+/* labels may be NULL (labels continue from the index's running counter). */
+int hg_hnsw_add_items(void* index, const float* data, size_t rows, const uint64_t* labels, int num_threads);
+int hg_hnsw_save(void* index, const char* path);
+/* Writes at most `capacity` bytes; *size receives the full serialized size. */
+int hg_hnsw_serialize(void* index, void* buffer, size_t capacity, size_t* size);
+/* info[0..7] = count, max_elements, dim, M, ef_construction, maxlevel, entry, num_deleted */
+int hg_hnsw_info(void* index, int64_t* info);
+int hg_hnsw_set_ef(void* index, size_t ef);
+int hg_hnsw_get_ef(void* index, size_t* ef);
+int hg_hnsw_free(void* index);
+/* Index.normalize_vector (bindings.cpp:241-248) applied row by row (the cosine space). */
+int hg_normalize_rows(const float* in, float* out, size_t rows, size_t dim);
+
+/* ---- GPU copy of an HNSW index ---- */
+int hg_gpu_index_create(void* index, int device, void** out_gpu_index);
+int hg_gpu_index_free(void* gpu_index);
+/* Bytes resident on the device for this index. */
+int hg_gpu_index_bytes(void* gpu_index, size_t* bytes);
+
+/* knn_query on host buffers: queries [nq][dim] f32; labels/dists [nq][k], closest first.
+ * counters (optional) receives 4 int32 per query: level-0 expansions, level-0 distance
+ * evaluations (entry included), upper-layer hops, upper-layer evaluations (entry included).
+ * Returns HG_ERR_SHORT (after writing every row, short rows padded with UINT64_MAX / +inf)
+ * when a query finds fewer than k results. */
+int hg_gpu_search(void* gpu_index, const float* queries, size_t nq, size_t k, size_t ef, uint64_t* labels,
+                  float* dists, int32_t* counters);
+
+/* Device-resident batched search.  ef_per_query / k_per_query (device int32 [nq]) may be NULL,
+ * then ef / k apply to all queries.  Rows of labels/dists have stride k_stride (>= every k).
+ * stop_model (may be NULL) enables learned early termination (hg_stop_model_*).
+ * stream is a hipStream_t (NULL = default stream).  Asynchronous: no host synchronisation. */
+int hg_gpu_search_device(void* gpu_index, const float* d_queries, size_t nq, const int32_t* d_ef_per_query,
+                         const int32_t* d_k_per_query, size_t ef, size_t k, size_t k_stride, uint64_t* d_labels,
+                         float* d_dists, int32_t* d_counters, void* stop_model, void* stream);
+
+/* Launch configuration chosen by the autotuner for (max ef, max k) on this index:
+ * out[0] = waves per workgroup, out[1] = visited-hash slots per wave, out[2] = beam capacity,
+ * out[3] = LDS bytes per workgroup, out[4] = workgroups launched. */
+int hg_gpu_search_plan(void* gpu_index, size_t max_ef, size_t max_k, size_t nq, int32_t* out);
+/* Re-tune: time candidate (waves, hash) pairs on the given device-resident queries and keep the
+ * fastest for this (max ef, max k) bucket. */
+int hg_gpu_autotune(void* gpu_index, const float* d_queries, size_t nq, size_t ef, size_t k, void* stream);
+
+/* ---- Brute force ---- */
+int hg_bf_create(int space, size_t dim, size_t max_elements, int device, void** out_bf);
+/* labels may be NULL (row index continues from the current count). */
+int hg_bf_add(void* bf, const float* data, size_t rows, const uint64_t* labels);
+int hg_bf_count(void* bf, size_t* count);
+int hg_bf_free(void* bf);
+/* Exact FP32 BFIndex.knn_query on host buffers (closest first, ties by label). */
+int hg_bf_search(void* bf, const float* queries, size_t nq, size_t k, uint64_t* labels, float* dists);
+/* Device-resident search.  precision = HG_PREC_*; rerank_c = candidates kept per query by the
+ * reduced-precision pass (ignored for FP32).  stats (optional, host, 4 doubles): GEMM ms,
+ * select ms, re-rank ms, queries whose candidate pass had to be retried. */
+int hg_bf_search_device(void* bf, const float* d_queries, size_t nq, size_t k, int precision, size_t rerank_c,
+                        uint64_t* d_labels, float* d_dists, double* stats, void* stream);
+/* FP64 near-tie check on the final lists: for each query, pairs of neighbouring results whose
+ * FP32 distances differ by at most rel_tol (relative) are re-evaluated in FP64.  out[0] =
+ * near-tie pairs checked, out[1] = pairs whose FP64 order disagrees with the FP32 order,
+ * out[2] = max |d32 - d64| over all returned results, out[3] = max relative error. */
+int hg_bf_fp64_check(void* bf, const float* d_queries, size_t nq, size_t k, const uint64_t* d_labels,
+                     const float* d_dists, double rel_tol, double* out, void* stream);
+
+/* ---- Learned early termination (stop model) ---- */
+/* Create a stop model for a given feature count / hidden width (weights zero). */
+int hg_stop_model_create(int in_dim, int hidden, int device, void** out_model);
+int hg_stop_model_free(void* model);
+int hg_stop_model_param_count(void* model, size_t* count);
+/* Parameters in FP32 (host), layout W1[h][in], b1[h], W2[h][h], b2[h], w3[h], b3. */
+int hg_stop_model_set_params(void* model, const float* params, size_t count);
+int hg_stop_model_get_params(void* model, float* params, size_t count);
+/* Feature normalisation (mean, 1/std per feature) applied before the network. */
+int hg_stop_model_set_norm(void* model, const float* mean, const float* inv_std, size_t count);
+/* Inference thresholds: every `period` level-0 expansions the network runs; a query stops
+ * when sigmoid(logit) >= threshold, and after `max_ext` extensions of ef... (see DESIGN). */
+int hg_stop_model_config(void* model, int period, float threshold, int min_hops);
+/* Trace collection: run the plain search and write, per query, up to max_records feature
+ * vectors (every `period` expansions) plus the current top-k ids at each record.
+ * d_features [nq][max_records][in_dim] f32, d_topk [nq][max_records][k] u32,
+ * d_nrec [nq] int32.  */
+int hg_gpu_collect_traces(void* gpu_index, const float* d_queries, size_t nq, size_t ef, size_t k, int period,
+                          int max_records, float* d_features, uint32_t* d_topk, int32_t* d_nrec, void* stream);
+/* Forward pass (logits) on device features [n][in_dim]. */
+int hg_stop_model_forward(void* model, const float* d_x, size_t n, float* d_logits, void* stream);
+/* One Adam step on a minibatch (mean BCE-with-logits); returns the loss through *loss
+ * (host). d_grad_out (optional, device f32 [param_count]) receives the gradient. */
+int hg_stop_model_train_step(void* model, const float* d_x, const float* d_y, size_t n, float lr, float beta1,
+                             float beta2, float eps, float* loss, float* d_grad_out, void* stream);
+
+/* ---- Device plumbing (so hosts need no other GPU runtime binding) ---- */
+int hg_device_count(int* count);
+int hg_set_device(int device);
+int hg_malloc(void** ptr, size_t bytes);
+int hg_free(void* ptr);
+int hg_memcpy_h2d(void* dst, const void* src, size_t bytes, void* stream);
+int hg_memcpy_d2h(void* dst, const void* src, size_t bytes, void* stream);
+int hg_memset(void* dst, int value, size_t bytes, void* stream);
+int hg_stream_create(void** stream);
+int hg_stream_destroy(void* stream);
+int hg_stream_synchronize(void* stream);
+int hg_device_synchronize(void);
+/* Event timer on a stream: hg_timer_start records, hg_timer_stop records and waits, returning
+ * the elapsed milliseconds between the two events. */
+int hg_timer_create(void** timer);
+int hg_timer_destroy(void* timer);
+int hg_timer_start(void* timer, void* stream);
+int hg_timer_stop(void* timer, void* stream, float* ms);
+/* hipGraph capture of a device search: capture once, replay many times. */
+int hg_graph_capture_search(void* gpu_index, const float* d_queries, size_t nq, const int32_t* d_ef_per_query,
+                            const int32_t* d_k_per_query, size_t ef, size_t k, size_t k_stride, uint64_t* d_labels,
+                            float* d_dists, int32_t* d_counters, void* stop_model, void* stream, void** out_graph);
+int hg_graph_launch(void* graph, void* stream);
+int hg_graph_free(void* graph);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

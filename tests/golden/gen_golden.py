@@ -1,0 +1,105 @@
+"""Writes the golden fixtures of tests/golden/ from the reference compiled from its own
+sources (oracle/_ref: the reference Python module built from python_bindings/bindings.cpp and
+the hnswlib headers with setup.py's flags).  Run in a container where /root/reference exists:
+
+    make -C oracle ref && python tests/golden/gen_golden.py
+
+Every fixture stores its inputs' SHA-256 (or the inputs themselves when small), the reference's
+outputs and the SHA-256 of the reference's saveIndex bytes for single-threaded builds.
+"""
+import hashlib
+import os
+import sys
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.dirname(os.path.abspath(__file__))))
+sys.path.insert(0, ROOT)
+from hnsw_rabitq_amd import datasets  # noqa: E402
+from oracle import pyoracle  # noqa: E402
+
+OUT = os.path.dirname(os.path.abspath(__file__))
+
+
+def sha(a):
+    return hashlib.sha256(np.ascontiguousarray(a).tobytes()).hexdigest()
+
+
+def build_ref(hnswlib, space, data, M=16, efc=200, seed=100):
+    p = hnswlib.Index(space=space, dim=data.shape[1])
+    p.init_index(max_elements=data.shape[0], M=M, ef_construction=efc, random_seed=seed)
+    p.add_items(data, num_threads=1)
+    path = "/tmp/_golden_index.bin"
+    p.save_index(path)
+    with open(path, "rb") as f:
+        blob = f.read()
+    os.remove(path)
+    return p, hashlib.sha256(blob).hexdigest()
+
+
+def hnsw_case(hnswlib, name, space, data, queries, efs, ks):
+    p, ghash = build_ref(hnswlib, space, data)
+    out = dict(data_sha=sha(data), query_sha=sha(queries), index_sha=ghash, efs=np.array(efs), ks=np.array(ks))
+    for ef in efs:
+        for k in ks:
+            p.set_ef(ef)
+            lab, dis = p.knn_query(queries, k=k, num_threads=1)
+            out[f"ids_ef{ef}_k{k}"] = lab.astype(np.uint64)
+            out[f"dists_ef{ef}_k{k}"] = dis.astype(np.float32)
+    np.savez_compressed(os.path.join(OUT, name), **out)
+    print(name, ghash[:16])
+
+
+def main():
+    hnswlib = pyoracle.ref_module()
+    shim = pyoracle.ref_shim()
+    assert hnswlib is not None and shim is not None, "build the reference first: make -C oracle ref"
+
+    # F2: distance bits at every dimension class (SIMD16, SIMD4, residual forms, scalar).
+    rng = np.random.default_rng(123)
+    dims = list(range(1, 41)) + [48, 64, 96, 100, 128, 161, 768]
+    kat = {"dims": np.array(dims)}
+    for d in dims:
+        a = rng.standard_normal((16, d)).astype(np.float32)
+        b = rng.standard_normal((16, d)).astype(np.float32)
+        for sp, sname in ((0, "l2"), (1, "ip")):
+            out = np.empty(16, np.float32)
+            shim.ref_distance_batch(sp, d, a.ctypes.data, b.ctypes.data, 16, out.ctypes.data)
+            kat[f"{sname}_{d}"] = out
+        kat[f"a_{d}"] = a
+        kat[f"b_{d}"] = b
+    np.savez_compressed(os.path.join(OUT, "dist_kat.npz"), **kat)
+
+    # F4: level sequence of the insertion RNG for random_seed=100, M=16.
+    lv = np.empty(100_000, np.int32)
+    shim.ref_random_levels(16, 100, lv.shape[0], lv.ctypes.data)
+    np.savez_compressed(os.path.join(OUT, "levels_seed100.npz"), levels=lv)
+
+    # F1: C1 (10k x 16 uniform, L2), HNSW ef=50 k=10 and BF k=10.
+    data = datasets.c1_uniform()
+    queries = datasets.c1_uniform(n=1000, seed=1)
+    hnsw_case(hnswlib, "c1_golden.npz", "l2", data, queries, [50], [10])
+    bf = hnswlib.BFIndex(space="l2", dim=16)
+    bf.init_index(max_elements=data.shape[0])
+    bf.add_items(data)
+    lab, dis = bf.knn_query(queries, k=10, num_threads=1)
+    np.savez_compressed(os.path.join(OUT, "c1_bf_golden.npz"), ids=lab.astype(np.uint64), dists=dis,
+                        data_sha=sha(data), query_sha=sha(queries))
+
+    # F3: reduced C2 (integer-valued Gamma data, many distance ties) and reduced C4 (IP GMM).
+    data = datasets.c2_gamma(n=20_000)
+    queries = datasets.c2_gamma(n=200, seed=1)
+    hnsw_case(hnswlib, "c2s_golden.npz", "l2", data, queries, [10, 40], [10])
+    data, queries = datasets.c4_gmm(n=20_000, nq=200, n_centres=64)
+    hnsw_case(hnswlib, "c4s_golden.npz", "ip", data, queries, [16, 64], [1, 10, 100])
+
+    # Cosine space at an odd dimension (normalisation + residual kernels).
+    rng = np.random.default_rng(7)
+    data = rng.standard_normal((2000, 21)).astype(np.float32)
+    queries = rng.standard_normal((100, 21)).astype(np.float32)
+    hnsw_case(hnswlib, "cos21_golden.npz", "cosine", data, queries, [30], [5])
+    np.savez_compressed(os.path.join(OUT, "cos21_inputs.npz"), data=data, queries=queries)
+
+
+if __name__ == "__main__":
+    main()
