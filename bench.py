@@ -1,0 +1,249 @@
+#!/usr/bin/env python3
+"""Benchmark: QPS at recall@10 = 0.95 on 1M x 128 L2 (Gamma(0.5, 40) integer-valued descriptors,
+M=16, ef_construction=200), plain fixed-ef GPU search vs learned early termination; brute-force
+MFMA TFLOPS on 1M x 768 inner product.
+
+One "step" = one batched GPU search of the 10,000 queries at the operating point (the smallest
+swept ef whose recall@10 >= 0.95).  Inputs are resident in HBM before timing starts; timing
+uses HIP events on the stream the kernels run on.  Prints ONE JSON line (rank 0, N=1).
+
+    python bench.py [--steps K] [--warmup W] [--gpus 1] [--n N] [--nq Q] [--no-learned] [--no-bf]
+"""
+import argparse
+import ctypes
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_HBM_GBS = 8000.0
+PEAK_BF16_TFLOPS = 2500.0
+PEAK_FP8_TFLOPS = 5000.0
+EF_SWEEP = [10, 20, 40, 80, 160, 320, 640, 1280, 2560]
+T0 = time.time()
+
+
+def log(msg):
+    print(f"[bench {time.time() - T0:7.1f}s] {msg}", file=sys.stderr, flush=True)
+
+
+def recall_at(labels, gt, k):
+    hits = 0
+    for a, b in zip(labels[:, :k], gt[:, :k]):
+        hits += len(np.intersect1d(a, b, assume_unique=False))
+    return hits / (labels.shape[0] * k)
+
+
+class Timer:
+    def __init__(self, lib, stream):
+        self.lib = lib
+        self.stream = stream
+        self.t = ctypes.c_void_p()
+        lib.call("hg_timer_create", ctypes.byref(self.t))
+
+    def start(self):
+        self.lib.call("hg_timer_start", self.t, self.stream)
+
+    def stop(self):
+        ms = ctypes.c_float()
+        self.lib.call("hg_timer_stop", self.t, self.stream, ctypes.byref(ms))
+        return ms.value
+
+
+def algorithmic_bytes(ctr, dim, k, maxM, maxM0):
+    """Bytes the search must read per query in the GPU layout: the query, one upper link row
+    (4*maxM) per upper hop, one level-0 link row (4*maxM0) per expansion, one vector per distance
+    evaluation, and the k results written (8-byte label + 4-byte distance)."""
+    h0, e0, hu, eu = (ctr[:, i].astype(np.int64) for i in range(4))
+    return (4 * dim + 4 * maxM * hu + 4 * dim * eu + 4 * maxM0 * h0 + 4 * dim * e0 + 12 * k).astype(np.float64)
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--n", type=int, default=1_000_000)
+    ap.add_argument("--nq", type=int, default=10_000)
+    ap.add_argument("--build-threads", type=int, default=0)
+    ap.add_argument("--index-cache", default="")
+    ap.add_argument("--no-learned", action="store_true")
+    ap.add_argument("--no-bf", action="store_true")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--bf-n", type=int, default=1_000_000)
+    ap.add_argument("--bf-nq", type=int, default=10_000)
+    ap.add_argument("--cpu-queries", type=int, default=2000)
+    args = ap.parse_args()
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus != 1 or world != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path (queries shard with no "
+              "exchange; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import hnsw_rabitq_amd as H
+    from hnsw_rabitq_amd import _lib, datasets, DeviceArray
+
+    D, K = 128, 10
+    log(f"generating C2 data n={args.n} nq={args.nq}")
+    data = datasets.c2_gamma(n=args.n)
+    queries = datasets.c2_gamma(n=args.nq, seed=1)
+
+    idx = H.Index("l2", D)
+    if args.index_cache and os.path.exists(args.index_cache):
+        log(f"loading index {args.index_cache}")
+        idx.load_index(args.index_cache)
+    else:
+        idx.init_index(args.n, M=16, ef_construction=200, random_seed=100)
+        nt = args.build_threads or os.cpu_count() or 1
+        log(f"building HNSW (M=16, efC=200) with {nt} threads")
+        step = 100_000
+        for s in range(0, args.n, step):
+            idx.add_items(data[s:s + step], np.arange(s, min(args.n, s + step), dtype=np.uint64), num_threads=nt)
+            log(f"  inserted {min(args.n, s + step)}")
+        if args.index_cache:
+            idx.save_index(args.index_cache)
+    info = idx._info()
+    maxM, maxM0 = int(info[3]), 2 * int(info[3])
+
+    stream = ctypes.c_void_p()
+    _lib.call("hg_stream_create", ctypes.byref(stream))
+    gpu = idx.gpu()
+    log(f"GPU index resident: {gpu.nbytes() / 1e6:.0f} MB")
+    dq = DeviceArray.from_numpy(queries)
+    nq = args.nq
+    dl = DeviceArray(nq * K * 8)
+    dd = DeviceArray(nq * K * 4)
+    dc = DeviceArray(nq * 16)
+    timer = Timer(_lib, stream)
+
+    # ground truth: exact FP32 brute force on the GPU (reference distance order, ties by label)
+    log("ground truth (exact FP32 brute force on GPU)")
+    bf = H.BFIndex("l2", D)
+    bf.init_index(args.n)
+    bf.add_items(data, np.arange(args.n, dtype=np.uint64))
+    gt_l = DeviceArray(nq * K * 8)
+    gt_d = DeviceArray(nq * K * 4)
+    timer.start()
+    _lib.call("hg_bf_search_device", bf._h, dq.ptr, nq, K, _lib.PREC_FP32, 0, gt_l.ptr, gt_d.ptr, None, stream)
+    gt_ms = timer.stop()
+    gt = gt_l.to_numpy(np.uint64, (nq, K))
+    exact_tflops = 2.0 * nq * args.n * D / (gt_ms * 1e-3) / 1e12
+    log(f"  exact BF {gt_ms:.1f} ms ({exact_tflops:.1f} TFLOP/s FP32 exact order)")
+    del bf
+
+    # ef sweep (plain fixed ef)
+    sweep = []
+    for ef in EF_SWEEP:
+        gpu.search_device(dq, nq, K, ef, dl, dd, d_counters=dc, stream=stream)
+        timer.start()
+        gpu.search_device(dq, nq, K, ef, dl, dd, d_counters=dc, stream=stream)
+        ms = timer.stop()
+        lab = dl.to_numpy(np.uint64, (nq, K))
+        ctr = dc.to_numpy(np.int32, (nq, 4))
+        rec = recall_at(lab, gt, K)
+        byts = algorithmic_bytes(ctr, D, K, maxM, maxM0).sum()
+        sweep.append(dict(ef=ef, recall=rec, ms=ms, qps=nq / (ms * 1e-3), gbs=byts / (ms * 1e-3) / 1e9,
+                          bytes_per_query=byts / nq, hops=float(ctr[:, 0].mean()), evals=float(ctr[:, 1].mean())))
+        log(f"  ef={ef:4d} recall@10={rec:.4f} {ms:8.2f} ms  QPS={nq / (ms * 1e-3):11.0f}  "
+            f"{byts / (ms * 1e-3) / 1e9:7.1f} GB/s  hops={ctr[:, 0].mean():.0f} evals={ctr[:, 1].mean():.0f}")
+    op = next((s for s in sweep if s["recall"] >= 0.95), sweep[-1])
+    ef_op = op["ef"]
+    log(f"operating point ef={ef_op} recall={op['recall']:.4f}")
+    t = time.time()
+    gpu.autotune(dq, nq, ef_op, K, stream=stream)
+    plan = gpu.plan(ef_op, K, nq)
+    log(f"autotuned in {time.time() - t:.1f}s: {plan}")
+
+    # timed steps at the operating point
+    for _ in range(args.warmup):
+        gpu.search_device(dq, nq, K, ef_op, dl, dd, stream=stream)
+    _lib.call("hg_stream_synchronize", stream)
+    _lib.call("hg_device_synchronize")
+    kernel_ms = []
+    t_wall = time.perf_counter()
+    for _ in range(args.steps):
+        timer.start()
+        gpu.search_device(dq, nq, K, ef_op, dl, dd, stream=stream)
+        kernel_ms.append(timer.stop())
+    _lib.call("hg_device_synchronize")
+    wall = time.perf_counter() - t_wall
+    ms_step = wall * 1e3 / args.steps
+    ev_ms = float(np.mean(kernel_ms))
+    plain_qps = nq / (ev_ms * 1e-3)
+    # counters of the operating point for the roofline
+    gpu.search_device(dq, nq, K, ef_op, dl, dd, d_counters=dc, stream=stream)
+    ctr = dc.to_numpy(np.int32, (nq, 4))
+    bytes_launch = algorithmic_bytes(ctr, D, K, maxM, maxM0).sum()
+    achieved = bytes_launch / (ev_ms * 1e-3) / 1e9
+    log(f"plain: {plain_qps:.0f} QPS at ef={ef_op}; kernel {ev_ms:.3f} ms; {achieved:.1f} GB/s algorithmic")
+
+    result = dict(plain_qps_at_095=plain_qps, plain_ef=ef_op, plain_recall=op["recall"], plan=plan)
+
+    # learned early termination
+    learned = None
+    if not args.no_learned:
+        try:
+            from hnsw_rabitq_amd import learned_stop
+            learned = learned_stop.run(gpu, H, _lib, data_dim=D, queries=queries, dq=dq, gt=gt, stream=stream,
+                                       timer=timer, sweep=sweep, log=log, steps=args.steps, plain_qps=plain_qps)
+            result.update(learned)
+        except ImportError:
+            log("learned stop module not available")
+
+    # brute force MFMA TFLOPS (1M x 768 IP GMM)
+    if not args.no_bf:
+        from hnsw_rabitq_amd import bf_bench
+        result.update(bf_bench.run(H, _lib, n=args.bf_n, nq=args.bf_nq, stream=stream, timer=timer, log=log))
+
+    # CPU baseline: the oracle restatement of knn_query on all host cores, bounded sample
+    cpu = None
+    if not args.no_cpu:
+        from oracle import pyoracle
+        nthreads = os.cpu_count() or 1
+        o = pyoracle.OracleHnsw(blob=idx.serialize())
+        sample = queries[: args.cpu_queries]
+        o.search(0, sample[:200], K, ef_op, mode=pyoracle.MODE_HEAP, nthreads=nthreads)  # warm-up
+        times = []
+        for _ in range(3):
+            t = time.perf_counter()
+            o.search(0, sample, K, ef_op, mode=pyoracle.MODE_HEAP, nthreads=nthreads)
+            times.append(time.perf_counter() - t)
+        cpu_qps = sample.shape[0] / float(np.median(times))
+        cpu = dict(value=cpu_qps, unit="queries/s", cores=nthreads, kind="port",
+                   sample=f"{sample.shape[0]} of the {nq} queries at ef={ef_op}, k=10, median of 3 after warm-up")
+        log(f"CPU oracle port: {cpu_qps:.0f} QPS on {nthreads} threads")
+
+    value = result.get("learned_qps_at_095") or plain_qps
+    line = {
+        "metric": "QPS@recall10=0.95, 1M×128 L2: plain vs learned-stop GPU; brute-force TFLOPS",
+        "value": value,
+        "unit": "queries/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": f"hnsw_l2_gamma_{args.n}x{D}_q{nq}_M16_efC200_k10",
+                   "n": args.n, "dim": D, "queries": nq, "ef_sweep": EF_SWEEP, "k": K},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
+                     "frac": achieved / PEAK_HBM_GBS, "traffic": None,
+                     "kernel": "hnsw_search_kernel", "bytes_per_launch": bytes_launch, "kernel_ms": ev_ms},
+        "cpu_baseline": cpu,
+        "sweep": [{k2: (round(v, 5) if isinstance(v, float) else v) for k2, v in s.items()} for s in sweep],
+        "exact_bf_gt_ms": gt_ms,
+    }
+    line.update(result)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

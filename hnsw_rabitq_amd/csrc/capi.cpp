@@ -64,11 +64,12 @@ Plan make_plan(GpuIndex* gi, size_t max_ef, size_t max_k, size_t nq) {
     if (it != gi->tuned.end()) {
         p.hash_bits = it->second.first;
     } else {
-        // visited nodes grow roughly as 16-32 per beam entry at M=16; the hash may be lossy,
-        // so it is sized for occupancy rather than for the worst case
-        p.hash_bits = std::min<uint32_t>(13, std::max<uint32_t>(10, ceil_log2(16 * ef_eff)));
+        // visited nodes grow roughly as 16-32 per beam entry at M=16; the hash may be lossy
+        // (an overwritten slot only costs a re-evaluation), so it is sized for occupancy
+        p.hash_bits = std::min<uint32_t>(12, std::max<uint32_t>(10, ceil_log2(8 * ef_eff)));
     }
-    p.hash_bits = std::max(p.hash_bits, ceil_log2(2 * (uint64_t)p.cap));  // output staging reuse
+    // the output pass stages k labels (8 bytes each) in the hash area
+    p.hash_bits = std::max(p.hash_bits, ceil_log2(2 * (uint64_t)std::max<size_t>(max_k, 1)));
     hg::SearchParams sp{};
     sp.g = gi->g;
     sp.cap = p.cap;
@@ -451,14 +452,14 @@ HG_API int hg_gpu_autotune(void* gv, const float* dq, size_t nq, size_t ef, size
         od.reserve(nq * k * 4);
         size_t ef_eff = std::max(ef, k);
         uint32_t cap = (uint32_t)((ef_eff + 63) / 64 * 64);
-        uint32_t min_bits = ceil_log2(2 * (uint64_t)cap);
+        uint32_t min_bits = ceil_log2(2 * (uint64_t)k);
         hipEvent_t e0, e1;
         hip_check(hipEventCreate(&e0), "hipEventCreate");
         hip_check(hipEventCreate(&e1), "hipEventCreate");
         float best = 1e30f;
         std::pair<uint32_t, int> best_cfg(0, 0);
         gi->tuned.erase(cap);
-        for (uint32_t bits = std::max<uint32_t>(min_bits, 9); bits <= 14; bits++) {
+        for (uint32_t bits = std::max<uint32_t>(min_bits, 9); bits <= 13; bits++) {
             for (int per_cu : {4, 8, 12, 16, 24, 32}) {
                 gi->tuned[cap] = std::make_pair(bits, per_cu);
                 Plan pl = make_plan(gi, ef, k, nq);

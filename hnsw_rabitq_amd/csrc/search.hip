@@ -78,7 +78,7 @@ __device__ __forceinline__ int lanes_below(uint64_t m, int lane) {
 __device__ __forceinline__ bool hash_insert(uint32_t* hash, uint32_t mask, uint32_t id) {
     uint32_t h = (id * 2654435761u) & mask;
 #pragma unroll 1
-    for (int p = 0; p < 16; p++) {
+    for (int p = 0; p < 8; p++) {
         uint32_t slot = (h + p) & mask;
         uint32_t old = atomicCAS(&hash[slot], SENT, id);
         if (old == SENT) return true;

@@ -1,0 +1,80 @@
+"""GPU brute force: exact FP32 path bit-exact against the reference's golden vectors and the
+oracle; reduced-precision (BF16 / FP8 MFMA + exact re-rank) path against the exact path."""
+import numpy as np
+import pytest
+
+from conftest import golden
+from helpers import workload
+
+pytestmark = pytest.mark.gpu
+
+
+def _bf(hg, space, data, labels=None):
+    b = hg.BFIndex(space, data.shape[1])
+    b.init_index(data.shape[0])
+    b.add_items(data, labels)
+    return b
+
+
+def test_bf_exact_matches_reference_golden(hg):
+    g = golden("c1_bf_golden.npz")
+    space, data, queries = workload("c1")
+    print("case=c1 bf exact k=10", flush=True)
+    lab, dis = _bf(hg, space, data).knn_query(queries, k=10)
+    np.testing.assert_array_equal(lab, g["ids"])
+    np.testing.assert_array_equal(dis.view(np.uint32), g["dists"].view(np.uint32))
+
+
+@pytest.mark.parametrize("space,dim,n,k", [("l2", 21, 3000, 7), ("ip", 96, 5000, 100), ("l2", 128, 4000, 10),
+                                           ("ip", 768, 3000, 100), ("l2", 5, 500, 50)])
+def test_bf_exact_matches_oracle(hg, orc, space, dim, n, k):
+    print(f"bf exact space={space} dim={dim} n={n} k={k}", flush=True)
+    rng = np.random.default_rng(dim)
+    data = rng.standard_normal((n, dim)).astype(np.float32)
+    q = rng.standard_normal((37, dim)).astype(np.float32)
+    lab, dis = _bf(hg, space, data).knn_query(q, k=k)
+    ol, od = orc.bf_search(0 if space == "l2" else 1, data, q, k)
+    np.testing.assert_array_equal(lab, ol)
+    np.testing.assert_array_equal(dis.view(np.uint32), od.view(np.uint32))
+
+
+def test_bf_exact_ties_by_label(hg, orc):
+    """Integer data: many equal distances; ties resolve by the smaller label, with labels not in
+    row order."""
+    rng = np.random.default_rng(1)
+    data = rng.integers(0, 3, (2000, 16)).astype(np.float32)
+    labels = rng.permutation(2000).astype(np.uint64) + 1000
+    q = rng.integers(0, 3, (20, 16)).astype(np.float32)
+    lab, dis = _bf(hg, "l2", data, labels).knn_query(q, k=40)
+    ol, od = orc.bf_search(0, data, q, 40, labels=labels)
+    np.testing.assert_array_equal(lab, ol)
+    np.testing.assert_array_equal(dis, od)
+
+
+@pytest.mark.parametrize("prec", ["bf16", "fp8"])
+@pytest.mark.parametrize("space", ["ip", "l2"])
+def test_bf_lowp_matches_exact(hg, space, prec):
+    """BF16/FP8 candidates + exact FP32 re-rank: the final lists equal the exact path's."""
+    from hnsw_rabitq_amd import datasets, DeviceArray, _lib
+    import ctypes
+    print(f"bf lowp space={space} prec={prec}", flush=True)
+    data, q = datasets.c3_gmm(n=40_000, nq=300, dim=256, n_centres=64)
+    if space == "l2":
+        data = data * 3.0
+        q = q * 3.0
+    b = _bf(hg, space, data)
+    k = 100
+    el, ed = b.knn_query(q, k=k)
+    dq = DeviceArray.from_numpy(q)
+    dl = DeviceArray(q.shape[0] * k * 8)
+    dd = DeviceArray(q.shape[0] * k * 4)
+    stats = np.zeros(4, np.float64)
+    P = {"bf16": _lib.PREC_BF16, "fp8": _lib.PREC_FP8}[prec]
+    _lib.call("hg_bf_search_device", b._h, dq.ptr, q.shape[0], k, P, 2048, dl.ptr, dd.ptr,
+              ctypes.c_void_p(stats.ctypes.data), None)
+    _lib.call("hg_device_synchronize")
+    ll = dl.to_numpy(np.uint64, (q.shape[0], k))
+    ld = dd.to_numpy(np.float32, (q.shape[0], k))
+    np.testing.assert_array_equal(ll, el)
+    np.testing.assert_array_equal(ld.view(np.uint32), ed.view(np.uint32))
+    print(f"stats {stats}", flush=True)

@@ -1,0 +1,119 @@
+"""Learned early termination: the HIP training kernels' loss and gradients against the CPU
+oracle (FP64), the in-search network against the forward kernel, and the stop hook's fallback
+(a network that never fires leaves the plain search bit-identical)."""
+import ctypes
+
+import numpy as np
+import pytest
+
+from helpers import built_index
+
+pytestmark = pytest.mark.gpu
+
+
+def bf16_round(p):
+    u = np.ascontiguousarray(p, np.float32).view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) & 0xFFFF0000
+    return u.astype(np.uint32).view(np.float32)
+
+
+def effective(params, I, H):
+    """Weights as the kernels use them: BF16-rounded matrices, FP32 biases."""
+    p = params.copy()
+    o = 0
+    for size, is_w in ((H * I, True), (H, False), (H * H, True), (H, False), (H, True), (1, False)):
+        if is_w:
+            p[o:o + size] = bf16_round(p[o:o + size])
+        o += size
+    return p
+
+
+def test_mlp_forward_and_gradients_match_oracle(orc):
+    from hnsw_rabitq_amd import DeviceArray, learned_stop
+    print("mlp gradient check", flush=True)
+    I, H, n = 12, 64, 512
+    rng = np.random.default_rng(0)
+    m = learned_stop.StopModel()
+    m.init_params(3)
+    params = m.get_params()
+    x = rng.standard_normal((n, I)).astype(np.float32)
+    y = (rng.random(n) < 0.4).astype(np.float32)
+    dx, dy = DeviceArray.from_numpy(x), DeviceArray.from_numpy(y)
+    dlog = DeviceArray(n * 4)
+    m.forward(dx, n, dlog)
+    logits = dlog.to_numpy(np.float32, (n,))
+    eff = effective(params, I, H)
+    ref = orc.mlp_forward(eff, I, H, x)
+    np.testing.assert_allclose(logits, ref, rtol=1e-4, atol=1e-4)
+    g = DeviceArray(m.nparam * 4)
+    loss = m.train_step(dx, dy, n, lr=0.0, dgrad=g)
+    grad = g.to_numpy(np.float32, (m.nparam,))
+    rloss, rgrad = orc.mlp_loss_grad(eff, I, H, x, y)
+    assert abs(loss - rloss) < 1e-4 * max(1.0, abs(rloss))
+    # relative to the gradient's scale (FP32 ordered sums vs FP64)
+    scale = np.abs(rgrad).max()
+    assert np.abs(grad - rgrad).max() < 1e-4 * scale
+    # a wrong-sign or dropped term would be far outside: check directionally too
+    cos = float(grad @ rgrad / (np.linalg.norm(grad) * np.linalg.norm(rgrad)))
+    assert cos > 0.99999
+    m.free()
+
+
+def test_adam_step_decreases_loss():
+    from hnsw_rabitq_amd import DeviceArray, learned_stop
+    rng = np.random.default_rng(1)
+    n = 4096
+    x = rng.standard_normal((n, 12)).astype(np.float32)
+    y = (x[:, 0] + 0.5 * x[:, 3] > 0).astype(np.float32)
+    m = learned_stop.StopModel()
+    m.init_params(0)
+    dx, dy = DeviceArray.from_numpy(x), DeviceArray.from_numpy(y)
+    first = m.train_step(dx, dy, n, lr=3e-3)
+    for _ in range(200):
+        last = m.train_step(dx, dy, n, lr=3e-3)
+    assert last < 0.5 * first
+    m.free()
+
+
+def test_stop_model_that_never_fires_is_plain_search(hg):
+    from hnsw_rabitq_amd import DeviceArray, learned_stop
+    p, space, data, queries = built_index("c1")
+    gpu = p.gpu()
+    nq, k, ef = queries.shape[0], 10, 50
+    dq = DeviceArray.from_numpy(queries)
+    l0, d0 = DeviceArray(nq * k * 8), DeviceArray(nq * k * 4)
+    l1, d1 = DeviceArray(nq * k * 8), DeviceArray(nq * k * 4)
+    gpu.search_device(dq, nq, k, ef, l0, d0)
+    m = learned_stop.StopModel()
+    params = np.zeros(m.nparam, np.float32)
+    params[-1] = -30.0  # logit -30: confidence ~1e-13, never >= 0.9
+    m.set_params(params)
+    m.config(period=4, threshold=0.9, min_hops=4)
+    gpu.search_device(dq, nq, k, ef, l1, d1, stop_model=m.h)
+    np.testing.assert_array_equal(l0.to_numpy(np.uint64, (nq, k)), l1.to_numpy(np.uint64, (nq, k)))
+    np.testing.assert_array_equal(d0.to_numpy(np.float32, (nq, k)), d1.to_numpy(np.float32, (nq, k)))
+    # a network that always fires stops every query at its first check: fewer expansions
+    params[-1] = 30.0
+    m.set_params(params)
+    c0, c1 = DeviceArray(nq * 16), DeviceArray(nq * 16)
+    gpu.search_device(dq, nq, k, ef, l0, d0, d_counters=c0)
+    gpu.search_device(dq, nq, k, ef, l1, d1, d_counters=c1, stop_model=m.h)
+    h0 = c0.to_numpy(np.int32, (nq, 4))[:, 0]
+    h1 = c1.to_numpy(np.int32, (nq, 4))[:, 0]
+    assert (h1 <= np.maximum(h0, 4)).all() and (h1 <= 4).all()
+    m.free()
+
+
+def test_traces_record_features_and_topk(hg):
+    from hnsw_rabitq_amd import DeviceArray, learned_stop
+    p, space, data, queries = built_index("c1")
+    gpu = p.gpu()
+    nq, k, ef = 200, 10, 50
+    dq = DeviceArray.from_numpy(queries[:nq])
+    f, t, r = learned_stop.collect_traces(gpu, dq, nq, ef, k, period=4, max_records=64)
+    lab, dis, ctr, _ = gpu.search(queries[:nq], k, ef, counters=True)
+    # one record every 4 expansions
+    np.testing.assert_array_equal(r, np.minimum(ctr[:, 0] // 4, 64))
+    assert np.isfinite(f[np.arange(64)[None, :] < r[:, None]]).all()
+    # feature 0 is the query norm
+    np.testing.assert_allclose(f[:, 0, 0], np.linalg.norm(queries[:nq], axis=1), rtol=1e-5)
