@@ -24,7 +24,7 @@ sys.path.insert(0, ROOT)
 PEAK_HBM_GBS = 8000.0
 PEAK_BF16_TFLOPS = 2500.0
 PEAK_FP8_TFLOPS = 5000.0
-EF_SWEEP = [10, 20, 40, 80, 160, 320, 640, 1280, 2560]
+EF_SWEEP = [10, 20, 40, 80, 160, 320, 640, 960, 1280, 1600, 1920, 2560]
 T0 = time.time()
 
 
@@ -78,6 +78,9 @@ def main():
     ap.add_argument("--bf-n", type=int, default=1_000_000)
     ap.add_argument("--bf-nq", type=int, default=10_000)
     ap.add_argument("--cpu-queries", type=int, default=2000)
+    ap.add_argument("--learned-ef", type=int, default=0, help="ef of the learned search (default 2x plain)")
+    ap.add_argument("--copy-probe", type=int, default=0,
+                    help="only run the streaming copy kernel over this many MiB (counter calibration)")
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
@@ -87,6 +90,15 @@ def main():
 
     import hnsw_rabitq_amd as H
     from hnsw_rabitq_amd import _lib, datasets, DeviceArray
+
+    if args.copy_probe:
+        nbytes = args.copy_probe << 20
+        a, b = DeviceArray(nbytes), DeviceArray(nbytes)
+        for _ in range(5):
+            _lib.call("hg_copy_kernel", b.ptr, a.ptr, nbytes, None)
+        _lib.call("hg_device_synchronize")
+        print(json.dumps({"copy_probe_bytes": nbytes, "launches": 5}))
+        return
 
     D, K = 128, 10
     log(f"generating C2 data n={args.n} nq={args.nq}")
@@ -190,7 +202,8 @@ def main():
         try:
             from hnsw_rabitq_amd import learned_stop
             learned = learned_stop.run(gpu, H, _lib, data_dim=D, queries=queries, dq=dq, gt=gt, stream=stream,
-                                       timer=timer, sweep=sweep, log=log, steps=args.steps, plain_qps=plain_qps)
+                                       timer=timer, sweep=sweep, log=log, steps=args.steps, plain_qps=plain_qps,
+                                       ef_trace=args.learned_ef or None)
             result.update(learned)
         except ImportError:
             log("learned stop module not available")

@@ -43,6 +43,7 @@ SIGNATURES = {
     "hg_bf_free": [_vp],
     "hg_bf_search": [_vp, _vp, _sz, _sz, _vp, _vp],
     "hg_bf_search_device": [_vp, _vp, _sz, _sz, _i32, _sz, _vp, _vp, _vp, _vp],
+    "hg_bf_gemm_variant": [_i32],
     "hg_bf_fp64_check": [_vp, _vp, _sz, _sz, _vp, _vp, ctypes.c_double, _vp, _vp],
     "hg_stop_model_create": [_i32, _i32, _i32, _pvp],
     "hg_stop_model_free": [_vp],
@@ -51,7 +52,7 @@ SIGNATURES = {
     "hg_stop_model_get_params": [_vp, _vp, _sz],
     "hg_stop_model_set_norm": [_vp, _vp, _vp, _sz],
     "hg_stop_model_config": [_vp, _i32, ctypes.c_float, _i32],
-    "hg_gpu_collect_traces": [_vp, _vp, _sz, _sz, _sz, _i32, _i32, _vp, _vp, _vp, _vp],
+    "hg_gpu_collect_traces": [_vp, _vp, _sz, _sz, _sz, _i32, _i32, _vp, _vp, _vp, _vp, _vp],
     "hg_stop_model_forward": [_vp, _vp, _sz, _vp, _vp],
     "hg_stop_model_train_step": [_vp, _vp, _vp, _sz, ctypes.c_float, ctypes.c_float, ctypes.c_float,
                                  ctypes.c_float, ctypes.POINTER(ctypes.c_float), _vp, _vp],
@@ -62,6 +63,7 @@ SIGNATURES = {
     "hg_memcpy_h2d": [_vp, _vp, _sz, _vp],
     "hg_memcpy_d2h": [_vp, _vp, _sz, _vp],
     "hg_memset": [_vp, _i32, _sz, _vp],
+    "hg_copy_kernel": [_vp, _vp, _sz, _vp],
     "hg_stream_create": [_pvp],
     "hg_stream_destroy": [_vp],
     "hg_stream_synchronize": [_vp],

@@ -498,6 +498,156 @@ __global__ __launch_bounds__(256, 2) void lowp_gemm_kernel(LowpGemmArgs g) {
     }
 }
 
+// Larger tiles: each wave owns (32*TM) x (32*TN) outputs (TM x TN MFMA blocks), the workgroup
+// 2 x 2 waves.  One LDS stage (k-slice of 128 bytes per row for both operands); the next slice
+// is prefetched into registers during the MFMAs and stored after a barrier, so a workgroup
+// needs (64*TM + 64*TN) * 144 bytes of LDS and two of them fit on a CU.
+template <int PREC, bool FILTER, int TM, int TN>
+__global__ __launch_bounds__(256, 2) void lowp_gemm_wide_kernel(LowpGemmArgs g) {
+    constexpr int BM = 64 * TM, BN = 64 * TN;
+    constexpr int LA = BM * 8 / 256, LB = BN * 8 / 256;  // 16-byte chunks per thread
+    __shared__ __align__(16) uint8_t As[BM][GKB + GPAD];
+    __shared__ __align__(16) uint8_t Bs[BN][GKB + GPAD];
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    const uint32_t tiles_m = (g.nq + BM - 1) / BM, tiles_n = (g.n + BN - 1) / BN;
+    const uint32_t ntiles = tiles_m * tiles_n;
+    const uint32_t b = blockIdx.x;
+    const uint32_t per_xcd = (ntiles + 7) / 8;
+    uint32_t t = (b % 8) * per_xcd + b / 8;
+    if (t >= ntiles) return;
+    const uint32_t tm = t % tiles_m, tn = t / tiles_m;
+    const uint32_t m0 = tm * BM, n0 = tn * BN;
+    const size_t row_bytes = (size_t)g.dim * (PREC == 1 ? 2 : 1);
+
+    f32x16_t acc[TM][TN];
+#pragma unroll
+    for (int i = 0; i < TM; i++)
+#pragma unroll
+        for (int j = 0; j < TN; j++)
+#pragma unroll
+            for (int r = 0; r < 16; r++) acc[i][j][r] = 0.f;
+
+    uint4 ra[LA], rb[LB];
+    auto gload = [&](size_t kb) {
+#pragma unroll
+        for (int i = 0; i < LA; i++) {
+            int idx = tid + 256 * i;
+            int row = idx >> 3, ch = idx & 7;
+            uint32_t qa = m0 + row;
+            ra[i] = qa < g.nq ? *reinterpret_cast<const uint4*>(g.A + qa * row_bytes + kb + ch * 16)
+                              : make_uint4(0, 0, 0, 0);
+        }
+#pragma unroll
+        for (int i = 0; i < LB; i++) {
+            int idx = tid + 256 * i;
+            int row = idx >> 3, ch = idx & 7;
+            uint32_t qb = n0 + row;
+            rb[i] = qb < g.n ? *reinterpret_cast<const uint4*>(g.B + qb * row_bytes + kb + ch * 16)
+                             : make_uint4(0, 0, 0, 0);
+        }
+    };
+    auto lstore = [&]() {
+#pragma unroll
+        for (int i = 0; i < LA; i++) {
+            int idx = tid + 256 * i;
+            *reinterpret_cast<uint4*>(&As[idx >> 3][(idx & 7) * 16]) = ra[i];
+        }
+#pragma unroll
+        for (int i = 0; i < LB; i++) {
+            int idx = tid + 256 * i;
+            *reinterpret_cast<uint4*>(&Bs[idx >> 3][(idx & 7) * 16]) = rb[i];
+        }
+    };
+    const size_t kbytes = row_bytes;
+    gload(0);
+    lstore();
+    __syncthreads();
+    for (size_t kb = 0; kb < kbytes; kb += GKB) {
+        const bool more = kb + GKB < kbytes;
+        if (more) gload(kb + GKB);
+        if (PREC == 1) {
+#pragma unroll
+            for (int ks = 0; ks < GKB; ks += 32) {
+                bf16x8_t fa[TM], fb[TN];
+#pragma unroll
+                for (int i = 0; i < TM; i++)
+                    fa[i] = *reinterpret_cast<const bf16x8_t*>(&As[wm * 32 * TM + i * 32 + (lane & 31)][ks + (lane >> 5) * 16]);
+#pragma unroll
+                for (int j = 0; j < TN; j++)
+                    fb[j] = *reinterpret_cast<const bf16x8_t*>(&Bs[wn * 32 * TN + j * 32 + (lane & 31)][ks + (lane >> 5) * 16]);
+#pragma unroll
+                for (int i = 0; i < TM; i++)
+#pragma unroll
+                    for (int j = 0; j < TN; j++)
+                        acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[i], fb[j], acc[i][j], 0, 0, 0);
+            }
+        } else {
+#pragma unroll
+            for (int ks = 0; ks < GKB; ks += 64) {
+                i32x8_t fa[TM], fb[TN];
+#pragma unroll
+                for (int i = 0; i < TM; i++)
+                    fa[i] = *reinterpret_cast<const i32x8_t*>(&As[wm * 32 * TM + i * 32 + (lane & 31)][ks + (lane >> 5) * 32]);
+#pragma unroll
+                for (int j = 0; j < TN; j++)
+                    fb[j] = *reinterpret_cast<const i32x8_t*>(&Bs[wn * 32 * TN + j * 32 + (lane & 31)][ks + (lane >> 5) * 32]);
+#pragma unroll
+                for (int i = 0; i < TM; i++)
+#pragma unroll
+                    for (int j = 0; j < TN; j++)
+                        acc[i][j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(fa[i], fb[j], acc[i][j], 0, 0, 0,
+                                                                                    127, 0, 127);
+            }
+        }
+        if (more) {
+            __syncthreads();
+            lstore();
+            __syncthreads();
+        }
+    }
+
+#pragma unroll
+    for (int i = 0; i < TM; i++) {
+#pragma unroll
+        for (int j = 0; j < TN; j++) {
+            const uint32_t col = n0 + wn * 32 * TN + j * 32 + (lane & 31);
+            const bool col_ok = col < g.n;
+            const float bsc = (PREC == 2 && col_ok) ? g.b_scale[col] : 1.0f;
+            const float bn = (g.space == 0 && col_ok) ? g.b_norm[col] : 0.f;
+#pragma unroll
+            for (int r = 0; r < 16; r++) {
+                const uint32_t row = m0 + wm * 32 * TM + i * 32 + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3);
+                const bool ok = col_ok && row < g.nq;
+                float s = acc[i][j][r];
+                if (PREC == 2 && ok) s = s * g.a_scale[row] * bsc;
+                float d = g.space == 0 ? (ok ? g.a_norm[row] : 0.f) + bn - 2.0f * s : 1.0f - s;
+                if (!FILTER) {
+                    if (ok) g.out_mat[(size_t)row * g.ld_out + col] = d;
+                } else {
+                    bool pass = ok && d < g.tau[row < g.nq ? row : 0];
+                    uint64_t m = __ballot(pass);
+                    uint32_t half = lane >> 5;
+                    uint32_t hm = (uint32_t)(m >> (32 * half));
+                    if (hm) {
+                        uint32_t leader = __ffs(hm) - 1;
+                        uint32_t base = 0;
+                        if ((lane & 31) == leader) base = atomicAdd(&g.cand_cnt[row], (uint32_t)__popc(hm));
+                        base = __shfl(base, (int)(half * 32 + leader));
+                        if (pass) {
+                            uint32_t slot = base + __popc(hm & ((1u << (lane & 31)) - 1u));
+                            if (slot < g.cap) {
+                                g.cand_rows[(size_t)row * g.cap + slot] = col;
+                                g.cand_d[(size_t)row * g.cap + slot] = d;
+                            }
+                        }
+                    }
+                }
+            }
+        }
+    }
+}
+
 // Copy every `step`-th row of a low-precision matrix (and its scale / norm) to a compact sample.
 __global__ void gather_rows_kernel(const uint8_t* __restrict__ src, size_t row_bytes, uint32_t step, uint32_t s_rows,
                                    uint8_t* __restrict__ dst, const float* sc, float* dsc, const float* nm,
@@ -580,7 +730,7 @@ __global__ __launch_bounds__(256) void kth_kernel(const float* __restrict__ mat,
 // the k smallest (distance, label) closest first.  A query whose candidate count overflowed the
 // buffer is flagged for the exact path.
 constexpr int RR_THREADS = 256;
-constexpr int RR_CAP = 8192;
+constexpr int RR_CAP = 16384;
 
 template <int SPACE>
 __global__ __launch_bounds__(RR_THREADS) void rerank_kernel(const float* __restrict__ Q, const float* __restrict__ X,
@@ -678,6 +828,37 @@ hipError_t launch_bf_convert(const BfData& d, uint16_t* vec_bf16, uint8_t* vec_f
 
 static inline size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
 
+// GEMM variant: 0 = 128x128 workgroup tiles (2x2 blocks per wave, double-buffered LDS),
+// 1 = 256x128 workgroup tiles (4x2 blocks per wave, register-prefetched single LDS stage).
+static int g_gemm_variant = 1;
+void set_gemm_variant(int v) { g_gemm_variant = v; }
+
+static void launch_gemm(LowpGemmArgs ga, int precision, bool filter, hipStream_t s) {
+    if (g_gemm_variant == 0) {
+        ga.tiles_m = (ga.nq + GM - 1) / GM;
+        ga.tiles_n = (ga.n + GN - 1) / GN;
+        uint32_t nt = ga.tiles_m * ga.tiles_n, grid = (nt + 7) / 8 * 8;
+        if (precision == 1) {
+            if (filter) hipLaunchKernelGGL((lowp_gemm_kernel<1, true>), dim3(grid), dim3(256), 0, s, ga);
+            else hipLaunchKernelGGL((lowp_gemm_kernel<1, false>), dim3(grid), dim3(256), 0, s, ga);
+        } else {
+            if (filter) hipLaunchKernelGGL((lowp_gemm_kernel<2, true>), dim3(grid), dim3(256), 0, s, ga);
+            else hipLaunchKernelGGL((lowp_gemm_kernel<2, false>), dim3(grid), dim3(256), 0, s, ga);
+        }
+        return;
+    }
+    constexpr int TM = 4, TN = 2;
+    uint32_t tm = (ga.nq + 64 * TM - 1) / (64 * TM), tn = (ga.n + 64 * TN - 1) / (64 * TN);
+    uint32_t nt = tm * tn, grid = (nt + 7) / 8 * 8;
+    if (precision == 1) {
+        if (filter) hipLaunchKernelGGL((lowp_gemm_wide_kernel<1, true, TM, TN>), dim3(grid), dim3(256), 0, s, ga);
+        else hipLaunchKernelGGL((lowp_gemm_wide_kernel<1, false, TM, TN>), dim3(grid), dim3(256), 0, s, ga);
+    } else {
+        if (filter) hipLaunchKernelGGL((lowp_gemm_wide_kernel<2, true, TM, TN>), dim3(grid), dim3(256), 0, s, ga);
+        else hipLaunchKernelGGL((lowp_gemm_wide_kernel<2, false, TM, TN>), dim3(grid), dim3(256), 0, s, ga);
+    }
+}
+
 hipError_t launch_bf_lowp(const BfData& d, int precision, const float* q, uint32_t nq, uint32_t k, uint32_t c,
                           uint64_t* labels, float* dists, void* scratch, hipStream_t s, double* stats) {
     const uint32_t n = d.n, dim = d.dim;
@@ -733,14 +914,9 @@ hipError_t launch_bf_lowp(const BfData& d, int precision, const float* q, uint32
     ga.ld_out = srows;
     ga.tiles_m = (nq + GM - 1) / GM;
     ga.tiles_n = (srows + GN - 1) / GN;
-    uint32_t nt = ga.tiles_m * ga.tiles_n;
-    uint32_t grid = (nt + 7) / 8 * 8;
-    if (precision == 1)
-        hipLaunchKernelGGL((lowp_gemm_kernel<1, false>), dim3(grid), dim3(256), 0, s, ga);
-    else
-        hipLaunchKernelGGL((lowp_gemm_kernel<2, false>), dim3(grid), dim3(256), 0, s, ga);
-    // ~1.5 c candidates expected; never below k + 1 sample ranks
-    double want = 1.5 * (double)c * (double)srows / (double)n;
+    launch_gemm(ga, precision, false, s);
+    // ~1.2 c candidates expected (buffer holds 4c); never below k + 1 sample ranks
+    double want = 1.2 * (double)c * (double)srows / (double)n;
     uint32_t rank = (uint32_t)std::min<double>((double)srows, std::max<double>((double)k + 1, std::ceil(want)));
     hipLaunchKernelGGL(kth_kernel, dim3(nq), dim3(256), 0, s, smat, (size_t)srows, srows, rank, tau);
     (void)hipMemsetAsync(ccnt, 0, (size_t)nq * 4, s);
@@ -756,13 +932,7 @@ hipError_t launch_bf_lowp(const BfData& d, int precision, const float* q, uint32
     ga.cand_d = cd;
     ga.cand_cnt = ccnt;
     ga.cap = RR_CAP;
-    ga.tiles_n = (n + GN - 1) / GN;
-    nt = ga.tiles_m * ga.tiles_n;
-    grid = (nt + 7) / 8 * 8;
-    if (precision == 1)
-        hipLaunchKernelGGL((lowp_gemm_kernel<1, true>), dim3(grid), dim3(256), 0, s, ga);
-    else
-        hipLaunchKernelGGL((lowp_gemm_kernel<2, true>), dim3(grid), dim3(256), 0, s, ga);
+    launch_gemm(ga, precision, true, s);
     (void)hipEventRecord(ev[2], s);
     size_t lds = (size_t)RR_CAP * sizeof(SelEntry) + (size_t)dim * 4;
     if (d.space == 0) {

@@ -390,9 +390,10 @@ HG_API int hg_gpu_search_device(void* gv, const float* dq, size_t nq, const int3
 // Trace collection for the learned stop: the plain search, recording every `period`
 // expansions the 12 traversal features and the current top-k ids.
 HG_API int hg_gpu_collect_traces(void* gv, const float* dq, size_t nq, size_t ef, size_t k, int period,
-                                 int max_records, float* d_features, uint32_t* d_topk, int32_t* d_nrec, void* stream) {
+                                 int max_records, float* d_features, uint32_t* d_topk, int32_t* d_nrec,
+                                 uint32_t* d_final, void* stream) {
     return guarded([&] {
-        require(gv && dq && d_features && d_topk && d_nrec, "null argument");
+        require(gv && dq && d_features && d_topk && d_nrec && d_final, "null argument");
         require(period >= 1 && max_records >= 1, "period and max_records must be positive");
         auto* gi = static_cast<GpuIndex*>(gv);
         check_search_args(gi, nq, ef, k, k);
@@ -415,6 +416,7 @@ HG_API int hg_gpu_collect_traces(void* gv, const float* dq, size_t nq, size_t ef
         sp.trace_feat = d_features;
         sp.trace_topk = d_topk;
         sp.trace_nrec = d_nrec;
+        sp.trace_final = d_final;
         sp.trace_period = period;
         sp.trace_max_rec = max_records;
         hip_check(hg::launch_search(sp, pl.blocks, s), "trace launch");
@@ -577,6 +579,8 @@ HG_API int hg_memcpy_h2d(void* dst, const void* src, size_t bytes, void* stream)
 }
 HG_API int hg_memcpy_d2h(void* dst, const void* src, size_t bytes, void* stream) {
     return guarded([&] {
+        // results may still be produced by work on another (non-blocking) stream
+        hip_check(hipDeviceSynchronize(), "hipDeviceSynchronize");
         hip_check(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, (hipStream_t)stream), "hipMemcpy D2H");
         hip_check(hipStreamSynchronize((hipStream_t)stream), "hipStreamSynchronize");
         return HG_OK;
@@ -585,6 +589,13 @@ HG_API int hg_memcpy_d2h(void* dst, const void* src, size_t bytes, void* stream)
 HG_API int hg_memset(void* dst, int value, size_t bytes, void* stream) {
     return guarded([&] {
         hip_check(hipMemsetAsync(dst, value, bytes, (hipStream_t)stream), "hipMemsetAsync");
+        return HG_OK;
+    });
+}
+HG_API int hg_copy_kernel(void* dst, const void* src, size_t bytes, void* stream) {
+    return guarded([&] {
+        require(dst && src && bytes % 16 == 0, "copy needs device pointers and a multiple of 16 bytes");
+        hip_check(hg::launch_copy(dst, src, bytes, (hipStream_t)stream), "copy launch");
         return HG_OK;
     });
 }

@@ -200,7 +200,7 @@ HG_API int hg_bf_search_device(void* bv, const float* dq, size_t nq, size_t k, i
         }
         require(b->dim % 64 == 0, "the reduced-precision GEMM needs dim to be a multiple of 64");
         size_t c = std::max(rerank_c, k);
-        require(c <= 8192, "rerank_c above 8192 is not supported");
+        require(c <= 4096, "rerank_c above 4096 is not supported");
         c = std::min(c, b->count);
         ensure_lowp(b, precision, s);
         size_t sb = hg::bf_lowp_scratch_bytes((uint32_t)nq, (uint32_t)b->dim, (uint32_t)c, (uint32_t)b->count);
@@ -222,6 +222,14 @@ HG_API int hg_bf_fp64_check(void* bv, const float* dq, size_t nq, size_t k, cons
         hip_check(hg::launch_fp64_check(b->data(), dq, (uint32_t)nq, (uint32_t)k, dl, dd, rel_tol, out, b->cand.p,
                                         (hipStream_t)stream),
                   "fp64 check launch");
+        return HG_OK;
+    });
+}
+
+HG_API int hg_bf_gemm_variant(int variant) {
+    return guarded([&] {
+        require(variant == 0 || variant == 1, "GEMM variant must be 0 (128x128 tiles) or 1 (256x128 tiles)");
+        hg::set_gemm_variant(variant);
         return HG_OK;
     });
 }

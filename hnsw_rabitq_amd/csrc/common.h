@@ -60,6 +60,7 @@ struct SearchParams {
     float* trace_feat;          // [nq][max_rec][in_dim]
     uint32_t* trace_topk;       // [nq][max_rec][k]
     int32_t* trace_nrec;        // [nq]
+    uint32_t* trace_final;      // [nq][k] internal ids of the final top-k
     int32_t trace_period, trace_max_rec;
 };
 

@@ -400,7 +400,12 @@ __global__ __launch_bounds__(64) void hnsw_search_kernel(SearchParams P) {
                     if (m > 0) {
                         // first staged position in the merged list
                         int first_pos = __shfl(pos, __ffsll((unsigned long long)km) - 1);
-                        // move beam entries [first_pos, n) right, last chunk first
+                        // Move beam entries [first_pos, n) right, last chunk first.  Entry i
+                        // moves by the number of staged keys below it, i.e. of staged
+                        // insertion points pos_s <= i (distinct keys).  The insertion points are
+                        // read lane by lane from registers; reads of a chunk precede its writes
+                        // (in-order LDS within the wave) and a chunk only writes at or above its
+                        // own base, so chunks need no barrier between them.
                         int new_n = n + m < ef_eff ? n + m : ef_eff;
                         int hi_chunk = (n - 1) & ~63;
                         int lo_chunk = first_pos & ~63;
@@ -409,18 +414,25 @@ __global__ __launch_bounds__(64) void hnsw_search_kernel(SearchParams P) {
                             bool mv = i >= first_pos && i < n;
                             float vd = 0.f;
                             uint32_t vi = 0;
-                            int np = 0;
                             if (mv) {
                                 vd = bd[i];
                                 vi = bi[i];
-                                np = i + stage_lower_bound(sd, si, m, vd, vi & ID_MASK);
                             }
-                            __syncthreads();
+                            int sh = 0;
+                            uint64_t t = km;
+                            while (t) {
+                                int L = __ffsll((unsigned long long)t) - 1;
+                                t &= t - 1;
+                                int pj = __builtin_amdgcn_readlane(pos, L);
+                                sh += pj <= i ? 1 : 0;
+                            }
+                            int np = i + sh;
+                            __asm__ volatile("" ::: "memory");
                             if (mv && np < ef_eff) {
                                 bd[np] = vd;
                                 bi[np] = vi;
                             }
-                            __syncthreads();
+                            __asm__ volatile("" ::: "memory");
                         }
                         int fp = ok ? pos + j : 0x7FFFFFFF;
                         if (ok && fp < ef_eff) {
@@ -513,9 +525,31 @@ __global__ __launch_bounds__(64) void hnsw_search_kernel(SearchParams P) {
             c[2] = Hu;
             c[3] = Eu;
         }
-        if (TRACE && lane == 0) P.trace_nrec[qi] = nrec;
+        if (TRACE) {
+            if (lane == 0) P.trace_nrec[qi] = nrec;
+            uint32_t* tf = P.trace_final + (size_t)qi * P.k_stride;
+            for (int i = lane; i < kq; i += 64) tf[i] = i < n ? (bi[i] & ID_MASK) : SENT;
+        }
         __syncthreads();
     }
+}
+
+// Plain streaming copy (16 bytes per lane per step): the reference kernel of known traffic
+// used to calibrate the FETCH_SIZE / WRITE_SIZE hardware counters.
+__global__ __launch_bounds__(256) void copy_kernel(uint4* __restrict__ dst, const uint4* __restrict__ src, size_t n16) {
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n16; i += stride) dst[i] = src[i];
+}
+
+hipError_t launch_copy(void* dst, const void* src, size_t bytes, hipStream_t s) {
+    if (bytes % 16 != 0) return hipErrorInvalidValue;
+    size_t n16 = bytes / 16;
+    size_t blocks = (n16 + 255) / 256;
+    if (blocks > 8192) blocks = 8192;
+    if (blocks == 0) return hipSuccess;
+    hipLaunchKernelGGL(copy_kernel, dim3((uint32_t)blocks), dim3(256), 0, s, (uint4*)dst, (const uint4*)src, n16);
+    return hipGetLastError();
 }
 
 __global__ void zero_u32_kernel(uint32_t* p, uint32_t n) {

@@ -103,8 +103,8 @@ int hg_bf_count(void* bf, size_t* count);
 int hg_bf_free(void* bf);
 /* Exact FP32 BFIndex.knn_query on host buffers (closest first, ties by label). */
 int hg_bf_search(void* bf, const float* queries, size_t nq, size_t k, uint64_t* labels, float* dists);
-/* Device-resident search.  precision = HG_PREC_*; rerank_c = candidates kept per query by the
- * reduced-precision pass (ignored for FP32).  stats (optional, host, 4 doubles): GEMM ms,
+/* Device-resident search.  precision = HG_PREC_*; rerank_c (<= 4096) = candidates kept per
+ * query by the reduced-precision pass and re-ranked in exact FP32 (ignored for FP32).  stats (optional, host, 4 doubles): GEMM ms,
  * select ms, re-rank ms, queries whose candidate pass had to be retried. */
 int hg_bf_search_device(void* bf, const float* d_queries, size_t nq, size_t k, int precision, size_t rerank_c,
                         uint64_t* d_labels, float* d_dists, double* stats, void* stream);
@@ -114,6 +114,11 @@ int hg_bf_search_device(void* bf, const float* d_queries, size_t nq, size_t k, i
  * out[2] = max |d32 - d64| over all returned results, out[3] = max relative error. */
 int hg_bf_fp64_check(void* bf, const float* d_queries, size_t nq, size_t k, const uint64_t* d_labels,
                      const float* d_dists, double rel_tol, double* out, void* stream);
+
+/* Tile configuration of the reduced-precision distance GEMM (process-wide): 0 = 128x128
+ * workgroup tiles with 2x2 MFMA blocks per wave and double-buffered LDS; 1 (default) =
+ * 256x128 tiles with 4x2 blocks per wave and a register-prefetched LDS stage. */
+int hg_bf_gemm_variant(int variant);
 
 /* ---- Learned early termination (stop model) ---- */
 /* Create a stop model for a given feature count / hidden width (weights zero). */
@@ -129,11 +134,12 @@ int hg_stop_model_set_norm(void* model, const float* mean, const float* inv_std,
  * when sigmoid(logit) >= threshold, and after `max_ext` extensions of ef... (see DESIGN). */
 int hg_stop_model_config(void* model, int period, float threshold, int min_hops);
 /* Trace collection: run the plain search and write, per query, up to max_records feature
- * vectors (every `period` expansions) plus the current top-k ids at each record.
- * d_features [nq][max_records][in_dim] f32, d_topk [nq][max_records][k] u32,
- * d_nrec [nq] int32.  */
+ * vectors (every `period` level-0 expansions) plus the current top-k internal ids at each
+ * record, and the final top-k internal ids.  d_features [nq][max_records][12] f32,
+ * d_topk [nq][max_records][k] u32, d_nrec [nq] int32, d_final [nq][k] u32. */
 int hg_gpu_collect_traces(void* gpu_index, const float* d_queries, size_t nq, size_t ef, size_t k, int period,
-                          int max_records, float* d_features, uint32_t* d_topk, int32_t* d_nrec, void* stream);
+                          int max_records, float* d_features, uint32_t* d_topk, int32_t* d_nrec, uint32_t* d_final,
+                          void* stream);
 /* Forward pass (logits) on device features [n][in_dim]. */
 int hg_stop_model_forward(void* model, const float* d_x, size_t n, float* d_logits, void* stream);
 /* One Adam step on a minibatch (mean BCE-with-logits); returns the loss through *loss
@@ -147,8 +153,12 @@ int hg_set_device(int device);
 int hg_malloc(void** ptr, size_t bytes);
 int hg_free(void* ptr);
 int hg_memcpy_h2d(void* dst, const void* src, size_t bytes, void* stream);
+/* Waits for all work on the device before copying (results may come from any stream). */
 int hg_memcpy_d2h(void* dst, const void* src, size_t bytes, void* stream);
 int hg_memset(void* dst, int value, size_t bytes, void* stream);
+/* Device-to-device copy by a plain streaming kernel (bytes % 16 == 0); also the calibration
+ * kernel of known traffic for the FETCH_SIZE / WRITE_SIZE counters. */
+int hg_copy_kernel(void* dst, const void* src, size_t bytes, void* stream);
 int hg_stream_create(void** stream);
 int hg_stream_destroy(void* stream);
 int hg_stream_synchronize(void* stream);

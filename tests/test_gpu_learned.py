@@ -110,8 +110,10 @@ def test_traces_record_features_and_topk(hg):
     gpu = p.gpu()
     nq, k, ef = 200, 10, 50
     dq = DeviceArray.from_numpy(queries[:nq])
-    f, t, r = learned_stop.collect_traces(gpu, dq, nq, ef, k, period=4, max_records=64)
+    f, t, r, fin = learned_stop.collect_traces(gpu, dq, nq, ef, k, period=4, max_records=64)
     lab, dis, ctr, _ = gpu.search(queries[:nq], k, ef, counters=True)
+    # the final ids are the search's own result (labels == ids here), as sets
+    np.testing.assert_array_equal(np.sort(fin.astype(np.uint64), 1), np.sort(lab, 1))
     # one record every 4 expansions
     np.testing.assert_array_equal(r, np.minimum(ctr[:, 0] // 4, 64))
     assert np.isfinite(f[np.arange(64)[None, :] < r[:, None]]).all()
