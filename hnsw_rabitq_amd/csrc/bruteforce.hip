@@ -327,6 +327,34 @@ __global__ __launch_bounds__(256) void convert_rows_kernel(const float* __restri
     }
 }
 
+// Tile order shared by the GEMM kernels.  Workgroup b runs on XCD b % 8.  The query tiles are
+// split into 8 groups, one per XCD, so a group's query rows stay in that XCD's L2; inside a
+// group the row tile varies slowest and every XCD walks the row tiles in the same order, so a
+// row tile is read from HBM about once and shared through the memory-side cache.
+struct TileOrder {
+    uint32_t tiles_m, tiles_n, G, groups;
+    __host__ __device__ static TileOrder make(uint32_t tm, uint32_t tn) {
+        TileOrder o;
+        o.tiles_m = tm;
+        o.tiles_n = tn;
+        o.G = (tm + 7) / 8;
+        o.groups = (tm + o.G - 1) / o.G;
+        return o;
+    }
+    __host__ __device__ uint32_t grid() const { return 8u * ((groups + 7) / 8) * G * tiles_n; }
+    // false when this workgroup has no tile
+    __device__ bool map(uint32_t b, uint32_t& tm, uint32_t& tn) const {
+        const uint32_t x = b % 8, local = b / 8;
+        const uint32_t per_group = G * tiles_n;
+        const uint32_t g = x + 8 * (local / per_group);
+        const uint32_t rem = local % per_group;
+        if (g >= groups) return false;
+        tn = rem / G;
+        tm = g * G + rem % G;
+        return tm < tiles_m;
+    }
+};
+
 // GEMM tile: 128 queries x 128 rows per workgroup of 4 waves (2 x 2 waves of 64 x 64), each
 // wave 2 x 2 MFMA 32x32 blocks.  LDS holds 128-byte k-slices of both operands (64 bf16 or 128
 // fp8 per row), double buffered.  MFMA operand layout (gfx950): lane l supplies row l % 32 and
@@ -361,15 +389,8 @@ __global__ __launch_bounds__(256, 2) void lowp_gemm_kernel(LowpGemmArgs g) {
     __shared__ __align__(16) uint8_t Bs[2][GN][GKB + GPAD];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave >> 1, wn = wave & 1;
-    // XCD-aware tile order: consecutive workgroups land on different XCDs, so each XCD takes a
-    // contiguous range of tiles; inside it the query tile varies fastest so the row tile is
-    // shared through that XCD's L2.
-    const uint32_t ntiles = g.tiles_m * g.tiles_n;
-    const uint32_t b = blockIdx.x;
-    const uint32_t per_xcd = (ntiles + 7) / 8;
-    uint32_t t = (b % 8) * per_xcd + b / 8;
-    if (t >= ntiles) return;
-    const uint32_t tm = t % g.tiles_m, tn = t / g.tiles_m;
+    uint32_t tm, tn;
+    if (!TileOrder::make(g.tiles_m, g.tiles_n).map(blockIdx.x, tm, tn)) return;
     const uint32_t m0 = tm * GM, n0 = tn * GN;
     const size_t row_bytes = (size_t)g.dim * (PREC == 1 ? 2 : 1);
 
@@ -510,13 +531,8 @@ __global__ __launch_bounds__(256, 2) void lowp_gemm_wide_kernel(LowpGemmArgs g) 
     __shared__ __align__(16) uint8_t Bs[BN][GKB + GPAD];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int wm = wave >> 1, wn = wave & 1;
-    const uint32_t tiles_m = (g.nq + BM - 1) / BM, tiles_n = (g.n + BN - 1) / BN;
-    const uint32_t ntiles = tiles_m * tiles_n;
-    const uint32_t b = blockIdx.x;
-    const uint32_t per_xcd = (ntiles + 7) / 8;
-    uint32_t t = (b % 8) * per_xcd + b / 8;
-    if (t >= ntiles) return;
-    const uint32_t tm = t % tiles_m, tn = t / tiles_m;
+    uint32_t tm, tn;
+    if (!TileOrder::make((g.nq + BM - 1) / BM, (g.n + BN - 1) / BN).map(blockIdx.x, tm, tn)) return;
     const uint32_t m0 = tm * BM, n0 = tn * BN;
     const size_t row_bytes = (size_t)g.dim * (PREC == 1 ? 2 : 1);
 
@@ -837,7 +853,7 @@ static void launch_gemm(LowpGemmArgs ga, int precision, bool filter, hipStream_t
     if (g_gemm_variant == 0) {
         ga.tiles_m = (ga.nq + GM - 1) / GM;
         ga.tiles_n = (ga.n + GN - 1) / GN;
-        uint32_t nt = ga.tiles_m * ga.tiles_n, grid = (nt + 7) / 8 * 8;
+        uint32_t grid = TileOrder::make(ga.tiles_m, ga.tiles_n).grid();
         if (precision == 1) {
             if (filter) hipLaunchKernelGGL((lowp_gemm_kernel<1, true>), dim3(grid), dim3(256), 0, s, ga);
             else hipLaunchKernelGGL((lowp_gemm_kernel<1, false>), dim3(grid), dim3(256), 0, s, ga);
@@ -849,7 +865,7 @@ static void launch_gemm(LowpGemmArgs ga, int precision, bool filter, hipStream_t
     }
     constexpr int TM = 4, TN = 2;
     uint32_t tm = (ga.nq + 64 * TM - 1) / (64 * TM), tn = (ga.n + 64 * TN - 1) / (64 * TN);
-    uint32_t nt = tm * tn, grid = (nt + 7) / 8 * 8;
+    uint32_t grid = TileOrder::make(tm, tn).grid();
     if (precision == 1) {
         if (filter) hipLaunchKernelGGL((lowp_gemm_wide_kernel<1, true, TM, TN>), dim3(grid), dim3(256), 0, s, ga);
         else hipLaunchKernelGGL((lowp_gemm_wide_kernel<1, false, TM, TN>), dim3(grid), dim3(256), 0, s, ga);

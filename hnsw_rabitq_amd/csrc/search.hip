@@ -16,8 +16,7 @@
 //            may be evaluated twice.  That cannot change results: a re-evaluated node is either
 //            still in the beam (dropped by the duplicate check of the merge) or was rejected or
 //            evicted under a lowerBound that has only decreased since.
-//   stage  : the neighbours of the expanded node that were not visited, then their distances,
-//            then the sorted batch that is merged into the beam.
+//   stage  : the neighbours of the expanded node that were not visited and their distances.
 // A batch of neighbours is merged set-wise: under a total order, processing the neighbours
 // one by one with the admission test `size < ef || d < lowerBound` and evicting the maximum
 // leaves exactly the ef smallest keys of (beam U batch) -- what the merge computes.
@@ -88,27 +87,6 @@ __device__ __forceinline__ bool hash_insert(uint32_t* hash, uint32_t mask, uint3
     return true;
 }
 
-// Bitonic sort of one (d, id) pair per lane across the 64 lanes, ascending by (d, id).
-__device__ __forceinline__ void wave_sort64(float& d, uint32_t& id, int lane) {
-#pragma unroll
-    for (int k = 2; k <= 64; k <<= 1) {
-#pragma unroll
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            float od = __shfl_xor(d, j);
-            uint32_t oi = __shfl_xor(id, j);
-            bool up = (lane & k) == 0;
-            bool lower = (lane & j) == 0;
-            bool o_less = key_less(od, oi, d, id);
-            // the lower lane keeps the smaller key in ascending blocks
-            bool take = (lower == up) ? o_less : !o_less && !(od == d && oi == id);
-            if (take) {
-                d = od;
-                id = oi;
-            }
-        }
-    }
-}
-
 // Number of beam entries with key < (d, id), over beam[0, n).
 __device__ __forceinline__ int beam_lower_bound(const float* bd, const uint32_t* bi, int n, float d, uint32_t id) {
     int lo = 0, hi = n;
@@ -122,37 +100,28 @@ __device__ __forceinline__ int beam_lower_bound(const float* bd, const uint32_t*
     return lo;
 }
 
-// Number of staged entries with key < (d, id), over s[0, m).
-__device__ __forceinline__ int stage_lower_bound(const float* sd, const uint32_t* si, int m, float d, uint32_t id) {
-    int lo = 0, hi = m;
-    while (lo < hi) {
-        int mid = (lo + hi) >> 1;
-        if (key_less(sd[mid], si[mid], d, id))
-            lo = mid + 1;
-        else
-            hi = mid;
-    }
-    return lo;
-}
-
 // ------------------------------------------------------------------------------------------
 // Learned stop: 12 traversal features and an in-wave MLP evaluation.
 // ------------------------------------------------------------------------------------------
 struct StopState {
     float qnorm, d_entry;
     int improved_recent;  // bitmask of the last 16 expansions that changed the top-k
+    int last_change;      // expansion count at the last change of the top-k
 };
 
 __device__ __forceinline__ float safe_div(float a, float b) { return a / (fabsf(b) + 1e-20f); }
 
 // Features (same order as the trace collector and the host-side training code):
-//  0 query norm            1 entry distance at level 0     2 d_1 / d_entry
-//  3 d_k / d_entry         4 d_last / d_entry              5 d_k / d_1
-//  6 d_last / d_k          7 hops / ef_eff                 8 log(1 + hops)
+//  0 query norm                      1 entry distance at level 0
+//  2 d_1 / d_entry                   3 d_k / d_entry
+//  4 d_last / d_entry                5 d_k / d_1
+//  6 d_next / d_k (next expansion)   7 hops / ef_eff
+//  8 hops since the top-k last changed / (hops + 1)
 //  9 fraction of the last 16 expansions that changed the top-k
-// 10 distance evaluations per hop                          11 next candidate distance / d_k
+// 10 distance evaluations per hop    11 log(1 + hops since the top-k last changed)
 __device__ __forceinline__ float stop_feature(int f, const StopState& st, float d1, float dk, float dlast,
                                               float dnext, int hops, int evals, int ef_eff) {
+    const int since = hops - st.last_change;
     switch (f) {
         case 0: return st.qnorm;
         case 1: return st.d_entry;
@@ -160,12 +129,12 @@ __device__ __forceinline__ float stop_feature(int f, const StopState& st, float 
         case 3: return safe_div(dk, st.d_entry);
         case 4: return safe_div(dlast, st.d_entry);
         case 5: return safe_div(dk, d1);
-        case 6: return safe_div(dlast, dk);
+        case 6: return safe_div(dnext, dk);
         case 7: return (float)hops / (float)ef_eff;
-        case 8: return logf(1.0f + (float)hops);
+        case 8: return (float)since / (float)(hops + 1);
         case 9: return (float)__popc((uint32_t)st.improved_recent & 0xFFFFu) / 16.0f;
         case 10: return (float)evals / (float)(hops + 1);
-        default: return safe_div(dnext, dk);
+        default: return logf(1.0f + (float)since);
     }
 }
 
@@ -324,6 +293,7 @@ __global__ __launch_bounds__(64) void hnsw_search_kernel(SearchParams P) {
         StopState st;
         st.d_entry = curdist;
         st.improved_recent = 0;
+        st.last_change = 0;
         if (STOP || TRACE) {
             float s = 0.f;
             for (uint32_t i = lane; i < g.dim; i += 64) s = fma_(qs[i], qs[i], s);
@@ -380,77 +350,78 @@ __global__ __launch_bounds__(64) void hnsw_search_kernel(SearchParams P) {
                 }
                 uint64_t live = ballot(id != SENT);
                 if (live) {
-                    wave_sort64(d, id, lane);
-                    // drop nodes already in the beam (re-evaluated after a hash overwrite)
+                    // Drop nodes already in the beam (re-evaluated after a hash overwrite).  pos =
+                    // number of beam keys below this key: its insertion point.
                     int pos = 0;
                     bool ok = id != SENT;
                     if (ok) {
                         pos = beam_lower_bound(bd, bi, n, d, id);
                         if (pos < n && bd[pos] == d && (bi[pos] & ID_MASK) == id) ok = false;
                     }
-                    uint64_t km = ballot(ok);
-                    int m = __popcll(km);
-                    __syncthreads();
-                    int j = lanes_below(km, lane);
-                    if (ok) {
-                        sd[j] = d;
-                        si[j] = id;
-                    }
-                    __syncthreads();
+                    const uint64_t km = ballot(ok);
+                    const int m = __popcll(km);
                     if (m > 0) {
-                        // first staged position in the merged list
-                        int first_pos = __shfl(pos, __ffsll((unsigned long long)km) - 1);
-                        // Move beam entries [first_pos, n) right, last chunk first.  Entry i
-                        // moves by the number of staged keys below it, i.e. of staged
-                        // insertion points pos_s <= i (distinct keys).  The insertion points are
-                        // read lane by lane from registers; reads of a chunk precede its writes
-                        // (in-order LDS within the wave) and a chunk only writes at or above its
+                        // Rank of each staged key among the staged keys (keys are distinct):
+                        // a scalar loop over the staged lanes, no cross-lane sort network.
+                        int r = 0;
+                        for (uint64_t t = km; t; t &= t - 1) {
+                            const int L = __ffsll((unsigned long long)t) - 1;
+                            const float dl = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(d), L));
+                            const uint32_t il = (uint32_t)__builtin_amdgcn_readlane((int)id, L);
+                            r += key_less(dl, il, d, id) ? 1 : 0;
+                        }
+                        // insertion points in rank order: lane r holds the point of rank r (the
+                        // other lanes park their value in lanes m..63)
+                        const int dst = ok ? r : m + lanes_below(~km, lane);
+                        const int spos = __builtin_amdgcn_ds_permute(dst * 4, pos);
+                        const int first_pos = __builtin_amdgcn_readlane(spos, 0);
+                        // Move beam entries [first_pos, n) right, last chunk first.  Entry i moves
+                        // by the number of staged keys below it = staged insertion points <= i.
+                        // Points of rank < lo are below the chunk base b; ranks [lo, hi) fall
+                        // inside the chunk (usually none).  A chunk's reads precede its writes
+                        // (in-order LDS within the wave) and a chunk writes only at or above its
                         // own base, so chunks need no barrier between them.
-                        int new_n = n + m < ef_eff ? n + m : ef_eff;
-                        int hi_chunk = (n - 1) & ~63;
-                        int lo_chunk = first_pos & ~63;
+                        const int new_n = n + m < ef_eff ? n + m : ef_eff;
+                        const int hi_chunk = (n - 1) & ~63;
+                        const int lo_chunk = first_pos & ~63;
+                        int hi = m;
                         for (int b = hi_chunk; b >= lo_chunk; b -= 64) {
-                            int i = b + lane;
-                            bool mv = i >= first_pos && i < n;
+                            int lo = hi;
+                            while (lo > 0 && __builtin_amdgcn_readlane(spos, lo - 1) >= b) lo--;
+                            const int i = b + lane;
+                            const bool mv = i >= first_pos && i < n;
                             float vd = 0.f;
                             uint32_t vi = 0;
                             if (mv) {
                                 vd = bd[i];
                                 vi = bi[i];
                             }
-                            int sh = 0;
-                            uint64_t t = km;
-                            while (t) {
-                                int L = __ffsll((unsigned long long)t) - 1;
-                                t &= t - 1;
-                                int pj = __builtin_amdgcn_readlane(pos, L);
-                                sh += pj <= i ? 1 : 0;
-                            }
-                            int np = i + sh;
+                            int sh = lo;
+                            for (int rr = lo; rr < hi; rr++) sh += __builtin_amdgcn_readlane(spos, rr) <= i ? 1 : 0;
+                            const int np = i + sh;
                             __asm__ volatile("" ::: "memory");
                             if (mv && np < ef_eff) {
                                 bd[np] = vd;
                                 bi[np] = vi;
                             }
                             __asm__ volatile("" ::: "memory");
+                            hi = lo;
                         }
-                        int fp = ok ? pos + j : 0x7FFFFFFF;
+                        const int fp = pos + r;
                         if (ok && fp < ef_eff) {
                             bd[fp] = d;
                             bi[fp] = id;
                         }
-                        int min_fp = fp;
-#pragma unroll
-                        for (int o = 32; o > 0; o >>= 1) {
-                            int t = __shfl_xor(min_fp, o);
-                            min_fp = t < min_fp ? t : min_fp;
-                        }
+                        const int min_fp = first_pos;
                         // Every position below min(first insert, p) holds an old entry whose old
                         // index was below p, hence already expanded.
                         scan_lo = min_fp < p ? min_fp : p;
                         if (STOP || TRACE) {
                             int kk = kq < new_n ? kq : new_n;
-                            if (min_fp < kk) st.improved_recent |= 1;
+                            if (min_fp < kk) {
+                                st.improved_recent |= 1;
+                                st.last_change = H0;
+                            }
                         }
                         n = new_n;
                         __syncthreads();
@@ -469,7 +440,17 @@ __global__ __launch_bounds__(64) void hnsw_search_kernel(SearchParams P) {
                     int kk = kq < n ? kq : n;
                     float d1 = bd[0], dk = bd[kk - 1], dlast = bd[n - 1];
                     // next candidate: first unexpanded from scan_lo (approximate: scan_lo entry)
-                    float dnext = scan_lo < n ? bd[scan_lo] : dlast;
+                    // the next expansion: first unexpanded entry (the search's stop rule ends
+                    // the loop when there is none)
+                    float dnext = dlast;
+                    for (int b = scan_lo; b < n; b += 64) {
+                        int i = b + lane;
+                        uint64_t m = ballot(i < n && (bi[i] & XFLAG) == 0);
+                        if (m) {
+                            dnext = bd[b + __ffsll((unsigned long long)m) - 1];
+                            break;
+                        }
+                    }
                     if (lane < kStopFeatures)
                         feat[lane] = stop_feature(lane, st, d1, dk, dlast, dnext, H0, E0, ef_eff);
                     __syncthreads();

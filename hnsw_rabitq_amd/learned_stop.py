@@ -177,9 +177,10 @@ def run(gpu, H, lib, data_dim, queries, dq, gt, stream, timer, sweep, log, steps
     dc = DeviceArray(nq * 16)
     best = None
     table = []
-    efs = sorted(set([ef_l, min(int(ef_l * 1.5), ef_max)]))
+    efs = sorted(set([ef_l, min(int(ef_l * 1.25), ef_max)]))
     for ef in efs:
-        for thr in (threshold, 0.8, 0.95):
+        gpu.autotune(dq, nq, ef, k, stream=stream)
+        for thr in (threshold, 0.5, 0.7, 0.8, 0.95):
             model.config(period=period, threshold=thr, min_hops=period)
             gpu.search_device(dq, nq, k, ef, dl, dd, d_counters=dc, stop_model=model.h, stream=stream)
             hops = float(dc.to_numpy(np.int32, (nq, 4))[:, 0].mean())
