@@ -664,6 +664,201 @@ __global__ __launch_bounds__(256, 2) void lowp_gemm_wide_kernel(LowpGemmArgs g) 
     }
 }
 
+// Direct-to-LDS pipeline: rows are fetched with global_load_lds_dwordx4 (64 lanes x 16 bytes
+// = 1 KiB of LDS per wave instruction, no register staging) into a ring of three 64-byte
+// k-slices, two slices ahead of the MFMAs.  Inside a 64-byte row the four 16-byte chunks are
+// XOR-swizzled by (row >> 1) & 3 so that the 32 rows read by one MFMA operand spread over all
+// LDS banks.  Workgroup tile 256 x 128 (4 waves of 4 x 2 MFMA 32x32 blocks).
+constexpr int LKB = 64, LNST = 3;
+
+__device__ __forceinline__ int swz(int row, int chunk) { return row * LKB + ((chunk ^ ((row >> 1) & 3)) << 4); }
+
+// Three slices per operand as distinct LDS objects, so the compiler's wait-count pass can tell
+// which in-flight direct-to-LDS loads an LDS read depends on (only the slice being read).
+template <int TM, int TN>
+struct LdsRing {
+    __attribute__((address_space(3))) uint8_t* a[LNST];
+    __attribute__((address_space(3))) uint8_t* b[LNST];
+};
+
+template <int PREC, int TM, int TN>
+__device__ __forceinline__ void gemm_lds_issue(const LowpGemmArgs& g, __attribute__((address_space(3))) uint8_t* dA,
+                                               __attribute__((address_space(3))) uint8_t* dB, int st, uint32_t m0,
+                                               uint32_t n0, size_t row_bytes, int wave, int lane) {
+    const int lrow = lane >> 2, lslot = lane & 3;
+    const size_t kb = (size_t)st * LKB;
+#pragma unroll
+    for (int i = 0; i < TM; i++) {  // A: 4*TM blocks of 16 rows, TM per wave
+        const int blk = wave * TM + i;
+        const int row = blk * 16 + lrow;
+        uint32_t q = m0 + row;
+        q = q < g.nq ? q : g.nq - 1;
+        const int c = lslot ^ ((row >> 1) & 3);
+        __builtin_amdgcn_global_load_lds((const void*)(g.A + (size_t)q * row_bytes + kb + c * 16), dA + blk * 1024, 16,
+                                         0, 0);
+    }
+#pragma unroll
+    for (int i = 0; i < TN; i++) {  // B: 4*TN blocks of 16 rows, TN per wave
+        const int blk = wave * TN + i;
+        const int row = blk * 16 + lrow;
+        uint32_t r = n0 + row;
+        r = r < g.n ? r : g.n - 1;
+        const int c = lslot ^ ((row >> 1) & 3);
+        __builtin_amdgcn_global_load_lds((const void*)(g.B + (size_t)r * row_bytes + kb + c * 16), dB + blk * 1024, 16,
+                                         0, 0);
+    }
+}
+
+typedef int i32x4_t __attribute__((ext_vector_type(4)));
+
+// Six 16-byte LDS reads completed inside one asm block (reads and their wait together, outputs
+// early-clobbered), so the compiler's wait-count pass does not make the read wait for the
+// direct-to-LDS loads still in flight for later slices; the caller orders the slice itself
+// (vmcnt wait + barrier).
+__device__ __forceinline__ void lds_read6(uint32_t a0, uint32_t a1, uint32_t a2, uint32_t a3, uint32_t a4, uint32_t a5,
+                                          i32x4_t& r0, i32x4_t& r1, i32x4_t& r2, i32x4_t& r3, i32x4_t& r4,
+                                          i32x4_t& r5) {
+    asm volatile(
+        "ds_read_b128 %0, %6\n\t"
+        "ds_read_b128 %1, %7\n\t"
+        "ds_read_b128 %2, %8\n\t"
+        "ds_read_b128 %3, %9\n\t"
+        "ds_read_b128 %4, %10\n\t"
+        "ds_read_b128 %5, %11\n\t"
+        "s_waitcnt lgkmcnt(0)"
+        : "=&v"(r0), "=&v"(r1), "=&v"(r2), "=&v"(r3), "=&v"(r4), "=&v"(r5)
+        : "v"(a0), "v"(a1), "v"(a2), "v"(a3), "v"(a4), "v"(a5)
+        : "memory");
+}
+
+template <int PREC, int TM, int TN>
+__device__ __forceinline__ void gemm_lds_mfma(uint32_t A, uint32_t B, f32x16_t (&acc)[TM][TN], int wm, int wn,
+                                              int lane) {
+    static_assert(TM == 4 && TN == 2, "fragment reads are written for 4 x 2 blocks per wave");
+    if (PREC == 1) {
+#pragma unroll
+        for (int ks = 0; ks < 2; ks++) {  // two 32x32x16 steps per 64-byte slice
+            const int c = ks * 2 + (lane >> 5);
+            const int ra = wm * 128 + (lane & 31), rb = wn * 64 + (lane & 31);
+            i32x4_t f0, f1, f2, f3, g0, g1;
+            lds_read6(A + swz(ra, c), A + swz(ra + 32, c), A + swz(ra + 64, c), A + swz(ra + 96, c), B + swz(rb, c),
+                      B + swz(rb + 32, c), f0, f1, f2, f3, g0, g1);
+            const bf16x8_t fa[4] = {__builtin_bit_cast(bf16x8_t, f0), __builtin_bit_cast(bf16x8_t, f1),
+                                    __builtin_bit_cast(bf16x8_t, f2), __builtin_bit_cast(bf16x8_t, f3)};
+            const bf16x8_t fb[2] = {__builtin_bit_cast(bf16x8_t, g0), __builtin_bit_cast(bf16x8_t, g1)};
+#pragma unroll
+            for (int i = 0; i < TM; i++)
+#pragma unroll
+                for (int j = 0; j < TN; j++)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(fa[i], fb[j], acc[i][j], 0, 0, 0);
+        }
+    } else {
+        const int c = (lane >> 5) * 2;
+        const int ra = wm * 128 + (lane & 31), rb = wn * 64 + (lane & 31);
+        i32x4_t l0, l1, l2, l3, m0, m1, h0, h1, h2, h3, n0, n1;
+        lds_read6(A + swz(ra, c), A + swz(ra + 32, c), A + swz(ra + 64, c), A + swz(ra + 96, c), B + swz(rb, c),
+                  B + swz(rb + 32, c), l0, l1, l2, l3, m0, m1);
+        lds_read6(A + swz(ra, c + 1), A + swz(ra + 32, c + 1), A + swz(ra + 64, c + 1), A + swz(ra + 96, c + 1),
+                  B + swz(rb, c + 1), B + swz(rb + 32, c + 1), h0, h1, h2, h3, n0, n1);
+        const i32x8_t fa[4] = {i32x8_t{l0.x, l0.y, l0.z, l0.w, h0.x, h0.y, h0.z, h0.w},
+                               i32x8_t{l1.x, l1.y, l1.z, l1.w, h1.x, h1.y, h1.z, h1.w},
+                               i32x8_t{l2.x, l2.y, l2.z, l2.w, h2.x, h2.y, h2.z, h2.w},
+                               i32x8_t{l3.x, l3.y, l3.z, l3.w, h3.x, h3.y, h3.z, h3.w}};
+        const i32x8_t fb[2] = {i32x8_t{m0.x, m0.y, m0.z, m0.w, n0.x, n0.y, n0.z, n0.w},
+                               i32x8_t{m1.x, m1.y, m1.z, m1.w, n1.x, n1.y, n1.z, n1.w}};
+#pragma unroll
+        for (int i = 0; i < TM; i++)
+#pragma unroll
+            for (int j = 0; j < TN; j++)
+                acc[i][j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(fa[i], fb[j], acc[i][j], 0, 0, 0, 127, 0,
+                                                                            127);
+    }
+}
+
+template <int PREC, bool FILTER>
+__global__ __launch_bounds__(256, 2) void lowp_gemm_lds_kernel(LowpGemmArgs g) {
+    constexpr int TM = 4, TN = 2, BM = 64 * TM, BN = 64 * TN;
+    __shared__ __align__(1024) uint8_t sA0[BM * LKB], sA1[BM * LKB], sA2[BM * LKB];
+    __shared__ __align__(1024) uint8_t sB0[BN * LKB], sB1[BN * LKB], sB2[BN * LKB];
+    typedef __attribute__((address_space(3))) uint8_t* lp;
+    lp A0 = (lp)sA0, A1 = (lp)sA1, A2 = (lp)sA2, B0 = (lp)sB0, B1 = (lp)sB1, B2 = (lp)sB2;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int wm = wave >> 1, wn = wave & 1;
+    uint32_t tm, tn;
+    if (!TileOrder::make((g.nq + BM - 1) / BM, (g.n + BN - 1) / BN).map(blockIdx.x, tm, tn)) return;
+    const uint32_t m0 = tm * BM, n0 = tn * BN;
+    const size_t row_bytes = (size_t)g.dim * (PREC == 1 ? 2 : 1);
+    const int nst = (int)(row_bytes / LKB);
+    constexpr int PER_STAGE = TM + TN;  // direct-to-LDS loads per wave per slice
+
+    f32x16_t acc[TM][TN];
+#pragma unroll
+    for (int i = 0; i < TM; i++)
+#pragma unroll
+        for (int j = 0; j < TN; j++)
+#pragma unroll
+            for (int r = 0; r < 16; r++) acc[i][j][r] = 0.f;
+
+    gemm_lds_issue<PREC, TM, TN>(g, A0, B0, 0, m0, n0, row_bytes, wave, lane);
+    if (nst > 1) gemm_lds_issue<PREC, TM, TN>(g, A1, B1, 1, m0, n0, row_bytes, wave, lane);
+    // one step: wait for slice st (the loads of slice st+1 may stay in flight), barrier, issue
+    // slice st+2 into the ring slot freed by slice st-1, MFMAs on slice st
+    auto step = [&](int st, lp rA, lp rB, lp nA, lp nB) {
+        if (st + 1 < nst)
+            __builtin_amdgcn_s_waitcnt(0x0F70 | PER_STAGE);  // vmcnt <= loads of one slice
+        else
+            __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt == 0
+        __builtin_amdgcn_s_barrier();
+        if (st + 2 < nst) gemm_lds_issue<PREC, TM, TN>(g, nA, nB, st + 2, m0, n0, row_bytes, wave, lane);
+        gemm_lds_mfma<PREC, TM, TN>((uint32_t)(uintptr_t)rA, (uint32_t)(uintptr_t)rB, acc, wm, wn, lane);
+    };
+    for (int st = 0; st < nst; st += 3) {
+        step(st, A0, B0, A2, B2);
+        if (st + 1 < nst) step(st + 1, A1, B1, A0, B0);
+        if (st + 2 < nst) step(st + 2, A2, B2, A1, B1);
+    }
+
+#pragma unroll
+    for (int i = 0; i < TM; i++) {
+#pragma unroll
+        for (int j = 0; j < TN; j++) {
+            const uint32_t col = n0 + wn * 32 * TN + j * 32 + (lane & 31);
+            const bool col_ok = col < g.n;
+            const float bsc = (PREC == 2 && col_ok) ? g.b_scale[col] : 1.0f;
+            const float bn = (g.space == 0 && col_ok) ? g.b_norm[col] : 0.f;
+#pragma unroll
+            for (int r = 0; r < 16; r++) {
+                const uint32_t row = m0 + wm * 32 * TM + i * 32 + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3);
+                const bool ok = col_ok && row < g.nq;
+                float s = acc[i][j][r];
+                if (PREC == 2 && ok) s = s * g.a_scale[row] * bsc;
+                float d = g.space == 0 ? (ok ? g.a_norm[row] : 0.f) + bn - 2.0f * s : 1.0f - s;
+                if (!FILTER) {
+                    if (ok) g.out_mat[(size_t)row * g.ld_out + col] = d;
+                } else {
+                    bool pass = ok && d < g.tau[row < g.nq ? row : 0];
+                    uint64_t m = __ballot(pass);
+                    uint32_t half = lane >> 5;
+                    uint32_t hm = (uint32_t)(m >> (32 * half));
+                    if (hm) {
+                        uint32_t leader = __ffs(hm) - 1;
+                        uint32_t base = 0;
+                        if ((lane & 31) == leader) base = atomicAdd(&g.cand_cnt[row], (uint32_t)__popc(hm));
+                        base = __shfl(base, (int)(half * 32 + leader));
+                        if (pass) {
+                            uint32_t slot = base + __popc(hm & ((1u << (lane & 31)) - 1u));
+                            if (slot < g.cap) {
+                                g.cand_rows[(size_t)row * g.cap + slot] = col;
+                                g.cand_d[(size_t)row * g.cap + slot] = d;
+                            }
+                        }
+                    }
+                }
+            }
+        }
+    }
+}
+
 // Copy every `step`-th row of a low-precision matrix (and its scale / norm) to a compact sample.
 __global__ void gather_rows_kernel(const uint8_t* __restrict__ src, size_t row_bytes, uint32_t step, uint32_t s_rows,
                                    uint8_t* __restrict__ dst, const float* sc, float* dsc, const float* nm,
@@ -845,8 +1040,9 @@ hipError_t launch_bf_convert(const BfData& d, uint16_t* vec_bf16, uint8_t* vec_f
 static inline size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
 
 // GEMM variant: 0 = 128x128 workgroup tiles (2x2 blocks per wave, double-buffered LDS),
-// 1 = 256x128 workgroup tiles (4x2 blocks per wave, register-prefetched single LDS stage).
-static int g_gemm_variant = 1;
+// 1 = 256x128 workgroup tiles (4x2 blocks per wave, register-prefetched single LDS stage),
+// 2 = 256x128 workgroup tiles fed by direct-to-LDS loads through a 3-slice ring.
+static int g_gemm_variant = 2;
 void set_gemm_variant(int v) { g_gemm_variant = v; }
 
 static void launch_gemm(LowpGemmArgs ga, int precision, bool filter, hipStream_t s) {
@@ -866,6 +1062,16 @@ static void launch_gemm(LowpGemmArgs ga, int precision, bool filter, hipStream_t
     constexpr int TM = 4, TN = 2;
     uint32_t tm = (ga.nq + 64 * TM - 1) / (64 * TM), tn = (ga.n + 64 * TN - 1) / (64 * TN);
     uint32_t grid = TileOrder::make(tm, tn).grid();
+    if (g_gemm_variant == 2) {
+        if (precision == 1) {
+            if (filter) hipLaunchKernelGGL((lowp_gemm_lds_kernel<1, true>), dim3(grid), dim3(256), 0, s, ga);
+            else hipLaunchKernelGGL((lowp_gemm_lds_kernel<1, false>), dim3(grid), dim3(256), 0, s, ga);
+        } else {
+            if (filter) hipLaunchKernelGGL((lowp_gemm_lds_kernel<2, true>), dim3(grid), dim3(256), 0, s, ga);
+            else hipLaunchKernelGGL((lowp_gemm_lds_kernel<2, false>), dim3(grid), dim3(256), 0, s, ga);
+        }
+        return;
+    }
     if (precision == 1) {
         if (filter) hipLaunchKernelGGL((lowp_gemm_wide_kernel<1, true, TM, TN>), dim3(grid), dim3(256), 0, s, ga);
         else hipLaunchKernelGGL((lowp_gemm_wide_kernel<1, false, TM, TN>), dim3(grid), dim3(256), 0, s, ga);

@@ -228,7 +228,7 @@ HG_API int hg_bf_fp64_check(void* bv, const float* dq, size_t nq, size_t k, cons
 
 HG_API int hg_bf_gemm_variant(int variant) {
     return guarded([&] {
-        require(variant == 0 || variant == 1, "GEMM variant must be 0 (128x128 tiles) or 1 (256x128 tiles)");
+        require(variant >= 0 && variant <= 2, "GEMM variant must be 0, 1 or 2");
         hg::set_gemm_variant(variant);
         return HG_OK;
     });

@@ -77,7 +77,7 @@ __device__ __forceinline__ int lanes_below(uint64_t m, int lane) {
 __device__ __forceinline__ bool hash_insert(uint32_t* hash, uint32_t mask, uint32_t id) {
     uint32_t h = (id * 2654435761u) & mask;
 #pragma unroll 1
-    for (int p = 0; p < 8; p++) {
+    for (int p = 0; p < 4; p++) {
         uint32_t slot = (h + p) & mask;
         uint32_t old = atomicCAS(&hash[slot], SENT, id);
         if (old == SENT) return true;
@@ -114,26 +114,29 @@ __device__ __forceinline__ float safe_div(float a, float b) { return a / (fabsf(
 // Features (same order as the trace collector and the host-side training code):
 //  0 query norm                      1 entry distance at level 0
 //  2 d_1 / d_entry                   3 d_k / d_entry
-//  4 d_last / d_entry                5 d_k / d_1
-//  6 d_next / d_k (next expansion)   7 hops / ef_eff
+//  4 position of the next expansion in the beam / ef_eff
+//  5 d_k / d_1                       6 d_next / d_k (next expansion)
+//  7 hops / ef_eff
 //  8 hops since the top-k last changed / (hops + 1)
 //  9 fraction of the last 16 expansions that changed the top-k
-// 10 distance evaluations per hop    11 log(1 + hops since the top-k last changed)
+// 10 min(position of the next expansion, 4k) / k
+// 11 log(1 + hops since the top-k last changed)
 __device__ __forceinline__ float stop_feature(int f, const StopState& st, float d1, float dk, float dlast,
-                                              float dnext, int hops, int evals, int ef_eff) {
+                                              float dnext, int pnext, int k, int hops, int ef_eff) {
     const int since = hops - st.last_change;
+    (void)dlast;
     switch (f) {
         case 0: return st.qnorm;
         case 1: return st.d_entry;
         case 2: return safe_div(d1, st.d_entry);
         case 3: return safe_div(dk, st.d_entry);
-        case 4: return safe_div(dlast, st.d_entry);
+        case 4: return (float)pnext / (float)ef_eff;
         case 5: return safe_div(dk, d1);
         case 6: return safe_div(dnext, dk);
         case 7: return (float)hops / (float)ef_eff;
         case 8: return (float)since / (float)(hops + 1);
         case 9: return (float)__popc((uint32_t)st.improved_recent & 0xFFFFu) / 16.0f;
-        case 10: return (float)evals / (float)(hops + 1);
+        case 10: return (float)(pnext < 4 * k ? pnext : 4 * k) / (float)k;
         default: return logf(1.0f + (float)since);
     }
 }
@@ -303,15 +306,20 @@ __global__ __launch_bounds__(64) void hnsw_search_kernel(SearchParams P) {
         }
         int nrec = 0;
         bool stopped = false;
+        uint32_t pf_node = SENT, pf_row = SENT;
         for (;;) {
-            // first unexpanded entry at or after scan_lo
+            // first unexpanded entry at or after scan_lo, and the one after it (whose link row is
+            // prefetched: it is the next expansion unless a better neighbour lands before it)
             int p = -1;
+            uint32_t node2 = SENT;
             for (int b = scan_lo; b < n; b += 64) {
                 int i = b + lane;
                 bool un = i < n && (bi[i] & XFLAG) == 0;
                 uint64_t m = ballot(un);
                 if (m) {
                     p = b + __ffsll((unsigned long long)m) - 1;
+                    const uint64_t m2 = m & (m - 1);
+                    if (m2) node2 = bi[b + __ffsll((unsigned long long)m2) - 1] & ID_MASK;
                     break;
                 }
             }
@@ -322,15 +330,39 @@ __global__ __launch_bounds__(64) void hnsw_search_kernel(SearchParams P) {
             H0++;
 
             // neighbours of node: keep those not visited, in list order
-            const uint32_t* row = g.links0 + (size_t)node * g.maxM0;
             int cnt = 0;
-            for (uint32_t b = 0; b < g.maxM0; b += 64) {
-                uint32_t j = b + lane;
-                uint32_t id = j < g.maxM0 ? row[j] : SENT;
-                bool fresh = id != SENT && hash_insert(hash, hmask, id);
-                uint64_t m = ballot(fresh);
-                if (fresh) si[cnt + lanes_below(m, lane)] = id;
-                cnt += __popcll(m);
+            if (g.maxM0 <= 32) {
+                // lanes 0..31 take node's row (from the previous prefetch when it matches),
+                // lanes 32..63 fetch node2's row for the next expansion
+                const int lr = lane & 31;
+                const bool upper = lane >= 32;
+                const uint32_t prev = (uint32_t)__shfl_xor((int)pf_row, 32);
+                const bool hit = node == pf_node;
+                uint32_t v = SENT;
+                const bool need = upper ? node2 != SENT : !hit;
+                if (need && lr < (int)g.maxM0) v = g.links0[(size_t)(upper ? node2 : node) * g.maxM0 + lr];
+                uint32_t cur;
+                if (hit)
+                    cur = prev;
+                else
+                    cur = v;
+                pf_row = v;
+                pf_node = node2;
+                const uint32_t id = (!upper && lr < (int)g.maxM0) ? cur : SENT;
+                const bool fresh = id != SENT && hash_insert(hash, hmask, id);
+                const uint64_t m = ballot(fresh);
+                if (fresh) si[lanes_below(m, lane)] = id;
+                cnt = __popcll(m);
+            } else {
+                const uint32_t* row = g.links0 + (size_t)node * g.maxM0;
+                for (uint32_t b = 0; b < g.maxM0; b += 64) {
+                    uint32_t j = b + lane;
+                    uint32_t id = j < g.maxM0 ? row[j] : SENT;
+                    bool fresh = id != SENT && hash_insert(hash, hmask, id);
+                    uint64_t m = ballot(fresh);
+                    if (fresh) si[cnt + lanes_below(m, lane)] = id;
+                    cnt += __popcll(m);
+                }
             }
             __syncthreads();
             if (cnt > 0) {
@@ -385,16 +417,27 @@ __global__ __launch_bounds__(64) void hnsw_search_kernel(SearchParams P) {
                         const int hi_chunk = (n - 1) & ~63;
                         const int lo_chunk = first_pos & ~63;
                         int hi = m;
+                        // reads of the next (lower) chunk are issued before the writes of the
+                        // current one: the current chunk writes only at or above its base,
+                        // which the lower chunk never reads
+                        float vd = 0.f, nvd = 0.f;
+                        uint32_t vi = 0, nvi = 0;
+                        {
+                            const int i = hi_chunk + lane;
+                            if (i >= first_pos && i < n) {
+                                vd = bd[i];
+                                vi = bi[i];
+                            }
+                        }
                         for (int b = hi_chunk; b >= lo_chunk; b -= 64) {
                             int lo = hi;
                             while (lo > 0 && __builtin_amdgcn_readlane(spos, lo - 1) >= b) lo--;
                             const int i = b + lane;
                             const bool mv = i >= first_pos && i < n;
-                            float vd = 0.f;
-                            uint32_t vi = 0;
-                            if (mv) {
-                                vd = bd[i];
-                                vi = bi[i];
+                            const int ni = i - 64;
+                            if (b - 64 >= lo_chunk && ni >= first_pos) {
+                                nvd = bd[ni];
+                                nvi = bi[ni];
                             }
                             int sh = lo;
                             for (int rr = lo; rr < hi; rr++) sh += __builtin_amdgcn_readlane(spos, rr) <= i ? 1 : 0;
@@ -405,6 +448,8 @@ __global__ __launch_bounds__(64) void hnsw_search_kernel(SearchParams P) {
                                 bi[np] = vi;
                             }
                             __asm__ volatile("" ::: "memory");
+                            vd = nvd;
+                            vi = nvi;
                             hi = lo;
                         }
                         const int fp = pos + r;
@@ -443,16 +488,18 @@ __global__ __launch_bounds__(64) void hnsw_search_kernel(SearchParams P) {
                     // the next expansion: first unexpanded entry (the search's stop rule ends
                     // the loop when there is none)
                     float dnext = dlast;
+                    int pnext = n;
                     for (int b = scan_lo; b < n; b += 64) {
                         int i = b + lane;
                         uint64_t m = ballot(i < n && (bi[i] & XFLAG) == 0);
                         if (m) {
-                            dnext = bd[b + __ffsll((unsigned long long)m) - 1];
+                            pnext = b + __ffsll((unsigned long long)m) - 1;
+                            dnext = bd[pnext];
                             break;
                         }
                     }
                     if (lane < kStopFeatures)
-                        feat[lane] = stop_feature(lane, st, d1, dk, dlast, dnext, H0, E0, ef_eff);
+                        feat[lane] = stop_feature(lane, st, d1, dk, dlast, dnext, pnext, kq, H0, ef_eff);
                     __syncthreads();
                     if (TRACE && nrec < P.trace_max_rec) {
                         float* fo = P.trace_feat + ((size_t)qi * P.trace_max_rec + nrec) * kStopFeatures;

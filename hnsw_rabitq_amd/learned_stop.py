@@ -177,7 +177,10 @@ def run(gpu, H, lib, data_dim, queries, dq, gt, stream, timer, sweep, log, steps
     dc = DeviceArray(nq * 16)
     best = None
     table = []
-    efs = sorted(set([ef_l, min(int(ef_l * 1.25), ef_max)]))
+    # candidate ef of the learned search: from just above the plain operating point up to the
+    # traced ef (a larger beam costs LDS, hence concurrency)
+    base = plain_op["ef"]
+    efs = sorted(set(min(ef_max, int(round(base * f / 64.0)) * 64) for f in (1.2, 1.6)) | {ef_l})
     for ef in efs:
         gpu.autotune(dq, nq, ef, k, stream=stream)
         for thr in (threshold, 0.5, 0.7, 0.8, 0.95):

@@ -116,8 +116,9 @@ int hg_bf_fp64_check(void* bf, const float* d_queries, size_t nq, size_t k, cons
                      const float* d_dists, double rel_tol, double* out, void* stream);
 
 /* Tile configuration of the reduced-precision distance GEMM (process-wide): 0 = 128x128
- * workgroup tiles with 2x2 MFMA blocks per wave and double-buffered LDS; 1 (default) =
- * 256x128 tiles with 4x2 blocks per wave and a register-prefetched LDS stage. */
+ * workgroup tiles with 2x2 MFMA blocks per wave and double-buffered LDS; 1 = 256x128 tiles
+ * with 4x2 blocks per wave and a register-prefetched LDS stage; 2 (default) = 256x128 tiles
+ * fed by direct-to-LDS loads through a ring of three k-slices. */
 int hg_bf_gemm_variant(int variant);
 
 /* ---- Learned early termination (stop model) ---- */
