@@ -8,6 +8,12 @@ swept ef whose recall@10 >= 0.95).  Inputs are resident in HBM before timing sta
 uses HIP events on the stream the kernels run on.  Prints ONE JSON line (rank 0, N=1).
 
     python bench.py [--steps K] [--warmup W] [--gpus 1] [--n N] [--nq Q] [--no-learned] [--no-bf]
+
+--ef E skips the ef sweep (E is the operating point) and --plan SLOTS,PER_CU skips the autotuner;
+together they make every search launch of the run identical, which is how the rocprofv3 summaries
+under profiles/ are taken.  The roofline's "traffic" is the HBM bytes per launch measured with
+rocprofv3 --pmc (FETCH_SIZE, WRITE_SIZE) and recorded in profiles/search_traffic.json; it is
+reported when that record matches this run's workload and ef, else null.
 """
 import argparse
 import ctypes
@@ -79,6 +85,8 @@ def main():
     ap.add_argument("--bf-nq", type=int, default=10_000)
     ap.add_argument("--cpu-queries", type=int, default=2000)
     ap.add_argument("--learned-ef", type=int, default=0, help="ef of the learned search (default 2x plain)")
+    ap.add_argument("--ef", type=int, default=0, help="operating-point ef (skips the sweep)")
+    ap.add_argument("--plan", default="", help="SLOTS,PER_CU launch plan (skips the autotuner)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run the streaming copy kernel over this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -148,9 +156,7 @@ def main():
     log(f"  exact BF {gt_ms:.1f} ms ({exact_tflops:.1f} TFLOP/s FP32 exact order)")
     del bf
 
-    # ef sweep (plain fixed ef)
-    sweep = []
-    for ef in EF_SWEEP:
+    def measure(ef):
         gpu.search_device(dq, nq, K, ef, dl, dd, d_counters=dc, stream=stream)
         timer.start()
         gpu.search_device(dq, nq, K, ef, dl, dd, d_counters=dc, stream=stream)
@@ -159,17 +165,45 @@ def main():
         ctr = dc.to_numpy(np.int32, (nq, 4))
         rec = recall_at(lab, gt, K)
         byts = algorithmic_bytes(ctr, D, K, maxM, maxM0).sum()
-        sweep.append(dict(ef=ef, recall=rec, ms=ms, qps=nq / (ms * 1e-3), gbs=byts / (ms * 1e-3) / 1e9,
-                          bytes_per_query=byts / nq, hops=float(ctr[:, 0].mean()), evals=float(ctr[:, 1].mean())))
         log(f"  ef={ef:4d} recall@10={rec:.4f} {ms:8.2f} ms  QPS={nq / (ms * 1e-3):11.0f}  "
             f"{byts / (ms * 1e-3) / 1e9:7.1f} GB/s  hops={ctr[:, 0].mean():.0f} evals={ctr[:, 1].mean():.0f}")
-    op = next((s for s in sweep if s["recall"] >= 0.95), sweep[-1])
+        return dict(ef=ef, recall=rec, ms=ms, qps=nq / (ms * 1e-3), gbs=byts / (ms * 1e-3) / 1e9,
+                    bytes_per_query=byts / nq, hops=float(ctr[:, 0].mean()), evals=float(ctr[:, 1].mean()))
+
+    if args.plan:
+        slots, per_cu = (int(x) for x in args.plan.split(","))
+        for ef in ([args.ef] if args.ef else EF_SWEEP):
+            gpu.set_plan(ef, K, slots, per_cu)
+    if args.ef:
+        sweep = [measure(args.ef)]
+        op = sweep[0]
+    else:
+        # ef sweep (plain fixed ef); the operating point is the smallest ef with recall@10 >= 0.95
+        sweep = [measure(ef) for ef in EF_SWEEP]
+        op = next((s for s in sweep if s["recall"] >= 0.95), sweep[-1])
     ef_op = op["ef"]
     log(f"operating point ef={ef_op} recall={op['recall']:.4f}")
-    t = time.time()
-    gpu.autotune(dq, nq, ef_op, K, stream=stream)
+    # distance evaluations with a visited hash of 2^15 slots (one workgroup per CU), which
+    # forgets almost no visited node: the algorithmic count.  The tuned plan's smaller hash
+    # re-evaluates a node whose slot was overwritten; those extra reads are not counted.  The
+    # count runs through the stop-model kernel with a model that is never evaluated (same
+    # traversal), so that its launch is not averaged with the plain kernel's in a profile.
+    from hnsw_rabitq_amd.learned_stop import StopModel
+    never = StopModel()
+    never.init_params(0)
+    never.config(period=1 << 30, threshold=0.999, min_hops=1 << 30)
+    gpu.set_plan(ef_op, K, 1 << 15, 1)
+    gpu.search_device(dq, nq, K, ef_op, dl, dd, d_counters=dc, stop_model=never.h, stream=stream)
+    ctr_alg = dc.to_numpy(np.int32, (nq, 4))
+    never.free()
+    if args.plan:
+        gpu.set_plan(ef_op, K, slots, per_cu)
+    if not args.plan:
+        t = time.time()
+        gpu.autotune(dq, nq, ef_op, K, stream=stream)
+        log(f"autotuned in {time.time() - t:.1f}s")
     plan = gpu.plan(ef_op, K, nq)
-    log(f"autotuned in {time.time() - t:.1f}s: {plan}")
+    log(f"plan: {plan}")
 
     # timed steps at the operating point
     for _ in range(args.warmup):
@@ -187,14 +221,18 @@ def main():
     ms_step = wall * 1e3 / args.steps
     ev_ms = float(np.mean(kernel_ms))
     plain_qps = nq / (ev_ms * 1e-3)
-    # counters of the operating point for the roofline
+    # counters of the tuned plan (evaluations including re-evaluations) for comparison
     gpu.search_device(dq, nq, K, ef_op, dl, dd, d_counters=dc, stream=stream)
     ctr = dc.to_numpy(np.int32, (nq, 4))
-    bytes_launch = algorithmic_bytes(ctr, D, K, maxM, maxM0).sum()
+    bytes_launch = algorithmic_bytes(ctr_alg, D, K, maxM, maxM0).sum()
+    bytes_read_tuned = algorithmic_bytes(ctr, D, K, maxM, maxM0).sum()
     achieved = bytes_launch / (ev_ms * 1e-3) / 1e9
-    log(f"plain: {plain_qps:.0f} QPS at ef={ef_op}; kernel {ev_ms:.3f} ms; {achieved:.1f} GB/s algorithmic")
+    log(f"plain: {plain_qps:.0f} QPS at ef={ef_op}; kernel {ev_ms:.3f} ms; {achieved:.1f} GB/s algorithmic "
+        f"(evaluations/query {ctr_alg[:, 1].mean():.0f}; tuned plan {ctr[:, 1].mean():.0f})")
 
-    result = dict(plain_qps_at_095=plain_qps, plain_ef=ef_op, plain_recall=op["recall"], plan=plan)
+    result = dict(plain_qps_at_095=plain_qps, plain_ef=ef_op, plain_recall=op["recall"], plan=plan,
+                  evals_per_query=float(ctr_alg[:, 1].mean()), evals_per_query_tuned=float(ctr[:, 1].mean()),
+                  bytes_per_launch_with_reevaluations=bytes_read_tuned)
 
     # learned early termination
     learned = None
@@ -231,7 +269,18 @@ def main():
                    sample=f"{sample.shape[0]} of the {nq} queries at ef={ef_op}, k=10, median of 3 after warm-up")
         log(f"CPU oracle port: {cpu_qps:.0f} QPS on {nthreads} threads")
 
-    value = result.get("learned_qps_at_095") or plain_qps
+    # headline: the faster of plain and learned at recall@10 >= 0.95 (plain is the fallback)
+    learned_qps = result.get("learned_qps_at_095") or 0.0
+    value = max(plain_qps, learned_qps)
+    result["value_method"] = "learned" if learned_qps > plain_qps else "plain"
+    workload = f"hnsw_l2_gamma_{args.n}x{D}_q{nq}_M16_efC200_k10"
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "search_traffic.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            rec = json.load(f)
+        if rec.get("workload") == workload and rec.get("ef") == ef_op:
+            traffic = rec.get("traffic_bytes_per_launch")
     line = {
         "metric": "QPS@recall10=0.95, 1M×128 L2: plain vs learned-stop GPU; brute-force TFLOPS",
         "value": value,
@@ -245,10 +294,10 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic",
-        "config": {"workload": f"hnsw_l2_gamma_{args.n}x{D}_q{nq}_M16_efC200_k10",
+        "config": {"workload": workload,
                    "n": args.n, "dim": D, "queries": nq, "ef_sweep": EF_SWEEP, "k": K},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": achieved / PEAK_HBM_GBS, "traffic": None,
+                     "frac": achieved / PEAK_HBM_GBS, "traffic": traffic,
                      "kernel": "hnsw_search_kernel", "bytes_per_launch": bytes_launch, "kernel_ms": ev_ms},
         "cpu_baseline": cpu,
         "sweep": [{k2: (round(v, 5) if isinstance(v, float) else v) for k2, v in s.items()} for s in sweep],

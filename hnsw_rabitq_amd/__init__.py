@@ -123,6 +123,13 @@ class GpuHnsw:
     def autotune(self, d_queries, nq, ef, k, stream=None):
         _lib.call("hg_gpu_autotune", self.handle, d_queries.ptr, nq, ef, k, stream)
 
+    def set_plan(self, ef, k, hash_slots, workgroups_per_cu):
+        """Fix the launch plan for (ef, k): hash_slots must be a power of two."""
+        bits = int(hash_slots).bit_length() - 1
+        if hash_slots != 1 << bits:
+            raise ValueError("hash_slots must be a power of two")
+        _lib.call("hg_gpu_set_plan", self.handle, ef, k, bits, workgroups_per_cu)
+
     def nbytes(self):
         b = ctypes.c_size_t()
         _lib.call("hg_gpu_index_bytes", self.handle, ctypes.byref(b))

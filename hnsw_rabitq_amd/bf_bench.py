@@ -72,3 +72,49 @@ def run(H, lib, n, nq, stream, timer, log, dim=768, k=100, c=4096):
             f"prep {stats[1]:.2f} ms, re-rank {stats[2]:.2f} ms, end-to-end {e2e:.2f} ms; recall@100 {rec:.4f}; "
             f"rows identical to exact {same:.4f}; fp64 near-tie pairs {chk[0]:.0f}, disagreements {chk[1]:.0f}")
     return out
+
+
+def main():
+    """Standalone run of the brute-force measurement (one JSON line on stdout):
+
+        python -m hnsw_rabitq_amd.bf_bench [--n 1000000] [--nq 10000]
+    """
+    import argparse
+    import json
+    import sys
+    import time
+
+    import hnsw_rabitq_amd as H
+    from hnsw_rabitq_amd import _lib
+
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--n", type=int, default=1_000_000)
+    ap.add_argument("--nq", type=int, default=10_000)
+    args = ap.parse_args()
+    t0 = time.time()
+
+    def log(m):
+        print(f"[bf {time.time() - t0:7.1f}s] {m}", file=sys.stderr, flush=True)
+
+    class Timer:
+        def __init__(self, stream):
+            self.stream = stream
+            self.t = ctypes.c_void_p()
+            _lib.call("hg_timer_create", ctypes.byref(self.t))
+
+        def start(self):
+            _lib.call("hg_timer_start", self.t, self.stream)
+
+        def stop(self):
+            ms = ctypes.c_float()
+            _lib.call("hg_timer_stop", self.t, self.stream, ctypes.byref(ms))
+            return ms.value
+
+    stream = ctypes.c_void_p()
+    _lib.call("hg_stream_create", ctypes.byref(stream))
+    out = run(H, _lib, n=args.n, nq=args.nq, stream=stream, timer=Timer(stream), log=log)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -144,14 +144,15 @@ __device__ __forceinline__ bool sel_less(const SelEntry& a, const SelEntry& b, c
 }
 
 // Bitonic sort of buf[0, cnt) (padded to a power of two with sentinels) by (distance, label).
+template <int NT = SEL_THREADS>
 __device__ void sel_sort(SelEntry* buf, int cnt, const uint64_t* labels) {
     int size = 1;
     while (size < cnt) size <<= 1;
-    for (int i = cnt + (int)threadIdx.x; i < size; i += SEL_THREADS) buf[i] = SelEntry{INFINITY, SENT};
+    for (int i = cnt + (int)threadIdx.x; i < size; i += NT) buf[i] = SelEntry{INFINITY, SENT};
     __syncthreads();
     for (int k = 2; k <= size; k <<= 1) {
         for (int j = k >> 1; j > 0; j >>= 1) {
-            for (int i = threadIdx.x; i < size; i += SEL_THREADS) {
+            for (int i = threadIdx.x; i < size; i += NT) {
                 int p = i ^ j;
                 if (p > i) {
                     bool up = (i & k) == 0;
@@ -383,6 +384,121 @@ struct LowpGemmArgs {
     uint32_t tiles_m, tiles_n;
 };
 
+// Per-row values of one workgroup tile, staged in LDS for the epilogue.
+template <int BM>
+struct EpiLds {
+    float tau[BM], a_norm[BM], a_scale[BM];
+    uint32_t cnt[2][BM];  // passes of each row in the column half of wave wn
+    uint32_t base[BM];    // start of the row's reserved candidate slots
+};
+
+// Epilogue of the GEMM variants.  The workgroup tile is rows [m0, m0 + 64*TM) x columns
+// [n0, n0 + 64*TN); wave (wm, wn) holds TM x TN 32x32 blocks in acc, so each of its 32*TM rows
+// sits in one (block row i, register r, half-wave h) and the two waves wn = 0, 1 share a row.
+// Sample mode writes every approximate distance.  Filter mode appends the columns with
+// d < tau[row] to the row's candidate list with one device-scope atomic per (row, workgroup):
+// pass 1 counts each row's passes (ballot + popcount), one thread per row reserves the row's
+// total, pass 2 recomputes d and writes each pass at the row base + the other wave's count +
+// its rank among the row's passes.
+template <int PREC, bool FILTER, int TM, int TN>
+__device__ __forceinline__ void gemm_epilogue(const LowpGemmArgs& g, f32x16_t (&acc)[TM][TN], uint32_t m0,
+                                              uint32_t n0, int wm, int wn, int lane, int tid,
+                                              EpiLds<64 * TM>& e) {
+    constexpr int BM = 64 * TM;
+    for (int r = tid; r < BM; r += 256) {
+        const uint32_t row = m0 + r;
+        const bool ok = row < g.nq;
+        e.tau[r] = (FILTER && ok) ? g.tau[row] : -INFINITY;
+        e.a_norm[r] = (g.space == 0 && ok) ? g.a_norm[row] : 0.f;
+        e.a_scale[r] = (PREC == 2 && ok) ? g.a_scale[row] : 1.f;
+    }
+    __syncthreads();
+    float bsc[TN], bn[TN];
+    uint32_t col[TN];
+    bool col_ok[TN];
+#pragma unroll
+    for (int j = 0; j < TN; j++) {
+        col[j] = n0 + wn * 32 * TN + j * 32 + (lane & 31);
+        col_ok[j] = col[j] < g.n;
+        bsc[j] = (PREC == 2 && col_ok[j]) ? g.b_scale[col[j]] : 1.0f;
+        bn[j] = (g.space == 0 && col_ok[j]) ? g.b_norm[col[j]] : 0.f;
+    }
+    auto dist = [&](int i, int j, int r, int lr) {
+        float sv = acc[i][j][r];
+        if (PREC == 2) sv = sv * e.a_scale[lr] * bsc[j];
+        return g.space == 0 ? e.a_norm[lr] + bn[j] - 2.0f * sv : 1.0f - sv;
+    };
+    const int half = lane >> 5;
+    if (!FILTER) {
+#pragma unroll
+        for (int i = 0; i < TM; i++)
+#pragma unroll
+            for (int r = 0; r < 16; r++) {
+                const int lr = wm * 32 * TM + i * 32 + (r >> 2) * 8 + half * 4 + (r & 3);
+                const uint32_t row = m0 + lr;
+#pragma unroll
+                for (int j = 0; j < TN; j++)
+                    if (col_ok[j] && row < g.nq) g.out_mat[(size_t)row * g.ld_out + col[j]] = dist(i, j, r, lr);
+            }
+        return;
+    }
+    // pass 1: per-row pass counts of this wave (each row belongs to one half-wave, so lanes 0
+    // and 32 store them without atomics)
+#pragma unroll
+    for (int i = 0; i < TM; i++)
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            const int lr = wm * 32 * TM + i * 32 + (r >> 2) * 8 + half * 4 + (r & 3);
+            const float t = e.tau[lr];
+            uint32_t c = 0;
+#pragma unroll
+            for (int j = 0; j < TN; j++)
+                c += (uint32_t)__popc((uint32_t)(__ballot(col_ok[j] && dist(i, j, r, lr) < t) >> (32 * half)));
+            if ((lane & 31) == 0) e.cnt[wn][lr] = c;
+        }
+    __syncthreads();
+    // one reservation per row
+    for (int r = tid; r < BM; r += 256) {
+        const uint32_t c = e.cnt[0][r] + e.cnt[1][r];
+        if (c) e.base[r] = atomicAdd(&g.cand_cnt[m0 + r], c);
+    }
+    __syncthreads();
+    // pass 2: write the candidates into the reserved slots
+    const uint32_t below = (lane & 31) ? (1u << (lane & 31)) - 1u : 0u;
+#pragma unroll
+    for (int i = 0; i < TM; i++)
+#pragma unroll
+        for (int r = 0; r < 16; r++) {
+            const int lr = wm * 32 * TM + i * 32 + (r >> 2) * 8 + half * 4 + (r & 3);
+            const uint32_t row = m0 + lr;
+            const float t = e.tau[lr];
+            float d[TN];
+            bool pass[TN];
+            uint32_t hm[TN];
+            bool any = false;
+#pragma unroll
+            for (int j = 0; j < TN; j++) {
+                d[j] = dist(i, j, r, lr);
+                pass[j] = col_ok[j] && d[j] < t;
+                hm[j] = (uint32_t)(__ballot(pass[j]) >> (32 * half));
+                any |= hm[j] != 0;
+            }
+            if (!any) continue;
+            uint32_t off = e.base[lr] + (wn ? e.cnt[0][lr] : 0u);
+#pragma unroll
+            for (int j = 0; j < TN; j++) {
+                if (pass[j]) {
+                    const uint32_t slot = off + (uint32_t)__popc(hm[j] & below);
+                    if (slot < g.cap) {
+                        g.cand_rows[(size_t)row * g.cap + slot] = col[j];
+                        g.cand_d[(size_t)row * g.cap + slot] = d[j];
+                    }
+                }
+                off += (uint32_t)__popc(hm[j]);
+            }
+        }
+}
+
 template <int PREC, bool FILTER>
 __global__ __launch_bounds__(256, 2) void lowp_gemm_kernel(LowpGemmArgs g) {
     __shared__ __align__(16) uint8_t As[2][GM][GKB + GPAD];
@@ -477,46 +593,8 @@ __global__ __launch_bounds__(256, 2) void lowp_gemm_kernel(LowpGemmArgs g) {
         }
     }
 
-    // epilogue
-#pragma unroll
-    for (int i = 0; i < 2; i++) {
-#pragma unroll
-        for (int j = 0; j < 2; j++) {
-            const uint32_t col = n0 + wn * 64 + j * 32 + (lane & 31);
-            const bool col_ok = col < g.n;
-            const float bsc = (PREC == 2 && col_ok) ? g.b_scale[col] : 1.0f;
-            const float bn = (g.space == 0 && col_ok) ? g.b_norm[col] : 0.f;
-#pragma unroll
-            for (int r = 0; r < 16; r++) {
-                const uint32_t row = m0 + wm * 64 + i * 32 + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3);
-                const bool ok = col_ok && row < g.nq;
-                float s = acc[i][j][r];
-                if (PREC == 2 && ok) s = s * g.a_scale[row] * bsc;
-                float d = g.space == 0 ? (ok ? g.a_norm[row] : 0.f) + bn - 2.0f * s : 1.0f - s;
-                if (!FILTER) {
-                    if (ok) g.out_mat[(size_t)row * g.ld_out + col] = d;
-                } else {
-                    bool pass = ok && d < g.tau[row < g.nq ? row : 0];
-                    uint64_t m = __ballot(pass);
-                    uint32_t half = lane >> 5;
-                    uint32_t hm = (uint32_t)(m >> (32 * half));
-                    if (hm) {
-                        uint32_t leader = __ffs(hm) - 1;
-                        uint32_t base = 0;
-                        if ((lane & 31) == leader) base = atomicAdd(&g.cand_cnt[row], (uint32_t)__popc(hm));
-                        base = __shfl(base, (int)(half * 32 + leader));
-                        if (pass) {
-                            uint32_t slot = base + __popc(hm & ((1u << (lane & 31)) - 1u));
-                            if (slot < g.cap) {
-                                g.cand_rows[(size_t)row * g.cap + slot] = col;
-                                g.cand_d[(size_t)row * g.cap + slot] = d;
-                            }
-                        }
-                    }
-                }
-            }
-        }
-    }
+    __shared__ EpiLds<64 * 2> epi;
+    gemm_epilogue<PREC, FILTER, 2, 2>(g, acc, m0, n0, wm, wn, lane, tid, epi);
 }
 
 // Larger tiles: each wave owns (32*TM) x (32*TN) outputs (TM x TN MFMA blocks), the workgroup
@@ -623,45 +701,8 @@ __global__ __launch_bounds__(256, 2) void lowp_gemm_wide_kernel(LowpGemmArgs g) 
         }
     }
 
-#pragma unroll
-    for (int i = 0; i < TM; i++) {
-#pragma unroll
-        for (int j = 0; j < TN; j++) {
-            const uint32_t col = n0 + wn * 32 * TN + j * 32 + (lane & 31);
-            const bool col_ok = col < g.n;
-            const float bsc = (PREC == 2 && col_ok) ? g.b_scale[col] : 1.0f;
-            const float bn = (g.space == 0 && col_ok) ? g.b_norm[col] : 0.f;
-#pragma unroll
-            for (int r = 0; r < 16; r++) {
-                const uint32_t row = m0 + wm * 32 * TM + i * 32 + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3);
-                const bool ok = col_ok && row < g.nq;
-                float s = acc[i][j][r];
-                if (PREC == 2 && ok) s = s * g.a_scale[row] * bsc;
-                float d = g.space == 0 ? (ok ? g.a_norm[row] : 0.f) + bn - 2.0f * s : 1.0f - s;
-                if (!FILTER) {
-                    if (ok) g.out_mat[(size_t)row * g.ld_out + col] = d;
-                } else {
-                    bool pass = ok && d < g.tau[row < g.nq ? row : 0];
-                    uint64_t m = __ballot(pass);
-                    uint32_t half = lane >> 5;
-                    uint32_t hm = (uint32_t)(m >> (32 * half));
-                    if (hm) {
-                        uint32_t leader = __ffs(hm) - 1;
-                        uint32_t base = 0;
-                        if ((lane & 31) == leader) base = atomicAdd(&g.cand_cnt[row], (uint32_t)__popc(hm));
-                        base = __shfl(base, (int)(half * 32 + leader));
-                        if (pass) {
-                            uint32_t slot = base + __popc(hm & ((1u << (lane & 31)) - 1u));
-                            if (slot < g.cap) {
-                                g.cand_rows[(size_t)row * g.cap + slot] = col;
-                                g.cand_d[(size_t)row * g.cap + slot] = d;
-                            }
-                        }
-                    }
-                }
-            }
-        }
-    }
+    __shared__ EpiLds<64 * TM> epi;
+    gemm_epilogue<PREC, FILTER, TM, TN>(g, acc, m0, n0, wm, wn, lane, tid, epi);
 }
 
 // Direct-to-LDS pipeline: rows are fetched with global_load_lds_dwordx4 (64 lanes x 16 bytes
@@ -818,45 +859,8 @@ __global__ __launch_bounds__(256, 2) void lowp_gemm_lds_kernel(LowpGemmArgs g) {
         if (st + 2 < nst) step(st + 2, A2, B2, A1, B1);
     }
 
-#pragma unroll
-    for (int i = 0; i < TM; i++) {
-#pragma unroll
-        for (int j = 0; j < TN; j++) {
-            const uint32_t col = n0 + wn * 32 * TN + j * 32 + (lane & 31);
-            const bool col_ok = col < g.n;
-            const float bsc = (PREC == 2 && col_ok) ? g.b_scale[col] : 1.0f;
-            const float bn = (g.space == 0 && col_ok) ? g.b_norm[col] : 0.f;
-#pragma unroll
-            for (int r = 0; r < 16; r++) {
-                const uint32_t row = m0 + wm * 32 * TM + i * 32 + (r >> 2) * 8 + (lane >> 5) * 4 + (r & 3);
-                const bool ok = col_ok && row < g.nq;
-                float s = acc[i][j][r];
-                if (PREC == 2 && ok) s = s * g.a_scale[row] * bsc;
-                float d = g.space == 0 ? (ok ? g.a_norm[row] : 0.f) + bn - 2.0f * s : 1.0f - s;
-                if (!FILTER) {
-                    if (ok) g.out_mat[(size_t)row * g.ld_out + col] = d;
-                } else {
-                    bool pass = ok && d < g.tau[row < g.nq ? row : 0];
-                    uint64_t m = __ballot(pass);
-                    uint32_t half = lane >> 5;
-                    uint32_t hm = (uint32_t)(m >> (32 * half));
-                    if (hm) {
-                        uint32_t leader = __ffs(hm) - 1;
-                        uint32_t base = 0;
-                        if ((lane & 31) == leader) base = atomicAdd(&g.cand_cnt[row], (uint32_t)__popc(hm));
-                        base = __shfl(base, (int)(half * 32 + leader));
-                        if (pass) {
-                            uint32_t slot = base + __popc(hm & ((1u << (lane & 31)) - 1u));
-                            if (slot < g.cap) {
-                                g.cand_rows[(size_t)row * g.cap + slot] = col;
-                                g.cand_d[(size_t)row * g.cap + slot] = d;
-                            }
-                        }
-                    }
-                }
-            }
-        }
-    }
+    __shared__ EpiLds<64 * TM> epi;
+    gemm_epilogue<PREC, FILTER, TM, TN>(g, acc, m0, n0, wm, wn, lane, tid, epi);
 }
 
 // Copy every `step`-th row of a low-precision matrix (and its scale / norm) to a compact sample.
@@ -892,6 +896,11 @@ __device__ __forceinline__ uint32_t f2key(float f) {
 __device__ __forceinline__ float key2f(uint32_t k) {
     uint32_t u = (k & 0x80000000u) ? (k & 0x7FFFFFFFu) : ~k;
     return __uint_as_float(u);
+}
+
+__global__ void fill_u32_kernel(uint32_t* __restrict__ p, uint32_t n, uint32_t v) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = v;
 }
 
 // tau[q] = the rank-th smallest value of row q of the sample matrix (radix select, 4 x 8 bits),
@@ -940,9 +949,16 @@ __global__ __launch_bounds__(256) void kth_kernel(const float* __restrict__ mat,
 // approximate distance, recompute their exact FP32 distances in the reference order, and write
 // the k smallest (distance, label) closest first.  A query whose candidate count overflowed the
 // buffer is flagged for the exact path.
-constexpr int RR_THREADS = 256;
-constexpr int RR_CAP = 16384;
+constexpr int RR_THREADS = 512;
+constexpr int RR_CAP = 16384;  // candidate slots per query
+constexpr int RR_TIES = 2048;  // candidates tied at the c-th approximate distance one query can resolve
 
+// Re-rank of one query per workgroup.  The c candidates with the smallest approximate distance
+// (ties by label) are chosen by a radix select that reads the candidate list from global memory,
+// so LDS holds only the c chosen rows; their exact FP32 distances (reference lane order) are
+// computed four lanes per row and the k smallest by (distance, label) are written.  A query
+// whose candidate list overflowed, came up short, or has more than RR_TIES ties at the cut is
+// flagged for the exact path.
 template <int SPACE>
 __global__ __launch_bounds__(RR_THREADS) void rerank_kernel(const float* __restrict__ Q, const float* __restrict__ X,
                                                             const uint64_t* __restrict__ labels, uint32_t dim,
@@ -953,59 +969,152 @@ __global__ __launch_bounds__(RR_THREADS) void rerank_kernel(const float* __restr
                                                             float* __restrict__ out_d, size_t ld_out,
                                                             uint32_t* __restrict__ overflow) {
     extern __shared__ __align__(16) unsigned char smem[];
-    SelEntry* buf = reinterpret_cast<SelEntry*>(smem);  // RR_CAP entries
-    float* qs = reinterpret_cast<float*>(buf + RR_CAP);  // dim floats
+    uint32_t cpow = 2;  // >= 2 keeps the query copy 16-byte aligned
+    while (cpow < c) cpow <<= 1;
+    SelEntry* buf = reinterpret_cast<SelEntry*>(smem);    // cpow entries
+    float* qs = reinterpret_cast<float*>(buf + cpow);      // dim floats
+    uint64_t* ties = reinterpret_cast<uint64_t*>(qs + ((dim + 1) & ~1u));  // RR_TIES labels
+    __shared__ uint32_t hist[256];
+    __shared__ uint32_t prefix, want, nsel, ntie, tfill, bad, cut_on;
+    __shared__ uint64_t cut_label;
     const uint32_t q = blockIdx.x;
     const uint32_t cnt = cand_cnt[q];
-    uint64_t* ol = out_l + (size_t)q * ld_out;
-    float* od = out_d + (size_t)q * ld_out;
     if (cnt > cap || cnt < k) {
         if (threadIdx.x == 0) overflow[q] = 1;
         return;
     }
+    const float* cdq = cand_d + (size_t)q * cap;
+    const uint32_t* crq = cand_rows + (size_t)q * cap;
     for (uint32_t i = threadIdx.x; i < dim; i += RR_THREADS) qs[i] = Q[(size_t)q * dim + i];
-    for (uint32_t i = threadIdx.x; i < cnt; i += RR_THREADS)
-        buf[i] = SelEntry{cand_d[(size_t)q * cap + i], cand_rows[(size_t)q * cap + i]};
+    const bool all = cnt <= c;
+    const uint32_t m = all ? cnt : c;
+    if (threadIdx.x == 0) {
+        prefix = 0;
+        want = c;
+        nsel = 0;
+        ntie = 0;
+        tfill = 0;
+        cut_label = 0;
+        cut_on = 0;
+        bad = 0;
+    }
     __syncthreads();
-    uint32_t m = cnt;
-    if (cnt > c) {
-        // order by approximate distance, ties by row (labels are not needed for this cut)
-        int size = 1;
-        while (size < (int)cnt) size <<= 1;
-        for (int i = cnt + threadIdx.x; i < size; i += RR_THREADS) buf[i] = SelEntry{INFINITY, SENT};
-        __syncthreads();
-        for (int kk = 2; kk <= size; kk <<= 1)
-            for (int j = kk >> 1; j > 0; j >>= 1) {
-                for (int i = threadIdx.x; i < size; i += RR_THREADS) {
-                    int p = i ^ j;
-                    if (p > i) {
-                        SelEntry a = buf[i], b = buf[p];
-                        bool a_gt = a.d > b.d || (a.d == b.d && a.r > b.r);
-                        bool up = (i & kk) == 0;
-                        if (up ? a_gt : !a_gt) {
-                            buf[i] = b;
-                            buf[p] = a;
+    if (!all) {
+        // key of the c-th smallest approximate distance; `want` ends as the number of entries
+        // with that key inside the first c
+        for (int shift = 24; shift >= 0; shift -= 8) {
+            for (int b = threadIdx.x; b < 256; b += RR_THREADS) hist[b] = 0;
+            __syncthreads();
+            const uint32_t pre = prefix;
+            const uint32_t mask_hi = shift == 24 ? 0u : (0xFFFFFFFFu << (shift + 8));
+            for (uint32_t i = threadIdx.x; i < cnt; i += RR_THREADS) {
+                const uint32_t key = f2key(cdq[i]);
+                if ((key & mask_hi) == pre) atomicAdd(&hist[(key >> shift) & 255u], 1u);
+            }
+            __syncthreads();
+            if (threadIdx.x == 0) {
+                uint32_t w = want, acc = 0, bsel = 255;
+                for (uint32_t bkt = 0; bkt < 256; bkt++) {
+                    if (acc + hist[bkt] >= w) {
+                        bsel = bkt;
+                        break;
+                    }
+                    acc += hist[bkt];
+                }
+                want = w - acc;
+                prefix = pre | (bsel << shift);
+                // last digit: hist[bsel] entries carry the cut key
+                if (shift == 0 && hist[bsel] > want) ntie = hist[bsel];
+            }
+            __syncthreads();
+        }
+        const uint32_t kc = prefix;
+        const uint32_t nt = ntie;
+        if (nt > 0) {
+            // more entries carry the cut key than are taken: keep the `want` smallest labels
+            // (the order the exact lists use among equal distances)
+            if (nt > (uint32_t)RR_TIES) {
+                if (threadIdx.x == 0) overflow[q] = 1;
+                return;
+            }
+            for (uint32_t i = threadIdx.x; i < cnt; i += RR_THREADS)
+                if (f2key(cdq[i]) == kc) ties[atomicAdd(&tfill, 1u)] = labels[crq[i]];
+            __syncthreads();
+            uint32_t tp = 1;
+            while (tp < nt) tp <<= 1;
+            for (uint32_t i = nt + threadIdx.x; i < tp; i += RR_THREADS) ties[i] = ~0ull;
+            __syncthreads();
+            for (uint32_t kk = 2; kk <= tp; kk <<= 1)
+                for (uint32_t j = kk >> 1; j > 0; j >>= 1) {
+                    for (uint32_t i = threadIdx.x; i < tp; i += RR_THREADS) {
+                        const uint32_t pp = i ^ j;
+                        if (pp > i) {
+                            const uint64_t a = ties[i], b = ties[pp];
+                            if (((i & kk) == 0) ? a > b : a < b) {
+                                ties[i] = b;
+                                ties[pp] = a;
+                            }
                         }
                     }
+                    __syncthreads();
                 }
-                __syncthreads();
+            if (threadIdx.x == 0) {
+                cut_label = ties[want - 1];
+                cut_on = 1;
             }
-        m = c;
+            __syncthreads();
+        }
+        // compaction of the chosen rows
+        const uint64_t cl = cut_label;
+        const bool con = cut_on != 0;
+        for (uint32_t i0 = 0; i0 < cnt; i0 += RR_THREADS) {
+            const uint32_t i = i0 + threadIdx.x;
+            bool take = false;
+            uint32_t r = 0;
+            if (i < cnt) {
+                const uint32_t key = f2key(cdq[i]);
+                r = crq[i];
+                take = key < kc || (key == kc && (!con || labels[r] <= cl));
+            }
+            const uint64_t bm = __ballot(take);
+            uint32_t base = 0;
+            const int lane = threadIdx.x & 63;
+            if (lane == 0 && bm) base = atomicAdd(&nsel, (uint32_t)__popcll(bm));
+            base = (uint32_t)__shfl((int)base, 0);
+            if (take) {
+                const uint32_t slot = base + (uint32_t)__popcll(bm & ((lane ? ~0ull >> (64 - lane) : 0ull)));
+                if (slot < m)
+                    buf[slot] = SelEntry{0.f, r};
+                else
+                    bad = 1;
+            }
+        }
+        __syncthreads();
+        if (nsel != m || bad) {  // cannot happen when the select is right; the exact path answers
+            if (threadIdx.x == 0) overflow[q] = 1;
+            return;
+        }
+    } else {
+        for (uint32_t i = threadIdx.x; i < cnt; i += RR_THREADS) buf[i] = SelEntry{0.f, crq[i]};
+        __syncthreads();
     }
     // exact distances, four lanes per row (reference order)
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int grp = lane >> 2, sub = lane & 3;
     const int nch = (int)(dim >> 4);
-    for (uint32_t base = wave * 16; base < m; base += 4 * 16) {
-        uint32_t idx = base + grp;
-        bool act = idx < m;
-        uint32_t r = act ? buf[idx].r : 0u;
+    constexpr int NW = RR_THREADS / 64;
+    for (uint32_t base = wave * 16; base < m; base += NW * 16) {
+        const uint32_t idx = base + grp;
+        const bool act = idx < m;
+        const uint32_t r = act ? buf[idx].r : 0u;
         float res;
         dist_group16<SPACE, 0>(X + (size_t)r * dim, qs, nch, act, sub, res, lane);
         if (act && sub == 0) buf[idx].d = res;
     }
     __syncthreads();
-    sel_sort(buf, (int)m, labels);
+    sel_sort<RR_THREADS>(buf, (int)m, labels);
+    uint64_t* ol = out_l + (size_t)q * ld_out;
+    float* od = out_d + (size_t)q * ld_out;
     for (uint32_t i = threadIdx.x; i < k; i += RR_THREADS) {
         ol[i] = labels[buf[i].r];
         od[i] = buf[i].d;
@@ -1103,8 +1212,8 @@ hipError_t launch_bf_lowp(const BfData& d, int precision, const float* q, uint32
     float* smat = (float*)take((size_t)nq * srows * 4);
     uint32_t* crow = (uint32_t*)take((size_t)nq * RR_CAP * 4);
     float* cd = (float*)take((size_t)nq * RR_CAP * 4);
-    uint32_t* ccnt = (uint32_t*)take((size_t)nq * 4);
-    uint32_t* ovf = (uint32_t*)take((size_t)nq * 4);
+    uint32_t* ccnt = (uint32_t*)take((size_t)nq * 8);  // counts, then overflow flags
+    uint32_t* ovf = ccnt + nq;
     float* xnm = (float*)take((size_t)n * 4);
     const uint8_t* xlp = precision == 1 ? (const uint8_t*)d.vec_bf16 : d.vec_fp8;
     hipEvent_t ev[4];
@@ -1141,8 +1250,7 @@ hipError_t launch_bf_lowp(const BfData& d, int precision, const float* q, uint32
     double want = 1.2 * (double)c * (double)srows / (double)n;
     uint32_t rank = (uint32_t)std::min<double>((double)srows, std::max<double>((double)k + 1, std::ceil(want)));
     hipLaunchKernelGGL(kth_kernel, dim3(nq), dim3(256), 0, s, smat, (size_t)srows, srows, rank, tau);
-    (void)hipMemsetAsync(ccnt, 0, (size_t)nq * 4, s);
-    (void)hipMemsetAsync(ovf, 0, (size_t)nq * 4, s);
+    hipLaunchKernelGGL(fill_u32_kernel, dim3((2 * nq + 255) / 256), dim3(256), 0, s, ccnt, 2 * nq, 0u);
     (void)hipEventRecord(ev[1], s);
     // filter pass over every row
     ga.B = xlp;
@@ -1156,7 +1264,9 @@ hipError_t launch_bf_lowp(const BfData& d, int precision, const float* q, uint32
     ga.cap = RR_CAP;
     launch_gemm(ga, precision, true, s);
     (void)hipEventRecord(ev[2], s);
-    size_t lds = (size_t)RR_CAP * sizeof(SelEntry) + (size_t)dim * 4;
+    uint32_t cpow = 2;
+    while (cpow < c) cpow <<= 1;
+    size_t lds = (size_t)cpow * sizeof(SelEntry) + (size_t)((dim + 1) & ~1u) * 4 + (size_t)RR_TIES * 8;
     if (d.space == 0) {
         (void)hipFuncSetAttribute((const void*)rerank_kernel<0>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         hipLaunchKernelGGL(rerank_kernel<0>, dim3(nq), dim3(RR_THREADS), lds, s, q, d.vec, d.labels, dim, crow, cd,

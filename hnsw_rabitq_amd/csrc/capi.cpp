@@ -491,6 +491,25 @@ HG_API int hg_gpu_autotune(void* gv, const float* dq, size_t nq, size_t ef, size
     });
 }
 
+// Fixes the launch plan for the beam capacity of (ef, k), as the autotuner would store it.
+HG_API int hg_gpu_set_plan(void* gv, size_t ef, size_t k, uint32_t hash_bits, int workgroups_per_cu) {
+    return guarded([&] {
+        require(gv != nullptr, "null index");
+        auto* gi = static_cast<GpuIndex*>(gv);
+        require(k >= 1 && ef >= 1, "ef and k must be positive");
+        require(hash_bits >= 9 && hash_bits <= 16 && hash_bits >= ceil_log2(2 * (uint64_t)k),
+                "hash_bits must be in [max(9, log2(2k)), 16]");
+        require(workgroups_per_cu >= 1 && workgroups_per_cu <= 32, "workgroups_per_cu must be in [1, 32]");
+        std::lock_guard<std::mutex> lk(gi->mu);
+        size_t ef_eff = std::max(ef, k);
+        uint32_t cap = (uint32_t)((ef_eff + 63) / 64 * 64);
+        require(hg::search_lds_bytes(gi->g.dim, cap, hash_bits) <= 160 * 1024,
+                "beam and visited hash exceed the 160 KiB of LDS of a CU");
+        gi->tuned[cap] = std::make_pair(hash_bits, workgroups_per_cu);
+        return HG_OK;
+    });
+}
+
 // ---- hipGraph of a device search ----
 struct GraphHandle {
     hipGraph_t graph = nullptr;

@@ -199,6 +199,7 @@ HG_API int hg_bf_search_device(void* bv, const float* dq, size_t nq, size_t k, i
             return HG_OK;
         }
         require(b->dim % 64 == 0, "the reduced-precision GEMM needs dim to be a multiple of 64");
+        require(b->dim <= 8192, "the reduced-precision path supports dim up to 8192");
         size_t c = std::max(rerank_c, k);
         require(c <= 4096, "rerank_c above 4096 is not supported");
         c = std::min(c, b->count);

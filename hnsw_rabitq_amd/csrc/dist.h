@@ -156,6 +156,7 @@ __device__ __forceinline__ void dist_group16(const float* __restrict__ row, cons
         }
     }
     if (NCH == 0) {
+#pragma unroll 4
         for (int t = 0; t < n; t++) {
             float4 x = act ? r4[4 * t + g] : make_float4(0.f, 0.f, 0.f, 0.f);
             float4 q = *reinterpret_cast<const float4*>(qs + 16 * t + 4 * g);

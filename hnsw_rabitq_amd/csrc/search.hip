@@ -29,8 +29,9 @@
 
 namespace hg {
 
+// (da, ia) < (db, ib) in (distance, id) order; evaluated without branches.
 __device__ __forceinline__ bool key_less(float da, uint32_t ia, float db, uint32_t ib) {
-    return da < db || (da == db && ia < ib);
+    return (int)(da < db) | ((int)(da == db) & (int)(ia < ib));
 }
 
 // Distances of the c vectors whose ids are ids[0..c) to the query in LDS; results to out[].
@@ -91,7 +92,7 @@ __device__ __forceinline__ bool hash_insert(uint32_t* hash, uint32_t mask, uint3
 __device__ __forceinline__ int beam_lower_bound(const float* bd, const uint32_t* bi, int n, float d, uint32_t id) {
     int lo = 0, hi = n;
     while (lo < hi) {
-        int mid = (lo + hi) >> 1;
+        const int mid = (lo + hi) >> 1;
         if (key_less(bd[mid], bi[mid] & ID_MASK, d, id))
             lo = mid + 1;
         else
@@ -407,50 +408,45 @@ __global__ __launch_bounds__(64) void hnsw_search_kernel(SearchParams P) {
                         const int dst = ok ? r : m + lanes_below(~km, lane);
                         const int spos = __builtin_amdgcn_ds_permute(dst * 4, pos);
                         const int first_pos = __builtin_amdgcn_readlane(spos, 0);
-                        // Move beam entries [first_pos, n) right, last chunk first.  Entry i moves
-                        // by the number of staged keys below it = staged insertion points <= i.
-                        // Points of rank < lo are below the chunk base b; ranks [lo, hi) fall
-                        // inside the chunk (usually none).  A chunk's reads precede its writes
-                        // (in-order LDS within the wave) and a chunk writes only at or above its
-                        // own base, so chunks need no barrier between them.
+                        // Move beam entries [first_pos, n) right, highest chunks first, eight
+                        // chunks per batch with every read of a batch issued before its writes.
+                        // Entry i moves by the number of staged insertion points <= i.  A batch
+                        // writes only at or above its lowest chunk base, which the next (lower)
+                        // batch never reads, and a wave's LDS operations execute in order.
                         const int new_n = n + m < ef_eff ? n + m : ef_eff;
                         const int hi_chunk = (n - 1) & ~63;
                         const int lo_chunk = first_pos & ~63;
-                        int hi = m;
-                        // reads of the next (lower) chunk are issued before the writes of the
-                        // current one: the current chunk writes only at or above its base,
-                        // which the lower chunk never reads
-                        float vd = 0.f, nvd = 0.f;
-                        uint32_t vi = 0, nvi = 0;
-                        {
-                            const int i = hi_chunk + lane;
-                            if (i >= first_pos && i < n) {
-                                vd = bd[i];
-                                vi = bi[i];
+                        for (int b0 = hi_chunk; b0 >= lo_chunk; b0 -= 8 * 64) {
+                            float vd[8];
+                            uint32_t vi[8];
+                            int sh[8];
+#pragma unroll
+                            for (int u = 0; u < 8; u++) {
+                                const int i = b0 - 64 * u + lane;
+                                vd[u] = 0.f;
+                                vi[u] = 0;
+                                sh[u] = 0;
+                                if (b0 - 64 * u >= lo_chunk && i >= first_pos && i < n) {
+                                    vd[u] = bd[i];
+                                    vi[u] = bi[i];
+                                }
                             }
-                        }
-                        for (int b = hi_chunk; b >= lo_chunk; b -= 64) {
-                            int lo = hi;
-                            while (lo > 0 && __builtin_amdgcn_readlane(spos, lo - 1) >= b) lo--;
-                            const int i = b + lane;
-                            const bool mv = i >= first_pos && i < n;
-                            const int ni = i - 64;
-                            if (b - 64 >= lo_chunk && ni >= first_pos) {
-                                nvd = bd[ni];
-                                nvi = bi[ni];
-                            }
-                            int sh = lo;
-                            for (int rr = lo; rr < hi; rr++) sh += __builtin_amdgcn_readlane(spos, rr) <= i ? 1 : 0;
-                            const int np = i + sh;
-                            __asm__ volatile("" ::: "memory");
-                            if (mv && np < ef_eff) {
-                                bd[np] = vd;
-                                bi[np] = vi;
+                            for (int rr = 0; rr < m; rr++) {
+                                const int sp = __builtin_amdgcn_readlane(spos, rr);
+#pragma unroll
+                                for (int u = 0; u < 8; u++) sh[u] += sp <= b0 - 64 * u + lane ? 1 : 0;
                             }
                             __asm__ volatile("" ::: "memory");
-                            vd = nvd;
-                            vi = nvi;
-                            hi = lo;
+#pragma unroll
+                            for (int u = 0; u < 8; u++) {
+                                const int i = b0 - 64 * u + lane;
+                                const int np = i + sh[u];
+                                if (b0 - 64 * u >= lo_chunk && i >= first_pos && i < n && np < ef_eff) {
+                                    bd[np] = vd[u];
+                                    bi[np] = vi[u];
+                                }
+                            }
+                            __asm__ volatile("" ::: "memory");
                         }
                         const int fp = pos + r;
                         if (ok && fp < ef_eff) {

@@ -3,10 +3,10 @@ hnswlib.h:134-150, filled by a small network).
 
 Pipeline (all compute on the GPU through the C ABI):
   1. training queries from the benchmark generator with their own seed;
-  2. labels: the ef=1024 top-k of each training query (plain search);
-  3. traces: the plain search at the learned search's ef, recording every `period` level-0
-     expansions the 12 traversal features and the current top-k; a record is positive when its
-     top-k already equals the ef=1024 top-k as a set;
+  2. traces: the plain search at the traced ef, recording every `period` level-0 expansions the
+     12 traversal features and the current top-k, plus the search's final top-k;
+  3. labels: a record is positive when its top-k already equals the traced search's final top-k
+     as a set (the search has converged to what it will return);
   4. training: the project's HIP forward/backward/Adam kernels (hg_stop_model_train_step);
   5. inference: the search kernel evaluates the network every `period` expansions and stops a
      query when sigmoid(logit) >= threshold; otherwise the query runs to the plain fixed-ef end.
@@ -180,7 +180,7 @@ def run(gpu, H, lib, data_dim, queries, dq, gt, stream, timer, sweep, log, steps
     # candidate ef of the learned search: from just above the plain operating point up to the
     # traced ef (a larger beam costs LDS, hence concurrency)
     base = plain_op["ef"]
-    efs = sorted(set(min(ef_max, int(round(base * f / 64.0)) * 64) for f in (1.2, 1.6)) | {ef_l})
+    efs = sorted(set(min(ef_max, int(round(base * f / 64.0)) * 64) for f in (1.2, 1.6)) | {ef_l, base})
     for ef in efs:
         gpu.autotune(dq, nq, ef, k, stream=stream)
         for thr in (threshold, 0.5, 0.7, 0.8, 0.95):
@@ -202,9 +202,28 @@ def run(gpu, H, lib, data_dim, queries, dq, gt, stream, timer, sweep, log, steps
             # the configured confidence (0.9) is the reported operating rule
             if thr == threshold and rec >= 0.95 and (best is None or row["qps"] > best["qps"]):
                 best = row
+    # bound for any stop rule on these traces: each benchmark query stopped at the first record
+    # from which its top-k no longer changes (recall = that of the traced search)
+    feats, topk, nrec, final_ids = collect_traces(gpu, dq, nq, ef_l, k, period, max_records, stream)
+    eq, valid = label_records(topk, nrec, final_ids)
+    R = eq.shape[1]
+    # first record index after which every valid record is positive
+    tail_ok = np.flip(np.logical_and.accumulate(np.flip(eq | ~valid, axis=1), axis=1), axis=1)
+    first = np.where(tail_ok.any(axis=1), np.argmax(tail_ok, axis=1), R)
+    oracle_hops = np.minimum(first * period, nrec * period).astype(np.float64)
+    del feats, topk
+    gpu.search_device(dq, nq, k, ef_l, dl, dd, d_counters=dc, stream=stream)
+    lab = dl.to_numpy(np.uint64, (nq, k))
+    traced_recall = float(np.mean([len(np.intersect1d(a, b)) for a, b in zip(lab, gt)]) / k)
+    traced_hops = float(dc.to_numpy(np.int32, (nq, 4))[:, 0].mean())
+    log(f"  stop-rule bound: traced ef={ef_l} recall@10={traced_recall:.4f} hops={traced_hops:.0f}; stopping each "
+        f"query where its top-{k} stops changing needs {oracle_hops.mean():.0f} hops on average")
     out = dict(learned_table=table, learned_train_loss_first=float(np.mean(losses[:20])),
                learned_train_loss_last=float(np.mean(losses[-50:])), learned_records=int(X.shape[0]),
-               learned_positive_fraction=float(Y.mean()), learned_train_s=train_s, learned_trace_ef=ef_l)
+               learned_positive_fraction=float(Y.mean()), learned_train_s=train_s, learned_trace_ef=ef_l,
+               learned_bound_traced_recall=traced_recall, learned_bound_traced_hops=traced_hops,
+               learned_bound_converged_hops=float(oracle_hops.mean()),
+               plain_hops_at_op=float(plain_op["hops"]))
     if best is not None:
         out.update(learned_qps_at_095=best["qps"], learned_ef=best["ef"], learned_recall=best["recall"],
                    learned_over_plain=best["qps"] / plain_qps)

@@ -94,6 +94,10 @@ int hg_gpu_search_plan(void* gpu_index, size_t max_ef, size_t max_k, size_t nq, 
 /* Re-tune: time candidate (waves, hash) pairs on the given device-resident queries and keep the
  * fastest for this (max ef, max k) bucket. */
 int hg_gpu_autotune(void* gpu_index, const float* d_queries, size_t nq, size_t ef, size_t k, void* stream);
+/* Stores a launch plan (visited-hash slots = 1 << hash_bits, persistent workgroups per CU) for
+ * the beam capacity of (ef, k), in place of an autotuned one.  No counterpart in the reference:
+ * a tuning knob of the GPU search for callers that already know their plan. */
+int hg_gpu_set_plan(void* gpu_index, size_t ef, size_t k, uint32_t hash_bits, int workgroups_per_cu);
 
 /* ---- Brute force ---- */
 int hg_bf_create(int space, size_t dim, size_t max_elements, int device, void** out_bf);

@@ -78,3 +78,41 @@ def test_bf_lowp_matches_exact(hg, space, prec):
     np.testing.assert_array_equal(ll, el)
     np.testing.assert_array_equal(ld.view(np.uint32), ed.view(np.uint32))
     print(f"stats {stats}", flush=True)
+
+
+@pytest.mark.parametrize("prec", ["bf16", "fp8"])
+def test_bf_lowp_ties_at_rerank_cut(hg, prec):
+    """Duplicated rows tie in the approximate distance at the re-rank cut.  Queries 0-31 sit on a
+    group of 1,500 copies with c = 1,000: the cut falls inside the group and keeps the smallest
+    labels (labels are shuffled), so the lists still equal the exact path's.  Queries 32-63 sit
+    on a group of 3,000 copies, more ties than the re-rank resolves: those queries take the
+    exact path (reported as retried)."""
+    from hnsw_rabitq_amd import datasets, DeviceArray, _lib
+    import ctypes
+    print(f"bf lowp ties prec={prec}", flush=True)
+    data, q = datasets.c3_gmm(n=20_000, nq=64, dim=128, n_centres=16)
+    rng = np.random.default_rng(5)
+    data[:1500] = data[0]
+    data[5000:8000] = data[5000]
+    q[:32] = data[0] + 1e-3 * rng.standard_normal((32, 128)).astype(np.float32)
+    q[32:] = data[5000] + 1e-3 * rng.standard_normal((32, 128)).astype(np.float32)
+    labels = (rng.permutation(20_000) + 7).astype(np.uint64)
+    b = _bf(hg, "ip", data, labels)
+    k = 100
+    el, ed = b.knn_query(q, k=k)
+    dq = DeviceArray.from_numpy(q)
+    dl = DeviceArray(q.shape[0] * k * 8)
+    dd = DeviceArray(q.shape[0] * k * 4)
+    stats = np.zeros(4, np.float64)
+    P = {"bf16": _lib.PREC_BF16, "fp8": _lib.PREC_FP8}[prec]
+    _lib.call("hg_bf_search_device", b._h, dq.ptr, q.shape[0], k, P, 1000, dl.ptr, dd.ptr,
+              ctypes.c_void_p(stats.ctypes.data), None)
+    _lib.call("hg_device_synchronize")
+    ll = dl.to_numpy(np.uint64, (q.shape[0], k))
+    ld = dd.to_numpy(np.float32, (q.shape[0], k))
+    print(f"stats {stats}", flush=True)
+    np.testing.assert_array_equal(ll, el)
+    np.testing.assert_array_equal(ld.view(np.uint32), ed.view(np.uint32))
+    # the first group's lists are its 100 smallest labels
+    np.testing.assert_array_equal(ll[0], np.sort(labels[:1500])[:k])
+    assert stats[3] == 32
