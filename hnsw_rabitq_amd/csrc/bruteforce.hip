@@ -385,31 +385,32 @@ struct LowpGemmArgs {
 };
 
 // Per-row values of one workgroup tile, staged in LDS for the epilogue.
-template <int BM>
-struct EpiLds {
+template <int BM, int WN>
+struct __align__(16) EpiLds {
     float tau[BM], a_norm[BM], a_scale[BM];
-    uint32_t cnt[2][BM];  // passes of each row in the column half of wave wn
-    uint32_t base[BM];    // start of the row's reserved candidate slots
+    uint32_t cnt[WN][BM];  // passes of each row in the columns of wave wn
+    uint32_t base[BM];     // start of the row's reserved candidate slots
 };
 
 // Epilogue of the GEMM variants.  The workgroup tile is rows [m0, m0 + 64*TM) x columns
-// [n0, n0 + 64*TN); wave (wm, wn) holds TM x TN 32x32 blocks in acc, so each of its 32*TM rows
-// sits in one (block row i, register r, half-wave h) and the two waves wn = 0, 1 share a row.
+// [n0, n0 + 32*TN*WN); wave (wm, wn) holds TM x TN 32x32 blocks in acc, so each of its 32*TM
+// rows sits in one (block row i, register r, half-wave h) and the WN waves wn share a row.
 // Sample mode writes every approximate distance.  Filter mode appends the columns with
 // d < tau[row] to the row's candidate list with one device-scope atomic per (row, workgroup):
 // pass 1 counts each row's passes (ballot + popcount), one thread per row reserves the row's
-// total, pass 2 recomputes d and writes each pass at the row base + the other wave's count +
-// its rank among the row's passes.
-template <int PREC, bool FILTER, int TM, int TN>
-__device__ __forceinline__ void gemm_epilogue(const LowpGemmArgs& g, f32x16_t (&acc)[TM][TN], uint32_t m0,
-                                              uint32_t n0, int wm, int wn, int lane, int tid,
-                                              EpiLds<64 * TM>& e) {
+// total, pass 2 recomputes d and writes each pass at the row base + the counts of the waves
+// before it + its rank among the row's passes.  NT threads; two waves along the rows.  SPACE
+// is fixed at compile time so that the unrolled loops carry no branch.
+template <int PREC, bool FILTER, int TM, int TN, int WN, int NT, int SPACE>
+__device__ __forceinline__ void gemm_epilogue_s(const LowpGemmArgs& g, f32x16_t (&acc)[TM][TN], uint32_t m0,
+                                                uint32_t n0, int wm, int wn, int lane, int tid,
+                                                EpiLds<64 * TM, WN>& e) {
     constexpr int BM = 64 * TM;
-    for (int r = tid; r < BM; r += 256) {
+    for (int r = tid; r < BM; r += NT) {
         const uint32_t row = m0 + r;
         const bool ok = row < g.nq;
         e.tau[r] = (FILTER && ok) ? g.tau[row] : -INFINITY;
-        e.a_norm[r] = (g.space == 0 && ok) ? g.a_norm[row] : 0.f;
+        e.a_norm[r] = (SPACE == 0 && ok) ? g.a_norm[row] : 0.f;
         e.a_scale[r] = (PREC == 2 && ok) ? g.a_scale[row] : 1.f;
     }
     __syncthreads();
@@ -421,12 +422,13 @@ __device__ __forceinline__ void gemm_epilogue(const LowpGemmArgs& g, f32x16_t (&
         col[j] = n0 + wn * 32 * TN + j * 32 + (lane & 31);
         col_ok[j] = col[j] < g.n;
         bsc[j] = (PREC == 2 && col_ok[j]) ? g.b_scale[col[j]] : 1.0f;
-        bn[j] = (g.space == 0 && col_ok[j]) ? g.b_norm[col[j]] : 0.f;
+        bn[j] = (SPACE == 0 && col_ok[j]) ? g.b_norm[col[j]] : 0.f;
     }
     auto dist = [&](int i, int j, int r, int lr) {
         float sv = acc[i][j][r];
         if (PREC == 2) sv = sv * e.a_scale[lr] * bsc[j];
-        return g.space == 0 ? e.a_norm[lr] + bn[j] - 2.0f * sv : 1.0f - sv;
+        if (SPACE == 0) return e.a_norm[lr] + bn[j] - 2.0f * sv;
+        return 1.0f - sv;
     };
     const int half = lane >> 5;
     if (!FILTER) {
@@ -443,23 +445,45 @@ __device__ __forceinline__ void gemm_epilogue(const LowpGemmArgs& g, f32x16_t (&
         return;
     }
     // pass 1: per-row pass counts of this wave (each row belongs to one half-wave, so lanes 0
-    // and 32 store them without atomics)
+    // and 32 store them without atomics); `hit` marks the (i, r) pairs with any pass (the
+    // counts are wave-uniform, so the mask lives in scalar registers)
+    uint64_t hit = 0, colm[TN];
+#pragma unroll
+    for (int j = 0; j < TN; j++) colm[j] = __builtin_amdgcn_ballot_w64(col_ok[j]);
 #pragma unroll
     for (int i = 0; i < TM; i++)
 #pragma unroll
-        for (int r = 0; r < 16; r++) {
-            const int lr = wm * 32 * TM + i * 32 + (r >> 2) * 8 + half * 4 + (r & 3);
-            const float t = e.tau[lr];
-            uint32_t c = 0;
+        for (int rq = 0; rq < 4; rq++) {
+            // rows lr0 + rr (half-wave 0) and lr0 + 4 + rr (half-wave 1), rr = 0..3
+            const int lr0 = wm * 32 * TM + i * 32 + rq * 8;
+            const float4 t4 = *reinterpret_cast<const float4*>(&e.tau[lr0 + half * 4]);
+            const float tq[4] = {t4.x, t4.y, t4.z, t4.w};
+            uint32_t clo[4], chi[4];
 #pragma unroll
-            for (int j = 0; j < TN; j++)
-                c += (uint32_t)__popc((uint32_t)(__ballot(col_ok[j] && dist(i, j, r, lr) < t) >> (32 * half)));
-            if ((lane & 31) == 0) e.cnt[wn][lr] = c;
+            for (int rr = 0; rr < 4; rr++) {
+                const int r = rq * 4 + rr, lr = lr0 + half * 4 + rr;
+                uint32_t lo = 0, hi = 0;
+#pragma unroll
+                for (int j = 0; j < TN; j++) {
+                    const uint64_t m = __builtin_amdgcn_ballot_w64(dist(i, j, r, lr) < tq[rr]) & colm[j];
+                    lo += (uint32_t)__popcll(m & 0xFFFFFFFFull);
+                    hi += (uint32_t)__popcll(m >> 32);
+                }
+                clo[rr] = lo;
+                chi[rr] = hi;
+                if (lo + hi) hit |= 1ull << (i * 16 + r);
+            }
+            if (lane == 0) {
+                *reinterpret_cast<uint4*>(&e.cnt[wn][lr0]) = make_uint4(clo[0], clo[1], clo[2], clo[3]);
+                *reinterpret_cast<uint4*>(&e.cnt[wn][lr0 + 4]) = make_uint4(chi[0], chi[1], chi[2], chi[3]);
+            }
         }
     __syncthreads();
     // one reservation per row
-    for (int r = tid; r < BM; r += 256) {
-        const uint32_t c = e.cnt[0][r] + e.cnt[1][r];
+    for (int r = tid; r < BM; r += NT) {
+        uint32_t c = 0;
+#pragma unroll
+        for (int w = 0; w < WN; w++) c += e.cnt[w][r];
         if (c) e.base[r] = atomicAdd(&g.cand_cnt[m0 + r], c);
     }
     __syncthreads();
@@ -469,22 +493,22 @@ __device__ __forceinline__ void gemm_epilogue(const LowpGemmArgs& g, f32x16_t (&
     for (int i = 0; i < TM; i++)
 #pragma unroll
         for (int r = 0; r < 16; r++) {
+            if (!((hit >> (i * 16 + r)) & 1ull)) continue;  // no pass in this pair: scalar skip
             const int lr = wm * 32 * TM + i * 32 + (r >> 2) * 8 + half * 4 + (r & 3);
             const uint32_t row = m0 + lr;
             const float t = e.tau[lr];
             float d[TN];
             bool pass[TN];
             uint32_t hm[TN];
-            bool any = false;
 #pragma unroll
             for (int j = 0; j < TN; j++) {
                 d[j] = dist(i, j, r, lr);
-                pass[j] = col_ok[j] && d[j] < t;
+                pass[j] = (int)col_ok[j] & (int)(d[j] < t);
                 hm[j] = (uint32_t)(__ballot(pass[j]) >> (32 * half));
-                any |= hm[j] != 0;
             }
-            if (!any) continue;
-            uint32_t off = e.base[lr] + (wn ? e.cnt[0][lr] : 0u);
+            uint32_t off = e.base[lr];
+#pragma unroll
+            for (int w = 0; w < WN - 1; w++) off += w < wn ? e.cnt[w][lr] : 0u;
 #pragma unroll
             for (int j = 0; j < TN; j++) {
                 if (pass[j]) {
@@ -497,6 +521,16 @@ __device__ __forceinline__ void gemm_epilogue(const LowpGemmArgs& g, f32x16_t (&
                 off += (uint32_t)__popc(hm[j]);
             }
         }
+}
+
+template <int PREC, bool FILTER, int TM, int TN, int WN, int NT>
+__device__ __forceinline__ void gemm_epilogue(const LowpGemmArgs& g, f32x16_t (&acc)[TM][TN], uint32_t m0,
+                                              uint32_t n0, int wm, int wn, int lane, int tid,
+                                              EpiLds<64 * TM, WN>& e) {
+    if (g.space == 0)
+        gemm_epilogue_s<PREC, FILTER, TM, TN, WN, NT, 0>(g, acc, m0, n0, wm, wn, lane, tid, e);
+    else
+        gemm_epilogue_s<PREC, FILTER, TM, TN, WN, NT, 1>(g, acc, m0, n0, wm, wn, lane, tid, e);
 }
 
 template <int PREC, bool FILTER>
@@ -593,8 +627,8 @@ __global__ __launch_bounds__(256, 2) void lowp_gemm_kernel(LowpGemmArgs g) {
         }
     }
 
-    __shared__ EpiLds<64 * 2> epi;
-    gemm_epilogue<PREC, FILTER, 2, 2>(g, acc, m0, n0, wm, wn, lane, tid, epi);
+    __shared__ EpiLds<64 * 2, 2> epi;
+    gemm_epilogue<PREC, FILTER, 2, 2, 2, 256>(g, acc, m0, n0, wm, wn, lane, tid, epi);
 }
 
 // Larger tiles: each wave owns (32*TM) x (32*TN) outputs (TM x TN MFMA blocks), the workgroup
@@ -701,8 +735,8 @@ __global__ __launch_bounds__(256, 2) void lowp_gemm_wide_kernel(LowpGemmArgs g) 
         }
     }
 
-    __shared__ EpiLds<64 * TM> epi;
-    gemm_epilogue<PREC, FILTER, TM, TN>(g, acc, m0, n0, wm, wn, lane, tid, epi);
+    __shared__ EpiLds<64 * TM, 2> epi;
+    gemm_epilogue<PREC, FILTER, TM, TN, 2, 256>(g, acc, m0, n0, wm, wn, lane, tid, epi);
 }
 
 // Direct-to-LDS pipeline: rows are fetched with global_load_lds_dwordx4 (64 lanes x 16 bytes
@@ -712,7 +746,10 @@ __global__ __launch_bounds__(256, 2) void lowp_gemm_wide_kernel(LowpGemmArgs g) 
 // LDS banks.  Workgroup tile 256 x 128 (4 waves of 4 x 2 MFMA 32x32 blocks).
 constexpr int LKB = 64, LNST = 3;
 
-__device__ __forceinline__ int swz(int row, int chunk) { return row * LKB + ((chunk ^ ((row >> 1) & 3)) << 4); }
+// 16-byte chunk permutation of a 64-byte row: the rows read together by 8 (or 16) lanes of a
+// ds_read_b128 cover every bank once with 32 (or 64) four-byte banks.
+__device__ __forceinline__ int swzf(int row) { return ((row >> 1) & 3) ^ ((row >> 3) & 1); }
+__device__ __forceinline__ int swz(int row, int chunk) { return row * LKB + ((chunk ^ swzf(row)) << 4); }
 
 // Three slices per operand as distinct LDS objects, so the compiler's wait-count pass can tell
 // which in-flight direct-to-LDS loads an LDS read depends on (only the slice being read).
@@ -722,29 +759,29 @@ struct LdsRing {
     __attribute__((address_space(3))) uint8_t* b[LNST];
 };
 
-template <int PREC, int TM, int TN>
+template <int PREC, int TM, int TN>  // TM: A blocks of 16 rows per wave, TN: B blocks per wave
 __device__ __forceinline__ void gemm_lds_issue(const LowpGemmArgs& g, __attribute__((address_space(3))) uint8_t* dA,
                                                __attribute__((address_space(3))) uint8_t* dB, int st, uint32_t m0,
                                                uint32_t n0, size_t row_bytes, int wave, int lane) {
     const int lrow = lane >> 2, lslot = lane & 3;
     const size_t kb = (size_t)st * LKB;
 #pragma unroll
-    for (int i = 0; i < TM; i++) {  // A: 4*TM blocks of 16 rows, TM per wave
+    for (int i = 0; i < TM; i++) {  // A: TM blocks of 16 rows per wave
         const int blk = wave * TM + i;
         const int row = blk * 16 + lrow;
         uint32_t q = m0 + row;
         q = q < g.nq ? q : g.nq - 1;
-        const int c = lslot ^ ((row >> 1) & 3);
+        const int c = lslot ^ swzf(row);
         __builtin_amdgcn_global_load_lds((const void*)(g.A + (size_t)q * row_bytes + kb + c * 16), dA + blk * 1024, 16,
                                          0, 0);
     }
 #pragma unroll
-    for (int i = 0; i < TN; i++) {  // B: 4*TN blocks of 16 rows, TN per wave
+    for (int i = 0; i < TN; i++) {  // B: TN blocks of 16 rows per wave
         const int blk = wave * TN + i;
         const int row = blk * 16 + lrow;
         uint32_t r = n0 + row;
         r = r < g.n ? r : g.n - 1;
-        const int c = lslot ^ ((row >> 1) & 3);
+        const int c = lslot ^ swzf(row);
         __builtin_amdgcn_global_load_lds((const void*)(g.B + (size_t)r * row_bytes + kb + c * 16), dB + blk * 1024, 16,
                                          0, 0);
     }
@@ -816,21 +853,25 @@ __device__ __forceinline__ void gemm_lds_mfma(uint32_t A, uint32_t B, f32x16_t (
     }
 }
 
-template <int PREC, bool FILTER>
-__global__ __launch_bounds__(256, 2) void lowp_gemm_lds_kernel(LowpGemmArgs g) {
-    constexpr int TM = 4, TN = 2, BM = 64 * TM, BN = 64 * TN;
+// WNW waves along the columns (2: 256 x 128 tile, 256 threads, two workgroups per CU; 4: 256 x 256
+// tile, 512 threads, one workgroup per CU and half the operand bytes per MFMA of the former).
+template <int PREC, bool FILTER, int WNW>
+__global__ __launch_bounds__(128 * WNW, WNW == 2 ? 2 : 1) void lowp_gemm_lds_kernel(LowpGemmArgs g) {
+    constexpr int NT = 128 * WNW, NW = 2 * WNW;
+    constexpr int TM = 4, TN = 2, BM = 64 * TM, BN = 32 * TN * WNW;
+    constexpr int APW = BM / 16 / NW, BPW = BN / 16 / NW;  // 16-row blocks per wave and slice
     __shared__ __align__(1024) uint8_t sA0[BM * LKB], sA1[BM * LKB], sA2[BM * LKB];
     __shared__ __align__(1024) uint8_t sB0[BN * LKB], sB1[BN * LKB], sB2[BN * LKB];
     typedef __attribute__((address_space(3))) uint8_t* lp;
     lp A0 = (lp)sA0, A1 = (lp)sA1, A2 = (lp)sA2, B0 = (lp)sB0, B1 = (lp)sB1, B2 = (lp)sB2;
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-    const int wm = wave >> 1, wn = wave & 1;
+    const int wm = wave / WNW, wn = wave % WNW;
     uint32_t tm, tn;
     if (!TileOrder::make((g.nq + BM - 1) / BM, (g.n + BN - 1) / BN).map(blockIdx.x, tm, tn)) return;
     const uint32_t m0 = tm * BM, n0 = tn * BN;
     const size_t row_bytes = (size_t)g.dim * (PREC == 1 ? 2 : 1);
     const int nst = (int)(row_bytes / LKB);
-    constexpr int PER_STAGE = TM + TN;  // direct-to-LDS loads per wave per slice
+    constexpr int PER_STAGE = APW + BPW;  // direct-to-LDS loads per wave per slice
 
     f32x16_t acc[TM][TN];
 #pragma unroll
@@ -840,8 +881,8 @@ __global__ __launch_bounds__(256, 2) void lowp_gemm_lds_kernel(LowpGemmArgs g) {
 #pragma unroll
             for (int r = 0; r < 16; r++) acc[i][j][r] = 0.f;
 
-    gemm_lds_issue<PREC, TM, TN>(g, A0, B0, 0, m0, n0, row_bytes, wave, lane);
-    if (nst > 1) gemm_lds_issue<PREC, TM, TN>(g, A1, B1, 1, m0, n0, row_bytes, wave, lane);
+    gemm_lds_issue<PREC, APW, BPW>(g, A0, B0, 0, m0, n0, row_bytes, wave, lane);
+    if (nst > 1) gemm_lds_issue<PREC, APW, BPW>(g, A1, B1, 1, m0, n0, row_bytes, wave, lane);
     // one step: wait for slice st (the loads of slice st+1 may stay in flight), barrier, issue
     // slice st+2 into the ring slot freed by slice st-1, MFMAs on slice st
     auto step = [&](int st, lp rA, lp rB, lp nA, lp nB) {
@@ -850,7 +891,7 @@ __global__ __launch_bounds__(256, 2) void lowp_gemm_lds_kernel(LowpGemmArgs g) {
         else
             __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt == 0
         __builtin_amdgcn_s_barrier();
-        if (st + 2 < nst) gemm_lds_issue<PREC, TM, TN>(g, nA, nB, st + 2, m0, n0, row_bytes, wave, lane);
+        if (st + 2 < nst) gemm_lds_issue<PREC, APW, BPW>(g, nA, nB, st + 2, m0, n0, row_bytes, wave, lane);
         gemm_lds_mfma<PREC, TM, TN>((uint32_t)(uintptr_t)rA, (uint32_t)(uintptr_t)rB, acc, wm, wn, lane);
     };
     for (int st = 0; st < nst; st += 3) {
@@ -859,8 +900,8 @@ __global__ __launch_bounds__(256, 2) void lowp_gemm_lds_kernel(LowpGemmArgs g) {
         if (st + 2 < nst) step(st + 2, A2, B2, A1, B1);
     }
 
-    __shared__ EpiLds<64 * TM> epi;
-    gemm_epilogue<PREC, FILTER, TM, TN>(g, acc, m0, n0, wm, wn, lane, tid, epi);
+    __shared__ EpiLds<64 * TM, WNW> epi;
+    gemm_epilogue<PREC, FILTER, TM, TN, WNW, NT>(g, acc, m0, n0, wm, wn, lane, tid, epi);
 }
 
 // Copy every `step`-th row of a low-precision matrix (and its scale / norm) to a compact sample.
@@ -1168,16 +1209,27 @@ static void launch_gemm(LowpGemmArgs ga, int precision, bool filter, hipStream_t
         }
         return;
     }
+    if (g_gemm_variant == 3) {
+        uint32_t grid = TileOrder::make((ga.nq + 255) / 256, (ga.n + 255) / 256).grid();
+        if (precision == 1) {
+            if (filter) hipLaunchKernelGGL((lowp_gemm_lds_kernel<1, true, 4>), dim3(grid), dim3(512), 0, s, ga);
+            else hipLaunchKernelGGL((lowp_gemm_lds_kernel<1, false, 4>), dim3(grid), dim3(512), 0, s, ga);
+        } else {
+            if (filter) hipLaunchKernelGGL((lowp_gemm_lds_kernel<2, true, 4>), dim3(grid), dim3(512), 0, s, ga);
+            else hipLaunchKernelGGL((lowp_gemm_lds_kernel<2, false, 4>), dim3(grid), dim3(512), 0, s, ga);
+        }
+        return;
+    }
     constexpr int TM = 4, TN = 2;
     uint32_t tm = (ga.nq + 64 * TM - 1) / (64 * TM), tn = (ga.n + 64 * TN - 1) / (64 * TN);
     uint32_t grid = TileOrder::make(tm, tn).grid();
     if (g_gemm_variant == 2) {
         if (precision == 1) {
-            if (filter) hipLaunchKernelGGL((lowp_gemm_lds_kernel<1, true>), dim3(grid), dim3(256), 0, s, ga);
-            else hipLaunchKernelGGL((lowp_gemm_lds_kernel<1, false>), dim3(grid), dim3(256), 0, s, ga);
+            if (filter) hipLaunchKernelGGL((lowp_gemm_lds_kernel<1, true, 2>), dim3(grid), dim3(256), 0, s, ga);
+            else hipLaunchKernelGGL((lowp_gemm_lds_kernel<1, false, 2>), dim3(grid), dim3(256), 0, s, ga);
         } else {
-            if (filter) hipLaunchKernelGGL((lowp_gemm_lds_kernel<2, true>), dim3(grid), dim3(256), 0, s, ga);
-            else hipLaunchKernelGGL((lowp_gemm_lds_kernel<2, false>), dim3(grid), dim3(256), 0, s, ga);
+            if (filter) hipLaunchKernelGGL((lowp_gemm_lds_kernel<2, true, 2>), dim3(grid), dim3(256), 0, s, ga);
+            else hipLaunchKernelGGL((lowp_gemm_lds_kernel<2, false, 2>), dim3(grid), dim3(256), 0, s, ga);
         }
         return;
     }

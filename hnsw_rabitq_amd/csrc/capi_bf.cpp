@@ -229,7 +229,7 @@ HG_API int hg_bf_fp64_check(void* bv, const float* dq, size_t nq, size_t k, cons
 
 HG_API int hg_bf_gemm_variant(int variant) {
     return guarded([&] {
-        require(variant >= 0 && variant <= 2, "GEMM variant must be 0, 1 or 2");
+        require(variant >= 0 && variant <= 3, "GEMM variant must be 0, 1, 2 or 3");
         hg::set_gemm_variant(variant);
         return HG_OK;
     });

@@ -156,34 +156,77 @@ __device__ __forceinline__ float bf16_round(float x) {
     return __uint_as_float(u & 0xFFFF0000u);
 }
 
-__device__ float mlp_logit(const DevStopModel& m, const float* feat, float* hbuf, int lane) {
-    const int in = m.in_dim, H = m.hidden;
+// The network's BF16 weights held in registers for the life of a persistent wave: lane j keeps
+// hidden unit j's rows of W1 (kStopFeatures values) and W2 (64 values, two per register) and
+// its biases and output weight.  Loaded once; an evaluation then reads only LDS.
+struct MlpRegs {
+    uint32_t w1[kStopFeatures / 2];
+    uint32_t w2[32];
+    float b1, b2, w3, b3;
+};
+
+__device__ __forceinline__ uint32_t bf16_bits(float x) { return __float_as_uint(bf16_round(x)) >> 16; }
+__device__ __forceinline__ float bf16_lo(uint32_t p) { return __uint_as_float(p << 16); }
+__device__ __forceinline__ float bf16_hi(uint32_t p) { return __uint_as_float(p & 0xFFFF0000u); }
+
+__device__ __forceinline__ void mlp_load(const DevStopModel& m, int lane, MlpRegs& w) {
+    const int H = m.hidden;
     const float* W1 = m.params;
-    const float* b1 = W1 + H * in;
+    const float* b1 = W1 + H * kStopFeatures;
     const float* W2 = b1 + H;
     const float* b2 = W2 + H * H;
     const float* w3 = b2 + H;
-    const float* b3 = w3 + H;
+    const bool u = lane < H;
+#pragma unroll
+    for (int i = 0; i < kStopFeatures / 2; i++) {
+        const float a = u ? W1[lane * kStopFeatures + 2 * i] : 0.f;
+        const float b = u ? W1[lane * kStopFeatures + 2 * i + 1] : 0.f;
+        w.w1[i] = bf16_bits(a) | (bf16_bits(b) << 16);
+    }
+#pragma unroll
+    for (int i = 0; i < 32; i++) {
+        const float a = u && 2 * i < H ? W2[lane * H + 2 * i] : 0.f;
+        const float b = u && 2 * i + 1 < H ? W2[lane * H + 2 * i + 1] : 0.f;
+        w.w2[i] = bf16_bits(a) | (bf16_bits(b) << 16);
+    }
+    w.b1 = u ? b1[lane] : 0.f;
+    w.b2 = u ? b2[lane] : 0.f;
+    w.w3 = u ? bf16_round(w3[lane]) : 0.f;
+    w.b3 = w3[H];
+}
+
+// Logit of the network for the (normalised) features in LDS; the same operation order as the
+// training forward pass (sequential FMAs over inputs, output accumulated in unit order).
+__device__ float mlp_logit(const DevStopModel& m, const MlpRegs& w, const float* feat, float* hbuf, int lane) {
+    const int H = m.hidden;
     if (lane < H) {
-        float s = b1[lane];
-        for (int i = 0; i < in; i++) s = fma_(bf16_round(W1[lane * in + i]), feat[i], s);
+        float s = w.b1;
+#pragma unroll
+        for (int i = 0; i < kStopFeatures / 2; i++) {
+            s = fma_(bf16_lo(w.w1[i]), feat[2 * i], s);
+            s = fma_(bf16_hi(w.w1[i]), feat[2 * i + 1], s);
+        }
         hbuf[lane] = gelu_tanh(s);
     }
     __syncthreads();
     float h2 = 0.f;
     if (lane < H) {
-        float s = b2[lane];
-        for (int i = 0; i < H; i++) s = fma_(bf16_round(W2[lane * H + i]), hbuf[i], s);
+        float s = w.b2;
+#pragma unroll
+        for (int i = 0; i < 32; i++) {
+            if (2 * i < H) s = fma_(bf16_lo(w.w2[i]), hbuf[2 * i], s);
+            if (2 * i + 1 < H) s = fma_(bf16_hi(w.w2[i]), hbuf[2 * i + 1], s);
+        }
         h2 = gelu_tanh(s);
     }
     __syncthreads();
     if (lane < H) hbuf[lane] = h2;
     __syncthreads();
-    // output layer accumulated in unit order, as the training forward pass does
     float o = 0.f;
-    for (int j = 0; j < H; j++) o = fma_(hbuf[j], bf16_round(w3[j]), o);
+    for (int j = 0; j < H; j++)
+        o = fma_(hbuf[j], __int_as_float(__builtin_amdgcn_readlane(__float_as_int(w.w3), j)), o);
     __syncthreads();
-    return o + b3[0];
+    return o + w.b3;
 }
 
 // ------------------------------------------------------------------------------------------
@@ -210,6 +253,8 @@ __global__ __launch_bounds__(64) void hnsw_search_kernel(SearchParams P) {
     float* hbuf = feat + 16;
 
     __shared__ uint32_t s_slot;
+    MlpRegs mw;
+    if (STOP) mlp_load(P.stop, lane, mw);
 
     for (;;) {
         if (lane == 0) s_slot = atomicAdd(P.work, 1u);
@@ -508,7 +553,7 @@ __global__ __launch_bounds__(64) void hnsw_search_kernel(SearchParams P) {
                         if (lane < kStopFeatures)
                             feat[lane] = (feat[lane] - P.stop.mean[lane]) * P.stop.inv_std[lane];
                         __syncthreads();
-                        float logit = mlp_logit(P.stop, feat, hbuf, lane);
+                        float logit = mlp_logit(P.stop, mw, feat, hbuf, lane);
                         if (logit >= P.stop.logit_threshold) stopped = true;
                     }
                     __syncthreads();

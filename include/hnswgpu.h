@@ -122,7 +122,8 @@ int hg_bf_fp64_check(void* bf, const float* d_queries, size_t nq, size_t k, cons
 /* Tile configuration of the reduced-precision distance GEMM (process-wide): 0 = 128x128
  * workgroup tiles with 2x2 MFMA blocks per wave and double-buffered LDS; 1 = 256x128 tiles
  * with 4x2 blocks per wave and a register-prefetched LDS stage; 2 (default) = 256x128 tiles
- * fed by direct-to-LDS loads through a ring of three k-slices. */
+ * fed by direct-to-LDS loads through a ring of three k-slices; 3 = 256x256 tiles, eight waves,
+ * the same direct-to-LDS ring. */
 int hg_bf_gemm_variant(int variant);
 
 /* ---- Learned early termination (stop model) ---- */

@@ -116,3 +116,29 @@ def test_bf_lowp_ties_at_rerank_cut(hg, prec):
     # the first group's lists are its 100 smallest labels
     np.testing.assert_array_equal(ll[0], np.sort(labels[:1500])[:k])
     assert stats[3] == 32
+
+
+@pytest.mark.parametrize("variant", [0, 1, 2, 3])
+def test_bf_lowp_gemm_variants_agree(hg, variant):
+    """Every GEMM tile configuration gives the exact path's lists (BF16 and FP8, inner product,
+    a row count that is not a multiple of any tile and a dim of 320 = five 64-element slices)."""
+    from hnsw_rabitq_amd import datasets, DeviceArray, _lib
+    import ctypes
+    print(f"bf lowp gemm variant={variant}", flush=True)
+    data, q = datasets.c3_gmm(n=30_011, nq=517, dim=320, n_centres=32)
+    b = _bf(hg, "ip", data)
+    k = 50
+    el, ed = b.knn_query(q, k=k)
+    dq = DeviceArray.from_numpy(q)
+    try:
+        _lib.call("hg_bf_gemm_variant", variant)
+        for P in (_lib.PREC_BF16, _lib.PREC_FP8):
+            dl = DeviceArray(q.shape[0] * k * 8)
+            dd = DeviceArray(q.shape[0] * k * 4)
+            _lib.call("hg_bf_search_device", b._h, dq.ptr, q.shape[0], k, P, 1024, dl.ptr, dd.ptr, None, None)
+            _lib.call("hg_device_synchronize")
+            np.testing.assert_array_equal(dl.to_numpy(np.uint64, (q.shape[0], k)), el)
+            np.testing.assert_array_equal(dd.to_numpy(np.float32, (q.shape[0], k)).view(np.uint32),
+                                          ed.view(np.uint32))
+    finally:
+        _lib.call("hg_bf_gemm_variant", 2)
