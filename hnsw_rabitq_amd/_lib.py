@@ -38,12 +38,14 @@ SIGNATURES = {
     "hg_gpu_search_plan": [_vp, _sz, _sz, _sz, _vp],
     "hg_gpu_autotune": [_vp, _vp, _sz, _sz, _sz, _vp],
     "hg_gpu_set_plan": [_vp, _sz, _sz, ctypes.c_uint32, _i32],
+    "hg_gpu_id_labels": [_vp, _vp, _sz, _vp, _vp],
     "hg_bf_create": [_i32, _sz, _sz, _i32, _pvp],
     "hg_bf_add": [_vp, _vp, _sz, _vp],
     "hg_bf_count": [_vp, ctypes.POINTER(_sz)],
     "hg_bf_free": [_vp],
     "hg_bf_search": [_vp, _vp, _sz, _sz, _vp, _vp],
     "hg_bf_search_device": [_vp, _vp, _sz, _sz, _i32, _sz, _vp, _vp, _vp, _vp],
+    "hg_bf_autotune": [_vp, _vp, _sz, _sz, _i32, _sz, _vp, _vp, _vp, _vp, _vp],
     "hg_bf_gemm_variant": [_i32],
     "hg_bf_fp64_check": [_vp, _vp, _sz, _sz, _vp, _vp, ctypes.c_double, _vp, _vp],
     "hg_stop_model_create": [_i32, _i32, _i32, _pvp],

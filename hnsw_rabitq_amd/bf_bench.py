@@ -28,22 +28,21 @@ def run(H, lib, n, nq, stream, timer, log, dim=768, k=100, c=4096):
     out = dict(bf_config=f"ip_gmm_{n}x{dim}_q{nq}_k{k}_c{c}", bf_exact_ms=exact_ms,
                bf_exact_tflops=flops / (exact_ms * 1e-3) / 1e12)
     log(f"  exact FP32: {exact_ms:.1f} ms ({out['bf_exact_tflops']:.1f} TFLOP/s)")
-    for variant in (0, 1, 2, 3):
-        lib.call("hg_bf_gemm_variant", variant)
-        for name, prec in (("bf16", lib.PREC_BF16), ("fp8", lib.PREC_FP8)):
-            ll, ld = DeviceArray(nq * k * 8), DeviceArray(nq * k * 4)
-            stats = np.zeros(4, np.float64)
-            lib.call("hg_bf_search_device", bf._h, dq.ptr, nq, k, prec, c, ll.ptr, ld.ptr,
-                     ctypes.c_void_p(stats.ctypes.data), stream)
-            lib.call("hg_bf_search_device", bf._h, dq.ptr, nq, k, prec, c, ll.ptr, ld.ptr,
-                     ctypes.c_void_p(stats.ctypes.data), stream)
-            got = ll.to_numpy(np.uint64, (nq, k))
-            log(f"  variant {variant} {name}: GEMM {stats[0]:.2f} ms = {flops / (stats[0] * 1e-3) / 1e12:.0f} "
-                f"TFLOP/s, rows identical to exact {float((got == exact).all(1).mean()):.4f}")
-            out[f"bf_{name}_gemm_ms_variant{variant}"] = stats[0]
-    best_variant = min((0, 1, 2, 3), key=lambda v: out[f"bf_bf16_gemm_ms_variant{v}"])
-    out["bf_gemm_variant"] = best_variant
-    lib.call("hg_bf_gemm_variant", best_variant)
+    # the autotuner times every GEMM tile configuration on these queries and keeps the fastest
+    # for this index and precision
+    for name, prec in (("bf16", lib.PREC_BF16), ("fp8", lib.PREC_FP8)):
+        ll, ld = DeviceArray(nq * k * 8), DeviceArray(nq * k * 4)
+        times = np.zeros(5, np.float64)
+        chosen = ctypes.c_int()
+        lib.call("hg_bf_autotune", bf._h, dq.ptr, nq, k, prec, c, ll.ptr, ld.ptr,
+                 ctypes.c_void_p(times.ctypes.data), ctypes.byref(chosen), stream)
+        got = ll.to_numpy(np.uint64, (nq, k))
+        for v in range(5):
+            log(f"  variant {v} {name}: GEMM {times[v]:.2f} ms = {flops / (times[v] * 1e-3) / 1e12:.0f} TFLOP/s")
+            out[f"bf_{name}_gemm_ms_variant{v}"] = float(times[v])
+        out[f"bf_{name}_gemm_variant"] = chosen.value
+        log(f"  autotuned {name} variant {chosen.value}; last run's rows identical to exact "
+            f"{float((got == exact).all(1).mean()):.4f}")
     for name, prec, peak in (("bf16", lib.PREC_BF16, 2500.0), ("fp8", lib.PREC_FP8, 5000.0)):
         ll, ld = DeviceArray(nq * k * 8), DeviceArray(nq * k * 4)
         stats = np.zeros(4, np.float64)

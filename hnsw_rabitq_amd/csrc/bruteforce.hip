@@ -809,6 +809,66 @@ __device__ __forceinline__ void lds_read6(uint32_t a0, uint32_t a1, uint32_t a2,
         : "memory");
 }
 
+__device__ __forceinline__ void lds_read8(const uint32_t (&a)[8], i32x4_t (&r)[8]) {
+    asm volatile(
+        "ds_read_b128 %0, %8\n\t"
+        "ds_read_b128 %1, %9\n\t"
+        "ds_read_b128 %2, %10\n\t"
+        "ds_read_b128 %3, %11\n\t"
+        "ds_read_b128 %4, %12\n\t"
+        "ds_read_b128 %5, %13\n\t"
+        "ds_read_b128 %6, %14\n\t"
+        "ds_read_b128 %7, %15\n\t"
+        "s_waitcnt lgkmcnt(0)"
+        : "=&v"(r[0]), "=&v"(r[1]), "=&v"(r[2]), "=&v"(r[3]), "=&v"(r[4]), "=&v"(r[5]), "=&v"(r[6]), "=&v"(r[7])
+        : "v"(a[0]), "v"(a[1]), "v"(a[2]), "v"(a[3]), "v"(a[4]), "v"(a[5]), "v"(a[6]), "v"(a[7])
+        : "memory");
+}
+
+// 4 x 4 blocks per wave (128 x 128 outputs): eight operand reads feed sixteen MFMAs.
+template <int PREC>
+__device__ __forceinline__ void gemm_lds_mfma44(uint32_t A, uint32_t B, f32x16_t (&acc)[4][4], int wm, int wn,
+                                                int lane) {
+    const int ra = wm * 128 + (lane & 31), rb = wn * 128 + (lane & 31);
+    if (PREC == 1) {
+#pragma unroll
+        for (int ks = 0; ks < 2; ks++) {
+            const int c = ks * 2 + (lane >> 5);
+            const uint32_t ad[8] = {A + swz(ra, c), A + swz(ra + 32, c), A + swz(ra + 64, c), A + swz(ra + 96, c),
+                                    B + swz(rb, c), B + swz(rb + 32, c), B + swz(rb + 64, c), B + swz(rb + 96, c)};
+            i32x4_t f[8];
+            lds_read8(ad, f);
+#pragma unroll
+            for (int i = 0; i < 4; i++)
+#pragma unroll
+                for (int j = 0; j < 4; j++)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(__builtin_bit_cast(bf16x8_t, f[i]),
+                                                                        __builtin_bit_cast(bf16x8_t, f[4 + j]),
+                                                                        acc[i][j], 0, 0, 0);
+        }
+    } else {
+        const int c = (lane >> 5) * 2;
+        const uint32_t al[8] = {A + swz(ra, c), A + swz(ra + 32, c), A + swz(ra + 64, c), A + swz(ra + 96, c),
+                                B + swz(rb, c), B + swz(rb + 32, c), B + swz(rb + 64, c), B + swz(rb + 96, c)};
+        const uint32_t ah[8] = {A + swz(ra, c + 1), A + swz(ra + 32, c + 1), A + swz(ra + 64, c + 1),
+                                A + swz(ra + 96, c + 1), B + swz(rb, c + 1), B + swz(rb + 32, c + 1),
+                                B + swz(rb + 64, c + 1), B + swz(rb + 96, c + 1)};
+        i32x4_t lo[8], hi[8];
+        lds_read8(al, lo);
+        lds_read8(ah, hi);
+        i32x8_t op[8];
+#pragma unroll
+        for (int t = 0; t < 8; t++)
+            op[t] = i32x8_t{lo[t].x, lo[t].y, lo[t].z, lo[t].w, hi[t].x, hi[t].y, hi[t].z, hi[t].w};
+#pragma unroll
+        for (int i = 0; i < 4; i++)
+#pragma unroll
+            for (int j = 0; j < 4; j++)
+                acc[i][j] = __builtin_amdgcn_mfma_scale_f32_32x32x64_f8f6f4(op[i], op[4 + j], acc[i][j], 0, 0, 0,
+                                                                            127, 0, 127);
+    }
+}
+
 template <int PREC, int TM, int TN>
 __device__ __forceinline__ void gemm_lds_mfma(uint32_t A, uint32_t B, f32x16_t (&acc)[TM][TN], int wm, int wn,
                                               int lane) {
@@ -853,12 +913,14 @@ __device__ __forceinline__ void gemm_lds_mfma(uint32_t A, uint32_t B, f32x16_t (
     }
 }
 
-// WNW waves along the columns (2: 256 x 128 tile, 256 threads, two workgroups per CU; 4: 256 x 256
-// tile, 512 threads, one workgroup per CU and half the operand bytes per MFMA of the former).
-template <int PREC, bool FILTER, int WNW>
-__global__ __launch_bounds__(128 * WNW, WNW == 2 ? 2 : 1) void lowp_gemm_lds_kernel(LowpGemmArgs g) {
+// WNW waves along the columns, TN 32-column blocks per wave (WNW 2, TN 2: 256 x 128 tile, 256
+// threads, two workgroups per CU; WNW 4, TN 2: 256 x 256, 512 threads; WNW 2, TN 4: 256 x 256,
+// 256 threads with 4 x 4 blocks per wave, accumulators beyond the VGPRs held in AGPRs, a third
+// fewer operand reads per MFMA).
+template <int PREC, bool FILTER, int WNW, int TN>
+__global__ __launch_bounds__(128 * WNW, (WNW == 2 && TN == 2) ? 2 : 1) void lowp_gemm_lds_kernel(LowpGemmArgs g) {
     constexpr int NT = 128 * WNW, NW = 2 * WNW;
-    constexpr int TM = 4, TN = 2, BM = 64 * TM, BN = 32 * TN * WNW;
+    constexpr int TM = 4, BM = 64 * TM, BN = 32 * TN * WNW;
     constexpr int APW = BM / 16 / NW, BPW = BN / 16 / NW;  // 16-row blocks per wave and slice
     __shared__ __align__(1024) uint8_t sA0[BM * LKB], sA1[BM * LKB], sA2[BM * LKB];
     __shared__ __align__(1024) uint8_t sB0[BN * LKB], sB1[BN * LKB], sB2[BN * LKB];
@@ -892,7 +954,10 @@ __global__ __launch_bounds__(128 * WNW, WNW == 2 ? 2 : 1) void lowp_gemm_lds_ker
             __builtin_amdgcn_s_waitcnt(0x0F70);  // vmcnt == 0
         __builtin_amdgcn_s_barrier();
         if (st + 2 < nst) gemm_lds_issue<PREC, APW, BPW>(g, nA, nB, st + 2, m0, n0, row_bytes, wave, lane);
-        gemm_lds_mfma<PREC, TM, TN>((uint32_t)(uintptr_t)rA, (uint32_t)(uintptr_t)rB, acc, wm, wn, lane);
+        if constexpr (TN == 4)
+            gemm_lds_mfma44<PREC>((uint32_t)(uintptr_t)rA, (uint32_t)(uintptr_t)rB, acc, wm, wn, lane);
+        else
+            gemm_lds_mfma<PREC, TM, TN>((uint32_t)(uintptr_t)rA, (uint32_t)(uintptr_t)rB, acc, wm, wn, lane);
     };
     for (int st = 0; st < nst; st += 3) {
         step(st, A0, B0, A2, B2);
@@ -1191,11 +1256,14 @@ static inline size_t align_up(size_t x) { return (x + 255) & ~(size_t)255; }
 
 // GEMM variant: 0 = 128x128 workgroup tiles (2x2 blocks per wave, double-buffered LDS),
 // 1 = 256x128 workgroup tiles (4x2 blocks per wave, register-prefetched single LDS stage),
-// 2 = 256x128 workgroup tiles fed by direct-to-LDS loads through a 3-slice ring.
+// 2 = 256x128 workgroup tiles fed by direct-to-LDS loads through a 3-slice ring, 3 = 256x256
+// tiles with eight waves on the same ring, 4 = 256x256 tiles with four waves of 4x4 blocks.
+// The process default applies when a caller passes variant < 0.
 static int g_gemm_variant = 2;
 void set_gemm_variant(int v) { g_gemm_variant = v; }
 
-static void launch_gemm(LowpGemmArgs ga, int precision, bool filter, hipStream_t s) {
+static void launch_gemm(LowpGemmArgs ga, int precision, bool filter, int variant, hipStream_t s) {
+    const int g_gemm_variant = variant >= 0 ? variant : hg::g_gemm_variant;
     if (g_gemm_variant == 0) {
         ga.tiles_m = (ga.nq + GM - 1) / GM;
         ga.tiles_n = (ga.n + GN - 1) / GN;
@@ -1209,14 +1277,25 @@ static void launch_gemm(LowpGemmArgs ga, int precision, bool filter, hipStream_t
         }
         return;
     }
+    if (g_gemm_variant == 4) {
+        uint32_t grid = TileOrder::make((ga.nq + 255) / 256, (ga.n + 255) / 256).grid();
+        if (precision == 1) {
+            if (filter) hipLaunchKernelGGL((lowp_gemm_lds_kernel<1, true, 2, 4>), dim3(grid), dim3(256), 0, s, ga);
+            else hipLaunchKernelGGL((lowp_gemm_lds_kernel<1, false, 2, 4>), dim3(grid), dim3(256), 0, s, ga);
+        } else {
+            if (filter) hipLaunchKernelGGL((lowp_gemm_lds_kernel<2, true, 2, 4>), dim3(grid), dim3(256), 0, s, ga);
+            else hipLaunchKernelGGL((lowp_gemm_lds_kernel<2, false, 2, 4>), dim3(grid), dim3(256), 0, s, ga);
+        }
+        return;
+    }
     if (g_gemm_variant == 3) {
         uint32_t grid = TileOrder::make((ga.nq + 255) / 256, (ga.n + 255) / 256).grid();
         if (precision == 1) {
-            if (filter) hipLaunchKernelGGL((lowp_gemm_lds_kernel<1, true, 4>), dim3(grid), dim3(512), 0, s, ga);
-            else hipLaunchKernelGGL((lowp_gemm_lds_kernel<1, false, 4>), dim3(grid), dim3(512), 0, s, ga);
+            if (filter) hipLaunchKernelGGL((lowp_gemm_lds_kernel<1, true, 4, 2>), dim3(grid), dim3(512), 0, s, ga);
+            else hipLaunchKernelGGL((lowp_gemm_lds_kernel<1, false, 4, 2>), dim3(grid), dim3(512), 0, s, ga);
         } else {
-            if (filter) hipLaunchKernelGGL((lowp_gemm_lds_kernel<2, true, 4>), dim3(grid), dim3(512), 0, s, ga);
-            else hipLaunchKernelGGL((lowp_gemm_lds_kernel<2, false, 4>), dim3(grid), dim3(512), 0, s, ga);
+            if (filter) hipLaunchKernelGGL((lowp_gemm_lds_kernel<2, true, 4, 2>), dim3(grid), dim3(512), 0, s, ga);
+            else hipLaunchKernelGGL((lowp_gemm_lds_kernel<2, false, 4, 2>), dim3(grid), dim3(512), 0, s, ga);
         }
         return;
     }
@@ -1225,11 +1304,11 @@ static void launch_gemm(LowpGemmArgs ga, int precision, bool filter, hipStream_t
     uint32_t grid = TileOrder::make(tm, tn).grid();
     if (g_gemm_variant == 2) {
         if (precision == 1) {
-            if (filter) hipLaunchKernelGGL((lowp_gemm_lds_kernel<1, true, 2>), dim3(grid), dim3(256), 0, s, ga);
-            else hipLaunchKernelGGL((lowp_gemm_lds_kernel<1, false, 2>), dim3(grid), dim3(256), 0, s, ga);
+            if (filter) hipLaunchKernelGGL((lowp_gemm_lds_kernel<1, true, 2, 2>), dim3(grid), dim3(256), 0, s, ga);
+            else hipLaunchKernelGGL((lowp_gemm_lds_kernel<1, false, 2, 2>), dim3(grid), dim3(256), 0, s, ga);
         } else {
-            if (filter) hipLaunchKernelGGL((lowp_gemm_lds_kernel<2, true, 2>), dim3(grid), dim3(256), 0, s, ga);
-            else hipLaunchKernelGGL((lowp_gemm_lds_kernel<2, false, 2>), dim3(grid), dim3(256), 0, s, ga);
+            if (filter) hipLaunchKernelGGL((lowp_gemm_lds_kernel<2, true, 2, 2>), dim3(grid), dim3(256), 0, s, ga);
+            else hipLaunchKernelGGL((lowp_gemm_lds_kernel<2, false, 2, 2>), dim3(grid), dim3(256), 0, s, ga);
         }
         return;
     }
@@ -1243,7 +1322,7 @@ static void launch_gemm(LowpGemmArgs ga, int precision, bool filter, hipStream_t
 }
 
 hipError_t launch_bf_lowp(const BfData& d, int precision, const float* q, uint32_t nq, uint32_t k, uint32_t c,
-                          uint64_t* labels, float* dists, void* scratch, hipStream_t s, double* stats) {
+                          uint64_t* labels, float* dists, void* scratch, hipStream_t s, double* stats, int variant) {
     const uint32_t n = d.n, dim = d.dim;
     const size_t elem = precision == 1 ? 2 : 1;
     const uint32_t step = n > 16384 ? (n + 16383) / 16384 : 1;
@@ -1297,7 +1376,7 @@ hipError_t launch_bf_lowp(const BfData& d, int precision, const float* q, uint32
     ga.ld_out = srows;
     ga.tiles_m = (nq + GM - 1) / GM;
     ga.tiles_n = (srows + GN - 1) / GN;
-    launch_gemm(ga, precision, false, s);
+    launch_gemm(ga, precision, false, variant, s);
     // ~1.2 c candidates expected (buffer holds 4c); never below k + 1 sample ranks
     double want = 1.2 * (double)c * (double)srows / (double)n;
     uint32_t rank = (uint32_t)std::min<double>((double)srows, std::max<double>((double)k + 1, std::ceil(want)));
@@ -1314,7 +1393,7 @@ hipError_t launch_bf_lowp(const BfData& d, int precision, const float* q, uint32
     ga.cand_d = cd;
     ga.cand_cnt = ccnt;
     ga.cap = RR_CAP;
-    launch_gemm(ga, precision, true, s);
+    launch_gemm(ga, precision, true, variant, s);
     (void)hipEventRecord(ev[2], s);
     uint32_t cpow = 2;
     while (cpow < c) cpow <<= 1;

@@ -510,6 +510,17 @@ HG_API int hg_gpu_set_plan(void* gv, size_t ef, size_t k, uint32_t hash_bits, in
     });
 }
 
+// External labels of internal ids on the device (trace records and final top-k carry internal ids).
+HG_API int hg_gpu_id_labels(void* gv, const uint32_t* d_ids, size_t n, uint64_t* d_labels, void* stream) {
+    return guarded([&] {
+        require(gv && (n == 0 || (d_ids && d_labels)), "null argument");
+        auto* gi = static_cast<GpuIndex*>(gv);
+        DeviceGuard dg(gi->device);
+        hip_check(hg::launch_id_labels(gi->g, d_ids, n, d_labels, (hipStream_t)stream), "label gather launch");
+        return HG_OK;
+    });
+}
+
 // ---- hipGraph of a device search ----
 struct GraphHandle {
     hipGraph_t graph = nullptr;

@@ -24,6 +24,7 @@ struct BfIndex {
     DevBuf vec, labels, vec_lp, scale, scratch, cand;
     DevBuf q, ol, od;
     int lp_kind = 0;  // precision of vec_lp (0 = not built)
+    int tuned[3] = {-1, -1, -1};  // GEMM variant chosen by the autotuner, per precision
     size_t lp_count = 0;
     hipStream_t stream = nullptr;
     std::unordered_map<uint64_t, size_t> label_row;
@@ -207,7 +208,7 @@ HG_API int hg_bf_search_device(void* bv, const float* dq, size_t nq, size_t k, i
         size_t sb = hg::bf_lowp_scratch_bytes((uint32_t)nq, (uint32_t)b->dim, (uint32_t)c, (uint32_t)b->count);
         b->scratch.reserve(sb);
         hip_check(hg::launch_bf_lowp(b->data(), precision, dq, (uint32_t)nq, (uint32_t)k, (uint32_t)c, dl, dd,
-                                     b->scratch.p, s, stats),
+                                     b->scratch.p, s, stats, b->tuned[precision]),
                   "reduced-precision brute-force launch");
         return HG_OK;
     });
@@ -227,9 +228,46 @@ HG_API int hg_bf_fp64_check(void* bv, const float* dq, size_t nq, size_t k, cons
     });
 }
 
+// Autotuner of the distance GEMM's tile configuration: runs the reduced-precision search on the
+// caller's queries with every variant (each twice, the second timed), keeps the variant with the
+// shortest filter GEMM for this index and precision, and reports the times.
+HG_API int hg_bf_autotune(void* bv, const float* dq, size_t nq, size_t k, int precision, size_t rerank_c,
+                          uint64_t* dl, float* dd, double* times_ms, int* chosen, void* stream) {
+    return guarded([&] {
+        require(bv && dq && dl && dd, "null argument");
+        auto* b = static_cast<BfIndex*>(bv);
+        require(precision == HG_PREC_BF16 || precision == HG_PREC_FP8, "autotune needs BF16 or FP8");
+        require(k >= 1 && k <= 2048 && k <= b->count && nq > 0, "invalid k or empty batch");
+        require(b->dim % 64 == 0 && b->dim <= 8192, "the reduced-precision GEMM needs dim % 64 == 0, dim <= 8192");
+        DeviceGuard dg(b->device);
+        hipStream_t s = (hipStream_t)stream;
+        size_t c = std::min(std::max(rerank_c, k), b->count);
+        require(c <= 4096, "rerank_c above 4096 is not supported");
+        ensure_lowp(b, precision, s);
+        b->scratch.reserve(hg::bf_lowp_scratch_bytes((uint32_t)nq, (uint32_t)b->dim, (uint32_t)c, (uint32_t)b->count));
+        double best = 1e30;
+        int best_v = -1;
+        for (int v = 0; v < hg::kGemmVariants; v++) {
+            double st[4] = {0, 0, 0, 0};
+            for (int rep = 0; rep < 2; rep++)
+                hip_check(hg::launch_bf_lowp(b->data(), precision, dq, (uint32_t)nq, (uint32_t)k, (uint32_t)c, dl, dd,
+                                             b->scratch.p, s, st, v),
+                          "reduced-precision brute-force launch");
+            if (times_ms) times_ms[v] = st[0];
+            if (st[0] < best) {
+                best = st[0];
+                best_v = v;
+            }
+        }
+        b->tuned[precision] = best_v;
+        if (chosen) *chosen = best_v;
+        return HG_OK;
+    });
+}
+
 HG_API int hg_bf_gemm_variant(int variant) {
     return guarded([&] {
-        require(variant >= 0 && variant <= 3, "GEMM variant must be 0, 1, 2 or 3");
+        require(variant >= 0 && variant < hg::kGemmVariants, "GEMM variant must be in [0, 4]");
         hg::set_gemm_variant(variant);
         return HG_OK;
     });

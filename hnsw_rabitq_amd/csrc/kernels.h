@@ -13,6 +13,7 @@ size_t search_lds_bytes(uint32_t dim, uint32_t cap, uint32_t hash_bits);
 int search_blocks_per_cu(const SearchParams& p);
 hipError_t launch_search(const SearchParams& p, int blocks, hipStream_t s);
 hipError_t launch_copy(void* dst, const void* src, size_t bytes, hipStream_t s);
+hipError_t launch_id_labels(const DevGraph& g, const uint32_t* ids, size_t n, uint64_t* out, hipStream_t s);
 
 // ---- brute force (bruteforce.hip) ----
 struct BfData {
@@ -37,7 +38,9 @@ size_t bf_lowp_scratch_bytes(uint32_t nq, uint32_t dim, uint32_t c, uint32_t n);
 hipError_t launch_bf_convert(const BfData& d, uint16_t* vec_bf16, uint8_t* vec_fp8, float* fp8_scale, float* norms,
                              hipStream_t s);
 hipError_t launch_bf_lowp(const BfData& d, int precision, const float* q, uint32_t nq, uint32_t k, uint32_t c,
-                          uint64_t* labels, float* dists, void* scratch, hipStream_t s, double* stats);
+                          uint64_t* labels, float* dists, void* scratch, hipStream_t s, double* stats,
+                          int variant = -1);
+constexpr int kGemmVariants = 5;
 void set_gemm_variant(int v);
 hipError_t launch_fp64_check(const BfData& d, const float* q, uint32_t nq, uint32_t k, const uint64_t* labels,
                              const float* dists, double rel_tol, double* out_host, void* scratch, hipStream_t s);

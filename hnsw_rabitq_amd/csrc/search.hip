@@ -621,6 +621,21 @@ hipError_t launch_copy(void* dst, const void* src, size_t bytes, hipStream_t s) 
     return hipGetLastError();
 }
 
+// labels[i] = external label of internal id ids[i] (SENT -> all ones).
+__global__ void id_labels_kernel(const uint64_t* __restrict__ lab, uint32_t n_nodes, const uint32_t* __restrict__ ids,
+                                 size_t n, uint64_t* __restrict__ out) {
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const uint32_t id = ids[i] & ID_MASK;
+    out[i] = (ids[i] != SENT && id < n_nodes) ? lab[id] : ~0ull;
+}
+
+hipError_t launch_id_labels(const DevGraph& g, const uint32_t* ids, size_t n, uint64_t* out, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(id_labels_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, g.labels, g.n, ids, n, out);
+    return hipGetLastError();
+}
+
 __global__ void zero_u32_kernel(uint32_t* p, uint32_t n) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0u;

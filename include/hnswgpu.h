@@ -123,8 +123,16 @@ int hg_bf_fp64_check(void* bf, const float* d_queries, size_t nq, size_t k, cons
  * workgroup tiles with 2x2 MFMA blocks per wave and double-buffered LDS; 1 = 256x128 tiles
  * with 4x2 blocks per wave and a register-prefetched LDS stage; 2 (default) = 256x128 tiles
  * fed by direct-to-LDS loads through a ring of three k-slices; 3 = 256x256 tiles, eight waves,
- * the same direct-to-LDS ring. */
+ * the same direct-to-LDS ring; 4 = 256x256 tiles, four waves of 4x4 blocks, the same ring. */
 int hg_bf_gemm_variant(int variant);
+/* Autotuner of the GEMM tile configuration for this index and precision (BF16 or FP8): runs the
+ * reduced-precision search on the device-resident queries with every variant, keeps the one
+ * with the shortest filter GEMM for later hg_bf_search_device calls on this index (it overrides
+ * hg_bf_gemm_variant there), writes the five GEMM times (ms) to times_ms[5] and the choice to
+ * *chosen (either may be NULL).  dl/dd receive the last run's results ([nq][k]).  No counterpart
+ * in the reference: the tile shape is a GPU concern. */
+int hg_bf_autotune(void* bf_index, const float* d_queries, size_t nq, size_t k, int precision, size_t rerank_c,
+                   uint64_t* d_labels, float* d_dists, double* times_ms, int* chosen, void* stream);
 
 /* ---- Learned early termination (stop model) ---- */
 /* Create a stop model for a given feature count / hidden width (weights zero). */
@@ -146,6 +154,10 @@ int hg_stop_model_config(void* model, int period, float threshold, int min_hops)
 int hg_gpu_collect_traces(void* gpu_index, const float* d_queries, size_t nq, size_t ef, size_t k, int period,
                           int max_records, float* d_features, uint32_t* d_topk, int32_t* d_nrec, uint32_t* d_final,
                           void* stream);
+/* External labels of n internal ids (as in trace records and d_final), on the device; an empty
+ * slot (0xFFFFFFFF) maps to all ones.  Counterpart of the reference's getExternalLabel
+ * (hnswalg.h:185-189) for device arrays. */
+int hg_gpu_id_labels(void* gpu_index, const uint32_t* d_ids, size_t n, uint64_t* d_labels, void* stream);
 /* Forward pass (logits) on device features [n][in_dim]. */
 int hg_stop_model_forward(void* model, const float* d_x, size_t n, float* d_logits, void* stream);
 /* One Adam step on a minibatch (mean BCE-with-logits); returns the loss through *loss

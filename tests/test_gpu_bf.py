@@ -118,7 +118,7 @@ def test_bf_lowp_ties_at_rerank_cut(hg, prec):
     assert stats[3] == 32
 
 
-@pytest.mark.parametrize("variant", [0, 1, 2, 3])
+@pytest.mark.parametrize("variant", [0, 1, 2, 3, 4])
 def test_bf_lowp_gemm_variants_agree(hg, variant):
     """Every GEMM tile configuration gives the exact path's lists (BF16 and FP8, inner product,
     a row count that is not a multiple of any tile and a dim of 320 = five 64-element slices)."""
@@ -142,3 +142,29 @@ def test_bf_lowp_gemm_variants_agree(hg, variant):
                                           ed.view(np.uint32))
     finally:
         _lib.call("hg_bf_gemm_variant", 2)
+
+
+def test_bf_gemm_autotune(hg):
+    """The tile autotuner times every variant, keeps one per index and precision, and the search
+    then runs with it: lists equal the exact path's before and after."""
+    from hnsw_rabitq_amd import datasets, DeviceArray, _lib
+    import ctypes
+    data, q = datasets.c3_gmm(n=20_000, nq=300, dim=128, n_centres=16)
+    b = _bf(hg, "ip", data)
+    k = 20
+    el, ed = b.knn_query(q, k=k)
+    dq = DeviceArray.from_numpy(q)
+    dl, dd = DeviceArray(q.shape[0] * k * 8), DeviceArray(q.shape[0] * k * 4)
+    times = np.zeros(5, np.float64)
+    chosen = ctypes.c_int(-1)
+    _lib.call("hg_bf_autotune", b._h, dq.ptr, q.shape[0], k, _lib.PREC_BF16, 512, dl.ptr, dd.ptr,
+              ctypes.c_void_p(times.ctypes.data), ctypes.byref(chosen), None)
+    _lib.call("hg_device_synchronize")
+    print(f"autotune times {times} chosen {chosen.value}", flush=True)
+    assert 0 <= chosen.value < 5 and (times > 0).all()
+    assert times[chosen.value] == times.min()
+    np.testing.assert_array_equal(dl.to_numpy(np.uint64, (q.shape[0], k)), el)
+    _lib.call("hg_bf_search_device", b._h, dq.ptr, q.shape[0], k, _lib.PREC_BF16, 512, dl.ptr, dd.ptr, None, None)
+    _lib.call("hg_device_synchronize")
+    np.testing.assert_array_equal(dl.to_numpy(np.uint64, (q.shape[0], k)), el)
+    np.testing.assert_array_equal(dd.to_numpy(np.float32, (q.shape[0], k)).view(np.uint32), ed.view(np.uint32))
