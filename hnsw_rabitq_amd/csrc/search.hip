@@ -108,37 +108,42 @@ struct StopState {
     float qnorm, d_entry;
     int improved_recent;  // bitmask of the last 16 expansions that changed the top-k
     int last_change;      // expansion count at the last change of the top-k
+    int win_pos;          // smallest beam position a new key took in the current window
+    int prev_pos[3];      // the same for the three windows before
 };
 
 __device__ __forceinline__ float safe_div(float a, float b) { return a / (fabsf(b) + 1e-20f); }
 
 // Features (same order as the trace collector and the host-side training code):
-//  0 query norm                      1 entry distance at level 0
-//  2 d_1 / d_entry                   3 d_k / d_entry
-//  4 position of the next expansion in the beam / ef_eff
-//  5 d_k / d_1                       6 d_next / d_k (next expansion)
-//  7 hops / ef_eff
-//  8 hops since the top-k last changed / (hops + 1)
-//  9 fraction of the last 16 expansions that changed the top-k
-// 10 min(position of the next expansion, 4k) / k
-// 11 log(1 + hops since the top-k last changed)
+//  0 query norm                       1 log(1 + entry distance at level 0)
+//  2 d_k / d_1                        3 d_ef / d_k (last beam entry)
+//  4 hops / ef_eff                    5 fraction of the last 16 expansions that changed the top-k
+//  6 d_next / d_k (next expansion)    7 position of the next expansion in the beam / ef_eff
+//  8 d_k / d_entry
+//  9 log(1 + smallest position a new key took in the current window / k)   (near misses)
+// 10 the same over the last four windows
+// 11 log(1 + number of beam entries closer than 1.2 d_k / k)                 (crowding at d_k)
+// A window is the `period` expansions between two evaluations; a window without insertions
+// counts as position ef_eff.
 __device__ __forceinline__ float stop_feature(int f, const StopState& st, float d1, float dk, float dlast,
-                                              float dnext, int pnext, int k, int hops, int ef_eff) {
-    const int since = hops - st.last_change;
-    (void)dlast;
+                                              float dnext, int pnext, int k, int hops, int ef_eff, int crowd) {
+    const float fk = (float)k;
+    const int w1 = st.win_pos < ef_eff ? st.win_pos : ef_eff;
+    int w4 = w1;
+    for (int j = 0; j < 3; j++) w4 = st.prev_pos[j] < w4 ? st.prev_pos[j] : w4;
     switch (f) {
         case 0: return st.qnorm;
-        case 1: return st.d_entry;
-        case 2: return safe_div(d1, st.d_entry);
-        case 3: return safe_div(dk, st.d_entry);
-        case 4: return (float)pnext / (float)ef_eff;
-        case 5: return safe_div(dk, d1);
+        case 1: return logf(1.0f + st.d_entry);
+        case 2: return safe_div(dk, d1);
+        case 3: return safe_div(dlast, dk);
+        case 4: return (float)hops / (float)ef_eff;
+        case 5: return (float)__popc((uint32_t)st.improved_recent & 0xFFFFu) / 16.0f;
         case 6: return safe_div(dnext, dk);
-        case 7: return (float)hops / (float)ef_eff;
-        case 8: return (float)since / (float)(hops + 1);
-        case 9: return (float)__popc((uint32_t)st.improved_recent & 0xFFFFu) / 16.0f;
-        case 10: return (float)(pnext < 4 * k ? pnext : 4 * k) / (float)k;
-        default: return logf(1.0f + (float)since);
+        case 7: return (float)pnext / (float)ef_eff;
+        case 8: return safe_div(dk, st.d_entry);
+        case 9: return logf(1.0f + (float)w1 / fk);
+        case 10: return logf(1.0f + (float)w4 / fk);
+        default: return logf(1.0f + (float)crowd / fk);
     }
 }
 
@@ -343,6 +348,8 @@ __global__ __launch_bounds__(64) void hnsw_search_kernel(SearchParams P) {
         st.d_entry = curdist;
         st.improved_recent = 0;
         st.last_change = 0;
+        st.win_pos = 1 << 30;
+        st.prev_pos[0] = st.prev_pos[1] = st.prev_pos[2] = 1 << 30;
         if (STOP || TRACE) {
             float s = 0.f;
             for (uint32_t i = lane; i < g.dim; i += 64) s = fma_(qs[i], qs[i], s);
@@ -508,6 +515,7 @@ __global__ __launch_bounds__(64) void hnsw_search_kernel(SearchParams P) {
                                 st.improved_recent |= 1;
                                 st.last_change = H0;
                             }
+                            st.win_pos = min_fp < st.win_pos ? min_fp : st.win_pos;
                         }
                         n = new_n;
                         __syncthreads();
@@ -539,8 +547,13 @@ __global__ __launch_bounds__(64) void hnsw_search_kernel(SearchParams P) {
                             break;
                         }
                     }
+                    const int crowd = beam_lower_bound(bd, bi, n, 1.2f * dk, 0u);
                     if (lane < kStopFeatures)
-                        feat[lane] = stop_feature(lane, st, d1, dk, dlast, dnext, pnext, kq, H0, ef_eff);
+                        feat[lane] = stop_feature(lane, st, d1, dk, dlast, dnext, pnext, kq, H0, ef_eff, crowd);
+                    st.prev_pos[2] = st.prev_pos[1];
+                    st.prev_pos[1] = st.prev_pos[0];
+                    st.prev_pos[0] = st.win_pos < ef_eff ? st.win_pos : ef_eff;
+                    st.win_pos = 1 << 30;
                     __syncthreads();
                     if (TRACE && nrec < P.trace_max_rec) {
                         float* fo = P.trace_feat + ((size_t)qi * P.trace_max_rec + nrec) * kStopFeatures;
