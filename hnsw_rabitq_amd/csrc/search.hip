@@ -88,17 +88,17 @@ __device__ __forceinline__ bool hash_insert(uint32_t* hash, uint32_t mask, uint3
     return true;
 }
 
-// Number of beam entries with key < (d, id), over beam[0, n).
+// Number of beam entries with key < (d, id), over beam[0, n): a fixed-step lower bound (the
+// probe index is clamped, so every lane runs the same steps with no branch).
 __device__ __forceinline__ int beam_lower_bound(const float* bd, const uint32_t* bi, int n, float d, uint32_t id) {
-    int lo = 0, hi = n;
-    while (lo < hi) {
-        const int mid = (lo + hi) >> 1;
-        if (key_less(bd[mid], bi[mid] & ID_MASK, d, id))
-            lo = mid + 1;
-        else
-            hi = mid;
+    int pos = 0;
+    for (int step = 1 << (31 - __clz(n > 0 ? n : 1)); step > 0; step >>= 1) {
+        const int probe = pos + step - 1;
+        const int pc = probe < n ? probe : (n > 0 ? n - 1 : 0);
+        const bool lt = (probe < n) & key_less(bd[pc], bi[pc] & ID_MASK, d, id);
+        pos += lt ? step : 0;
     }
-    return lo;
+    return pos;
 }
 
 // ------------------------------------------------------------------------------------------
@@ -468,35 +468,36 @@ __global__ __launch_bounds__(64) void hnsw_search_kernel(SearchParams P) {
                         const int new_n = n + m < ef_eff ? n + m : ef_eff;
                         const int hi_chunk = (n - 1) & ~63;
                         const int lo_chunk = first_pos & ~63;
+                        // Reads are unconditional (indices below cap, clamped at 0); a lane whose
+                        // entry does not move writes to its own slot of the distance-staging area
+                        // (sd/si, which hold no live data during the merge), so no lane branches.
+                        const int trash_d = (int)(sd - bd) + lane, trash_i = (int)(si - bi) + lane;
                         for (int b0 = hi_chunk; b0 >= lo_chunk; b0 -= 8 * 64) {
                             float vd[8];
                             uint32_t vi[8];
                             int sh[8];
+                            const int base = b0 + lane;
 #pragma unroll
                             for (int u = 0; u < 8; u++) {
-                                const int i = b0 - 64 * u + lane;
-                                vd[u] = 0.f;
-                                vi[u] = 0;
+                                const int i = base - 64 * u;
+                                const int ic = i > 0 ? i : 0;
+                                vd[u] = bd[ic];
+                                vi[u] = bi[ic];
                                 sh[u] = 0;
-                                if (b0 - 64 * u >= lo_chunk && i >= first_pos && i < n) {
-                                    vd[u] = bd[i];
-                                    vi[u] = bi[i];
-                                }
                             }
                             for (int rr = 0; rr < m; rr++) {
                                 const int sp = __builtin_amdgcn_readlane(spos, rr);
 #pragma unroll
-                                for (int u = 0; u < 8; u++) sh[u] += sp <= b0 - 64 * u + lane ? 1 : 0;
+                                for (int u = 0; u < 8; u++) sh[u] += sp <= base - 64 * u ? 1 : 0;
                             }
                             __asm__ volatile("" ::: "memory");
 #pragma unroll
                             for (int u = 0; u < 8; u++) {
-                                const int i = b0 - 64 * u + lane;
+                                const int i = base - 64 * u;
                                 const int np = i + sh[u];
-                                if (b0 - 64 * u >= lo_chunk && i >= first_pos && i < n && np < ef_eff) {
-                                    bd[np] = vd[u];
-                                    bi[np] = vi[u];
-                                }
+                                const bool w = (b0 - 64 * u >= lo_chunk) & (i >= first_pos) & (i < n) & (np < ef_eff);
+                                bd[w ? np : trash_d] = vd[u];
+                                bi[w ? np : trash_i] = vi[u];
                             }
                             __asm__ volatile("" ::: "memory");
                         }
