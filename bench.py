@@ -230,7 +230,16 @@ def main():
     log(f"plain: {plain_qps:.0f} QPS at ef={ef_op}; kernel {ev_ms:.3f} ms; {achieved:.1f} GB/s algorithmic "
         f"(evaluations/query {ctr_alg[:, 1].mean():.0f}; tuned plan {ctr[:, 1].mean():.0f})")
 
+    # the same batch through the host-buffer entry point (queries H2D, search, results D2H):
+    # the PCIe-inclusive rate, reported beside the HBM-resident one
+    gpu.search(queries, K, ef_op)
+    t = time.perf_counter()
+    for _ in range(3):
+        gpu.search(queries, K, ef_op)
+    host_ms = (time.perf_counter() - t) * 1e3 / 3
+    log(f"host buffers (H2D + search + D2H): {host_ms:.2f} ms = {nq / (host_ms * 1e-3):.0f} QPS")
     result = dict(plain_qps_at_095=plain_qps, plain_ef=ef_op, plain_recall=op["recall"], plan=plan,
+                  plain_qps_host_buffers=nq / (host_ms * 1e-3), plain_ms_host_buffers=host_ms,
                   evals_per_query=float(ctr_alg[:, 1].mean()), evals_per_query_tuned=float(ctr[:, 1].mean()),
                   bytes_per_launch_with_reevaluations=bytes_read_tuned)
 
@@ -265,8 +274,15 @@ def main():
             o.search(0, sample, K, ef_op, mode=pyoracle.MODE_HEAP, nthreads=nthreads)
             times.append(time.perf_counter() - t)
         cpu_qps = sample.shape[0] / float(np.median(times))
+        model = "unknown CPU"
+        try:
+            with open("/proc/cpuinfo") as f:
+                model = next((ln.split(":", 1)[1].strip() for ln in f if ln.startswith("model name")), model)
+        except OSError:
+            pass
         cpu = dict(value=cpu_qps, unit="queries/s", cores=nthreads, kind="port",
-                   sample=f"{sample.shape[0]} of the {nq} queries at ef={ef_op}, k=10, median of 3 after warm-up")
+                   sample=f"{sample.shape[0]} of the {nq} queries at ef={ef_op}, k=10, median of 3 after warm-up; "
+                          f"{model}")
         log(f"CPU oracle port: {cpu_qps:.0f} QPS on {nthreads} threads")
 
     # headline: the faster of plain and learned at recall@10 >= 0.95 (plain is the fallback)

@@ -269,6 +269,19 @@ class Index:
             self._dirty = False
         return self._gpu
 
+    def mark_deleted(self, label):
+        """HierarchicalNSW::markDelete: the element stays in the graph and is never returned."""
+        if self._h is None:
+            raise RuntimeError("Index not initialized")
+        _lib.call("hg_hnsw_mark_deleted", self._h, int(label))
+        self._dirty = True
+
+    def unmark_deleted(self, label):
+        if self._h is None:
+            raise RuntimeError("Index not initialized")
+        _lib.call("hg_hnsw_unmark_deleted", self._h, int(label))
+        self._dirty = True
+
     def knn_query(self, data, k=1, num_threads=-1, filter=None):
         if filter is not None:
             raise RuntimeError("filter: the filtered (non-bare-bone) search is not implemented by the GPU engine")

@@ -29,6 +29,8 @@ SIGNATURES = {
     "hg_hnsw_set_ef": [_vp, _sz],
     "hg_hnsw_get_ef": [_vp, ctypes.POINTER(_sz)],
     "hg_hnsw_free": [_vp],
+    "hg_hnsw_mark_deleted": [_vp, ctypes.c_uint64],
+    "hg_hnsw_unmark_deleted": [_vp, ctypes.c_uint64],
     "hg_normalize_rows": [_vp, _vp, _sz, _sz],
     "hg_gpu_index_create": [_vp, _i32, _pvp],
     "hg_gpu_index_free": [_vp],

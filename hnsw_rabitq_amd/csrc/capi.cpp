@@ -202,6 +202,22 @@ HG_API int hg_hnsw_set_ef(void* h, size_t ef) {
     });
 }
 
+HG_API int hg_hnsw_mark_deleted(void* h, uint64_t label) {
+    return guarded([&] {
+        require(h != nullptr, "null index");
+        static_cast<hnswgpu::HnswIndex*>(h)->mark_deleted(label);
+        return HG_OK;
+    });
+}
+
+HG_API int hg_hnsw_unmark_deleted(void* h, uint64_t label) {
+    return guarded([&] {
+        require(h != nullptr, "null index");
+        static_cast<hnswgpu::HnswIndex*>(h)->unmark_deleted(label);
+        return HG_OK;
+    });
+}
+
 HG_API int hg_hnsw_get_ef(void* h, size_t* ef) {
     return guarded([&] {
         require(h && ef, "null argument");

@@ -195,6 +195,32 @@ size_t HnswIndex::num_deleted() const {
     return n;
 }
 
+void HnswIndex::mark_deleted(labeltype label) {
+    tableint id;
+    {
+        std::lock_guard<std::mutex> g(label_lookup_lock_);
+        auto it = label_lookup_.find(label);
+        if (it == label_lookup_.end()) throw std::runtime_error("Label not found");
+        id = it->second;
+    }
+    unsigned char* flags = (unsigned char*)ll0_mut(id) + 2;
+    if (*flags & 0x01) throw std::runtime_error("The requested to delete element is already deleted");
+    *flags |= 0x01;
+}
+
+void HnswIndex::unmark_deleted(labeltype label) {
+    tableint id;
+    {
+        std::lock_guard<std::mutex> g(label_lookup_lock_);
+        auto it = label_lookup_.find(label);
+        if (it == label_lookup_.end()) throw std::runtime_error("Label not found");
+        id = it->second;
+    }
+    unsigned char* flags = (unsigned char*)ll0_mut(id) + 2;
+    if (!(*flags & 0x01)) throw std::runtime_error("The requested to undelete element is not deleted");
+    *flags &= (unsigned char)~0x01;
+}
+
 int HnswIndex::random_level(double reverse_size) {
     std::uniform_real_distribution<double> distribution(0.0, 1.0);
     double r = -log(distribution(level_generator_)) * reverse_size;

@@ -77,6 +77,10 @@ class HnswIndex {
     tableint entry() const { return enterpoint_node_; }
     int element_level(tableint i) const { return element_levels_[i]; }
     size_t num_deleted() const;
+    // markDelete / unmarkDelete (hnswalg.h:852-927): set or clear the delete mark (bit 0 of byte
+    // 2 of the level-0 header); the graph is unchanged.
+    void mark_deleted(labeltype label);
+    void unmark_deleted(labeltype label);
 
     const uint32_t* linklist0(tableint i) const {
         return (const uint32_t*)(level0_ + i * size_data_per_element_ + offset_level0_);

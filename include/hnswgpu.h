@@ -60,6 +60,12 @@ int hg_hnsw_serialize(void* index, void* buffer, size_t capacity, size_t* size);
 /* info[0..7] = count, max_elements, dim, M, ef_construction, maxlevel, entry, num_deleted */
 int hg_hnsw_info(void* index, int64_t* info);
 int hg_hnsw_set_ef(void* index, size_t ef);
+/* HierarchicalNSW::markDelete / unmarkDelete (hnswalg.h:852-927): set / clear the delete mark of
+ * a label; deleted elements stay in the graph, are traversed, and are never returned.  Errors
+ * (HG_ERR_RUNTIME): "Label not found", "The requested to delete element is already deleted",
+ * "The requested to undelete element is not deleted". */
+int hg_hnsw_mark_deleted(void* index, uint64_t label);
+int hg_hnsw_unmark_deleted(void* index, uint64_t label);
 int hg_hnsw_get_ef(void* index, size_t* ef);
 int hg_hnsw_free(void* index);
 /* Index.normalize_vector (bindings.cpp:241-248) applied row by row (the cosine space). */

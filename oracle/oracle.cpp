@@ -194,6 +194,9 @@ struct Hnsw {
         return l;
     }
     static unsigned short count_of(const uint32_t* p) { return *(const unsigned short*)p; }
+    // isMarkedDeleted (hnswalg.h:930-935): bit 0 of byte 2 of the level-0 header
+    bool deleted(uint32_t i) const { return (((const unsigned char*)ll0(i))[2] & 0x01) != 0; }
+    size_t num_deleted = 0;
 };
 
 template <typename T>
@@ -238,6 +241,7 @@ Hnsw* parse(const char* buf, size_t len) {
         throw std::runtime_error("Index seems to be corrupted or unsupported");
     }
     h->dim = (h->label_offset - h->offsetData) / 4;
+    for (size_t i = 0; i < h->count; i++) h->num_deleted += h->deleted((uint32_t)i) ? 1 : 0;
     return h;
 }
 
@@ -246,9 +250,11 @@ struct Res {
     uint32_t id;
 };
 
-// One query, reference searchKnn (hnswalg.h:1269-1323) with searchBaseLayerST<true>
-// (hnswalg.h:308-440).  Mode HEAP uses std::priority_queue with CompareByFirst; mode TOTAL keys
-// every comparison by (dist, id).
+// One query, reference searchKnn (hnswalg.h:1269-1323) with searchBaseLayerST (hnswalg.h:308-440):
+// the bare-bone branch when the index has no deleted element and no filter is given, else the
+// non-bare-bone branch (deleted or filtered nodes are traversed but never enter the result set;
+// the search stops only once the result set holds ef elements).  Mode HEAP uses
+// std::priority_queue with CompareByFirst; mode TOTAL keys every comparison by (dist, id).
 struct Searcher {
     const Hnsw& h;
     int space;
@@ -276,7 +282,8 @@ struct Searcher {
     };
 
     template <int MODE>
-    void run(const float* q, size_t k, size_t ef, std::vector<Res>& out, int64_t* ctr, uint8_t* tie) {
+    void run(const float* q, size_t k, size_t ef, std::vector<Res>& out, int64_t* ctr, uint8_t* tie,
+             const uint8_t* allowed = nullptr) {
         out.clear();
         int64_t H0 = 0, E0 = 0, Hu = 0, Eu = 0;
         if (h.count == 0) return;
@@ -310,7 +317,89 @@ struct Searcher {
         std::vector<float> seen;  // every level-0 distance (tie detection)
         bool track = tie != nullptr;
 
-        if (MODE == OR_MODE_HEAP) {
+        const bool bare = h.num_deleted == 0 && allowed == nullptr;
+        auto ok = [&](uint32_t id) { return !h.deleted(id) && (allowed == nullptr || allowed[id] != 0); };
+        if (!bare && MODE == OR_MODE_HEAP) {
+            // searchBaseLayerST<false> (hnswalg.h:323-437)
+            std::priority_queue<P, std::vector<P>, ByFirst> top, cand;
+            float lowerBound;
+            if (ok(cur)) {
+                float dep = dist(space, dim, q, h.vec(cur));
+                E0++;
+                if (track) seen.push_back(dep);
+                lowerBound = dep;
+                top.emplace(dep, cur);
+                cand.emplace(-dep, cur);
+            } else {
+                lowerBound = std::numeric_limits<float>::max();
+                cand.emplace(-lowerBound, cur);
+            }
+            visited[cur] = tag;
+            while (!cand.empty()) {
+                P cp = cand.top();
+                if (-cp.first > lowerBound && top.size() == ef) break;
+                cand.pop();
+                const uint32_t* data = h.ll0(cp.second);
+                size_t size = Hnsw::count_of(data);
+                H0++;
+                for (size_t j = 1; j <= size; j++) {
+                    uint32_t cid = data[j];
+                    if (visited[cid] == tag) continue;
+                    visited[cid] = tag;
+                    float d = dist(space, dim, q, h.vec(cid));
+                    E0++;
+                    if (track) seen.push_back(d);
+                    if (top.size() < ef || lowerBound > d) {
+                        cand.emplace(-d, cid);
+                        if (ok(cid)) top.emplace(d, cid);
+                        while (top.size() > ef) top.pop();
+                        if (!top.empty()) lowerBound = top.top().first;
+                    }
+                }
+            }
+            while (top.size() > k) top.pop();
+            while (!top.empty()) {
+                out.push_back({top.top().first, top.top().second});
+                top.pop();
+            }
+        } else if (!bare) {
+            // the same branch in (dist, id) order: the entry point's distance is the descent's
+            // (counted once), and a disallowed entry point is expanded first
+            std::priority_queue<P, std::vector<P>, ByPair> top;
+            std::priority_queue<P, std::vector<P>, ByPairNeg> cand;
+            E0++;
+            if (track) seen.push_back(curdist);
+            if (ok(cur)) top.emplace(curdist, cur);
+            cand.emplace(-std::numeric_limits<float>::infinity(), cur);
+            visited[cur] = tag;
+            while (!cand.empty()) {
+                P cp = cand.top();
+                if (top.size() == ef && top.top() < cp) break;
+                cand.pop();
+                const uint32_t* data = h.ll0(cp.second);
+                size_t size = Hnsw::count_of(data);
+                H0++;
+                for (size_t j = 1; j <= size; j++) {
+                    uint32_t cid = data[j];
+                    if (visited[cid] == tag) continue;
+                    visited[cid] = tag;
+                    float d = dist(space, dim, q, h.vec(cid));
+                    E0++;
+                    if (track) seen.push_back(d);
+                    P np(d, cid);
+                    if (top.size() < ef || np < top.top()) {
+                        cand.push(np);
+                        if (ok(cid)) top.push(np);
+                        while (top.size() > ef) top.pop();
+                    }
+                }
+            }
+            while (top.size() > k) top.pop();
+            while (!top.empty()) {
+                out.push_back({top.top().first, top.top().second});
+                top.pop();
+            }
+        } else if (MODE == OR_MODE_HEAP) {
             std::priority_queue<P, std::vector<P>, ByFirst> top, cand;
             float dep = dist(space, dim, q, h.vec(cur));
             E0++;
@@ -443,6 +532,13 @@ void* or_hnsw_load_bytes(const void* buf, size_t len) {
 }
 
 void or_hnsw_free(void* h) { delete (Hnsw*)h; }
+void or_hnsw_labels(void* hv, uint64_t* labels_by_id, uint8_t* deleted_by_id) {
+    const Hnsw& h = *(const Hnsw*)hv;
+    for (size_t i = 0; i < h.count; i++) {
+        if (labels_by_id) labels_by_id[i] = h.label((uint32_t)i);
+        if (deleted_by_id) deleted_by_id[i] = h.deleted((uint32_t)i) ? 1 : 0;
+    }
+}
 size_t or_hnsw_count(void* h) { return ((Hnsw*)h)->count; }
 int or_hnsw_maxlevel(void* h) { return ((Hnsw*)h)->maxlevel; }
 uint32_t or_hnsw_entry(void* h) { return ((Hnsw*)h)->entry; }
@@ -450,6 +546,14 @@ uint32_t or_hnsw_entry(void* h) { return ((Hnsw*)h)->entry; }
 int or_hnsw_search(void* hv, int space, const float* q, size_t nq, size_t k, size_t ef,
                    const int32_t* ef_per_query, const int32_t* k_per_query, size_t k_stride, int mode,
                    int nthreads, uint64_t* labels, float* dists, int64_t* counters, uint8_t* tie_flags) {
+    return or_hnsw_search_allowed(hv, space, q, nq, k, ef, ef_per_query, k_per_query, k_stride, mode, nthreads,
+                                  nullptr, labels, dists, counters, tie_flags);
+}
+
+int or_hnsw_search_allowed(void* hv, int space, const float* q, size_t nq, size_t k, size_t ef,
+                           const int32_t* ef_per_query, const int32_t* k_per_query, size_t k_stride, int mode,
+                           int nthreads, const uint8_t* allowed, uint64_t* labels, float* dists, int64_t* counters,
+                           uint8_t* tie_flags) {
     const Hnsw& h = *(const Hnsw*)hv;
     if (k_stride == 0) k_stride = k;
     if (nthreads <= 0) nthreads = (int)std::thread::hardware_concurrency();
@@ -465,10 +569,10 @@ int or_hnsw_search(void* hv, int space, const float* q, size_t nq, size_t k, siz
         try {
             if (mode == OR_MODE_HEAP)
                 ss[t]->run<OR_MODE_HEAP>(q + r * h.dim, kq, efq, out, counters ? counters + 4 * r : nullptr,
-                                         tie_flags ? tie_flags + r : nullptr);
+                                         tie_flags ? tie_flags + r : nullptr, allowed);
             else
                 ss[t]->run<OR_MODE_TOTAL>(q + r * h.dim, kq, efq, out, counters ? counters + 4 * r : nullptr,
-                                          tie_flags ? tie_flags + r : nullptr);
+                                          tie_flags ? tie_flags + r : nullptr, allowed);
         } catch (std::exception& e) {
             std::lock_guard<std::mutex> g(err_mu);
             g_err = e.what();

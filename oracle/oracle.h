@@ -7,7 +7,7 @@
 //                          (kernel chosen by dim as on an AVX-512 host; GCC contracts the
 //                          intrinsic mul+add pairs into FMA, which is restated with fmaf)
 //   or_hnsw_load           hnswalg.h:715-821 (on-disk format; header field order 685-712)
-//   or_hnsw_search         hnswalg.h:1269-1323 (searchKnn) + 308-440 (searchBaseLayerST<true>)
+//   or_hnsw_search         hnswalg.h:1269-1323 (searchKnn) + 308-440 (searchBaseLayerST<true/false>)
 //                          + bindings.cpp:612-697 (row output, closest first)
 //   or_bf_search           bruteforce.h:106-125 + bindings.cpp:850-929
 //   or_mlp_*               the learned-stop network's forward/backward/Adam in FP64/FP32 scalar
@@ -55,6 +55,15 @@ uint32_t or_hnsw_entry(void* h);
 int or_hnsw_search(void* h, int space, const float* q, size_t nq, size_t k, size_t ef,
                    const int32_t* ef_per_query, const int32_t* k_per_query, size_t k_stride, int mode,
                    int nthreads, uint64_t* labels, float* dists, int64_t* counters, uint8_t* tie_flags);
+// The same with a filter: allowed[internal id] != 0 admits the element to the result set (null =
+// every element).  With a filter, or with delete marks in the index, the non-bare-bone branch of
+// searchBaseLayerST runs (hnswalg.h:323-437; searchKnn's choice, hnswalg.h:1304-1312).
+int or_hnsw_search_allowed(void* h, int space, const float* q, size_t nq, size_t k, size_t ef,
+                           const int32_t* ef_per_query, const int32_t* k_per_query, size_t k_stride, int mode,
+                           int nthreads, const uint8_t* allowed, uint64_t* labels, float* dists, int64_t* counters,
+                           uint8_t* tie_flags);
+// Internal id of each element's label and its delete mark, for building `allowed` masks.
+void or_hnsw_labels(void* h, uint64_t* labels_by_id, uint8_t* deleted_by_id);
 
 // Exact brute force: the k lexicographically smallest (distance, label) pairs per query,
 // closest first.  labels_in may be null (labels = row index).

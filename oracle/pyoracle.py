@@ -44,6 +44,9 @@ def lib():
         L.or_hnsw_entry.restype = ctypes.c_uint32
         L.or_hnsw_entry.argtypes = [_vp]
         L.or_hnsw_search.argtypes = [_vp, _i32, _vp, _sz, _sz, _sz, _vp, _vp, _sz, _i32, _i32, _vp, _vp, _vp, _vp]
+        L.or_hnsw_search_allowed.argtypes = [_vp, _i32, _vp, _sz, _sz, _sz, _vp, _vp, _sz, _i32, _i32, _vp, _vp,
+                                             _vp, _vp, _vp]
+        L.or_hnsw_labels.argtypes = [_vp, _vp, _vp]
         L.or_bf_search.argtypes = [_i32, _sz, _vp, _vp, _sz, _vp, _sz, _sz, _i32, _vp, _vp]
         L.or_fp64_pairs.argtypes = [_i32, _sz, _vp, _vp, _vp, _vp, _sz, _vp]
         L.or_mlp_forward.argtypes = [_vp, _i32, _i32, _vp, _sz, _vp]
@@ -83,8 +86,17 @@ class OracleHnsw:
     def count(self):
         return lib().or_hnsw_count(self.h)
 
+    def labels(self):
+        """(label of each internal id, delete mark of each internal id)."""
+        n = self.count()
+        lab = np.empty(n, np.uint64)
+        dele = np.empty(n, np.uint8)
+        lib().or_hnsw_labels(self.h, _p(lab), _p(dele))
+        return lab, dele
+
     def search(self, space, queries, k, ef, mode=MODE_HEAP, nthreads=0, ef_per_query=None, k_per_query=None,
-               counters=False, ties=False):
+               counters=False, ties=False, allowed=None):
+        """allowed: optional uint8 mask over internal ids (the filter of searchKnn)."""
         q = np.ascontiguousarray(queries, np.float32)
         nq = q.shape[0]
         kmax = int(k if k_per_query is None else max(k, int(np.max(k_per_query))))
@@ -94,8 +106,9 @@ class OracleHnsw:
         tf = np.zeros(nq, np.uint8) if ties else None
         efp = None if ef_per_query is None else np.ascontiguousarray(ef_per_query, np.int32)
         kp = None if k_per_query is None else np.ascontiguousarray(k_per_query, np.int32)
-        r = lib().or_hnsw_search(self.h, space, _p(q), nq, k, ef, _p(efp), _p(kp), kmax, mode, nthreads,
-                                 _p(labels), _p(dists), _p(ctr), _p(tf))
+        al = None if allowed is None else np.ascontiguousarray(allowed, np.uint8)
+        r = lib().or_hnsw_search_allowed(self.h, space, _p(q), nq, k, ef, _p(efp), _p(kp), kmax, mode, nthreads,
+                                         _p(al), _p(labels), _p(dists), _p(ctr), _p(tf))
         if r < 0:
             raise RuntimeError(lib().or_last_error().decode())
         return labels, dists, ctr, tf, r

@@ -116,6 +116,52 @@ REF_API int ref_hnsw_search(void* hv, const float* q, size_t nq, size_t k, size_
     return short_rows;
 }
 
+// markDelete (hnswalg.h:852-868); returns -1 with the reference's message on error.
+REF_API int ref_hnsw_mark_deleted(void* hv, uint64_t label) {
+    try {
+        static_cast<RefHnsw*>(hv)->alg->markDelete((hnswlib::labeltype)label);
+        return 0;
+    } catch (std::exception& e) {
+        g_err = e.what();
+        return -1;
+    }
+}
+
+namespace {
+// A label filter as a caller of searchKnn would pass it (hnswlib.h:128-132).
+struct LabelMask : public hnswlib::BaseFilterFunctor {
+    const uint8_t* mask;
+    size_t n;
+    LabelMask(const uint8_t* m, size_t nn) : mask(m), n(nn) {}
+    bool operator()(hnswlib::labeltype l) override { return l < n && mask[l] != 0; }
+};
+}  // namespace
+
+// ref_hnsw_search with a filter over labels (allowed[label] != 0; null = no filter): searchKnn
+// takes the non-bare-bone branch whenever a filter is given or elements are deleted.
+REF_API int ref_hnsw_search_filtered(void* hv, const float* q, size_t nq, size_t k, size_t ef,
+                                     const uint8_t* allowed, size_t n_labels, uint64_t* labels, float* dists) {
+    RefHnsw* h = static_cast<RefHnsw*>(hv);
+    h->alg->setEf(ef);
+    size_t dim = *(size_t*)h->space.sp->get_dist_func_param();
+    LabelMask f(allowed, n_labels);
+    int short_rows = 0;
+    for (size_t r = 0; r < nq; r++) {
+        auto res = h->alg->searchKnn(q + r * dim, k, allowed ? &f : nullptr);
+        for (size_t i = 0; i < k; i++) {
+            labels[r * k + i] = UINT64_MAX;
+            dists[r * k + i] = __builtin_inff();
+        }
+        if (res.size() != k) short_rows++;
+        for (int i = (int)res.size() - 1; i >= 0; i--) {
+            labels[r * k + i] = res.top().second;
+            dists[r * k + i] = res.top().first;
+            res.pop();
+        }
+    }
+    return short_rows;
+}
+
 // BFIndex.knn_query semantics (bindings.cpp:850-929 over bruteforce.h:106-125).
 REF_API int ref_bf_search(int space, size_t dim, const float* data, size_t n, const float* q, size_t nq,
                           size_t k, uint64_t* labels, float* dists) {

@@ -2,7 +2,8 @@
 sources (oracle/_ref: the reference Python module built from python_bindings/bindings.cpp and
 the hnswlib headers with setup.py's flags).  Run in a container where /root/reference exists:
 
-    make -C oracle ref && python tests/golden/gen_golden.py
+    make -C oracle ref && python tests/golden/gen_golden.py        (all fixtures)
+                          python tests/golden/gen_golden.py nbb    (the deletion/filter fixture only)
 
 Every fixture stores its inputs' SHA-256 (or the inputs themselves when small), the reference's
 outputs and the SHA-256 of the reference's saveIndex bytes for single-threaded builds.
@@ -101,5 +102,42 @@ def main():
     np.savez_compressed(os.path.join(OUT, "cos21_inputs.npz"), data=data, queries=queries)
 
 
+def nbb():
+    """F5: the non-bare-bone branch on C1: deletions (every 5th label and the entry point's),
+    then a label filter on top (label % 3 != 0), then a 2%-selective filter (label % 50 == 1);
+    ef=50, k=10.  Stores the reference's saveIndex hash after the deletions."""
+    hnswlib = pyoracle.ref_module()
+    assert hnswlib is not None, "build the reference first: make -C oracle ref"
+    data = datasets.c1_uniform()
+    queries = datasets.c1_uniform(n=1000, seed=1)
+    p, _ = build_ref(hnswlib, "l2", data)
+    path = "/tmp/_golden_index.bin"
+    p.save_index(path)
+    entry = pyoracle.OracleHnsw(path=path)
+    entry_label = int(pyoracle.lib().or_hnsw_entry(entry.h))  # labels = insertion order
+    deleted = sorted(set(range(0, data.shape[0], 5)) | {entry_label})
+    for lab in deleted:
+        p.mark_deleted(lab)
+    p.save_index(path)
+    with open(path, "rb") as f:
+        blob = f.read()
+    os.remove(path)
+    out = dict(data_sha=sha(data), query_sha=sha(queries), deleted=np.array(deleted, np.uint64),
+               entry_label=entry_label, index_sha=hashlib.sha256(blob).hexdigest())
+    p.set_ef(50)
+    lab, dis = p.knn_query(queries, k=10, num_threads=1)
+    out["ids_del"], out["dists_del"] = lab.astype(np.uint64), dis.astype(np.float32)
+    lab, dis = p.knn_query(queries, k=10, num_threads=1, filter=lambda l: l % 3 != 0)
+    out["ids_del_f3"], out["dists_del_f3"] = lab.astype(np.uint64), dis.astype(np.float32)
+    lab, dis = p.knn_query(queries, k=10, num_threads=1, filter=lambda l: l % 50 == 1)
+    out["ids_del_f50"], out["dists_del_f50"] = lab.astype(np.uint64), dis.astype(np.float32)
+    np.savez_compressed(os.path.join(OUT, "c1_nbb_golden.npz"), **out)
+    print("c1_nbb_golden.npz", out["index_sha"][:16], "entry label", entry_label)
+
+
 if __name__ == "__main__":
-    main()
+    if len(sys.argv) > 1 and sys.argv[1] == "nbb":
+        nbb()
+    else:
+        main()
+        nbb()
