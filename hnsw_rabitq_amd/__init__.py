@@ -97,16 +97,43 @@ class GpuHnsw:
         _lib.call("hg_gpu_index_create", cpu_handle, device, ctypes.byref(h))
         self.handle = h
 
-    def search(self, queries, k, ef, counters=False):
+    def search(self, queries, k, ef, counters=False, filter=None):
+        """filter: None or a GpuFilter of this index (non-bare-bone search)."""
         q = np.ascontiguousarray(queries, dtype=np.float32)
         nq = q.shape[0]
         labels = np.empty((nq, k), dtype=np.uint64)
         dists = np.empty((nq, k), dtype=np.float32)
         ctr = np.empty((nq, 4), dtype=np.int32) if counters else None
-        code = _lib.load().hg_gpu_search(self.handle, _ptr(q), nq, k, ef, _ptr(labels), _ptr(dists), _ptr(ctr))
+        lib = _lib.load()
+        if filter is None:
+            code = lib.hg_gpu_search(self.handle, _ptr(q), nq, k, ef, _ptr(labels), _ptr(dists), _ptr(ctr))
+        else:
+            if filter.owner is not self:
+                raise ValueError("the filter was built for another GPU index")
+            code = lib.hg_gpu_search_filtered(self.handle, _ptr(q), nq, k, ef, filter.handle, _ptr(labels),
+                                              _ptr(dists), _ptr(ctr))
         if code not in (_lib.HG_OK, _lib.HG_ERR_SHORT):
             _lib.check(code)
         return labels, dists, ctr, code == _lib.HG_ERR_SHORT
+
+    def labels(self):
+        """External label of every internal id (getExternalLabel, hnswalg.h:185-189)."""
+        n = ctypes.c_size_t()
+        _lib.call("hg_gpu_index_labels", self.handle, None, 0, ctypes.byref(n))
+        out = np.empty(n.value, dtype=np.uint64)
+        _lib.call("hg_gpu_index_labels", self.handle, _ptr(out), n.value, ctypes.byref(n))
+        return out
+
+    def make_filter(self, allowed_labels):
+        return GpuFilter(self, allowed_labels)
+
+    def refresh_deleted(self, cpu_handle):
+        _lib.call("hg_gpu_refresh_deleted", self.handle, cpu_handle)
+
+    def status(self):
+        st = ctypes.c_uint32()
+        _lib.call("hg_gpu_search_status", self.handle, ctypes.byref(st))
+        return st.value
 
     def search_device(self, d_queries, nq, k, ef, d_labels, d_dists, d_counters=None, d_ef=None, d_k=None,
                       k_stride=None, stop_model=None, stream=None):
@@ -147,6 +174,38 @@ class GpuHnsw:
             pass
 
 
+class GpuFilter:
+    """The allowed set of a filtered search over one GPU index, as a bitmap over its elements.
+
+    hnswlib evaluates its filter functor (hnswlib.h:16-20) on each element the search reaches;
+    the GPU search reads the same answers from this bitmap, built once from the allowed labels.
+    """
+
+    def __init__(self, gpu_index, allowed_labels):
+        lab = np.ascontiguousarray(np.asarray(allowed_labels, dtype=np.uint64).reshape(-1))
+        h = ctypes.c_void_p()
+        _lib.call("hg_gpu_filter_create", gpu_index.handle, _ptr(lab), lab.shape[0], ctypes.byref(h))
+        self.handle = h
+        self.owner = gpu_index
+
+    @property
+    def allowed(self):
+        c = ctypes.c_size_t()
+        _lib.call("hg_gpu_filter_count", self.handle, ctypes.byref(c))
+        return c.value
+
+    def free(self):
+        if self.handle is not None and self.handle.value:
+            _lib.load().hg_gpu_filter_free(self.handle)
+        self.handle = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
 class Index:
     """hnswlib.Index mirror (bindings.cpp:143-722, 933-1010) with the GPU query path."""
 
@@ -157,6 +216,7 @@ class Index:
         self._h = None
         self._gpu = None
         self._dirty = True
+        self._marks_dirty = False
         self.default_ef = 10
         self.num_threads = os.cpu_count() or 1
         self.device = 0
@@ -261,12 +321,17 @@ class Index:
 
     # -- query (GPU) --
     def gpu(self):
-        """The device copy of the index, exported on first use after a change."""
+        """The device copy of the index, exported on first use after a change (delete marks
+        alone are re-read without a new export)."""
         if self._gpu is None or self._dirty:
             if self._gpu is not None:
                 self._gpu.free()
             self._gpu = GpuHnsw(self._h, self.device)
             self._dirty = False
+            self._marks_dirty = False
+        elif self._marks_dirty:
+            self._gpu.refresh_deleted(self._h)
+            self._marks_dirty = False
         return self._gpu
 
     def mark_deleted(self, label):
@@ -274,26 +339,44 @@ class Index:
         if self._h is None:
             raise RuntimeError("Index not initialized")
         _lib.call("hg_hnsw_mark_deleted", self._h, int(label))
-        self._dirty = True
+        self._marks_dirty = True
 
     def unmark_deleted(self, label):
         if self._h is None:
             raise RuntimeError("Index not initialized")
         _lib.call("hg_hnsw_unmark_deleted", self._h, int(label))
-        self._dirty = True
+        self._marks_dirty = True
+
+    def make_filter(self, allowed_labels):
+        """A reusable GpuFilter of the current export that allows the given labels."""
+        return self.gpu().make_filter(allowed_labels)
 
     def knn_query(self, data, k=1, num_threads=-1, filter=None):
-        if filter is not None:
-            raise RuntimeError("filter: the filtered (non-bare-bone) search is not implemented by the GPU engine")
+        """filter: a callable label -> bool, as in hnswlib (evaluated here once per element),
+        or a GpuFilter from make_filter."""
         if self._h is None:
             raise RuntimeError("Index not initialized")
+        if filter is not None and not (isinstance(filter, GpuFilter) or callable(filter)):
+            raise TypeError("filter must be a callable taking a label, or a GpuFilter")
         arr = _as_rows(data, self.dim)
         if self.normalize:
             arr = normalize_rows(arr)
         k = int(k)
         if arr.shape[0] == 0:
             return np.empty((0, k), dtype=np.uint64), np.empty((0, k), dtype=np.float32)
-        labels, dists, _, short = self.gpu().search(arr, k, self.default_ef)
+        gpu = self.gpu()
+        gf = None
+        if isinstance(filter, GpuFilter):
+            gf = filter
+        elif filter is not None:
+            labs = gpu.labels()
+            keep = np.fromiter((bool(filter(int(x))) for x in labs), dtype=bool, count=labs.shape[0])
+            gf = gpu.make_filter(labs[keep])
+        try:
+            labels, dists, _, short = gpu.search(arr, k, self.default_ef, filter=gf)
+        finally:
+            if gf is not None and gf is not filter:
+                gf.free()
         if short:
             raise RuntimeError("Cannot return the results in a contiguous 2D array. Probably ef or M is too small")
         return labels, dists

@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <map>
@@ -27,11 +28,17 @@ namespace {
 // GPU copy of an HNSW graph.
 // ------------------------------------------------------------------------------------------
 struct GpuIndex {
+    uint64_t serial = 0;  // unique per export (filters name the export they were built for)
     int device = 0;
     int cus = 0;
     hg::DevGraph g{};
     DevBuf vec, links0, upper_off, upper, labels;
     DevBuf work;
+    // non-bare-bone search: delete marks as a bitmap over internal ids, the status word of the
+    // last filtered launch, and the host labels (filters are built from label lists)
+    DevBuf delbits, status;
+    size_t num_deleted = 0;
+    std::vector<uint64_t> host_labels;
     // host-path staging
     DevBuf q, ol, od, ctr;
     hipStream_t stream = nullptr;
@@ -56,12 +63,36 @@ uint32_t ceil_log2(uint64_t x) {
     return b;
 }
 
-Plan make_plan(GpuIndex* gi, size_t max_ef, size_t max_k, size_t nq) {
+// Membership filter of one exported index: bit i set = the label of internal id i is allowed
+// (the isIdAllowed functor of hnswalg.h:325-326, 406-407, evaluated once per element).
+struct GpuFilter {
+    GpuIndex* gi = nullptr;
+    uint64_t serial = 0;
+    int device = 0;
+    size_t n = 0;
+    DevBuf bits;
+    size_t allowed = 0;
+};
+
+std::atomic<uint64_t> g_export_serial{0};
+
+// Largest beam of the non-bare-bone search: every admitted key stays in the beam until the
+// result set holds ef allowed entries, so a selective filter needs about ef / (allowed fraction)
+// entries.  8192 entries = 64 KiB of LDS.
+constexpr uint32_t kFiltMaxCap = 8192;
+
+Plan make_plan(GpuIndex* gi, size_t max_ef, size_t max_k, size_t nq, bool filtered = false,
+               double allowed_frac = 1.0, uint32_t min_cap = 0) {
     size_t ef_eff = std::max(max_ef, max_k);
     Plan p;
     p.cap = (uint32_t)((ef_eff + 63) / 64 * 64);
-    auto it = gi->tuned.find(p.cap);
-    if (it != gi->tuned.end()) {
+    auto it = filtered ? gi->tuned.end() : gi->tuned.find(p.cap);
+    if (filtered) {
+        const double want = 2.0 * (double)ef_eff / std::max(allowed_frac, 1e-4) + 256.0;
+        const double c = std::min<double>(kFiltMaxCap, std::max<double>({want, (double)p.cap, (double)min_cap}));
+        p.cap = std::min<uint32_t>(kFiltMaxCap, ((uint32_t)c + 63) / 64 * 64);
+        p.hash_bits = std::min<uint32_t>(13, std::max<uint32_t>(10, ceil_log2(4 * (uint64_t)p.cap)));
+    } else if (it != gi->tuned.end()) {
         p.hash_bits = it->second.first;
     } else {
         // visited nodes grow roughly as 16-32 per beam entry at M=16; the hash may be lossy
@@ -69,13 +100,21 @@ Plan make_plan(GpuIndex* gi, size_t max_ef, size_t max_k, size_t nq) {
         p.hash_bits = std::min<uint32_t>(12, std::max<uint32_t>(10, ceil_log2(8 * ef_eff)));
     }
     // the output pass stages k labels (8 bytes each) in the hash area
-    p.hash_bits = std::max(p.hash_bits, ceil_log2(2 * (uint64_t)std::max<size_t>(max_k, 1)));
+    const uint32_t min_bits = ceil_log2(2 * (uint64_t)std::max<size_t>(max_k, 1));
+    p.hash_bits = std::max(p.hash_bits, min_bits);
+    if (filtered) {
+        // a CU holds 160 KiB of LDS: shrink the (lossy) hash, then the beam, to fit
+        while (hg::search_lds_bytes(gi->g.dim, p.cap, p.hash_bits) > 160 * 1024 && p.hash_bits > std::max(10u, min_bits))
+            p.hash_bits--;
+        while (hg::search_lds_bytes(gi->g.dim, p.cap, p.hash_bits) > 160 * 1024 && p.cap > 64) p.cap -= 64;
+        require(p.cap >= ef_eff, "ef and dimension too large for the filtered GPU search");
+    }
     hg::SearchParams sp{};
     sp.g = gi->g;
     sp.cap = p.cap;
     sp.hash_bits = p.hash_bits;
     p.lds = hg::search_lds_bytes(gi->g.dim, p.cap, p.hash_bits);
-    int per_cu = hg::search_blocks_per_cu(sp);
+    int per_cu = hg::search_blocks_per_cu(sp, filtered);
     if (it != gi->tuned.end() && it->second.second > 0) per_cu = std::min(per_cu, it->second.second);
     size_t blocks = (size_t)gi->cus * (size_t)per_cu;
     p.blocks = (int)std::max<size_t>(1, std::min(blocks, nq));
@@ -92,10 +131,24 @@ hg::SearchParams base_params(GpuIndex* gi, const Plan& pl) {
     return sp;
 }
 
+// Fraction of the elements a query may return (not deleted, allowed by the filter), used to
+// size the beam of the non-bare-bone search.
+double allowed_fraction(const GpuIndex* gi, const GpuFilter* f) {
+    const double n = std::max<double>(1.0, (double)gi->g.n);
+    double frac = ((double)gi->g.n - (double)gi->num_deleted) / n;
+    if (f) frac = std::min(frac, (double)f->allowed / n);
+    return frac;
+}
+
+// The non-bare-bone branch runs when the index holds deleted elements or a filter is given
+// (hnswalg.h:1304-1312).  status: device word, bit 0 set by a query that outgrew the beam.
 hipError_t run_search(GpuIndex* gi, const float* dq, size_t nq, const int32_t* def, const int32_t* dk, size_t ef,
                       size_t k, size_t k_stride, uint64_t* dl, float* dd, int32_t* dc, StopModel* stop,
-                      hipStream_t s) {
-    Plan pl = make_plan(gi, ef, k, nq);
+                      hipStream_t s, const GpuFilter* filt = nullptr, uint32_t* status = nullptr,
+                      uint32_t min_cap = 0) {
+    const bool filtered = gi->num_deleted > 0 || filt != nullptr;
+    require(!(filtered && stop), "the learned stop is not available with deleted elements or a filter");
+    Plan pl = make_plan(gi, ef, k, nq, filtered, filtered ? allowed_fraction(gi, filt) : 1.0, min_cap);
     hg::SearchParams sp = base_params(gi, pl);
     sp.queries = dq;
     sp.nq = (uint32_t)nq;
@@ -107,11 +160,24 @@ hipError_t run_search(GpuIndex* gi, const float* dq, size_t nq, const int32_t* d
     sp.out_labels = dl;
     sp.out_dists = dd;
     sp.counters = dc;
+    if (filtered) {
+        sp.del_bits = gi->num_deleted > 0 ? (const uint32_t*)gi->delbits.p : nullptr;
+        sp.allow_bits = filt ? (const uint32_t*)filt->bits.p : nullptr;
+        sp.status = status ? status : (uint32_t*)gi->status.p;
+    }
     if (stop) {
         sp.use_stop = 1;
         sp.stop = dev_stop(stop);
     }
     return hg::launch_search(sp, pl.blocks, s);
+}
+
+GpuFilter* as_filter(GpuIndex* gi, void* f) {
+    if (!f) return nullptr;
+    auto* gf = static_cast<GpuFilter*>(f);
+    require(gf->gi == gi && gf->serial == gi->serial && gf->n == gi->host_labels.size(),
+            "the filter was built for another GPU index");
+    return gf;
 }
 
 void check_search_args(GpuIndex* gi, size_t nq, size_t ef, size_t k, size_t k_stride) {
@@ -246,17 +312,14 @@ HG_API int hg_gpu_index_create(void* hv, int device, void** out) {
         auto* h = static_cast<hnswgpu::HnswIndex*>(hv);
         size_t n = h->count();
         require(n > 0, "the index is empty");
-        require(n < (1ull << 31), "more than 2^31 elements are not supported by the GPU search");
-        if (h->num_deleted() != 0)
-            throw Error(HG_ERR_INVALID,
-                        "indexes with deleted elements need the filtered (non-bare-bone) search, which the GPU "
-                        "engine does not implement");
+        require(n < (1ull << 30), "more than 2^30 elements are not supported by the GPU search");
         int ndev = 0;
         hip_check(hipGetDeviceCount(&ndev), "hipGetDeviceCount");
         require(device >= 0 && device < ndev, "invalid device");
         DeviceGuard dg(device);
         std::unique_ptr<GpuIndex> gi(new GpuIndex());
         gi->device = device;
+        gi->serial = ++g_export_serial;
         hipDeviceProp_t prop;
         hip_check(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
         gi->cus = prop.multiProcessorCount;
@@ -311,6 +374,17 @@ HG_API int hg_gpu_index_create(void* hv, int device, void** out) {
         upload(gi->upper_off, uoff.data(), uoff.size() * 4);
         upload(gi->upper, up.data(), up.size() * 4);
         upload(gi->labels, labels.data(), labels.size() * 8);
+        {
+            std::vector<uint32_t> del((n + 31) / 32, 0u);
+            for (size_t i = 0; i < n; i++)
+                if (h->is_deleted((uint32_t)i)) {
+                    del[i >> 5] |= 1u << (i & 31);
+                    gi->num_deleted++;
+                }
+            upload(gi->delbits, del.data(), del.size() * 4);
+        }
+        gi->status.reserve(256);
+        gi->host_labels = std::move(labels);
         gi->work.reserve(256);
         hip_check(hipStreamCreateWithFlags(&gi->stream, hipStreamNonBlocking), "hipStreamCreate");
         hg::DevGraph& g = gi->g;
@@ -351,36 +425,164 @@ HG_API int hg_gpu_index_bytes(void* g, size_t* bytes) {
     });
 }
 
+namespace {
+// Host-buffer search (the knn_query path): upload, one launch, download.  The non-bare-bone
+// search that outgrows its beam is run again at the largest beam; if a query still needs more,
+// the call fails rather than return a list that may differ from the reference's.
+int search_host(GpuIndex* gi, const float* queries, size_t nq, size_t k, size_t ef, const GpuFilter* filt,
+                uint64_t* labels, float* dists, int32_t* counters) {
+    check_search_args(gi, nq, ef, k, k);
+    if (nq == 0) return HG_OK;
+    require(queries != nullptr, "null queries");
+    DeviceGuard dg(gi->device);
+    const size_t dim = gi->g.dim;
+    gi->q.reserve(nq * dim * 4);
+    gi->ol.reserve(nq * k * 8);
+    gi->od.reserve(nq * k * 4);
+    if (counters) gi->ctr.reserve(nq * 16);
+    hipStream_t s = gi->stream;
+    const bool filtered = gi->num_deleted > 0 || filt != nullptr;
+    hip_check(hipMemcpyAsync(gi->q.p, queries, nq * dim * 4, hipMemcpyHostToDevice, s), "hipMemcpy H2D");
+    for (uint32_t min_cap = 0;;) {
+        hip_check(run_search(gi, (const float*)gi->q.p, nq, nullptr, nullptr, ef, k, k, (uint64_t*)gi->ol.p,
+                             (float*)gi->od.p, counters ? (int32_t*)gi->ctr.p : nullptr, nullptr, s, filt, nullptr,
+                             min_cap),
+                  "search launch");
+        if (!filtered) break;
+        uint32_t status = 0;
+        hip_check(hipMemcpyAsync(&status, gi->status.p, 4, hipMemcpyDeviceToHost, s), "hipMemcpy D2H");
+        hip_check(hipStreamSynchronize(s), "search");
+        if ((status & 1u) == 0) break;
+        const Plan used = make_plan(gi, ef, k, nq, true, allowed_fraction(gi, filt), min_cap);
+        if (used.cap >= kFiltMaxCap || min_cap >= kFiltMaxCap) {
+            g_err = "filtered search: a query needed more than " + std::to_string(kFiltMaxCap) +
+                    " beam entries (the allowed elements are too few for this ef)";
+            return HG_ERR_RUNTIME;
+        }
+        min_cap = kFiltMaxCap;
+    }
+    hip_check(hipMemcpyAsync(labels, gi->ol.p, nq * k * 8, hipMemcpyDeviceToHost, s), "hipMemcpy D2H");
+    hip_check(hipMemcpyAsync(dists, gi->od.p, nq * k * 4, hipMemcpyDeviceToHost, s), "hipMemcpy D2H");
+    if (counters) hip_check(hipMemcpyAsync(counters, gi->ctr.p, nq * 16, hipMemcpyDeviceToHost, s), "hipMemcpy D2H");
+    hip_check(hipStreamSynchronize(s), "search");
+    for (size_t r = 0; r < nq; r++)
+        if (labels[r * k + k - 1] == ~0ull) {
+            g_err = "Cannot return the results in a contiguous 2D array. Probably ef or M is too small";
+            return HG_ERR_SHORT;
+        }
+    return HG_OK;
+}
+}  // namespace
+
 HG_API int hg_gpu_search(void* gv, const float* queries, size_t nq, size_t k, size_t ef, uint64_t* labels,
                          float* dists, int32_t* counters) {
     return guarded([&] {
         require(gv && labels && dists, "null argument");
         auto* gi = static_cast<GpuIndex*>(gv);
-        check_search_args(gi, nq, ef, k, k);
-        if (nq == 0) return HG_OK;
-        require(queries != nullptr, "null queries");
+        std::lock_guard<std::mutex> lk(gi->mu);
+        return search_host(gi, queries, nq, k, ef, nullptr, labels, dists, counters);
+    });
+}
+
+HG_API int hg_gpu_search_filtered(void* gv, const float* queries, size_t nq, size_t k, size_t ef, void* filter,
+                                  uint64_t* labels, float* dists, int32_t* counters) {
+    return guarded([&] {
+        require(gv && labels && dists, "null argument");
+        auto* gi = static_cast<GpuIndex*>(gv);
+        std::lock_guard<std::mutex> lk(gi->mu);
+        return search_host(gi, queries, nq, k, ef, as_filter(gi, filter), labels, dists, counters);
+    });
+}
+
+HG_API int hg_gpu_filter_create(void* gv, const uint64_t* allowed_labels, size_t n_allowed, void** out) {
+    return guarded([&] {
+        require(gv && out && (n_allowed == 0 || allowed_labels), "null argument");
+        auto* gi = static_cast<GpuIndex*>(gv);
+        std::vector<uint64_t> sorted(allowed_labels, allowed_labels + n_allowed);
+        std::sort(sorted.begin(), sorted.end());
+        const size_t n = gi->host_labels.size();
+        std::vector<uint32_t> bits((n + 31) / 32, 0u);
+        std::unique_ptr<GpuFilter> f(new GpuFilter());
+        for (size_t i = 0; i < n; i++)
+            if (std::binary_search(sorted.begin(), sorted.end(), gi->host_labels[i])) {
+                bits[i >> 5] |= 1u << (i & 31);
+                f->allowed++;
+            }
+        DeviceGuard dg(gi->device);
+        f->gi = gi;
+        f->serial = gi->serial;
+        f->device = gi->device;
+        f->n = n;
+        f->bits.reserve(std::max<size_t>(bits.size() * 4, 16));
+        hip_check(hipMemcpy(f->bits.p, bits.data(), bits.size() * 4, hipMemcpyHostToDevice), "hipMemcpy H2D");
+        *out = f.release();
+        return HG_OK;
+    });
+}
+
+HG_API int hg_gpu_filter_free(void* f) {
+    if (!f) return HG_OK;
+    auto* gf = static_cast<GpuFilter*>(f);
+    int prev = -1;
+    (void)hipGetDevice(&prev);
+    (void)hipSetDevice(gf->device);
+    delete gf;
+    if (prev >= 0) (void)hipSetDevice(prev);
+    return HG_OK;
+}
+
+HG_API int hg_gpu_filter_count(void* f, size_t* allowed) {
+    return guarded([&] {
+        require(f && allowed, "null argument");
+        *allowed = static_cast<GpuFilter*>(f)->allowed;
+        return HG_OK;
+    });
+}
+
+HG_API int hg_gpu_search_status(void* gv, uint32_t* status) {
+    return guarded([&] {
+        require(gv && status, "null argument");
+        auto* gi = static_cast<GpuIndex*>(gv);
+        DeviceGuard dg(gi->device);
+        hip_check(hipMemcpy(status, gi->status.p, 4, hipMemcpyDeviceToHost), "hipMemcpy D2H");
+        return HG_OK;
+    });
+}
+
+// Re-reads the delete marks of the CPU index the GPU index was exported from.
+HG_API int hg_gpu_refresh_deleted(void* gv, void* cpu) {
+    return guarded([&] {
+        require(gv && cpu, "null argument");
+        auto* gi = static_cast<GpuIndex*>(gv);
+        auto* h = static_cast<hnswgpu::HnswIndex*>(cpu);
+        const size_t n = gi->host_labels.size();
+        require(h->count() == n, "the CPU index changed size since the export; export it again");
+        std::vector<uint32_t> del((n + 31) / 32, 0u);
+        size_t nd = 0;
+        for (size_t i = 0; i < n; i++) {
+            require(h->label((uint32_t)i) == gi->host_labels[i], "the CPU index changed since the export");
+            if (h->is_deleted((uint32_t)i)) {
+                del[i >> 5] |= 1u << (i & 31);
+                nd++;
+            }
+        }
         std::lock_guard<std::mutex> lk(gi->mu);
         DeviceGuard dg(gi->device);
-        const size_t dim = gi->g.dim;
-        gi->q.reserve(nq * dim * 4);
-        gi->ol.reserve(nq * k * 8);
-        gi->od.reserve(nq * k * 4);
-        if (counters) gi->ctr.reserve(nq * 16);
-        hipStream_t s = gi->stream;
-        hip_check(hipMemcpyAsync(gi->q.p, queries, nq * dim * 4, hipMemcpyHostToDevice, s), "hipMemcpy H2D");
-        hip_check(run_search(gi, (const float*)gi->q.p, nq, nullptr, nullptr, ef, k, k, (uint64_t*)gi->ol.p,
-                             (float*)gi->od.p, counters ? (int32_t*)gi->ctr.p : nullptr, nullptr, s),
-                  "search launch");
-        hip_check(hipMemcpyAsync(labels, gi->ol.p, nq * k * 8, hipMemcpyDeviceToHost, s), "hipMemcpy D2H");
-        hip_check(hipMemcpyAsync(dists, gi->od.p, nq * k * 4, hipMemcpyDeviceToHost, s), "hipMemcpy D2H");
-        if (counters)
-            hip_check(hipMemcpyAsync(counters, gi->ctr.p, nq * 16, hipMemcpyDeviceToHost, s), "hipMemcpy D2H");
-        hip_check(hipStreamSynchronize(s), "search");
-        for (size_t r = 0; r < nq; r++)
-            if (labels[r * k + k - 1] == ~0ull) {
-                g_err = "Cannot return the results in a contiguous 2D array. Probably ef or M is too small";
-                return HG_ERR_SHORT;
-            }
+        hip_check(hipMemcpy(gi->delbits.p, del.data(), del.size() * 4, hipMemcpyHostToDevice), "hipMemcpy H2D");
+        gi->num_deleted = nd;
+        return HG_OK;
+    });
+}
+
+HG_API int hg_gpu_index_labels(void* gv, uint64_t* labels, size_t capacity, size_t* count) {
+    return guarded([&] {
+        require(gv && count, "null argument");
+        auto* gi = static_cast<GpuIndex*>(gv);
+        *count = gi->host_labels.size();
+        if (labels) {
+            require(capacity >= gi->host_labels.size(), "label buffer too small");
+            std::copy(gi->host_labels.begin(), gi->host_labels.end(), labels);
+        }
         return HG_OK;
     });
 }
@@ -403,6 +605,24 @@ HG_API int hg_gpu_search_device(void* gv, const float* dq, size_t nq, const int3
     });
 }
 
+HG_API int hg_gpu_search_device_filtered(void* gv, const float* dq, size_t nq, const int32_t* def,
+                                         const int32_t* dk, size_t ef, size_t k, size_t k_stride, uint64_t* dl,
+                                         float* dd, int32_t* dc, void* filter, uint32_t* d_status, void* stream) {
+    return guarded([&] {
+        require(gv && dl && dd, "null argument");
+        auto* gi = static_cast<GpuIndex*>(gv);
+        check_search_args(gi, nq, ef, k, k_stride);
+        if (nq == 0) return HG_OK;
+        require(dq != nullptr, "null queries");
+        DeviceGuard dg(gi->device);
+        std::lock_guard<std::mutex> lk(gi->mu);
+        hip_check(run_search(gi, dq, nq, def, dk, ef, k, k_stride, dl, dd, dc, nullptr, (hipStream_t)stream,
+                             as_filter(gi, filter), d_status),
+                  "search launch");
+        return HG_OK;
+    });
+}
+
 // Trace collection for the learned stop: the plain search, recording every `period`
 // expansions the 12 traversal features and the current top-k ids.
 HG_API int hg_gpu_collect_traces(void* gv, const float* dq, size_t nq, size_t ef, size_t k, int period,
@@ -413,6 +633,7 @@ HG_API int hg_gpu_collect_traces(void* gv, const float* dq, size_t nq, size_t ef
         require(period >= 1 && max_records >= 1, "period and max_records must be positive");
         auto* gi = static_cast<GpuIndex*>(gv);
         check_search_args(gi, nq, ef, k, k);
+        require(gi->num_deleted == 0, "traces of the learned stop need an index without deleted elements");
         if (nq == 0) return HG_OK;
         DeviceGuard dg(gi->device);
         std::lock_guard<std::mutex> lk(gi->mu);
@@ -465,6 +686,7 @@ HG_API int hg_gpu_autotune(void* gv, const float* dq, size_t nq, size_t ef, size
         DeviceGuard dg(gi->device);
         std::lock_guard<std::mutex> lk(gi->mu);
         hipStream_t s = (hipStream_t)stream;
+        require(gi->num_deleted == 0, "the autotuner times the bare-bone search: the index has deleted elements");
         DevBuf ol, od;
         ol.reserve(nq * k * 8);
         od.reserve(nq * k * 4);

@@ -7,7 +7,8 @@ namespace hg {
 
 constexpr uint32_t SENT = 0xFFFFFFFFu;    // empty link slot / empty hash slot / no result
 constexpr uint32_t XFLAG = 0x80000000u;   // "expanded" mark carried in a beam id
-constexpr uint32_t ID_MASK = 0x7FFFFFFFu;
+constexpr uint32_t DFLAG = 0x40000000u;   // "not in the result set" (deleted or filtered out)
+constexpr uint32_t ID_MASK = 0x3FFFFFFFu; // ids below 2^30
 
 // GPU layout of an HNSW graph (exported once from the CPU index):
 //   vec      [n][stride] f32, stride = dim rounded up to 4 (16-byte rows), zero padded
@@ -62,6 +63,12 @@ struct SearchParams {
     int32_t* trace_nrec;        // [nq]
     uint32_t* trace_final;      // [nq][k] internal ids of the final top-k
     int32_t trace_period, trace_max_rec;
+    // non-bare-bone search (deletions / label filter): bitmaps over internal ids, bit set =
+    // deleted (del_bits) or allowed (allow_bits); either may be null.  status bit 0 is set when
+    // a query needed more beam entries than cap (its result is then not exact).
+    const uint32_t* del_bits;
+    const uint32_t* allow_bits;
+    uint32_t* status;
 };
 
 constexpr int kStopFeatures = 12;

@@ -10,7 +10,7 @@ namespace hg {
 
 // ---- graph search (search.hip) ----
 size_t search_lds_bytes(uint32_t dim, uint32_t cap, uint32_t hash_bits);
-int search_blocks_per_cu(const SearchParams& p);
+int search_blocks_per_cu(const SearchParams& p, bool filtered = false);
 hipError_t launch_search(const SearchParams& p, int blocks, hipStream_t s);
 hipError_t launch_copy(void* dst, const void* src, size_t bytes, hipStream_t s);
 hipError_t launch_id_labels(const DevGraph& g, const uint32_t* ids, size_t n, uint64_t* out, hipStream_t s);

@@ -238,7 +238,20 @@ __device__ float mlp_logit(const DevStopModel& m, const MlpRegs& w, const float*
 // The search kernel.  Block = one wave (64 lanes) = one query at a time; workgroups loop over
 // queries taken from a global counter until none are left.
 // ------------------------------------------------------------------------------------------
-template <int SPACE, int MODE, int NCH, bool STOP, bool TRACE>
+// Membership of an internal id in the result set of the non-bare-bone search (hnswalg.h:325-326,
+// 406-407): not marked deleted and allowed by the filter.
+__device__ __forceinline__ bool id_allowed(uint32_t wdel, uint32_t wal, uint32_t id) {
+    const uint32_t b = id & 31u;
+    return ((wdel >> b) & 1u) == 0u && ((wal >> b) & 1u) != 0u;
+}
+
+// FILT: the non-bare-bone branch of searchBaseLayerST (hnswalg.h:323-437) in (distance, id)
+// order.  Beam entries carry DFLAG when deleted or filtered out; such entries are traversed but
+// never returned.  The beam ends right after the ef-th allowed entry once there are ef of them
+// (nothing beyond it can be expanded: the reference stops at the first candidate farther than
+// the result set's worst once that set is full); before that every admitted key is kept, up to
+// the capacity `cap` (status bit 0 flags a query that needed more).
+template <int SPACE, int MODE, int NCH, bool STOP, bool TRACE, bool FILT>
 __global__ __launch_bounds__(64) void hnsw_search_kernel(SearchParams P) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int lane = threadIdx.x;
@@ -336,9 +349,15 @@ __global__ __launch_bounds__(64) void hnsw_search_kernel(SearchParams P) {
 
         // ---- level 0 ----
         int n = 1;
+        int nallowed = 1;  // allowed entries in the beam (FILT)
+        if (FILT) {
+            const uint32_t wdel = P.del_bits ? P.del_bits[cur >> 5] : 0u;
+            const uint32_t wal = P.allow_bits ? P.allow_bits[cur >> 5] : ~0u;
+            nallowed = id_allowed(wdel, wal, cur) ? 1 : 0;
+        }
         if (lane == 0) {
             bd[0] = curdist;
-            bi[0] = cur;
+            bi[0] = cur | ((FILT && nallowed == 0) ? DFLAG : 0u);
             hash_insert(hash, hmask, cur);
         }
         E0 = 1;
@@ -377,9 +396,10 @@ __global__ __launch_bounds__(64) void hnsw_search_kernel(SearchParams P) {
                 }
             }
             if (p < 0) break;
-            const uint32_t node = bi[p] & ID_MASK;
+            const uint32_t bip = bi[p];
+            const uint32_t node = bip & ID_MASK;
             __syncthreads();
-            if (lane == 0) bi[p] = node | XFLAG;
+            if (lane == 0) bi[p] = bip | XFLAG;
             H0++;
 
             // neighbours of node: keep those not visited, in list order
@@ -419,16 +439,32 @@ __global__ __launch_bounds__(64) void hnsw_search_kernel(SearchParams P) {
             }
             __syncthreads();
             if (cnt > 0) {
+                // FILT: the membership words of the fresh ids load with the vectors
+                uint32_t fw_del = 0u, fw_al = ~0u;
+                if (FILT && lane < cnt) {
+                    const uint32_t fid = si[lane];
+                    if (P.del_bits) fw_del = P.del_bits[fid >> 5];
+                    if (P.allow_bits) fw_al = P.allow_bits[fid >> 5];
+                }
                 dist_batch<SPACE, MODE, NCH>(g, qs, si, cnt, sd, lane);
                 __syncthreads();
+                if (FILT) {
+                    if (lane < cnt) {
+                        const uint32_t fid = si[lane];
+                        if (!id_allowed(fw_del, fw_al, fid)) si[lane] = fid | DFLAG;
+                    }
+                    __syncthreads();
+                }
                 E0 += cnt;
                 // admission against the current worst (when full) and sort of the batch
                 float d = INFINITY;
-                uint32_t id = SENT;
+                uint32_t id = SENT, idf = SENT;  // idf: id with its DFLAG (FILT)
                 if (lane < cnt) {
                     d = sd[lane];
-                    id = si[lane];
-                    if (n >= ef_eff && !key_less(d, id, bd[n - 1], bi[n - 1] & ID_MASK)) {
+                    idf = si[lane];
+                    id = idf & ID_MASK;
+                    const bool full = FILT ? nallowed >= ef_eff : n >= ef_eff;
+                    if (full && !key_less(d, id, bd[n - 1], bi[n - 1] & ID_MASK)) {
                         d = INFINITY;
                         id = SENT;
                     }
@@ -465,7 +501,9 @@ __global__ __launch_bounds__(64) void hnsw_search_kernel(SearchParams P) {
                         // Entry i moves by the number of staged insertion points <= i.  A batch
                         // writes only at or above its lowest chunk base, which the next (lower)
                         // batch never reads, and a wave's LDS operations execute in order.
-                        const int new_n = n + m < ef_eff ? n + m : ef_eff;
+                        const int lim = FILT ? (int)cap : ef_eff;  // entries kept by position
+                        const int new_n = n + m < lim ? n + m : lim;
+                        if (FILT && n + m > lim && lane == 0) atomicOr(P.status, 1u);
                         const int hi_chunk = (n - 1) & ~63;
                         const int lo_chunk = first_pos & ~63;
                         // Reads are unconditional (indices below cap, clamped at 0); a lane whose
@@ -495,16 +533,16 @@ __global__ __launch_bounds__(64) void hnsw_search_kernel(SearchParams P) {
                             for (int u = 0; u < 8; u++) {
                                 const int i = base - 64 * u;
                                 const int np = i + sh[u];
-                                const bool w = (b0 - 64 * u >= lo_chunk) & (i >= first_pos) & (i < n) & (np < ef_eff);
+                                const bool w = (b0 - 64 * u >= lo_chunk) & (i >= first_pos) & (i < n) & (np < lim);
                                 bd[w ? np : trash_d] = vd[u];
                                 bi[w ? np : trash_i] = vi[u];
                             }
                             __asm__ volatile("" ::: "memory");
                         }
                         const int fp = pos + r;
-                        if (ok && fp < ef_eff) {
+                        if (ok && fp < lim) {
                             bd[fp] = d;
-                            bi[fp] = id;
+                            bi[fp] = FILT ? idf : id;
                         }
                         const int min_fp = first_pos;
                         // Every position below min(first insert, p) holds an old entry whose old
@@ -520,6 +558,29 @@ __global__ __launch_bounds__(64) void hnsw_search_kernel(SearchParams P) {
                         }
                         n = new_n;
                         __syncthreads();
+                        if (FILT) {
+                            nallowed += __popcll(ballot(ok && (idf & DFLAG) == 0u));
+                            if (nallowed >= ef_eff) {
+                                // end the beam right after the ef-th allowed entry: drop the last
+                                // (nallowed - ef) allowed entries and whatever lies beyond
+                                int remaining = nallowed - ef_eff, newn = n;
+                                for (int b = (n - 1) & ~63; b >= 0; b -= 64) {
+                                    const int i = b + lane;
+                                    const uint64_t am = ballot(i < n && (bi[i] & DFLAG) == 0u);
+                                    const int c = __popcll(am);
+                                    if (c > remaining) {
+                                        uint64_t t = am;
+                                        for (int j = 1; j < c - remaining; j++) t &= t - 1;
+                                        newn = b + __ffsll((unsigned long long)t);
+                                        break;
+                                    }
+                                    remaining -= c;
+                                }
+                                n = newn;
+                                nallowed = ef_eff;
+                                scan_lo = scan_lo < n ? scan_lo : n;
+                            }
+                        }
                     } else {
                         scan_lo = p;
                     }
@@ -577,6 +638,32 @@ __global__ __launch_bounds__(64) void hnsw_search_kernel(SearchParams P) {
             }
         }
 
+        if (FILT) {
+            // the result set: allowed entries in beam order, compacted to the front in place
+            // (a chunk's reads precede its writes, and every entry moves down or stays)
+            int got = 0;
+            for (int b = 0; b < n && got < kq; b += 64) {
+                const int i = b + lane;
+                float dv = 0.f;
+                uint32_t iv = 0u;
+                if (i < n) {
+                    dv = bd[i];
+                    iv = bi[i];
+                }
+                const bool al = i < n && (iv & DFLAG) == 0u;
+                const uint64_t am = ballot(al);
+                const int slot = got + lanes_below(am, lane);
+                __asm__ volatile("" ::: "memory");
+                if (al && slot < kq) {
+                    bd[slot] = dv;
+                    bi[slot] = iv;
+                }
+                __asm__ volatile("" ::: "memory");
+                got += __popcll(am);
+            }
+            n = got < kq ? got : kq;
+            __syncthreads();
+        }
         // ---- output: first k entries, rows ordered by (distance, label) ----
         const int kk = kq < n ? kq : n;
         uint64_t* lab_s = reinterpret_cast<uint64_t*>(hash);  // hash no longer needed
@@ -663,9 +750,9 @@ size_t search_lds_bytes(uint32_t dim, uint32_t cap, uint32_t hash_bits) {
     return (size_t)dimpad * 4 + (size_t)cap * 8 + ((size_t)1 << hash_bits) * 4 + 64 * 8 + (16 + 128) * 4;
 }
 
-template <int SPACE, int MODE, int NCH, bool STOP, bool TRACE>
+template <int SPACE, int MODE, int NCH, bool STOP, bool TRACE, bool FILT = false>
 static hipError_t launch_t(const SearchParams& p, int blocks, size_t lds, hipStream_t s) {
-    auto k = hnsw_search_kernel<SPACE, MODE, NCH, STOP, TRACE>;
+    auto k = hnsw_search_kernel<SPACE, MODE, NCH, STOP, TRACE, FILT>;
     if (lds > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
@@ -692,6 +779,15 @@ hipError_t launch_search(const SearchParams& p, int blocks, hipStream_t s) {
     uint32_t* w = p.work;
     hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, s, w, 1u);
     bool stop = p.use_stop != 0, trace = p.trace_feat != nullptr;
+    if (p.del_bits || p.allow_bits) {
+        hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, s, p.status, 1u);
+        const bool v16 = p.g.dim % 16 == 0;
+        if (p.g.space == 0)
+            return v16 ? launch_t<0, 1, 0, false, false, true>(p, blocks, lds, s)
+                       : launch_t<0, 0, 0, false, false, true>(p, blocks, lds, s);
+        return v16 ? launch_t<1, 1, 0, false, false, true>(p, blocks, lds, s)
+                   : launch_t<1, 0, 0, false, false, true>(p, blocks, lds, s);
+    }
     if (p.g.space == 0) {
         if (trace) return launch_dim<0, false, true>(p, blocks, lds, s);
         if (stop) return launch_dim<0, true, false>(p, blocks, lds, s);
@@ -702,16 +798,20 @@ hipError_t launch_search(const SearchParams& p, int blocks, hipStream_t s) {
     return launch_dim<1, false, false>(p, blocks, lds, s);
 }
 
-int search_blocks_per_cu(const SearchParams& p) {
+int search_blocks_per_cu(const SearchParams& p, bool filtered) {
     size_t lds = search_lds_bytes(p.g.dim, p.cap, p.hash_bits);
     int nb = 0;
     hipError_t e;
-    if (p.g.dim % 16 == 0 && p.g.dim / 16 == 8 && p.g.space == 0)
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, hnsw_search_kernel<0, 1, 8, false, false>, 64, lds);
+    if (filtered)
+        e = p.g.space == 0
+                ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, hnsw_search_kernel<0, 1, 0, false, false, true>, 64, lds)
+                : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, hnsw_search_kernel<1, 1, 0, false, false, true>, 64, lds);
+    else if (p.g.dim % 16 == 0 && p.g.dim / 16 == 8 && p.g.space == 0)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, hnsw_search_kernel<0, 1, 8, false, false, false>, 64, lds);
     else if (p.g.space == 0)
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, hnsw_search_kernel<0, 1, 0, false, false>, 64, lds);
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, hnsw_search_kernel<0, 1, 0, false, false, false>, 64, lds);
     else
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, hnsw_search_kernel<1, 1, 0, false, false>, 64, lds);
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, hnsw_search_kernel<1, 1, 0, false, false, false>, 64, lds);
     if (e != hipSuccess || nb <= 0) nb = 1;
     return nb;
 }

@@ -13,7 +13,11 @@
  *   hg_hnsw_serialize     the same byte stream as saveIndex, into memory
  *   hg_gpu_index_create   (new) export of data_level0_memory_ / linkLists_ into the GPU layout
  *   hg_gpu_search         Index.knn_query bindings.cpp:612-697 over
- *                         HierarchicalNSW::searchKnn hnswalg.h:1269-1323 (bare-bone branch)
+ *                         HierarchicalNSW::searchKnn hnswalg.h:1269-1323 (bare-bone branch;
+ *                         the non-bare-bone branch when the index holds deleted elements)
+ *   hg_gpu_search_filtered  the same with a filter: knn_query(filter=) bindings.cpp:612-697 ->
+ *                         searchKnn(query, k, isIdAllowed), non-bare-bone searchBaseLayerST
+ *                         hnswalg.h:308-440
  *   hg_gpu_search_device  the same with device-resident queries/results, per-query ef and k
  *                         (extends the single ef_ of hnswalg.h:173-175)
  *   hg_bf_create / hg_bf_add   BruteforceSearch(space, maxElements) / addPoint
@@ -92,6 +96,37 @@ int hg_gpu_search(void* gpu_index, const float* queries, size_t nq, size_t k, si
 int hg_gpu_search_device(void* gpu_index, const float* d_queries, size_t nq, const int32_t* d_ef_per_query,
                          const int32_t* d_k_per_query, size_t ef, size_t k, size_t k_stride, uint64_t* d_labels,
                          float* d_dists, int32_t* d_counters, void* stop_model, void* stream);
+
+/* Filtered search (knn_query(..., filter=) bindings.cpp:612-697; BaseFilterFunctor
+ * hnswlib.h:16-20 evaluated by the non-bare-bone searchBaseLayerST hnswalg.h:308-440).
+ * A filter is built once per GPU index from the labels it allows (labels absent from the index
+ * are ignored) and may be reused by any number of searches.  Deleted elements (markDelete) are
+ * never returned, filter or not.  The beam holds every admitted element until ef allowed ones
+ * are found, so a very selective filter needs a large beam: the host search retries at the
+ * largest beam (8192 entries) and fails with HG_ERR_RUNTIME if a query still needs more. */
+int hg_gpu_filter_create(void* gpu_index, const uint64_t* allowed_labels, size_t n_allowed, void** out_filter);
+int hg_gpu_filter_free(void* filter);
+/* Elements of the index the filter allows. */
+int hg_gpu_filter_count(void* filter, size_t* allowed);
+/* hg_gpu_search with a filter (NULL = no filter). */
+int hg_gpu_search_filtered(void* gpu_index, const float* queries, size_t nq, size_t k, size_t ef, void* filter,
+                           uint64_t* labels, float* dists, int32_t* counters);
+/* Device-resident filtered search (as hg_gpu_search_device, no stop model).  d_status (device
+ * uint32, may be NULL = the index's own word) is cleared by the launch and gets bit 0 set when a
+ * query outgrew the beam; its results are then not those of the reference. */
+int hg_gpu_search_device_filtered(void* gpu_index, const float* d_queries, size_t nq, const int32_t* d_ef_per_query,
+                                  const int32_t* d_k_per_query, size_t ef, size_t k, size_t k_stride,
+                                  uint64_t* d_labels, float* d_dists, int32_t* d_counters, void* filter,
+                                  uint32_t* d_status, void* stream);
+/* The index's own status word (see above) after the last non-bare-bone launch that used it;
+ * synchronise the launch's stream first. */
+int hg_gpu_search_status(void* gpu_index, uint32_t* status);
+/* Re-reads the delete marks (markDelete / unmarkDelete) of the CPU index this GPU index was
+ * exported from, without a new export.  Fails if elements were added since the export. */
+int hg_gpu_refresh_deleted(void* gpu_index, void* index);
+/* Labels by internal id (getExternalLabel hnswalg.h:185-189 for every element); labels may be
+ * NULL to query the count. */
+int hg_gpu_index_labels(void* gpu_index, uint64_t* labels, size_t capacity, size_t* count);
 
 /* Launch configuration chosen by the autotuner for (max ef, max k) on this index:
  * out[0] = waves per workgroup, out[1] = visited-hash slots per wave, out[2] = beam capacity,

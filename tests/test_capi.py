@@ -61,8 +61,8 @@ def test_capacity_and_argument_errors(hg):
         p.add_items(np.zeros((2, 5), np.float32))
     with pytest.raises(RuntimeError, match="exceeds the specified limit"):
         p.add_items(np.random.default_rng(0).random((4, 4), dtype=np.float32))
-    with pytest.raises(RuntimeError, match="filter"):
-        p.knn_query(np.zeros(4, np.float32), k=1, filter=lambda i: True)
+    with pytest.raises(TypeError, match="filter must be a callable"):
+        p.knn_query(np.zeros(4, np.float32), k=1, filter=[1, 2])
 
 
 def test_multithreaded_build_is_valid(hg, orc):
