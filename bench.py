@@ -81,6 +81,7 @@ def main():
     ap.add_argument("--no-learned", action="store_true")
     ap.add_argument("--no-bf", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-filter", action="store_true", help="skip the filtered / deleted-element search")
     ap.add_argument("--bf-n", type=int, default=1_000_000)
     ap.add_argument("--bf-nq", type=int, default=10_000)
     ap.add_argument("--cpu-queries", type=int, default=2000)
@@ -254,6 +255,11 @@ def main():
             result.update(learned)
         except ImportError:
             log("learned stop module not available")
+
+    # non-bare-bone search: a 10% label filter and 10% deleted elements on the same index
+    if not args.no_filter:
+        from hnsw_rabitq_amd import filter_bench
+        result.update(filter_bench.run(H, _lib, idx, gpu, data, dq, nq, stream, timer, log, k=K))
 
     # brute force MFMA TFLOPS (1M x 768 IP GMM)
     if not args.no_bf:

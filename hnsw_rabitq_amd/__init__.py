@@ -14,6 +14,7 @@ built library or a device raises.
 """
 import ctypes
 import os
+import sys
 
 import numpy as np
 
@@ -131,15 +132,26 @@ class GpuHnsw:
         _lib.call("hg_gpu_refresh_deleted", self.handle, cpu_handle)
 
     def status(self):
-        st = ctypes.c_uint32()
-        _lib.call("hg_gpu_search_status", self.handle, ctypes.byref(st))
-        return st.value
+        """(status word, queries that needed the second pass) of the last launch on this index."""
+        st, second = ctypes.c_uint32(), ctypes.c_uint32()
+        _lib.call("hg_gpu_search_status", self.handle, ctypes.byref(st), ctypes.byref(second))
+        return st.value, second.value
 
     def search_device(self, d_queries, nq, k, ef, d_labels, d_dists, d_counters=None, d_ef=None, d_k=None,
                       k_stride=None, stop_model=None, stream=None):
         _lib.call("hg_gpu_search_device", self.handle, d_queries.ptr, nq, d_ef.ptr if d_ef else None,
                   d_k.ptr if d_k else None, ef, k, k_stride or k, d_labels.ptr, d_dists.ptr,
                   d_counters.ptr if d_counters else None, stop_model, stream)
+
+    def search_device_filtered(self, d_queries, nq, k, ef, d_labels, d_dists, filter=None, d_status=None,
+                               d_counters=None, d_ef=None, d_k=None, k_stride=None, stream=None):
+        """Device-resident non-bare-bone search (deleted elements, optional GpuFilter)."""
+        if filter is not None and filter.owner is not self:
+            raise ValueError("the filter was built for another GPU index")
+        _lib.call("hg_gpu_search_device_filtered", self.handle, d_queries.ptr, nq, d_ef.ptr if d_ef else None,
+                  d_k.ptr if d_k else None, ef, k, k_stride or k, d_labels.ptr, d_dists.ptr,
+                  d_counters.ptr if d_counters else None, filter.handle if filter is not None else None,
+                  d_status.ptr if d_status else None, stream)
 
     def plan(self, max_ef, max_k, nq):
         out = np.zeros(5, dtype=np.int32)
@@ -405,6 +417,8 @@ class BFIndex:
         self.space = space
         self.dim = int(dim)
         self._h = None
+        self._labels = set()
+        self._cur_l = 0
         self.num_threads = os.cpu_count() or 1
         self.device = 0
 
@@ -414,18 +428,23 @@ class BFIndex:
         h = ctypes.c_void_p()
         _lib.call("hg_bf_create", self.space_id, self.dim, int(max_elements), self.device, ctypes.byref(h))
         self._h = h
+        self._labels = set()
 
     def add_items(self, data, ids=None):
         arr = _as_rows(data, self.dim)
         if self.normalize:
             arr = normalize_rows(arr)
-        lab = None
         if ids is not None:
             lab = np.ascontiguousarray(np.asarray(ids, dtype=np.uint64).reshape(-1))
             if lab.shape[0] != arr.shape[0]:
                 raise RuntimeError("The input label shape %d does not match the input data vector shape %d"
                                    % (lab.ndim, arr.shape[0]))
+        else:
+            # default labels continue a counter of every row ever passed in (bindings.cpp:812-822)
+            lab = np.arange(self._cur_l, self._cur_l + arr.shape[0], dtype=np.uint64)
         _lib.call("hg_bf_add", self._h, _ptr(arr), arr.shape[0], _ptr(lab))
+        self._cur_l += arr.shape[0]
+        self._labels.update(int(x) for x in lab)
 
     def get_current_count(self):
         c = ctypes.c_size_t()
@@ -436,8 +455,9 @@ class BFIndex:
         self.num_threads = int(n)
 
     def knn_query(self, data, k=1, num_threads=-1, filter=None):
-        if filter is not None:
-            raise RuntimeError("filter: filtered brute force is not implemented by the GPU engine")
+        """filter: a callable label -> bool, as in hnswlib (evaluated here once per element)."""
+        if filter is not None and not callable(filter):
+            raise TypeError("filter must be a callable taking a label")
         arr = _as_rows(data, self.dim)
         if self.normalize:
             arr = normalize_rows(arr)
@@ -447,11 +467,49 @@ class BFIndex:
         dists = np.empty((nq, k), dtype=np.float32)
         if nq == 0:
             return labels, dists
-        code = _lib.load().hg_bf_search(self._h, _ptr(arr), nq, k, _ptr(labels), _ptr(dists))
+        lib = _lib.load()
+        if filter is None:
+            code = lib.hg_bf_search(self._h, _ptr(arr), nq, k, _ptr(labels), _ptr(dists))
+        else:
+            labs = np.array(list(self._labels), dtype=np.uint64)
+            keep = np.fromiter((bool(filter(int(x))) for x in labs), dtype=bool, count=labs.shape[0])
+            allowed = np.ascontiguousarray(labs[keep])
+            f = ctypes.c_void_p()
+            _lib.call("hg_bf_filter_create", self._h, _ptr(allowed), allowed.shape[0], ctypes.byref(f))
+            try:
+                code = lib.hg_bf_search_filtered(self._h, _ptr(arr), nq, k, f, _ptr(labels), _ptr(dists))
+            finally:
+                lib.hg_bf_filter_free(f)
         if code == _lib.HG_ERR_SHORT:
             raise RuntimeError("Cannot return the results in a contiguous 2D array. There are not enough elements.")
         _lib.check(code)
         return labels, dists
+
+    def delete_vector(self, label):
+        """BruteforceSearch::removePoint (bruteforce.h:85-104); an unknown label is ignored."""
+        _lib.call("hg_bf_delete", self._h, int(label))
+        self._labels.discard(int(label))
+
+    def save_index(self, path_to_index):
+        _lib.call("hg_bf_save", self._h, str(path_to_index).encode())
+
+    def load_index(self, path_to_index, max_elements=0):
+        """BruteforceSearch::loadIndex: the capacity comes from the file (max_elements is ignored,
+        as in the reference)."""
+        if self._h is not None:
+            print("Warning: Calling load_index for an already inited index. Old index is being deallocated.",
+                  file=sys.stderr)
+            _lib.load().hg_bf_free(self._h)
+            self._h = None
+        h = ctypes.c_void_p()
+        _lib.call("hg_bf_load", self.space_id, self.dim, str(path_to_index).encode(), self.device, ctypes.byref(h))
+        self._h = h
+        n = self.get_current_count()
+        labs = np.empty(n, dtype=np.uint64)
+        if n:
+            _lib.call("hg_bf_labels", self._h, _ptr(labs), n)
+        self._labels = set(int(x) for x in labs)
+        self._cur_l = n
 
     def __del__(self):
         try:

@@ -41,12 +41,20 @@ SIGNATURES = {
     "hg_gpu_autotune": [_vp, _vp, _sz, _sz, _sz, _vp],
     "hg_gpu_set_plan": [_vp, _sz, _sz, ctypes.c_uint32, _i32],
     "hg_gpu_id_labels": [_vp, _vp, _sz, _vp, _vp],
+    "hg_bf_delete": [_vp, ctypes.c_uint64],
+    "hg_bf_labels": [_vp, _vp, _sz],
+    "hg_bf_save": [_vp, ctypes.c_char_p],
+    "hg_bf_load": [_i32, _sz, ctypes.c_char_p, _i32, _vp],
+    "hg_bf_filter_create": [_vp, _vp, _sz, _vp],
+    "hg_bf_filter_free": [_vp],
+    "hg_bf_search_filtered": [_vp, _vp, _sz, _sz, _vp, _vp, _vp],
+    "hg_bf_search_device_filtered": [_vp, _vp, _sz, _sz, _vp, _vp, _vp, _vp],
     "hg_gpu_filter_create": [_vp, _vp, _sz, _vp],
     "hg_gpu_filter_free": [_vp],
     "hg_gpu_filter_count": [_vp, _vp],
     "hg_gpu_search_filtered": [_vp, _vp, _sz, _sz, _sz, _vp, _vp, _vp, _vp],
     "hg_gpu_search_device_filtered": [_vp, _vp, _sz, _vp, _vp, _sz, _sz, _sz, _vp, _vp, _vp, _vp, _vp, _vp],
-    "hg_gpu_search_status": [_vp, _vp],
+    "hg_gpu_search_status": [_vp, _vp, _vp],
     "hg_gpu_refresh_deleted": [_vp, _vp],
     "hg_gpu_index_labels": [_vp, _vp, _sz, _vp],
     "hg_bf_create": [_i32, _sz, _sz, _i32, _pvp],

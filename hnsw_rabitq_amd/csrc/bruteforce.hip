@@ -174,7 +174,7 @@ __global__ __launch_bounds__(SEL_THREADS) void bf_select_kernel(const float* __r
                                                                  size_t ld_rows, const uint64_t* __restrict__ labels,
                                                                  uint32_t k, uint32_t q_offset,
                                                                  uint64_t* __restrict__ out_l, float* __restrict__ out_d,
-                                                                 size_t ld_outp) {
+                                                                 size_t ld_outp, const uint32_t* __restrict__ allow) {
     __shared__ SelEntry buf[SEL_CAP + SEL_CHUNK];
     __shared__ int cnt;
     __shared__ SelEntry thr;
@@ -191,6 +191,9 @@ __global__ __launch_bounds__(SEL_THREADS) void bf_select_kernel(const float* __r
         for (uint32_t i = base + threadIdx.x; i < base + SEL_CHUNK && i < n; i += SEL_THREADS) {
             uint32_t r = rmap ? rmap[i] : i;
             if (r == SENT) continue;
+            // the filter of BruteforceSearch::searchKnn (bruteforce.h:115-117): a row whose label
+            // is not allowed never enters the result
+            if (allow && ((allow[r >> 5] >> (r & 31u)) & 1u) == 0u) continue;
             SelEntry e{row[i], r};
             if (sel_less(e, t, labels) || t.r == SENT) {
                 int pos = atomicAdd(&cnt, 1);
@@ -256,7 +259,7 @@ hipError_t launch_bf_exact(const BfData& d, const float* q, uint32_t nq, uint32_
         hipError_t e = hipGetLastError();
         if (e != hipSuccess) return e;
         hipLaunchKernelGGL(bf_select_kernel, dim3(cq), dim3(SEL_THREADS), 0, s, mat, (size_t)d.n, d.n,
-                           (const uint32_t*)nullptr, (size_t)0, d.labels, k, q0, labels, dists, (size_t)k);
+                           (const uint32_t*)nullptr, (size_t)0, d.labels, k, q0, labels, dists, (size_t)k, d.allow);
         e = hipGetLastError();
         if (e != hipSuccess) return e;
     }

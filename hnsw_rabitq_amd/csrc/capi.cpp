@@ -33,10 +33,18 @@ struct GpuIndex {
     int cus = 0;
     hg::DevGraph g{};
     DevBuf vec, links0, upper_off, upper, labels;
-    DevBuf work;
-    // non-bare-bone search: delete marks as a bitmap over internal ids, the status word of the
-    // last filtered launch, and the host labels (filters are built from label lists)
-    DevBuf delbits, status;
+    // Per-launch device words (status, work counters, overflow-list counts; layout at kWords)
+    // and ovf = two overflow lists of [nq] u32.  Launches take the next of kRing sets, so up to
+    // kRing launches on different streams may be in flight on one index.
+    struct LaunchWords {
+        DevBuf words, ovf;
+    };
+    static constexpr int kRing = 16;
+    LaunchWords ring[kRing];
+    unsigned next_ring = 0, last_ring = 0;
+    // non-bare-bone search: delete marks as a bitmap over internal ids, and the host labels
+    // (filters are built from label lists)
+    DevBuf delbits;
     size_t num_deleted = 0;
     std::vector<uint64_t> host_labels;
     // host-path staging
@@ -80,6 +88,9 @@ std::atomic<uint64_t> g_export_serial{0};
 // result set holds ef allowed entries, so a selective filter needs about ef / (allowed fraction)
 // entries.  8192 entries = 64 KiB of LDS.
 constexpr uint32_t kFiltMaxCap = 8192;
+// Launch words: [0] status, [1..8] work counters of the passes, [9..16] overflow-list counts.
+constexpr int kMaxPasses = 8, kWordStatus = 0, kWordWork = 1, kWordCount = 1 + kMaxPasses;
+constexpr uint32_t kWords = 1 + 2 * kMaxPasses;
 
 Plan make_plan(GpuIndex* gi, size_t max_ef, size_t max_k, size_t nq, bool filtered = false,
                double allowed_frac = 1.0, uint32_t min_cap = 0) {
@@ -88,10 +99,15 @@ Plan make_plan(GpuIndex* gi, size_t max_ef, size_t max_k, size_t nq, bool filter
     p.cap = (uint32_t)((ef_eff + 63) / 64 * 64);
     auto it = filtered ? gi->tuned.end() : gi->tuned.find(p.cap);
     if (filtered) {
-        const double want = 2.0 * (double)ef_eff / std::max(allowed_frac, 1e-4) + 256.0;
+        // ef allowed entries are found after about ef / (allowed fraction) admissions; the
+        // margin covers the spread of that count for elements allowed independently of their
+        // position (a query whose neighbourhood holds fewer allowed elements is flagged, and the
+        // host path runs it again at the largest beam)
+        const double want = 1.1 * (double)ef_eff / std::max(allowed_frac, 1e-4) + 128.0;
         const double c = std::min<double>(kFiltMaxCap, std::max<double>({want, (double)p.cap, (double)min_cap}));
         p.cap = std::min<uint32_t>(kFiltMaxCap, ((uint32_t)c + 63) / 64 * 64);
-        p.hash_bits = std::min<uint32_t>(13, std::max<uint32_t>(10, ceil_log2(4 * (uint64_t)p.cap)));
+        // lossy, like the bare-bone plan's: about one slot per two beam entries
+        p.hash_bits = std::min<uint32_t>(12, std::max<uint32_t>(10, ceil_log2(p.cap) - 1));
     } else if (it != gi->tuned.end()) {
         p.hash_bits = it->second.first;
     } else {
@@ -121,14 +137,20 @@ Plan make_plan(GpuIndex* gi, size_t max_ef, size_t max_k, size_t nq, bool filter
     return p;
 }
 
-hg::SearchParams base_params(GpuIndex* gi, const Plan& pl) {
+hg::SearchParams base_params(GpuIndex* gi, const Plan& pl, GpuIndex::LaunchWords* lw) {
     hg::SearchParams sp{};
     sp.g = gi->g;
     sp.cap = pl.cap;
     sp.hash_bits = pl.hash_bits;
-    sp.work = (uint32_t*)gi->work.p;
+    sp.work = (uint32_t*)lw->words.p + kWordWork;
     sp.trace_period = 1;
     return sp;
+}
+
+// The next set of launch words of the ring (host state under the index mutex).
+GpuIndex::LaunchWords* next_words(GpuIndex* gi) {
+    gi->last_ring = gi->next_ring++ % GpuIndex::kRing;
+    return &gi->ring[gi->last_ring];
 }
 
 // Fraction of the elements a query may return (not deleted, allowed by the filter), used to
@@ -140,16 +162,29 @@ double allowed_fraction(const GpuIndex* gi, const GpuFilter* f) {
     return frac;
 }
 
+bool is_filtered(const GpuIndex* gi, const GpuFilter* filt) { return gi->num_deleted > 0 || filt != nullptr; }
+
+// Reserves the launch words (and, for the non-bare-bone search, the overflow list) that a
+// launch of nq queries needs; done before a graph capture, where allocation is not allowed.
+void prepare_words(GpuIndex::LaunchWords* lw, size_t nq, bool filtered) {
+    lw->words.reserve(kWords * 4);
+    if (filtered) lw->ovf.reserve(std::max<size_t>(nq, 1) * 8);
+}
+
 // The non-bare-bone branch runs when the index holds deleted elements or a filter is given
-// (hnswalg.h:1304-1312).  status: device word, bit 0 set by a query that outgrew the beam.
+// (hnswalg.h:1304-1312), in passes at growing beams: the first planned from the allowed
+// fraction, each later one (x2.5, up to the largest) taking the queries that outgrew the previous
+// one from a list kept on the device.  All passes are asynchronous and graph-capturable.  status
+// (device word, may be null = the launch words' own) gets bit 0 when a query outgrew the largest
+// beam too.  lw: the launch words to use (prepared).
 hipError_t run_search(GpuIndex* gi, const float* dq, size_t nq, const int32_t* def, const int32_t* dk, size_t ef,
                       size_t k, size_t k_stride, uint64_t* dl, float* dd, int32_t* dc, StopModel* stop,
-                      hipStream_t s, const GpuFilter* filt = nullptr, uint32_t* status = nullptr,
-                      uint32_t min_cap = 0) {
-    const bool filtered = gi->num_deleted > 0 || filt != nullptr;
-    require(!(filtered && stop), "the learned stop is not available with deleted elements or a filter");
-    Plan pl = make_plan(gi, ef, k, nq, filtered, filtered ? allowed_fraction(gi, filt) : 1.0, min_cap);
-    hg::SearchParams sp = base_params(gi, pl);
+                      hipStream_t s, GpuIndex::LaunchWords* lw, const GpuFilter* filt = nullptr,
+                      uint32_t* status = nullptr) {
+    const bool filtered = is_filtered(gi, filt);
+    const double frac = filtered ? allowed_fraction(gi, filt) : 1.0;
+    Plan pl = make_plan(gi, ef, k, nq, filtered, frac);
+    hg::SearchParams sp = base_params(gi, pl, lw);
     sp.queries = dq;
     sp.nq = (uint32_t)nq;
     sp.ef_pq = def;
@@ -160,16 +195,46 @@ hipError_t run_search(GpuIndex* gi, const float* dq, size_t nq, const int32_t* d
     sp.out_labels = dl;
     sp.out_dists = dd;
     sp.counters = dc;
-    if (filtered) {
-        sp.del_bits = gi->num_deleted > 0 ? (const uint32_t*)gi->delbits.p : nullptr;
-        sp.allow_bits = filt ? (const uint32_t*)filt->bits.p : nullptr;
-        sp.status = status ? status : (uint32_t*)gi->status.p;
-    }
     if (stop) {
         sp.use_stop = 1;
         sp.stop = dev_stop(stop);
     }
-    return hg::launch_search(sp, pl.blocks, s);
+    if (!filtered) return hg::launch_search(sp, pl.blocks, s);
+
+    uint32_t* words = (uint32_t*)lw->words.p;
+    sp.del_bits = gi->num_deleted > 0 ? (const uint32_t*)gi->delbits.p : nullptr;
+    sp.allow_bits = filt ? (const uint32_t*)filt->bits.p : nullptr;
+    sp.status = status ? status : words + kWordStatus;
+    hipError_t e = hg::launch_zero_u32(words, kWords, s);  // status, work counters, list counts
+    if (e == hipSuccess && status) e = hg::launch_zero_u32(status, 1, s);
+    if (e != hipSuccess) return e;
+    // passes at growing beams: each lists the queries that outgrew it for the next (the lists
+    // alternate between two halves of ovf); the last, at the largest beam, sets the status
+    Plan cur = pl;
+    for (int pass = 0;; pass++) {
+        const bool last = cur.cap >= kFiltMaxCap || pass + 1 >= kMaxPasses;
+        hg::SearchParams sq = sp;
+        sq.cap = cur.cap;
+        sq.hash_bits = cur.hash_bits;
+        sq.work = words + kWordWork + pass;
+        if (pass > 0) {
+            sq.order = (const uint32_t*)lw->ovf.p + (size_t)((pass - 1) & 1) * nq;
+            sq.nq_dev = words + kWordCount + pass - 1;
+        }
+        if (!last) {
+            sq.ovf_list = (uint32_t*)lw->ovf.p + (size_t)(pass & 1) * nq;
+            sq.ovf_count = words + kWordCount + pass;
+        }
+        e = hg::launch_search(sq, cur.blocks, s);
+        if (e != hipSuccess || last) return e;
+        const uint32_t next = std::min<uint32_t>(kFiltMaxCap, (uint32_t)(cur.cap * 5 / 2 + 63) / 64 * 64);
+        cur = make_plan(gi, ef, k, nq, true, frac, next);
+    }
+}
+
+void check_stop_args(const GpuIndex* gi, const GpuFilter* filt, void* stop) {
+    require(!(stop && is_filtered(gi, filt)),
+            "the learned stop is not available with deleted elements or a filter");
 }
 
 GpuFilter* as_filter(GpuIndex* gi, void* f) {
@@ -383,9 +448,8 @@ HG_API int hg_gpu_index_create(void* hv, int device, void** out) {
                 }
             upload(gi->delbits, del.data(), del.size() * 4);
         }
-        gi->status.reserve(256);
+        for (auto& lw : gi->ring) lw.words.reserve(kWords * 4);
         gi->host_labels = std::move(labels);
-        gi->work.reserve(256);
         hip_check(hipStreamCreateWithFlags(&gi->stream, hipStreamNonBlocking), "hipStreamCreate");
         hg::DevGraph& g = gi->g;
         g.vec = (const float*)gi->vec.p;
@@ -426,9 +490,9 @@ HG_API int hg_gpu_index_bytes(void* g, size_t* bytes) {
 }
 
 namespace {
-// Host-buffer search (the knn_query path): upload, one launch, download.  The non-bare-bone
-// search that outgrows its beam is run again at the largest beam; if a query still needs more,
-// the call fails rather than return a list that may differ from the reference's.
+// Host-buffer search (the knn_query path): upload, launch, download.  If a query of the
+// non-bare-bone search outgrows even the largest beam, the call fails rather than return a list
+// that may differ from the reference's.
 int search_host(GpuIndex* gi, const float* queries, size_t nq, size_t k, size_t ef, const GpuFilter* filt,
                 uint64_t* labels, float* dists, int32_t* counters) {
     check_search_args(gi, nq, ef, k, k);
@@ -443,23 +507,21 @@ int search_host(GpuIndex* gi, const float* queries, size_t nq, size_t k, size_t 
     hipStream_t s = gi->stream;
     const bool filtered = gi->num_deleted > 0 || filt != nullptr;
     hip_check(hipMemcpyAsync(gi->q.p, queries, nq * dim * 4, hipMemcpyHostToDevice, s), "hipMemcpy H2D");
-    for (uint32_t min_cap = 0;;) {
-        hip_check(run_search(gi, (const float*)gi->q.p, nq, nullptr, nullptr, ef, k, k, (uint64_t*)gi->ol.p,
-                             (float*)gi->od.p, counters ? (int32_t*)gi->ctr.p : nullptr, nullptr, s, filt, nullptr,
-                             min_cap),
-                  "search launch");
-        if (!filtered) break;
+    GpuIndex::LaunchWords* lw = next_words(gi);
+    prepare_words(lw, nq, filtered);
+    hip_check(run_search(gi, (const float*)gi->q.p, nq, nullptr, nullptr, ef, k, k, (uint64_t*)gi->ol.p,
+                         (float*)gi->od.p, counters ? (int32_t*)gi->ctr.p : nullptr, nullptr, s, lw, filt),
+              "search launch");
+    if (filtered) {
         uint32_t status = 0;
-        hip_check(hipMemcpyAsync(&status, gi->status.p, 4, hipMemcpyDeviceToHost, s), "hipMemcpy D2H");
+        hip_check(hipMemcpyAsync(&status, (uint32_t*)lw->words.p + kWordStatus, 4, hipMemcpyDeviceToHost, s),
+                  "hipMemcpy D2H");
         hip_check(hipStreamSynchronize(s), "search");
-        if ((status & 1u) == 0) break;
-        const Plan used = make_plan(gi, ef, k, nq, true, allowed_fraction(gi, filt), min_cap);
-        if (used.cap >= kFiltMaxCap || min_cap >= kFiltMaxCap) {
+        if (status & 1u) {
             g_err = "filtered search: a query needed more than " + std::to_string(kFiltMaxCap) +
                     " beam entries (the allowed elements are too few for this ef)";
             return HG_ERR_RUNTIME;
         }
-        min_cap = kFiltMaxCap;
     }
     hip_check(hipMemcpyAsync(labels, gi->ol.p, nq * k * 8, hipMemcpyDeviceToHost, s), "hipMemcpy D2H");
     hip_check(hipMemcpyAsync(dists, gi->od.p, nq * k * 4, hipMemcpyDeviceToHost, s), "hipMemcpy D2H");
@@ -539,12 +601,15 @@ HG_API int hg_gpu_filter_count(void* f, size_t* allowed) {
     });
 }
 
-HG_API int hg_gpu_search_status(void* gv, uint32_t* status) {
+HG_API int hg_gpu_search_status(void* gv, uint32_t* status, uint32_t* second_pass_queries) {
     return guarded([&] {
-        require(gv && status, "null argument");
+        require(gv != nullptr, "null argument");
         auto* gi = static_cast<GpuIndex*>(gv);
         DeviceGuard dg(gi->device);
-        hip_check(hipMemcpy(status, gi->status.p, 4, hipMemcpyDeviceToHost), "hipMemcpy D2H");
+        uint32_t w[kWords] = {};
+        hip_check(hipMemcpy(w, gi->ring[gi->last_ring].words.p, sizeof(w), hipMemcpyDeviceToHost), "hipMemcpy D2H");
+        if (status) *status = w[kWordStatus];
+        if (second_pass_queries) *second_pass_queries = w[kWordCount];
         return HG_OK;
     });
 }
@@ -596,10 +661,13 @@ HG_API int hg_gpu_search_device(void* gv, const float* dq, size_t nq, const int3
         check_search_args(gi, nq, ef, k, k_stride);
         if (nq == 0) return HG_OK;
         require(dq != nullptr, "null queries");
+        check_stop_args(gi, nullptr, stop);
         DeviceGuard dg(gi->device);
         std::lock_guard<std::mutex> lk(gi->mu);
+        GpuIndex::LaunchWords* lw = next_words(gi);
+        prepare_words(lw, nq, is_filtered(gi, nullptr));
         hip_check(run_search(gi, dq, nq, def, dk, ef, k, k_stride, dl, dd, dc, static_cast<StopModel*>(stop),
-                             (hipStream_t)stream),
+                             (hipStream_t)stream, lw),
                   "search launch");
         return HG_OK;
     });
@@ -614,10 +682,13 @@ HG_API int hg_gpu_search_device_filtered(void* gv, const float* dq, size_t nq, c
         check_search_args(gi, nq, ef, k, k_stride);
         if (nq == 0) return HG_OK;
         require(dq != nullptr, "null queries");
+        GpuFilter* gf = as_filter(gi, filter);
         DeviceGuard dg(gi->device);
         std::lock_guard<std::mutex> lk(gi->mu);
-        hip_check(run_search(gi, dq, nq, def, dk, ef, k, k_stride, dl, dd, dc, nullptr, (hipStream_t)stream,
-                             as_filter(gi, filter), d_status),
+        GpuIndex::LaunchWords* lw = next_words(gi);
+        prepare_words(lw, nq, is_filtered(gi, gf));
+        hip_check(run_search(gi, dq, nq, def, dk, ef, k, k_stride, dl, dd, dc, nullptr, (hipStream_t)stream, lw, gf,
+                             d_status),
                   "search launch");
         return HG_OK;
     });
@@ -642,7 +713,9 @@ HG_API int hg_gpu_collect_traces(void* gv, const float* dq, size_t nq, size_t ef
         ol.reserve(nq * k * 8);
         od.reserve(nq * k * 4);
         Plan pl = make_plan(gi, ef, k, nq);
-        hg::SearchParams sp = base_params(gi, pl);
+        GpuIndex::LaunchWords* lw = next_words(gi);
+        prepare_words(lw, nq, false);
+        hg::SearchParams sp = base_params(gi, pl, lw);
         sp.queries = dq;
         sp.nq = (uint32_t)nq;
         sp.ef = (uint32_t)ef;
@@ -707,8 +780,10 @@ HG_API int hg_gpu_autotune(void* gv, const float* dq, size_t nq, size_t ef, size
                 float ms_sum = 0;
                 for (int rep = 0; rep < 2; rep++) {
                     hip_check(hipEventRecord(e0, s), "hipEventRecord");
+                    GpuIndex::LaunchWords* lw = next_words(gi);
+                    prepare_words(lw, nq, false);
                     hip_check(run_search(gi, dq, nq, nullptr, nullptr, ef, k, k, (uint64_t*)ol.p, (float*)od.p,
-                                         nullptr, nullptr, s),
+                                         nullptr, nullptr, s, lw),
                               "search launch");
                     hip_check(hipEventRecord(e1, s), "hipEventRecord");
                     hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
@@ -763,6 +838,7 @@ HG_API int hg_gpu_id_labels(void* gv, const uint32_t* d_ids, size_t n, uint64_t*
 struct GraphHandle {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
+    GpuIndex::LaunchWords lw;  // the graph's own launch words (replays reuse them)
     ~GraphHandle() {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
@@ -781,12 +857,24 @@ HG_API int hg_graph_capture_search(void* gv, const float* dq, size_t nq, const i
         DeviceGuard dg(gi->device);
         std::lock_guard<std::mutex> lk(gi->mu);
         hipStream_t s = (hipStream_t)stream;
+        check_stop_args(gi, nullptr, stop);
         std::unique_ptr<GraphHandle> gh(new GraphHandle());
-        // resolve kernel attributes outside the capture
-        Plan pl = make_plan(gi, ef, k, nq);
+        // allocate and resolve kernel attributes outside the capture
+        const bool filtered = is_filtered(gi, nullptr);
+        prepare_words(&gh->lw, nq, filtered);
+        Plan pl = make_plan(gi, ef, k, nq, filtered, allowed_fraction(gi, nullptr));
         (void)pl;
         hip_check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");
-        hipError_t le = run_search(gi, dq, nq, def, dk, ef, k, k_stride, dl, dd, dc, static_cast<StopModel*>(stop), s);
+        hipError_t le = hipSuccess;
+        try {
+            le = run_search(gi, dq, nq, def, dk, ef, k, k_stride, dl, dd, dc, static_cast<StopModel*>(stop), s,
+                            &gh->lw);
+        } catch (...) {
+            hipGraph_t g = nullptr;
+            (void)hipStreamEndCapture(s, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
         hipGraph_t graph = nullptr;
         hipError_t ce = hipStreamEndCapture(s, &graph);
         hip_check(le, "search launch (capture)");

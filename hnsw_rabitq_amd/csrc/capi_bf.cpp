@@ -2,7 +2,9 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <cstring>
+#include <fstream>
 #include <memory>
 #include <stdexcept>
 #include <string>
@@ -28,6 +30,9 @@ struct BfIndex {
     size_t lp_count = 0;
     hipStream_t stream = nullptr;
     std::unordered_map<uint64_t, size_t> label_row;
+    std::vector<uint64_t> host_labels;  // label of each row (filters are bitmaps over rows)
+    uint64_t serial = 0;   // unique per index
+    uint64_t version = 0;  // changes whenever rows move or are added
     ~BfIndex() {
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -39,6 +44,7 @@ struct BfIndex {
         d.vec_fp8 = lp_kind == HG_PREC_FP8 ? (const uint8_t*)vec_lp.p : nullptr;
         d.fp8_scale = (const float*)scale.p;
         d.norms = nullptr;
+        d.allow = nullptr;
         d.n = (uint32_t)count;
         d.dim = (uint32_t)dim;
         d.space = space;
@@ -46,9 +52,30 @@ struct BfIndex {
     }
 };
 
+std::atomic<uint64_t> g_bf_serial{0};
+
+// Label filter of one brute-force index: bitmap over its rows, valid for the row version it
+// was built from (adds and deletions move rows).
+struct BfFilter {
+    uint64_t serial = 0, version = 0;
+    int device = 0;
+    size_t allowed = 0;
+    DevBuf bits;
+};
+
+BfFilter* as_bf_filter(BfIndex* b, void* f) {
+    if (!f) return nullptr;
+    auto* bf = static_cast<BfFilter*>(f);
+    require(bf->serial == b->serial, "the filter was built for another brute-force index");
+    require(bf->version == b->version, "the index changed since the filter was built; build it again");
+    return bf;
+}
+
 // Exact FP32 search with device buffers on stream s.
-void exact_search(BfIndex* b, const float* dq, size_t nq, size_t k, uint64_t* dl, float* dd, hipStream_t s) {
+void exact_search(BfIndex* b, const float* dq, size_t nq, size_t k, uint64_t* dl, float* dd, hipStream_t s,
+                  const uint32_t* allow = nullptr) {
     hg::BfData d = b->data();
+    d.allow = allow;
     size_t sb = hg::bf_exact_scratch_bytes((uint32_t)nq, (uint32_t)k, d.n);
     b->scratch.reserve(sb);
     hip_check(hg::launch_bf_exact(d, dq, (uint32_t)nq, (uint32_t)k, dl, dd, b->scratch.p, s), "brute-force launch");
@@ -86,6 +113,7 @@ HG_API int hg_bf_create(int space, size_t dim, size_t max_elements, int device, 
         b->space = space;
         b->dim = dim;
         b->max_elements = max_elements;
+        b->serial = ++g_bf_serial;
         b->vec.reserve(std::max<size_t>(max_elements, 1) * dim * 4);
         b->labels.reserve(std::max<size_t>(max_elements, 1) * 8);
         hip_check(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking), "hipStreamCreate");
@@ -119,6 +147,9 @@ HG_API int hg_bf_add(void* bv, const float* data, size_t rows, const uint64_t* l
             }
             lab[r] = l;
         }
+        b->host_labels.resize(count);
+        for (size_t r2 = 0; r2 < rows; r2++) b->host_labels[target[r2]] = lab[r2];
+        b->version++;
         // contiguous runs are copied in one transfer
         size_t r = 0;
         while (r < rows) {
@@ -134,6 +165,17 @@ HG_API int hg_bf_add(void* bv, const float* data, size_t rows, const uint64_t* l
         }
         b->count = count;
         b->lp_count = (size_t)-1;  // reduced-precision copy is stale
+        return HG_OK;
+    });
+}
+
+// Label of every row, in row order.
+HG_API int hg_bf_labels(void* bv, uint64_t* labels, size_t capacity) {
+    return guarded([&] {
+        require(bv != nullptr, "null index");
+        auto* b = static_cast<BfIndex*>(bv);
+        require(capacity >= b->count && (b->count == 0 || labels), "label buffer too small");
+        std::copy(b->host_labels.begin(), b->host_labels.begin() + b->count, labels);
         return HG_OK;
     });
 }
@@ -261,6 +303,178 @@ HG_API int hg_bf_autotune(void* bv, const float* dq, size_t nq, size_t k, int pr
         }
         b->tuned[precision] = best_v;
         if (chosen) *chosen = best_v;
+        return HG_OK;
+    });
+}
+
+// BruteforceSearch::removePoint (bruteforce.h:85-104): the last row moves into the removed
+// row; an unknown label is ignored, as in the reference.
+HG_API int hg_bf_delete(void* bv, uint64_t label) {
+    return guarded([&] {
+        require(bv != nullptr, "null index");
+        auto* b = static_cast<BfIndex*>(bv);
+        auto it = b->label_row.find(label);
+        if (it == b->label_row.end()) return HG_OK;
+        const size_t row = it->second, last = b->count - 1;
+        b->label_row.erase(it);
+        DeviceGuard dg(b->device);
+        if (row != last) {
+            const uint64_t moved = b->host_labels[last];
+            b->label_row[moved] = row;
+            b->host_labels[row] = moved;
+            hip_check(hipMemcpy((char*)b->vec.p + row * b->dim * 4, (const char*)b->vec.p + last * b->dim * 4,
+                                b->dim * 4, hipMemcpyDeviceToDevice),
+                      "hipMemcpy D2D");
+            hip_check(hipMemcpy((char*)b->labels.p + row * 8, &moved, 8, hipMemcpyHostToDevice), "hipMemcpy H2D");
+        }
+        b->host_labels.pop_back();
+        b->count--;
+        b->version++;
+        b->lp_count = (size_t)-1;
+        return HG_OK;
+    });
+}
+
+// BruteforceSearch::saveIndex (bruteforce.h:128-141): max_elements, bytes per element and the
+// element count (size_t each), then max_elements records of [dim f32 | u64 label]; records past
+// the count are zero.
+HG_API int hg_bf_save(void* bv, const char* path) {
+    return guarded([&] {
+        require(bv && path, "null argument");
+        auto* b = static_cast<BfIndex*>(bv);
+        DeviceGuard dg(b->device);
+        std::ofstream out(path, std::ios::binary);
+        require((bool)out, std::string("Cannot open file ") + path);
+        const size_t per = b->dim * 4 + 8;
+        const size_t hdr[3] = {b->max_elements, per, b->count};
+        out.write((const char*)hdr, sizeof(hdr));
+        std::vector<float> vec(b->count * b->dim);
+        if (b->count)
+            hip_check(hipMemcpy(vec.data(), b->vec.p, vec.size() * 4, hipMemcpyDeviceToHost), "hipMemcpy D2H");
+        std::vector<char> rec(per);
+        for (size_t r = 0; r < b->count; r++) {
+            memcpy(rec.data(), &vec[r * b->dim], b->dim * 4);
+            memcpy(rec.data() + b->dim * 4, &b->host_labels[r], 8);
+            out.write(rec.data(), per);
+        }
+        std::fill(rec.begin(), rec.end(), 0);
+        for (size_t r = b->count; r < b->max_elements; r++) out.write(rec.data(), per);
+        require((bool)out, "write failed");
+        return HG_OK;
+    });
+}
+
+// BruteforceSearch::loadIndex (bruteforce.h:144-161) into a new GPU index; the label -> row
+// map is rebuilt from the records (the reference leaves it empty after a load).
+HG_API int hg_bf_load(int space, size_t dim, const char* path, int device, void** out) {
+    return guarded([&] {
+        require(path && out, "null argument");
+        std::ifstream in(path, std::ios::binary);
+        require((bool)in, std::string("Cannot open file ") + path);
+        size_t hdr[3] = {0, 0, 0};
+        in.read((char*)hdr, sizeof(hdr));
+        const size_t per = dim * 4 + 8;
+        require((bool)in && hdr[2] <= hdr[0], "Index seems to be corrupted or unsupported");
+        require(hdr[1] == per, "Index seems to be corrupted or unsupported");
+        void* h = nullptr;
+        int rc = hg_bf_create(space, dim, hdr[0], device, &h);
+        if (rc != HG_OK) return rc;
+        std::unique_ptr<BfIndex> b(static_cast<BfIndex*>(h));
+        std::vector<float> vec(hdr[2] * dim);
+        std::vector<uint64_t> lab(hdr[2]);
+        std::vector<char> rec(per);
+        for (size_t r = 0; r < hdr[2]; r++) {
+            in.read(rec.data(), per);
+            require((bool)in, "Index seems to be corrupted or unsupported");
+            memcpy(&vec[r * dim], rec.data(), dim * 4);
+            memcpy(&lab[r], rec.data() + dim * 4, 8);
+        }
+        rc = hg_bf_add(b.get(), vec.data(), hdr[2], lab.data());
+        if (rc != HG_OK) return rc;
+        *out = b.release();
+        return HG_OK;
+    });
+}
+
+HG_API int hg_bf_filter_create(void* bv, const uint64_t* allowed_labels, size_t n_allowed, void** out) {
+    return guarded([&] {
+        require(bv && out && (n_allowed == 0 || allowed_labels), "null argument");
+        auto* b = static_cast<BfIndex*>(bv);
+        std::vector<uint32_t> bits((b->count + 31) / 32 + 1, 0u);
+        std::unique_ptr<BfFilter> f(new BfFilter());
+        for (size_t i = 0; i < n_allowed; i++) {
+            auto it = b->label_row.find(allowed_labels[i]);
+            if (it == b->label_row.end()) continue;
+            const size_t r = it->second;
+            if (!((bits[r >> 5] >> (r & 31)) & 1u)) f->allowed++;
+            bits[r >> 5] |= 1u << (r & 31);
+        }
+        DeviceGuard dg(b->device);
+        f->serial = b->serial;
+        f->version = b->version;
+        f->device = b->device;
+        f->bits.reserve(bits.size() * 4);
+        hip_check(hipMemcpy(f->bits.p, bits.data(), bits.size() * 4, hipMemcpyHostToDevice), "hipMemcpy H2D");
+        *out = f.release();
+        return HG_OK;
+    });
+}
+
+HG_API int hg_bf_filter_free(void* f) {
+    if (!f) return HG_OK;
+    auto* bf = static_cast<BfFilter*>(f);
+    int prev = -1;
+    (void)hipGetDevice(&prev);
+    (void)hipSetDevice(bf->device);
+    delete bf;
+    if (prev >= 0) (void)hipSetDevice(prev);
+    return HG_OK;
+}
+
+// BFIndex.knn_query(..., filter=) on host buffers (exact FP32): the k smallest (distance, label)
+// pairs among the allowed rows; HG_ERR_SHORT when fewer than k rows are allowed.
+HG_API int hg_bf_search_filtered(void* bv, const float* queries, size_t nq, size_t k, void* filter,
+                                 uint64_t* labels, float* dists) {
+    return guarded([&] {
+        require(bv && labels && dists, "null argument");
+        auto* b = static_cast<BfIndex*>(bv);
+        require(k >= 1 && k <= 2048, "k must be in [1, 2048]");
+        BfFilter* f = as_bf_filter(b, filter);
+        if (nq == 0) return HG_OK;
+        require(queries != nullptr, "null queries");
+        DeviceGuard dg(b->device);
+        hipStream_t s = b->stream;
+        b->q.reserve(nq * b->dim * 4);
+        b->ol.reserve(nq * k * 8);
+        b->od.reserve(nq * k * 4);
+        hip_check(hipMemcpyAsync(b->q.p, queries, nq * b->dim * 4, hipMemcpyHostToDevice, s), "hipMemcpy H2D");
+        if (b->count > 0)
+            exact_search(b, (const float*)b->q.p, nq, k, (uint64_t*)b->ol.p, (float*)b->od.p, s,
+                         f ? (const uint32_t*)f->bits.p : nullptr);
+        hip_check(hipMemcpyAsync(labels, b->ol.p, nq * k * 8, hipMemcpyDeviceToHost, s), "hipMemcpy D2H");
+        hip_check(hipMemcpyAsync(dists, b->od.p, nq * k * 4, hipMemcpyDeviceToHost, s), "hipMemcpy D2H");
+        hip_check(hipStreamSynchronize(s), "brute force");
+        if (k > (f ? f->allowed : b->count)) {
+            g_err = "Cannot return the results in a contiguous 2D array. There are not enough elements.";
+            return HG_ERR_SHORT;
+        }
+        return HG_OK;
+    });
+}
+
+// Device-resident filtered exact search (FP32), asynchronous on stream.
+HG_API int hg_bf_search_device_filtered(void* bv, const float* dq, size_t nq, size_t k, void* filter, uint64_t* dl,
+                                        float* dd, void* stream) {
+    return guarded([&] {
+        require(bv && dq && dl && dd, "null argument");
+        auto* b = static_cast<BfIndex*>(bv);
+        require(k >= 1 && k <= 2048, "k must be in [1, 2048]");
+        BfFilter* f = as_bf_filter(b, filter);
+        require(k <= (f ? f->allowed : b->count),
+                "Cannot return the results in a contiguous 2D array. There are not enough elements.");
+        if (nq == 0) return HG_OK;
+        DeviceGuard dg(b->device);
+        exact_search(b, dq, nq, k, dl, dd, (hipStream_t)stream, f ? (const uint32_t*)f->bits.p : nullptr);
         return HG_OK;
     });
 }

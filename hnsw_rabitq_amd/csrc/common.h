@@ -69,6 +69,12 @@ struct SearchParams {
     const uint32_t* del_bits;
     const uint32_t* allow_bits;
     uint32_t* status;
+    // FILT, first pass: a query that outgrew the beam appends its index to ovf_list (counter
+    // ovf_count) instead of setting status; the second pass takes its queries from that list
+    // (order = ovf_list, nq_dev = ovf_count: the count is read on the device).
+    uint32_t* ovf_list;
+    uint32_t* ovf_count;
+    const uint32_t* nq_dev;
 };
 
 constexpr int kStopFeatures = 12;

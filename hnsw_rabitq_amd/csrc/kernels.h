@@ -12,6 +12,7 @@ namespace hg {
 size_t search_lds_bytes(uint32_t dim, uint32_t cap, uint32_t hash_bits);
 int search_blocks_per_cu(const SearchParams& p, bool filtered = false);
 hipError_t launch_search(const SearchParams& p, int blocks, hipStream_t s);
+hipError_t launch_zero_u32(uint32_t* p, uint32_t n, hipStream_t s);  // a kernel (graph-safe), not a memset
 hipError_t launch_copy(void* dst, const void* src, size_t bytes, hipStream_t s);
 hipError_t launch_id_labels(const DevGraph& g, const uint32_t* ids, size_t n, uint64_t* out, hipStream_t s);
 
@@ -23,6 +24,7 @@ struct BfData {
     const uint8_t* vec_fp8;    // [n][dim] e4m3 (optional)
     const float* fp8_scale;    // [n] per-vector scale (optional)
     const float* norms;        // [n] squared norms in FP32 (L2 GEMM form), optional
+    const uint32_t* allow;     // [ceil(n/32)] bitmap of rows a query may return (exact path), optional
     uint32_t n, dim;
     int32_t space;
 };

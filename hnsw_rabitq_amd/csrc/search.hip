@@ -250,7 +250,8 @@ __device__ __forceinline__ bool id_allowed(uint32_t wdel, uint32_t wal, uint32_t
 // never returned.  The beam ends right after the ef-th allowed entry once there are ef of them
 // (nothing beyond it can be expanded: the reference stops at the first candidate farther than
 // the result set's worst once that set is full); before that every admitted key is kept, up to
-// the capacity `cap` (status bit 0 flags a query that needed more).
+// the capacity `cap`; a query that needs more is abandoned and listed for a pass at a larger
+// capacity (ovf_list), or, in the last pass, sets status bit 0.
 template <int SPACE, int MODE, int NCH, bool STOP, bool TRACE, bool FILT>
 __global__ __launch_bounds__(64) void hnsw_search_kernel(SearchParams P) {
     extern __shared__ __align__(16) unsigned char smem[];
@@ -274,16 +275,21 @@ __global__ __launch_bounds__(64) void hnsw_search_kernel(SearchParams P) {
     MlpRegs mw;
     if (STOP) mlp_load(P.stop, lane, mw);
 
+    const uint32_t nq_all = P.nq_dev ? *P.nq_dev : P.nq;
     for (;;) {
         if (lane == 0) s_slot = atomicAdd(P.work, 1u);
         __syncthreads();
         const uint32_t slot = s_slot;
         __syncthreads();
-        if (slot >= P.nq) break;
+        if (slot >= nq_all) break;
         const uint32_t qi = P.order ? P.order[slot] : slot;
-        const int kq = P.k_pq ? P.k_pq[qi] : (int)P.k;
+        // per-query values are clamped to what the launch was sized for: k to the launch's k
+        // (at most the row stride), max(ef, k) to the beam capacity
+        int kq = P.k_pq ? P.k_pq[qi] : (int)P.k;
+        kq = kq < 1 ? 1 : (kq > (int)P.k ? (int)P.k : kq);
         const int efq = P.ef_pq ? P.ef_pq[qi] : (int)P.ef;
-        const int ef_eff = efq > kq ? efq : kq;
+        const int ef_want = efq > kq ? efq : kq;
+        const int ef_eff = ef_want < (int)cap ? ef_want : (int)cap;
 
         // query to LDS, hash cleared
         const float* qg = P.queries + (size_t)qi * g.dim;
@@ -378,6 +384,7 @@ __global__ __launch_bounds__(64) void hnsw_search_kernel(SearchParams P) {
         }
         int nrec = 0;
         bool stopped = false;
+        bool ovf = false;  // FILT: the beam overflowed (the result may differ from the reference's)
         uint32_t pf_node = SENT, pf_row = SENT;
         for (;;) {
             // first unexpanded entry at or after scan_lo, and the one after it (whose link row is
@@ -503,7 +510,7 @@ __global__ __launch_bounds__(64) void hnsw_search_kernel(SearchParams P) {
                         // batch never reads, and a wave's LDS operations execute in order.
                         const int lim = FILT ? (int)cap : ef_eff;  // entries kept by position
                         const int new_n = n + m < lim ? n + m : lim;
-                        if (FILT && n + m > lim && lane == 0) atomicOr(P.status, 1u);
+                        ovf |= FILT && n + m > lim;
                         const int hi_chunk = (n - 1) & ~63;
                         const int lo_chunk = first_pos & ~63;
                         // Reads are unconditional (indices below cap, clamped at 0); a lane whose
@@ -636,8 +643,15 @@ __global__ __launch_bounds__(64) void hnsw_search_kernel(SearchParams P) {
                 st.improved_recent <<= 1;
                 if (stopped) break;
             }
+            // a pass that lists overflowing queries for a larger beam abandons them at once
+            if (FILT && ovf && P.ovf_list) break;
         }
 
+        if (FILT && ovf && P.ovf_list) {
+            if (lane == 0) P.ovf_list[atomicAdd(P.ovf_count, 1u)] = qi;
+            __syncthreads();
+            continue;
+        }
         if (FILT) {
             // the result set: allowed entries in beam order, compacted to the front in place
             // (a chunk's reads precede its writes, and every entry moves down or stays)
@@ -662,6 +676,7 @@ __global__ __launch_bounds__(64) void hnsw_search_kernel(SearchParams P) {
                 got += __popcll(am);
             }
             n = got < kq ? got : kq;
+            if (lane == 0 && ovf) atomicOr(P.status, 1u);
             __syncthreads();
         }
         // ---- output: first k entries, rows ordered by (distance, label) ----
@@ -742,6 +757,12 @@ __global__ void zero_u32_kernel(uint32_t* p, uint32_t n) {
     if (i < n) p[i] = 0u;
 }
 
+hipError_t launch_zero_u32(uint32_t* p, uint32_t n, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(zero_u32_kernel, dim3((n + 63) / 64), dim3(64), 0, s, p, n);
+    return hipGetLastError();
+}
+
 // ------------------------------------------------------------------------------------------
 // Host-side launch
 // ------------------------------------------------------------------------------------------
@@ -780,7 +801,6 @@ hipError_t launch_search(const SearchParams& p, int blocks, hipStream_t s) {
     hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, s, w, 1u);
     bool stop = p.use_stop != 0, trace = p.trace_feat != nullptr;
     if (p.del_bits || p.allow_bits) {
-        hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, s, p.status, 1u);
         const bool v16 = p.g.dim % 16 == 0;
         if (p.g.space == 0)
             return v16 ? launch_t<0, 1, 0, false, false, true>(p, blocks, lds, s)

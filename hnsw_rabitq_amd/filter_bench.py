@@ -1,0 +1,89 @@
+"""Non-bare-bone search measurement on the 1M x 128 L2 index: a label filter that keeps 10% of
+the elements (label % 10 == 0), and 10% of the elements marked deleted (label % 10 == 1).
+Ground truth is the exact FP32 top-10 over the returnable rows (GPU brute force over that subset).
+Reports recall@10, kernel time and queries/s per ef; the operating point is the smallest ef with
+recall@10 >= 0.95."""
+import numpy as np
+
+FILTER_EFS = (10, 20, 40, 80, 160)
+DELETE_EFS = (400, 800, 1600)
+
+
+def _recall(labels, gt, k):
+    hits = 0
+    for a, b in zip(labels[:, :k], gt[:, :k]):
+        hits += len(np.intersect1d(a, b))
+    return hits / (labels.shape[0] * k)
+
+
+def _exact(H, lib, data, rows, dq, nq, k, stream):
+    from . import DeviceArray
+    bf = H.BFIndex("l2", data.shape[1])
+    bf.init_index(rows.shape[0])
+    bf.add_items(data[rows], rows.astype(np.uint64))
+    gl, gd = DeviceArray(nq * k * 8), DeviceArray(nq * k * 4)
+    lib.call("hg_bf_search_device", bf._h, dq.ptr, nq, k, lib.PREC_FP32, 0, gl.ptr, gd.ptr, None, stream)
+    return gl.to_numpy(np.uint64, (nq, k))
+
+
+def run(H, lib, idx, gpu, data, dq, nq, stream, timer, log, k=10, steps=3):
+    from . import DeviceArray
+    n = data.shape[0]
+    dl, dd, dst = DeviceArray(nq * k * 8), DeviceArray(nq * k * 4), DeviceArray(4)
+    dc = DeviceArray(nq * 16)
+    labels = np.arange(n, dtype=np.uint64)  # the bench index uses label = row
+
+    def sweep(name, efs, gt, flt):
+        pts = []
+        for ef in efs:
+            gpu.search_device_filtered(dq, nq, k, ef, dl, dd, filter=flt, d_status=dst, stream=stream)
+            ms = []
+            for _ in range(steps):
+                timer.start()
+                gpu.search_device_filtered(dq, nq, k, ef, dl, dd, filter=flt, d_status=dst, stream=stream)
+                ms.append(timer.stop())
+            lab = dl.to_numpy(np.uint64, (nq, k))
+            status = int(dst.to_numpy(np.uint32, (1,))[0])
+            second = gpu.status()[1]
+            gpu.search_device_filtered(dq, nq, k, ef, dl, dd, filter=flt, d_status=dst, d_counters=dc,
+                                       stream=stream)
+            ctr = dc.to_numpy(np.int32, (nq, 4))
+            rec = _recall(lab, gt, k)
+            t = float(np.mean(ms))
+            log(f"  {name} ef={ef:4d} recall@10={rec:.4f} {t:8.2f} ms QPS={nq / (t * 1e-3):10.0f} "
+                f"hops={ctr[:, 0].mean():.0f} evals={ctr[:, 1].mean():.0f} second-pass queries={second} "
+                f"status={status}")
+            pts.append(dict(ef=ef, recall=rec, ms=t, qps=nq / (t * 1e-3), second_pass=second,
+                            hops=float(ctr[:, 0].mean()), evals=float(ctr[:, 1].mean()),
+                            overflow=bool(status & 1)))
+        op = next((p for p in pts if p["recall"] >= 0.95 and not p["overflow"]), None)
+        return pts, op
+
+    out = {}
+    # label filter: 10% of the labels
+    keep = labels[labels % 10 == 0]
+    log(f"filtered search: filter keeps {keep.shape[0]} of {n} labels")
+    flt = gpu.make_filter(keep)
+    gt = _exact(H, lib, data, keep.astype(np.int64), dq, nq, k, stream)
+    pts, op = sweep("filter10%", FILTER_EFS, gt, flt)
+    flt.free()
+    out["filter10_sweep"] = pts
+    out["filter10_qps_at_095"] = op["qps"] if op else None
+    out["filter10_ef"] = op["ef"] if op else None
+
+    # deletions: 10% of the elements marked deleted, then unmarked again
+    dele = labels[labels % 10 == 1]
+    log(f"deleted elements: marking {dele.shape[0]}")
+    for lab in dele:
+        lib.call("hg_hnsw_mark_deleted", idx.cpu_handle(), int(lab))
+    gpu.refresh_deleted(idx.cpu_handle())
+    alive = np.nonzero(labels % 10 != 1)[0]
+    gt = _exact(H, lib, data, alive, dq, nq, k, stream)
+    pts, op = sweep("deleted10%", DELETE_EFS, gt, None)
+    for lab in dele:
+        lib.call("hg_hnsw_unmark_deleted", idx.cpu_handle(), int(lab))
+    gpu.refresh_deleted(idx.cpu_handle())
+    out["deleted10_sweep"] = pts
+    out["deleted10_qps_at_095"] = op["qps"] if op else None
+    out["deleted10_ef"] = op["ef"] if op else None
+    return out

@@ -26,6 +26,12 @@
  *                         BruteforceSearch::searchKnn bruteforce.h:106-125
  *   hg_bf_search_device   the same, device-resident, with reduced-precision distance GEMM
  *                         (BF16 / FP8) and an exact FP32 re-rank of the top-c candidates
+ *   hg_bf_search_filtered BFIndex.knn_query(filter=) -> searchKnn(q, k, isIdAllowed)
+ *                         bruteforce.h:106-125 (exact FP32)
+ *   hg_bf_delete          BFIndex.delete_vector bindings.cpp:829-831 -> removePoint
+ *                         bruteforce.h:85-104
+ *   hg_bf_save / _load    BFIndex.save_index / load_index -> saveIndex / loadIndex
+ *                         bruteforce.h:128-161
  */
 #ifndef HNSWGPU_H
 #define HNSWGPU_H
@@ -90,7 +96,9 @@ int hg_gpu_search(void* gpu_index, const float* queries, size_t nq, size_t k, si
                   float* dists, int32_t* counters);
 
 /* Device-resident batched search.  ef_per_query / k_per_query (device int32 [nq]) may be NULL,
- * then ef / k apply to all queries.  Rows of labels/dists have stride k_stride (>= every k).
+ * then ef / k apply to all queries; when given, ef and k must be at least every per-query value
+ * (they size the launch: a larger per-query k is clamped to k, a larger max(ef, k) to the beam
+ * capacity).  Rows of labels/dists have stride k_stride (>= every k).
  * stop_model (may be NULL) enables learned early termination (hg_stop_model_*).
  * stream is a hipStream_t (NULL = default stream).  Asynchronous: no host synchronisation. */
 int hg_gpu_search_device(void* gpu_index, const float* d_queries, size_t nq, const int32_t* d_ef_per_query,
@@ -111,16 +119,19 @@ int hg_gpu_filter_count(void* filter, size_t* allowed);
 /* hg_gpu_search with a filter (NULL = no filter). */
 int hg_gpu_search_filtered(void* gpu_index, const float* queries, size_t nq, size_t k, size_t ef, void* filter,
                            uint64_t* labels, float* dists, int32_t* counters);
-/* Device-resident filtered search (as hg_gpu_search_device, no stop model).  d_status (device
- * uint32, may be NULL = the index's own word) is cleared by the launch and gets bit 0 set when a
- * query outgrew the beam; its results are then not those of the reference. */
+/* Device-resident filtered search (as hg_gpu_search_device, no stop model).  Two launches on
+ * the stream: the beam planned from the allowed fraction, then, for the queries that outgrew
+ * it (listed on the device), the largest beam.  d_status (device uint32, may be NULL = the
+ * index's own word) is cleared by the call and gets bit 0 set when a query outgrew even the
+ * largest beam; its results are then not those of the reference. */
 int hg_gpu_search_device_filtered(void* gpu_index, const float* d_queries, size_t nq, const int32_t* d_ef_per_query,
                                   const int32_t* d_k_per_query, size_t ef, size_t k, size_t k_stride,
                                   uint64_t* d_labels, float* d_dists, int32_t* d_counters, void* filter,
                                   uint32_t* d_status, void* stream);
-/* The index's own status word (see above) after the last non-bare-bone launch that used it;
- * synchronise the launch's stream first. */
-int hg_gpu_search_status(void* gpu_index, uint32_t* status);
+/* After the last search launched on this index (synchronise its stream first): the index's own
+ * status word (see above) and the number of queries that needed the second, largest-beam pass.
+ * Either pointer may be NULL. */
+int hg_gpu_search_status(void* gpu_index, uint32_t* status, uint32_t* second_pass_queries);
 /* Re-reads the delete marks (markDelete / unmarkDelete) of the CPU index this GPU index was
  * exported from, without a new export.  Fails if elements were added since the export. */
 int hg_gpu_refresh_deleted(void* gpu_index, void* index);
@@ -159,6 +170,28 @@ int hg_bf_search_device(void* bf, const float* d_queries, size_t nq, size_t k, i
  * out[2] = max |d32 - d64| over all returned results, out[3] = max relative error. */
 int hg_bf_fp64_check(void* bf, const float* d_queries, size_t nq, size_t k, const uint64_t* d_labels,
                      const float* d_dists, double rel_tol, double* out, void* stream);
+
+/* Label of every row, in row order (capacity >= the count). */
+int hg_bf_labels(void* bf, uint64_t* labels, size_t capacity);
+/* removePoint: the last row moves into the removed row; an unknown label is ignored. */
+int hg_bf_delete(void* bf, uint64_t label);
+/* saveIndex format: size_t max_elements, size_t bytes per element (dim*4 + 8), size_t count,
+ * then max_elements records [dim f32 | u64 label] (records past the count are zero). */
+int hg_bf_save(void* bf, const char* path);
+/* loadIndex into a new GPU index (capacity from the file); the label -> row map is rebuilt, so
+ * later adds of a known label overwrite its row and deletions find it. */
+int hg_bf_load(int space, size_t dim, const char* path, int device, void** out_bf);
+/* Label filter of a brute-force index (bitmap over its rows; labels absent from the index are
+ * ignored).  Valid until the next add or delete on the index. */
+int hg_bf_filter_create(void* bf, const uint64_t* allowed_labels, size_t n_allowed, void** out_filter);
+int hg_bf_filter_free(void* filter);
+/* Exact FP32 search among the allowed rows (filter may be NULL): the k smallest
+ * (distance, label) pairs, closest first.  HG_ERR_SHORT when fewer than k rows are allowed. */
+int hg_bf_search_filtered(void* bf, const float* queries, size_t nq, size_t k, void* filter, uint64_t* labels,
+                          float* dists);
+/* The same on device buffers, asynchronous on stream. */
+int hg_bf_search_device_filtered(void* bf, const float* d_queries, size_t nq, size_t k, void* filter,
+                                 uint64_t* d_labels, float* d_dists, void* stream);
 
 /* Tile configuration of the reduced-precision distance GEMM (process-wide): 0 = 128x128
  * workgroup tiles with 2x2 MFMA blocks per wave and double-buffered LDS; 1 = 256x128 tiles

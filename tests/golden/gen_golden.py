@@ -4,6 +4,7 @@ the hnswlib headers with setup.py's flags).  Run in a container where /root/refe
 
     make -C oracle ref && python tests/golden/gen_golden.py        (all fixtures)
                           python tests/golden/gen_golden.py nbb    (the deletion/filter fixture only)
+                          python tests/golden/gen_golden.py bf_ops (the brute-force operations fixture)
 
 Every fixture stores its inputs' SHA-256 (or the inputs themselves when small), the reference's
 outputs and the SHA-256 of the reference's saveIndex bytes for single-threaded builds.
@@ -135,9 +136,50 @@ def nbb():
     print("c1_nbb_golden.npz", out["index_sha"][:16], "entry label", entry_label)
 
 
+def bf_ops():
+    """F6: BFIndex deletions, replacement, appends, a label filter and saveIndex on C1.  Stores
+    the reference's results and the SHA-256 of the saved file's header and used records (records
+    past the count are uninitialised memory in the reference's file)."""
+    hnswlib = pyoracle.ref_module()
+    assert hnswlib is not None, "build the reference first: make -C oracle ref"
+    data = datasets.c1_uniform()
+    queries = datasets.c1_uniform(n=1000, seed=1)
+    n = data.shape[0]
+    bf = hnswlib.BFIndex(space="l2", dim=16)
+    bf.init_index(max_elements=n + 100)
+    bf.add_items(data)
+    deleted = list(range(0, n, 7))
+    for lab in deleted:
+        bf.delete_vector(lab)
+    bf.delete_vector(10 ** 7)  # unknown label: ignored
+    extra = datasets.c1_uniform(n=60, seed=2)
+    bf.add_items(extra[:50], ids=np.arange(20000, 20050))
+    bf.add_items(extra[50:55], ids=np.array([3, 5, 8, 9, 10]))  # existing labels: rows overwritten
+    bf.add_items(extra[55:60])  # default labels continue the counter of rows passed in
+    out = dict(data_sha=sha(data), query_sha=sha(queries), deleted=np.array(deleted, np.uint64))
+    lab, dis = bf.knn_query(queries, k=10, num_threads=1)
+    out["ids_ops"], out["dists_ops"] = lab.astype(np.uint64), dis.astype(np.float32)
+    lab, dis = bf.knn_query(queries, k=10, num_threads=1, filter=lambda l: l % 3 != 0)
+    out["ids_ops_f3"], out["dists_ops_f3"] = lab.astype(np.uint64), dis.astype(np.float32)
+    path = "/tmp/_golden_bf.bin"
+    bf.save_index(path)
+    with open(path, "rb") as f:
+        blob = f.read()
+    os.remove(path)
+    hdr = np.frombuffer(blob[:24], np.uint64)
+    used = 24 + int(hdr[2]) * int(hdr[1])
+    out["file_header"] = hdr.copy()
+    out["file_used_sha"] = hashlib.sha256(blob[:used]).hexdigest()
+    np.savez_compressed(os.path.join(OUT, "c1_bf_ops_golden.npz"), **out)
+    print("c1_bf_ops_golden.npz", out["file_used_sha"][:16], "count", int(hdr[2]))
+
+
 if __name__ == "__main__":
     if len(sys.argv) > 1 and sys.argv[1] == "nbb":
         nbb()
+    elif len(sys.argv) > 1 and sys.argv[1] == "bf_ops":
+        bf_ops()
     else:
         main()
         nbb()
+        bf_ops()

@@ -63,7 +63,7 @@ def _check_against_oracle(orc, p, o, queries, case, gpu_filter=None):
     np.testing.assert_array_equal(ctr[:, 0], ct[:, 0])
     np.testing.assert_array_equal(ctr[:, 2:], ct[:, 2:])
     assert (ctr[:, 1] >= ct[:, 1]).all()
-    assert p.gpu().status() == 0
+    assert p.gpu().status()[0] == 0
 
 
 @pytest.mark.parametrize("case", ["del", "del_f3", "del_f50"])
@@ -184,8 +184,8 @@ def test_gpu_filtered_device_path_per_query(orc, hg):
     dl = hg.DeviceArray(nq * kmax * 8)
     dd = hg.DeviceArray(nq * kmax * 4)
     ds = hg.DeviceArray(4)
-    _lib.call("hg_gpu_search_device_filtered", gi.handle, dq.ptr, nq, de.ptr, dk.ptr, 0, kmax, kmax, dl.ptr, dd.ptr,
-              None, f.handle, ds.ptr, None)
+    _lib.call("hg_gpu_search_device_filtered", gi.handle, dq.ptr, nq, de.ptr, dk.ptr, int(efq.max()), kmax, kmax,
+              dl.ptr, dd.ptr, None, f.handle, ds.ptr, None)
     _lib.call("hg_device_synchronize")
     lab = dl.to_numpy(np.uint64, (nq, kmax))
     dis = dd.to_numpy(np.float32, (nq, kmax))
@@ -196,3 +196,41 @@ def test_gpu_filtered_device_path_per_query(orc, hg):
         kk = int(kq[r])
         np.testing.assert_array_equal(lab[r, :kk], lt[r, :kk])
         np.testing.assert_array_equal(dis[r, :kk].view(np.uint32), dt[r, :kk].view(np.uint32))
+
+
+def test_gpu_filter_correlated_with_position_retries_large_beam(orc, hg):
+    """A filter that keeps 30% of the elements, all far from the queries (first coordinate above
+    0.7, queries below 0.2): the beam planned from the global fraction overflows for most queries,
+    which the second pass runs again at the largest beam; device and host paths end equal to the
+    oracle."""
+    from hnsw_rabitq_amd import DeviceArray
+    space, data, queries = workload("c1")
+    p = hg.Index(space, data.shape[1])
+    p.init_index(data.shape[0], M=16, ef_construction=200, random_seed=100)
+    p.add_items(data, num_threads=1)
+    q = np.ascontiguousarray(queries[queries[:, 0] < 0.2])
+    nq = q.shape[0]
+    print(f"case=c1 correlated filter nq={nq}", flush=True)
+    gi = p.gpu()
+    keep = np.nonzero(data[:, 0] > 0.7)[0].astype(np.uint64)  # labels = row numbers
+    f = gi.make_filter(keep)
+    dq = DeviceArray.from_numpy(q)
+    dl, dd, ds = DeviceArray(nq * K * 8), DeviceArray(nq * K * 4), DeviceArray(4)
+    gi.search_device_filtered(dq, nq, K, EF, dl, dd, filter=f, d_status=ds)
+    assert int(ds.to_numpy(np.uint32, (1,))[0]) == 0
+    status, second = gi.status()
+    print(f"queries run again at the largest beam: {second} of {nq}", flush=True)
+    assert second > nq // 2
+    dev_lab = dl.to_numpy(np.uint64, (nq, K))
+    lab, dis, ctr, short = gi.search(q, K, EF, counters=True, filter=f)
+    assert not short
+    assert gi.status() == (0, second)
+    np.testing.assert_array_equal(dev_lab, lab)
+    o = orc.OracleHnsw(blob=p.serialize())
+    lab_by_id = o.labels()[0].astype(np.int64)
+    allowed = (data[lab_by_id, 0] > 0.7).astype(np.uint8)
+    lt, dt, ct, _, _ = o.search(0, q, K, EF, mode=orc.MODE_TOTAL, counters=True, allowed=allowed)
+    np.testing.assert_array_equal(lab, lt)
+    np.testing.assert_array_equal(dis.view(np.uint32), dt.view(np.uint32))
+    np.testing.assert_array_equal(ctr[:, 0], ct[:, 0])
+    assert (keep[np.searchsorted(keep, lab.reshape(-1))] == lab.reshape(-1)).all()
