@@ -88,8 +88,8 @@ std::atomic<uint64_t> g_export_serial{0};
 // result set holds ef allowed entries, so a selective filter needs about ef / (allowed fraction)
 // entries.  8192 entries = 64 KiB of LDS.
 constexpr uint32_t kFiltMaxCap = 8192;
-// Launch words: [0] status, [1..8] work counters of the passes, [9..16] overflow-list counts.
-constexpr int kMaxPasses = 8, kWordStatus = 0, kWordWork = 1, kWordCount = 1 + kMaxPasses;
+// Launch words: [0] status, [1..12] work counters of the passes, [13..24] overflow-list counts.
+constexpr int kMaxPasses = 12, kWordStatus = 0, kWordWork = 1, kWordCount = 1 + kMaxPasses;
 constexpr uint32_t kWords = 1 + 2 * kMaxPasses;
 
 Plan make_plan(GpuIndex* gi, size_t max_ef, size_t max_k, size_t nq, bool filtered = false,
@@ -106,8 +106,8 @@ Plan make_plan(GpuIndex* gi, size_t max_ef, size_t max_k, size_t nq, bool filter
         const double want = 1.1 * (double)ef_eff / std::max(allowed_frac, 1e-4) + 128.0;
         const double c = std::min<double>(kFiltMaxCap, std::max<double>({want, (double)p.cap, (double)min_cap}));
         p.cap = std::min<uint32_t>(kFiltMaxCap, ((uint32_t)c + 63) / 64 * 64);
-        // lossy, like the bare-bone plan's: about one slot per two beam entries
-        p.hash_bits = std::min<uint32_t>(12, std::max<uint32_t>(10, ceil_log2(p.cap) - 1));
+        // lossy, like the tuned bare-bone plans: LDS goes to the beam
+        p.hash_bits = 10;
     } else if (it != gi->tuned.end()) {
         p.hash_bits = it->second.first;
     } else {
@@ -124,6 +124,13 @@ Plan make_plan(GpuIndex* gi, size_t max_ef, size_t max_k, size_t nq, bool filter
             p.hash_bits--;
         while (hg::search_lds_bytes(gi->g.dim, p.cap, p.hash_bits) > 160 * 1024 && p.cap > 64) p.cap -= 64;
         require(p.cap >= ef_eff, "ef and dimension too large for the filtered GPU search");
+        // the largest beam that keeps the same number of workgroups per CU (a free margin)
+        const size_t per = (size_t)(160 * 1024) / hg::search_lds_bytes(gi->g.dim, p.cap, p.hash_bits);
+        if (per >= 1) {
+            const size_t budget = (size_t)(160 * 1024) / per - 512;
+            while (p.cap + 64 <= kFiltMaxCap && hg::search_lds_bytes(gi->g.dim, p.cap + 64, p.hash_bits) <= budget)
+                p.cap += 64;
+        }
     }
     hg::SearchParams sp{};
     sp.g = gi->g;
@@ -173,7 +180,7 @@ void prepare_words(GpuIndex::LaunchWords* lw, size_t nq, bool filtered) {
 
 // The non-bare-bone branch runs when the index holds deleted elements or a filter is given
 // (hnswalg.h:1304-1312), in passes at growing beams: the first planned from the allowed
-// fraction, each later one (x2.5, up to the largest) taking the queries that outgrew the previous
+// fraction, each later one (x1.5, up to the largest) taking the queries that outgrew the previous
 // one from a list kept on the device.  All passes are asynchronous and graph-capturable.  status
 // (device word, may be null = the launch words' own) gets bit 0 when a query outgrew the largest
 // beam too.  lw: the launch words to use (prepared).
@@ -227,7 +234,7 @@ hipError_t run_search(GpuIndex* gi, const float* dq, size_t nq, const int32_t* d
         }
         e = hg::launch_search(sq, cur.blocks, s);
         if (e != hipSuccess || last) return e;
-        const uint32_t next = std::min<uint32_t>(kFiltMaxCap, (uint32_t)(cur.cap * 5 / 2 + 63) / 64 * 64);
+        const uint32_t next = std::min<uint32_t>(kFiltMaxCap, (uint32_t)(cur.cap * 3 / 2 + 63) / 64 * 64);
         cur = make_plan(gi, ef, k, nq, true, frac, next);
     }
 }

@@ -782,16 +782,16 @@ static hipError_t launch_t(const SearchParams& p, int blocks, size_t lds, hipStr
     return hipGetLastError();
 }
 
-template <int SPACE, bool STOP, bool TRACE>
+template <int SPACE, bool STOP, bool TRACE, bool FILT = false>
 static hipError_t launch_dim(const SearchParams& p, int blocks, size_t lds, hipStream_t s) {
-    if (p.g.dim % 16 != 0) return launch_t<SPACE, 0, 0, STOP, TRACE>(p, blocks, lds, s);
+    if (p.g.dim % 16 != 0) return launch_t<SPACE, 0, 0, STOP, TRACE, FILT>(p, blocks, lds, s);
     switch (p.g.dim / 16) {
-        case 1: return launch_t<SPACE, 1, 1, STOP, TRACE>(p, blocks, lds, s);
-        case 2: return launch_t<SPACE, 1, 2, STOP, TRACE>(p, blocks, lds, s);
-        case 4: return launch_t<SPACE, 1, 4, STOP, TRACE>(p, blocks, lds, s);
-        case 6: return launch_t<SPACE, 1, 6, STOP, TRACE>(p, blocks, lds, s);
-        case 8: return launch_t<SPACE, 1, 8, STOP, TRACE>(p, blocks, lds, s);
-        default: return launch_t<SPACE, 1, 0, STOP, TRACE>(p, blocks, lds, s);
+        case 1: return launch_t<SPACE, 1, 1, STOP, TRACE, FILT>(p, blocks, lds, s);
+        case 2: return launch_t<SPACE, 1, 2, STOP, TRACE, FILT>(p, blocks, lds, s);
+        case 4: return launch_t<SPACE, 1, 4, STOP, TRACE, FILT>(p, blocks, lds, s);
+        case 6: return launch_t<SPACE, 1, 6, STOP, TRACE, FILT>(p, blocks, lds, s);
+        case 8: return launch_t<SPACE, 1, 8, STOP, TRACE, FILT>(p, blocks, lds, s);
+        default: return launch_t<SPACE, 1, 0, STOP, TRACE, FILT>(p, blocks, lds, s);
     }
 }
 
@@ -800,14 +800,9 @@ hipError_t launch_search(const SearchParams& p, int blocks, hipStream_t s) {
     uint32_t* w = p.work;
     hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, s, w, 1u);
     bool stop = p.use_stop != 0, trace = p.trace_feat != nullptr;
-    if (p.del_bits || p.allow_bits) {
-        const bool v16 = p.g.dim % 16 == 0;
-        if (p.g.space == 0)
-            return v16 ? launch_t<0, 1, 0, false, false, true>(p, blocks, lds, s)
-                       : launch_t<0, 0, 0, false, false, true>(p, blocks, lds, s);
-        return v16 ? launch_t<1, 1, 0, false, false, true>(p, blocks, lds, s)
-                   : launch_t<1, 0, 0, false, false, true>(p, blocks, lds, s);
-    }
+    if (p.del_bits || p.allow_bits)
+        return p.g.space == 0 ? launch_dim<0, false, false, true>(p, blocks, lds, s)
+                              : launch_dim<1, false, false, true>(p, blocks, lds, s);
     if (p.g.space == 0) {
         if (trace) return launch_dim<0, false, true>(p, blocks, lds, s);
         if (stop) return launch_dim<0, true, false>(p, blocks, lds, s);
@@ -822,7 +817,9 @@ int search_blocks_per_cu(const SearchParams& p, bool filtered) {
     size_t lds = search_lds_bytes(p.g.dim, p.cap, p.hash_bits);
     int nb = 0;
     hipError_t e;
-    if (filtered)
+    if (filtered && p.g.dim == 128 && p.g.space == 0)
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, hnsw_search_kernel<0, 1, 8, false, false, true>, 64, lds);
+    else if (filtered)
         e = p.g.space == 0
                 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, hnsw_search_kernel<0, 1, 0, false, false, true>, 64, lds)
                 : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, hnsw_search_kernel<1, 1, 0, false, false, true>, 64, lds);
