@@ -117,6 +117,22 @@ class GpuHnsw:
             _lib.check(code)
         return labels, dists, ctr, code == _lib.HG_ERR_SHORT
 
+    def search_epsilon(self, queries, epsilon, min_candidates, max_candidates, out_stride=None, filter=None):
+        """Range search (searchStopConditionClosest + EpsilonSearchStopCondition): (labels, dists,
+        counts); row q holds the closest min(counts[q], out_stride) results."""
+        q = np.ascontiguousarray(queries, dtype=np.float32)
+        nq = q.shape[0]
+        stride = int(out_stride or min(max_candidates, 8192))
+        if filter is not None and filter.owner is not self:
+            raise ValueError("the filter was built for another GPU index")
+        labels = np.empty((nq, stride), dtype=np.uint64)
+        dists = np.empty((nq, stride), dtype=np.float32)
+        counts = np.empty(nq, dtype=np.uint32)
+        _lib.call("hg_gpu_search_epsilon", self.handle, _ptr(q), nq, float(epsilon), int(min_candidates),
+                  int(max_candidates), filter.handle if filter is not None else None, stride, _ptr(labels),
+                  _ptr(dists), _ptr(counts))
+        return labels, dists, counts
+
     def labels(self):
         """External label of every internal id (getExternalLabel, hnswalg.h:185-189)."""
         n = ctypes.c_size_t()
@@ -362,6 +378,33 @@ class Index:
     def make_filter(self, allowed_labels):
         """A reusable GpuFilter of the current export that allows the given labels."""
         return self.gpu().make_filter(allowed_labels)
+
+    def range_query(self, data, epsilon, min_candidates, max_candidates, filter=None):
+        """The C++ API's searchStopConditionClosest with EpsilonSearchStopCondition (no Python
+        binding in hnswlib): per query, the elements within `epsilon` (distance as the space
+        reports it) found by the search, closest first, at most max_candidates.  Returns a list
+        of (labels, distances) pairs.  filter: a callable label -> bool or a GpuFilter."""
+        if self._h is None:
+            raise RuntimeError("Index not initialized")
+        arr = _as_rows(data, self.dim)
+        if self.normalize:
+            arr = normalize_rows(arr)
+        if arr.shape[0] == 0:
+            return []
+        gpu = self.gpu()
+        gf = filter if isinstance(filter, GpuFilter) or filter is None else None
+        if filter is not None and gf is None:
+            labs = gpu.labels()
+            keep = np.fromiter((bool(filter(int(x))) for x in labs), dtype=bool, count=labs.shape[0])
+            gf = gpu.make_filter(labs[keep])
+        try:
+            lab, dis, cnt = gpu.search_epsilon(arr, epsilon, min_candidates, max_candidates, filter=gf)
+        finally:
+            if gf is not None and gf is not filter:
+                gf.free()
+        if (cnt > lab.shape[1]).any():
+            raise RuntimeError("more than 8192 results in range; lower max_candidates")
+        return [(lab[i, :cnt[i]].copy(), dis[i, :cnt[i]].copy()) for i in range(arr.shape[0])]
 
     def knn_query(self, data, k=1, num_threads=-1, filter=None):
         """filter: a callable label -> bool, as in hnswlib (evaluated here once per element),

@@ -55,6 +55,9 @@ SIGNATURES = {
     "hg_gpu_search_filtered": [_vp, _vp, _sz, _sz, _sz, _vp, _vp, _vp, _vp],
     "hg_gpu_search_device_filtered": [_vp, _vp, _sz, _vp, _vp, _sz, _sz, _sz, _vp, _vp, _vp, _vp, _vp, _vp],
     "hg_gpu_search_status": [_vp, _vp, _vp],
+    "hg_gpu_search_epsilon": [_vp, _vp, _sz, ctypes.c_float, _sz, _sz, _vp, _sz, _vp, _vp, _vp],
+    "hg_gpu_search_epsilon_device": [_vp, _vp, _sz, ctypes.c_float, _sz, _sz, _vp, _sz, _vp, _vp, _vp, _vp, _vp,
+                                     _vp],
     "hg_gpu_refresh_deleted": [_vp, _vp],
     "hg_gpu_index_labels": [_vp, _vp, _sz, _vp],
     "hg_bf_create": [_i32, _sz, _sz, _i32, _pvp],

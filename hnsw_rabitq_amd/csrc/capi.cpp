@@ -48,7 +48,7 @@ struct GpuIndex {
     size_t num_deleted = 0;
     std::vector<uint64_t> host_labels;
     // host-path staging
-    DevBuf q, ol, od, ctr;
+    DevBuf q, ol, od, ctr, cnt;
     hipStream_t stream = nullptr;
     size_t bytes = 0;
     std::mutex mu;
@@ -115,12 +115,9 @@ Plan make_plan(GpuIndex* gi, size_t max_ef, size_t max_k, size_t nq, bool filter
         // (an overwritten slot only costs a re-evaluation), so it is sized for occupancy
         p.hash_bits = std::min<uint32_t>(12, std::max<uint32_t>(10, ceil_log2(8 * ef_eff)));
     }
-    // the output pass stages k labels (8 bytes each) in the hash area
-    const uint32_t min_bits = ceil_log2(2 * (uint64_t)std::max<size_t>(max_k, 1));
-    p.hash_bits = std::max(p.hash_bits, min_bits);
     if (filtered) {
         // a CU holds 160 KiB of LDS: shrink the (lossy) hash, then the beam, to fit
-        while (hg::search_lds_bytes(gi->g.dim, p.cap, p.hash_bits) > 160 * 1024 && p.hash_bits > std::max(10u, min_bits))
+        while (hg::search_lds_bytes(gi->g.dim, p.cap, p.hash_bits) > 160 * 1024 && p.hash_bits > 9)
             p.hash_bits--;
         while (hg::search_lds_bytes(gi->g.dim, p.cap, p.hash_bits) > 160 * 1024 && p.cap > 64) p.cap -= 64;
         require(p.cap >= ef_eff, "ef and dimension too large for the filtered GPU search");
@@ -171,6 +168,13 @@ double allowed_fraction(const GpuIndex* gi, const GpuFilter* f) {
 
 bool is_filtered(const GpuIndex* gi, const GpuFilter* filt) { return gi->num_deleted > 0 || filt != nullptr; }
 
+// Range search (EpsilonSearchStopCondition, stop_condition.h:207-275).
+struct EpsArgs {
+    float eps;
+    size_t min_c, max_c;
+    uint32_t* counts;  // device [nq]
+};
+
 // Reserves the launch words (and, for the non-bare-bone search, the overflow list) that a
 // launch of nq queries needs; done before a graph capture, where allocation is not allowed.
 void prepare_words(GpuIndex::LaunchWords* lw, size_t nq, bool filtered) {
@@ -187,10 +191,14 @@ void prepare_words(GpuIndex::LaunchWords* lw, size_t nq, bool filtered) {
 hipError_t run_search(GpuIndex* gi, const float* dq, size_t nq, const int32_t* def, const int32_t* dk, size_t ef,
                       size_t k, size_t k_stride, uint64_t* dl, float* dd, int32_t* dc, StopModel* stop,
                       hipStream_t s, GpuIndex::LaunchWords* lw, const GpuFilter* filt = nullptr,
-                      uint32_t* status = nullptr) {
-    const bool filtered = is_filtered(gi, filt);
+                      uint32_t* status = nullptr, const EpsArgs* eps = nullptr) {
+    const bool filtered = is_filtered(gi, filt) || eps != nullptr;
     const double frac = filtered ? allowed_fraction(gi, filt) : 1.0;
-    Plan pl = make_plan(gi, ef, k, nq, filtered, frac);
+    // the range search's beam holds at least min_c results before it may stop: the first pass
+    // is planned from that, later passes grow it
+    const size_t plan_ef = eps ? std::max<size_t>(eps->min_c, 1) : ef;
+    const size_t plan_k = eps ? 1 : k;
+    Plan pl = make_plan(gi, plan_ef, plan_k, nq, filtered, frac);
     hg::SearchParams sp = base_params(gi, pl, lw);
     sp.queries = dq;
     sp.nq = (uint32_t)nq;
@@ -205,6 +213,13 @@ hipError_t run_search(GpuIndex* gi, const float* dq, size_t nq, const int32_t* d
     if (stop) {
         sp.use_stop = 1;
         sp.stop = dev_stop(stop);
+    }
+    if (eps) {
+        sp.ef = (uint32_t)std::min<size_t>(eps->max_c, 0x7FFFFFFF);
+        sp.eps_mode = 1;
+        sp.eps = eps->eps;
+        sp.eps_min = (int)std::min<size_t>(eps->min_c, 0x7FFFFFFF);
+        sp.counts = eps->counts;
     }
     if (!filtered) return hg::launch_search(sp, pl.blocks, s);
 
@@ -235,7 +250,7 @@ hipError_t run_search(GpuIndex* gi, const float* dq, size_t nq, const int32_t* d
         e = hg::launch_search(sq, cur.blocks, s);
         if (e != hipSuccess || last) return e;
         const uint32_t next = std::min<uint32_t>(kFiltMaxCap, (uint32_t)(cur.cap * 3 / 2 + 63) / 64 * 64);
-        cur = make_plan(gi, ef, k, nq, true, frac, next);
+        cur = make_plan(gi, plan_ef, plan_k, nq, true, frac, next);
     }
 }
 
@@ -563,6 +578,82 @@ HG_API int hg_gpu_search_filtered(void* gv, const float* queries, size_t nq, siz
     });
 }
 
+namespace {
+void check_eps_args(float eps, size_t min_c, size_t max_c, size_t out_stride) {
+    require(max_c >= 1 && min_c <= max_c, "need 1 <= max_candidates and min_candidates <= max_candidates");
+    require(out_stride >= 1 && out_stride <= kFiltMaxCap, "out_stride must be in [1, 8192]");
+    require(!(eps != eps), "epsilon is NaN");
+}
+}  // namespace
+
+// searchStopConditionClosest with EpsilonSearchStopCondition on host buffers (hnswalg.h:1326-1377).
+HG_API int hg_gpu_search_epsilon(void* gv, const float* queries, size_t nq, float epsilon, size_t min_candidates,
+                                 size_t max_candidates, void* filter, size_t out_stride, uint64_t* labels,
+                                 float* dists, uint32_t* counts) {
+    return guarded([&] {
+        require(gv && labels && dists && counts, "null argument");
+        auto* gi = static_cast<GpuIndex*>(gv);
+        check_eps_args(epsilon, min_candidates, max_candidates, out_stride);
+        GpuFilter* gf = as_filter(gi, filter);
+        require(nq < (1ull << 31), "too many queries");
+        if (nq == 0) return HG_OK;
+        require(queries != nullptr, "null queries");
+        std::lock_guard<std::mutex> lk(gi->mu);
+        DeviceGuard dg(gi->device);
+        const size_t dim = gi->g.dim;
+        gi->q.reserve(nq * dim * 4);
+        gi->ol.reserve(nq * out_stride * 8);
+        gi->od.reserve(nq * out_stride * 4);
+        gi->cnt.reserve(nq * 4);
+        hipStream_t s = gi->stream;
+        hip_check(hipMemcpyAsync(gi->q.p, queries, nq * dim * 4, hipMemcpyHostToDevice, s), "hipMemcpy H2D");
+        GpuIndex::LaunchWords* lw = next_words(gi);
+        prepare_words(lw, nq, true);
+        EpsArgs ea{epsilon, min_candidates, max_candidates, (uint32_t*)gi->cnt.p};
+        hip_check(run_search(gi, (const float*)gi->q.p, nq, nullptr, nullptr, max_candidates, out_stride, out_stride,
+                             (uint64_t*)gi->ol.p, (float*)gi->od.p, nullptr, nullptr, s, lw, gf, nullptr, &ea),
+                  "search launch");
+        uint32_t status = 0;
+        hip_check(hipMemcpyAsync(&status, (uint32_t*)lw->words.p + kWordStatus, 4, hipMemcpyDeviceToHost, s),
+                  "hipMemcpy D2H");
+        hip_check(hipMemcpyAsync(labels, gi->ol.p, nq * out_stride * 8, hipMemcpyDeviceToHost, s), "hipMemcpy D2H");
+        hip_check(hipMemcpyAsync(dists, gi->od.p, nq * out_stride * 4, hipMemcpyDeviceToHost, s), "hipMemcpy D2H");
+        hip_check(hipMemcpyAsync(counts, gi->cnt.p, nq * 4, hipMemcpyDeviceToHost, s), "hipMemcpy D2H");
+        hip_check(hipStreamSynchronize(s), "search");
+        if (status & 1u) {
+            g_err = "range search: a query needed more than " + std::to_string(kFiltMaxCap) +
+                    " beam entries (lower min_candidates or epsilon)";
+            return HG_ERR_RUNTIME;
+        }
+        return HG_OK;
+    });
+}
+
+// The same on device buffers, asynchronous on stream (d_status as hg_gpu_search_device_filtered).
+HG_API int hg_gpu_search_epsilon_device(void* gv, const float* dq, size_t nq, float epsilon, size_t min_candidates,
+                                        size_t max_candidates, void* filter, size_t out_stride, uint64_t* dl,
+                                        float* dd, uint32_t* d_counts, int32_t* dc, uint32_t* d_status,
+                                        void* stream) {
+    return guarded([&] {
+        require(gv && dl && dd && d_counts, "null argument");
+        auto* gi = static_cast<GpuIndex*>(gv);
+        check_eps_args(epsilon, min_candidates, max_candidates, out_stride);
+        GpuFilter* gf = as_filter(gi, filter);
+        require(nq < (1ull << 31), "too many queries");
+        if (nq == 0) return HG_OK;
+        require(dq != nullptr, "null queries");
+        std::lock_guard<std::mutex> lk(gi->mu);
+        DeviceGuard dg(gi->device);
+        GpuIndex::LaunchWords* lw = next_words(gi);
+        prepare_words(lw, nq, true);
+        EpsArgs ea{epsilon, min_candidates, max_candidates, d_counts};
+        hip_check(run_search(gi, dq, nq, nullptr, nullptr, max_candidates, out_stride, out_stride, dl, dd, dc, nullptr,
+                             (hipStream_t)stream, lw, gf, d_status, &ea),
+                  "search launch");
+        return HG_OK;
+    });
+}
+
 HG_API int hg_gpu_filter_create(void* gv, const uint64_t* allowed_labels, size_t n_allowed, void** out) {
     return guarded([&] {
         require(gv && out && (n_allowed == 0 || allowed_labels), "null argument");
@@ -772,14 +863,13 @@ HG_API int hg_gpu_autotune(void* gv, const float* dq, size_t nq, size_t ef, size
         od.reserve(nq * k * 4);
         size_t ef_eff = std::max(ef, k);
         uint32_t cap = (uint32_t)((ef_eff + 63) / 64 * 64);
-        uint32_t min_bits = ceil_log2(2 * (uint64_t)k);
         hipEvent_t e0, e1;
         hip_check(hipEventCreate(&e0), "hipEventCreate");
         hip_check(hipEventCreate(&e1), "hipEventCreate");
         float best = 1e30f;
         std::pair<uint32_t, int> best_cfg(0, 0);
         gi->tuned.erase(cap);
-        for (uint32_t bits = std::max<uint32_t>(min_bits, 9); bits <= 13; bits++) {
+        for (uint32_t bits = 9; bits <= 13; bits++) {
             for (int per_cu : {4, 8, 12, 16, 24, 32}) {
                 gi->tuned[cap] = std::make_pair(bits, per_cu);
                 Plan pl = make_plan(gi, ef, k, nq);
@@ -817,8 +907,7 @@ HG_API int hg_gpu_set_plan(void* gv, size_t ef, size_t k, uint32_t hash_bits, in
         require(gv != nullptr, "null index");
         auto* gi = static_cast<GpuIndex*>(gv);
         require(k >= 1 && ef >= 1, "ef and k must be positive");
-        require(hash_bits >= 9 && hash_bits <= 16 && hash_bits >= ceil_log2(2 * (uint64_t)k),
-                "hash_bits must be in [max(9, log2(2k)), 16]");
+        require(hash_bits >= 9 && hash_bits <= 16, "hash_bits must be in [9, 16]");
         require(workgroups_per_cu >= 1 && workgroups_per_cu <= 32, "workgroups_per_cu must be in [1, 32]");
         std::lock_guard<std::mutex> lk(gi->mu);
         size_t ef_eff = std::max(ef, k);

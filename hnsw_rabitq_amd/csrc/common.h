@@ -75,6 +75,13 @@ struct SearchParams {
     uint32_t* ovf_list;
     uint32_t* ovf_count;
     const uint32_t* nq_dev;
+    // FILT, range search (EpsilonSearchStopCondition): ef = the result bound max_c; the search
+    // also stops at a candidate farther than eps once eps_min results are held; results are
+    // those within eps, counts[q] of them, the first k_stride written.
+    int eps_mode;
+    float eps;
+    int eps_min;
+    uint32_t* counts;
 };
 
 constexpr int kStopFeatures = 12;

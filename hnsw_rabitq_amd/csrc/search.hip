@@ -288,8 +288,11 @@ __global__ __launch_bounds__(64) void hnsw_search_kernel(SearchParams P) {
         int kq = P.k_pq ? P.k_pq[qi] : (int)P.k;
         kq = kq < 1 ? 1 : (kq > (int)P.k ? (int)P.k : kq);
         const int efq = P.ef_pq ? P.ef_pq[qi] : (int)P.ef;
-        const int ef_want = efq > kq ? efq : kq;
-        const int ef_eff = ef_want < (int)cap ? ef_want : (int)cap;
+        // the range search bounds its result set by ef alone (max_num_candidates)
+        const int ef_want = (FILT && P.eps_mode) ? efq : (efq > kq ? efq : kq);
+        // FILT keeps entries by the beam capacity (an overflow goes to a larger pass), so its
+        // result-set bound stays unclamped
+        const int ef_eff = (FILT || ef_want < (int)cap) ? ef_want : (int)cap;
 
         // query to LDS, hash cleared
         const float* qg = P.queries + (size_t)qi * g.dim;
@@ -404,6 +407,12 @@ __global__ __launch_bounds__(64) void hnsw_search_kernel(SearchParams P) {
             }
             if (p < 0) break;
             const uint32_t bip = bi[p];
+            if (FILT && P.eps_mode) {
+                // EpsilonSearchStopCondition::should_stop_search (stop_condition.h:232-243); a
+                // disallowed entry point is the reference's candidate at FLT_MAX
+                const float cd = (H0 == 0 && (bip & DFLAG)) ? __FLT_MAX__ : bd[p];
+                if (cd > P.eps && nallowed >= P.eps_min) break;
+            }
             const uint32_t node = bip & ID_MASK;
             __syncthreads();
             if (lane == 0) bi[p] = bip | XFLAG;
@@ -652,11 +661,13 @@ __global__ __launch_bounds__(64) void hnsw_search_kernel(SearchParams P) {
             __syncthreads();
             continue;
         }
+        // entries kept for the output: k, or for the range search the whole result set
+        const int keep = (FILT && P.eps_mode) ? ef_eff : kq;
         if (FILT) {
             // the result set: allowed entries in beam order, compacted to the front in place
             // (a chunk's reads precede its writes, and every entry moves down or stays)
             int got = 0;
-            for (int b = 0; b < n && got < kq; b += 64) {
+            for (int b = 0; b < n && got < keep; b += 64) {
                 const int i = b + lane;
                 float dv = 0.f;
                 uint32_t iv = 0u;
@@ -668,38 +679,55 @@ __global__ __launch_bounds__(64) void hnsw_search_kernel(SearchParams P) {
                 const uint64_t am = ballot(al);
                 const int slot = got + lanes_below(am, lane);
                 __asm__ volatile("" ::: "memory");
-                if (al && slot < kq) {
+                if (al && slot < keep) {
                     bd[slot] = dv;
                     bi[slot] = iv;
                 }
                 __asm__ volatile("" ::: "memory");
                 got += __popcll(am);
             }
-            n = got < kq ? got : kq;
+            n = got < keep ? got : keep;
             if (lane == 0 && ovf) atomicOr(P.status, 1u);
             __syncthreads();
         }
-        // ---- output: first k entries, rows ordered by (distance, label) ----
-        const int kk = kq < n ? kq : n;
-        uint64_t* lab_s = reinterpret_cast<uint64_t*>(hash);  // hash no longer needed
-        __syncthreads();
-        for (int i = lane; i < kk; i += 64) lab_s[i] = g.labels[bi[i] & ID_MASK];
+        // ---- output: rows ordered by (distance, label) ----
+        // The beam is sorted by (distance, id): entry i's rank by (distance, label) is the start
+        // of its run of equal distances plus the run entries with a smaller (label, position).
+        int nout = kq < n ? kq : n;
+        int row_len = kq;
+        if (FILT && P.eps_mode) {
+            // filter_results (stop_condition.h:262-269): the results within eps, a prefix
+            int neps = 0;
+            for (int b = 0; b < n; b += 64) {
+                const int i = b + lane;
+                const uint64_t m = ballot(i < n && bd[i] <= P.eps);
+                neps += __popcll(m);
+                if (m != ~0ull) break;
+            }
+            nout = neps;
+            row_len = (int)P.k_stride;
+            if (lane == 0) P.counts[qi] = (uint32_t)neps;
+        }
         __syncthreads();
         uint64_t* ol = P.out_labels + (size_t)qi * P.k_stride;
         float* od = P.out_dists + (size_t)qi * P.k_stride;
-        for (int i = lane; i < kk; i += 64) {
-            float di = bd[i];
-            uint64_t li = lab_s[i];
-            int r = 0;
-            for (int j = 0; j < kk; j++) {
-                float dj = bd[j];
-                uint64_t lj = lab_s[j];
-                r += (dj < di || (dj == di && lj < li)) ? 1 : 0;
+        for (int i = lane; i < nout; i += 64) {
+            const float di = bd[i];
+            const uint64_t li = g.labels[bi[i] & ID_MASK];
+            int s0 = i;
+            while (s0 > 0 && bd[s0 - 1] == di) s0--;
+            int r = s0;
+            for (int j = s0; j < nout && bd[j] == di; j++) {
+                if (j == i) continue;
+                const uint64_t lj = g.labels[bi[j] & ID_MASK];
+                r += (lj < li || (lj == li && j < i)) ? 1 : 0;
             }
-            ol[r] = li;
-            od[r] = di;
+            if (r < row_len) {
+                ol[r] = li;
+                od[r] = di;
+            }
         }
-        for (int i = kk + lane; i < kq; i += 64) {
+        for (int i = (nout < row_len ? nout : row_len) + lane; i < row_len; i += 64) {
             ol[i] = ~0ull;
             od[i] = INFINITY;
         }
@@ -800,7 +828,7 @@ hipError_t launch_search(const SearchParams& p, int blocks, hipStream_t s) {
     uint32_t* w = p.work;
     hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, s, w, 1u);
     bool stop = p.use_stop != 0, trace = p.trace_feat != nullptr;
-    if (p.del_bits || p.allow_bits)
+    if (p.del_bits || p.allow_bits || p.eps_mode)
         return p.g.space == 0 ? launch_dim<0, false, false, true>(p, blocks, lds, s)
                               : launch_dim<1, false, false, true>(p, blocks, lds, s);
     if (p.g.space == 0) {

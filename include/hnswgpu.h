@@ -20,6 +20,8 @@
  *                         hnswalg.h:308-440
  *   hg_gpu_search_device  the same with device-resident queries/results, per-query ef and k
  *                         (extends the single ef_ of hnswalg.h:173-175)
+ *   hg_gpu_search_epsilon searchStopConditionClosest(q, EpsilonSearchStopCondition, filter)
+ *                         hnswalg.h:1326-1377, stop_condition.h:207-275 (range search)
  *   hg_bf_create / hg_bf_add   BruteforceSearch(space, maxElements) / addPoint
  *                         bruteforce.h:46-83; BFIndex.init_index/add_items bindings.cpp:785-819
  *   hg_bf_search          BFIndex.knn_query bindings.cpp:850-929 over
@@ -132,6 +134,24 @@ int hg_gpu_search_device_filtered(void* gpu_index, const float* d_queries, size_
  * status word (see above) and the number of queries that needed the second, largest-beam pass.
  * Either pointer may be NULL. */
 int hg_gpu_search_status(void* gpu_index, uint32_t* status, uint32_t* second_pass_queries);
+/* Range search: searchStopConditionClosest with EpsilonSearchStopCondition(epsilon,
+ * min_candidates, max_candidates) (hnswalg.h:1326-1377, stop_condition.h:207-275), optionally
+ * filtered (filter may be NULL; deleted elements are never returned).  epsilon bounds the
+ * distance as the space reports it (squared L2, or 1 - inner product).  counts[q] = number of
+ * results of query q (at most max_candidates); row q of labels/dists (stride out_stride <= 8192)
+ * holds the closest min(counts[q], out_stride) of them, ordered by (distance, label), padded
+ * with UINT64_MAX / +inf.  The reference returns internal ids in place of labels here; the two
+ * agree when labels are the insertion order.  Fails (HG_ERR_RUNTIME) when a query's beam needs
+ * more than 8192 entries. */
+int hg_gpu_search_epsilon(void* gpu_index, const float* queries, size_t nq, float epsilon, size_t min_candidates,
+                          size_t max_candidates, void* filter, size_t out_stride, uint64_t* labels, float* dists,
+                          uint32_t* counts);
+/* The same on device buffers, asynchronous on stream; d_counters and d_status as for
+ * hg_gpu_search_device_filtered (each may be NULL). */
+int hg_gpu_search_epsilon_device(void* gpu_index, const float* d_queries, size_t nq, float epsilon,
+                                 size_t min_candidates, size_t max_candidates, void* filter, size_t out_stride,
+                                 uint64_t* d_labels, float* d_dists, uint32_t* d_counts, int32_t* d_counters,
+                                 uint32_t* d_status, void* stream);
 /* Re-reads the delete marks (markDelete / unmarkDelete) of the CPU index this GPU index was
  * exported from, without a new export.  Fails if elements were added since the export. */
 int hg_gpu_refresh_deleted(void* gpu_index, void* index);

@@ -483,6 +483,166 @@ struct Searcher {
             *tie = std::adjacent_find(seen.begin(), seen.end()) != seen.end() ? 1 : 0;
         }
     }
+
+    // searchStopConditionClosest (hnswalg.h:1326-1377) with EpsilonSearchStopCondition
+    // (stop_condition.h:207-275) driving searchBaseLayerST<false> (hnswalg.h:323-437): the result
+    // set holds at most max_c allowed elements; the search stops at a candidate farther than the
+    // result set's worst once it holds max_c elements, or at a candidate farther than eps once it
+    // holds min_c; the results farther than eps are dropped.  `out` is closest first.
+    template <int MODE>
+    void run_eps(const float* q, float eps, size_t min_c, size_t max_c, std::vector<Res>& out, int64_t* ctr,
+                 uint8_t* tie, const uint8_t* allowed) {
+        out.clear();
+        int64_t H0 = 0, E0 = 0, Hu = 0, Eu = 0;
+        if (h.count == 0) return;
+        const size_t dim = h.dim;
+        uint32_t cur = h.entry;
+        float curdist = dist(space, dim, q, h.vec(cur));
+        Eu++;
+        for (int level = h.maxlevel; level > 0; level--) {  // hnswalg.h:1335-1360
+            bool changed = true;
+            while (changed) {
+                changed = false;
+                const uint32_t* data = h.ll(cur, level);
+                int size = Hnsw::count_of(data);
+                Hu++;
+                for (int i = 0; i < size; i++) {
+                    uint32_t cand = data[1 + i];
+                    if (cand > h.max_elements) throw std::runtime_error("cand error");
+                    float d = dist(space, dim, q, h.vec(cand));
+                    Eu++;
+                    if (d < curdist) {
+                        curdist = d;
+                        cur = cand;
+                        changed = true;
+                    }
+                }
+            }
+        }
+        next_tag();
+        std::vector<float> seen;
+        const bool track = tie != nullptr;
+        auto ok = [&](uint32_t id) { return !h.deleted(id) && (allowed == nullptr || allowed[id] != 0); };
+        const float fmax = std::numeric_limits<float>::max();
+        size_t curr = 0;  // curr_num_items_
+        std::vector<P> res;
+        if (MODE == OR_MODE_HEAP) {
+            std::priority_queue<P, std::vector<P>, ByFirst> top, cand;
+            float lowerBound;
+            if (ok(cur)) {
+                float dep = dist(space, dim, q, h.vec(cur));
+                E0++;
+                if (track) seen.push_back(dep);
+                lowerBound = dep;
+                top.emplace(dep, cur);
+                curr++;
+                cand.emplace(-dep, cur);
+            } else {
+                lowerBound = fmax;
+                cand.emplace(-lowerBound, cur);
+            }
+            visited[cur] = tag;
+            while (!cand.empty()) {
+                P cp = cand.top();
+                const float cd = -cp.first;
+                if ((cd > lowerBound && curr == max_c) || (cd > eps && curr >= min_c)) break;
+                cand.pop();
+                const uint32_t* data = h.ll0(cp.second);
+                size_t size = Hnsw::count_of(data);
+                H0++;
+                for (size_t j = 1; j <= size; j++) {
+                    uint32_t cid = data[j];
+                    if (visited[cid] == tag) continue;
+                    visited[cid] = tag;
+                    float d = dist(space, dim, q, h.vec(cid));
+                    E0++;
+                    if (track) seen.push_back(d);
+                    if (curr < max_c || lowerBound > d) {
+                        cand.emplace(-d, cid);
+                        if (ok(cid)) {
+                            top.emplace(d, cid);
+                            curr++;
+                        }
+                        while (curr > max_c) {
+                            top.pop();
+                            curr--;
+                        }
+                        if (!top.empty()) lowerBound = top.top().first;
+                    }
+                }
+            }
+            res.resize(top.size());
+            for (size_t i = res.size(); i-- > 0;) {
+                res[i] = top.top();
+                top.pop();
+            }
+        } else {
+            // (dist, id) keys; a disallowed entry point is popped first with candidate distance
+            // FLT_MAX, as the reference's -lowerBound entry
+            std::priority_queue<P, std::vector<P>, ByPair> top;
+            std::priority_queue<P, std::vector<P>, ByPairNeg> cand;
+            E0++;
+            if (track) seen.push_back(curdist);
+            const bool ep_ok = ok(cur);
+            if (ep_ok) {
+                top.emplace(curdist, cur);
+                curr++;
+            }
+            cand.emplace(-std::numeric_limits<float>::infinity(), cur);
+            visited[cur] = tag;
+            bool first = true;
+            while (!cand.empty()) {
+                P cp = cand.top();
+                const float cd = first ? (ep_ok ? curdist : fmax) : cp.first;
+                const P key = first ? P(curdist, cur) : cp;
+                first = false;
+                if ((curr == max_c && top.top() < key) || (cd > eps && curr >= min_c)) break;
+                cand.pop();
+                const uint32_t* data = h.ll0(cp.second);
+                size_t size = Hnsw::count_of(data);
+                H0++;
+                for (size_t j = 1; j <= size; j++) {
+                    uint32_t cid = data[j];
+                    if (visited[cid] == tag) continue;
+                    visited[cid] = tag;
+                    float d = dist(space, dim, q, h.vec(cid));
+                    E0++;
+                    if (track) seen.push_back(d);
+                    P np(d, cid);
+                    if (curr < max_c || np < top.top()) {
+                        cand.push(np);
+                        if (ok(cid)) {
+                            top.push(np);
+                            curr++;
+                        }
+                        while (curr > max_c) {
+                            top.pop();
+                            curr--;
+                        }
+                    }
+                }
+            }
+            res.resize(top.size());
+            for (size_t i = res.size(); i-- > 0;) {
+                res[i] = top.top();
+                top.pop();
+            }
+        }
+        // filter_results (stop_condition.h:262-269)
+        while (!res.empty() && res.back().first > eps) res.pop_back();
+        while (res.size() > max_c) res.pop_back();
+        for (auto& r : res) out.push_back({r.first, r.second});
+        if (ctr) {
+            ctr[0] = H0;
+            ctr[1] = E0;
+            ctr[2] = Hu;
+            ctr[3] = Eu;
+        }
+        if (tie) {
+            std::sort(seen.begin(), seen.end());
+            *tie = std::adjacent_find(seen.begin(), seen.end()) != seen.end() ? 1 : 0;
+        }
+    }
 };
 
 }  // namespace
@@ -592,6 +752,47 @@ int or_hnsw_search_allowed(void* hv, int space, const float* q, size_t nq, size_
     });
     for (auto* s : ss) delete s;
     return failed ? -1 : short_rows.load();
+}
+
+int or_hnsw_search_epsilon(void* hv, int space, const float* q, size_t nq, float eps, size_t min_c, size_t max_c,
+                           int mode, int nthreads, const uint8_t* allowed, size_t out_stride, uint64_t* labels,
+                           float* dists, uint32_t* counts, int64_t* counters, uint8_t* tie_flags) {
+    const Hnsw& h = *(const Hnsw*)hv;
+    if (nthreads <= 0) nthreads = (int)std::thread::hardware_concurrency();
+    std::vector<Searcher*> ss(std::max(nthreads, 1), nullptr);
+    std::mutex err_mu;
+    bool failed = false;
+    parallel_for(nq, nthreads, [&](size_t r, int t) {
+        if (!ss[t]) ss[t] = new Searcher(h, space);
+        std::vector<Res> out;
+        try {
+            if (mode == OR_MODE_HEAP)
+                ss[t]->run_eps<OR_MODE_HEAP>(q + r * h.dim, eps, min_c, max_c, out,
+                                             counters ? counters + 4 * r : nullptr,
+                                             tie_flags ? tie_flags + r : nullptr, allowed);
+            else
+                ss[t]->run_eps<OR_MODE_TOTAL>(q + r * h.dim, eps, min_c, max_c, out,
+                                              counters ? counters + 4 * r : nullptr,
+                                              tie_flags ? tie_flags + r : nullptr, allowed);
+        } catch (std::exception& e) {
+            std::lock_guard<std::mutex> g(err_mu);
+            g_err = e.what();
+            failed = true;
+            return;
+        }
+        // the reference returns the result vector in heap order (closest first); rows here are
+        // ordered by (distance, label) so that equal distances have one order
+        std::vector<std::pair<float, uint64_t>> lab;
+        for (auto& o : out) lab.emplace_back(o.d, h.label(o.id));
+        if (mode == OR_MODE_TOTAL) std::sort(lab.begin(), lab.end());
+        counts[r] = (uint32_t)lab.size();
+        for (size_t i = 0; i < out_stride; i++) {
+            labels[r * out_stride + i] = i < lab.size() ? lab[i].second : UINT64_MAX;
+            dists[r * out_stride + i] = i < lab.size() ? lab[i].first : std::numeric_limits<float>::infinity();
+        }
+    });
+    for (auto* s : ss) delete s;
+    return failed ? -1 : 0;
 }
 
 int or_bf_search(int space, size_t dim, const float* data, const uint64_t* labels_in, size_t n,

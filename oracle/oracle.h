@@ -9,6 +9,7 @@
 //   or_hnsw_load           hnswalg.h:715-821 (on-disk format; header field order 685-712)
 //   or_hnsw_search         hnswalg.h:1269-1323 (searchKnn) + 308-440 (searchBaseLayerST<true/false>)
 //                          + bindings.cpp:612-697 (row output, closest first)
+//   or_hnsw_search_epsilon hnswalg.h:1326-1377 (searchStopConditionClosest) + stop_condition.h:207-275
 //   or_bf_search           bruteforce.h:106-125 + bindings.cpp:850-929
 //   or_mlp_*               the learned-stop network's forward/backward/Adam in FP64/FP32 scalar
 //                          form (no reference counterpart: checks the GPU training kernels)
@@ -62,6 +63,13 @@ int or_hnsw_search_allowed(void* h, int space, const float* q, size_t nq, size_t
                            const int32_t* ef_per_query, const int32_t* k_per_query, size_t k_stride, int mode,
                            int nthreads, const uint8_t* allowed, uint64_t* labels, float* dists, int64_t* counters,
                            uint8_t* tie_flags);
+// searchStopConditionClosest with EpsilonSearchStopCondition (hnswalg.h:1326-1377,
+// stop_condition.h:207-275): per query, counts[q] results (distance <= eps, at most max_c),
+// closest first, in rows of out_stride (the first out_stride of them).  Mode HEAP keeps the
+// reference's result order; mode TOTAL orders by (distance, label).  Returns 0, or -1 on error.
+int or_hnsw_search_epsilon(void* h, int space, const float* q, size_t nq, float eps, size_t min_c, size_t max_c,
+                           int mode, int nthreads, const uint8_t* allowed, size_t out_stride, uint64_t* labels,
+                           float* dists, uint32_t* counts, int64_t* counters, uint8_t* tie_flags);
 // Internal id of each element's label and its delete mark, for building `allowed` masks.
 void or_hnsw_labels(void* h, uint64_t* labels_by_id, uint8_t* deleted_by_id);
 

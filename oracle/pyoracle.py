@@ -47,6 +47,8 @@ def lib():
         L.or_hnsw_search_allowed.argtypes = [_vp, _i32, _vp, _sz, _sz, _sz, _vp, _vp, _sz, _i32, _i32, _vp, _vp,
                                              _vp, _vp, _vp]
         L.or_hnsw_labels.argtypes = [_vp, _vp, _vp]
+        L.or_hnsw_search_epsilon.argtypes = [_vp, _i32, _vp, _sz, ctypes.c_float, _sz, _sz, _i32, _i32, _vp, _sz,
+                                             _vp, _vp, _vp, _vp, _vp]
         L.or_bf_search.argtypes = [_i32, _sz, _vp, _vp, _sz, _vp, _sz, _sz, _i32, _vp, _vp]
         L.or_fp64_pairs.argtypes = [_i32, _sz, _vp, _vp, _vp, _vp, _sz, _vp]
         L.or_mlp_forward.argtypes = [_vp, _i32, _i32, _vp, _sz, _vp]
@@ -112,6 +114,24 @@ class OracleHnsw:
         if r < 0:
             raise RuntimeError(lib().or_last_error().decode())
         return labels, dists, ctr, tf, r
+
+    def search_epsilon(self, space, queries, eps, min_c, max_c, out_stride, mode=MODE_HEAP, nthreads=0,
+                       allowed=None, counters=False, ties=False):
+        """searchStopConditionClosest + EpsilonSearchStopCondition: (labels, dists, counts,
+        counters, tie flags); row q holds its first min(counts[q], out_stride) results."""
+        q = np.ascontiguousarray(queries, np.float32)
+        nq = q.shape[0]
+        labels = np.empty((nq, out_stride), np.uint64)
+        dists = np.empty((nq, out_stride), np.float32)
+        counts = np.empty(nq, np.uint32)
+        ctr = np.zeros((nq, 4), np.int64) if counters else None
+        tf = np.zeros(nq, np.uint8) if ties else None
+        al = None if allowed is None else np.ascontiguousarray(allowed, np.uint8)
+        r = lib().or_hnsw_search_epsilon(self.h, space, _p(q), nq, eps, min_c, max_c, mode, nthreads, _p(al),
+                                         out_stride, _p(labels), _p(dists), _p(counts), _p(ctr), _p(tf))
+        if r < 0:
+            raise RuntimeError(lib().or_last_error().decode())
+        return labels, dists, counts, ctr, tf
 
     def __del__(self):
         try:
@@ -188,4 +208,10 @@ def ref_shim():
         return None
     L.ref_distance_batch.argtypes = [_i32, _sz, _vp, _vp, _sz, _vp]
     L.ref_random_levels.argtypes = [_sz, _sz, _sz, _vp]
+    L.ref_last_error.restype = ctypes.c_char_p
+    L.ref_hnsw_build.restype = _vp
+    L.ref_hnsw_build.argtypes = [_i32, _sz, _vp, _sz, _sz, _sz, _sz]
+    L.ref_hnsw_free.argtypes = [_vp]
+    L.ref_hnsw_mark_deleted.argtypes = [_vp, ctypes.c_uint64]
+    L.ref_hnsw_search_epsilon.argtypes = [_vp, _vp, _sz, ctypes.c_float, _sz, _sz, _vp, _sz, _sz, _vp, _vp, _vp]
     return L

@@ -162,6 +162,33 @@ REF_API int ref_hnsw_search_filtered(void* hv, const float* q, size_t nq, size_t
     return short_rows;
 }
 
+// searchStopConditionClosest with EpsilonSearchStopCondition (hnswalg.h:1326-1377,
+// stop_condition.h:207-275), optionally filtered.  Row r holds min(counts[r], out_stride) of its
+// results in the reference's order; the second member of each pair is what the reference puts
+// there (the internal id of the result).
+REF_API int ref_hnsw_search_epsilon(void* hv, const float* q, size_t nq, float eps, size_t min_c, size_t max_c,
+                                    const uint8_t* allowed, size_t n_labels, size_t out_stride, uint64_t* labels,
+                                    float* dists, uint32_t* counts) {
+    try {
+        RefHnsw* h = static_cast<RefHnsw*>(hv);
+        size_t dim = *(size_t*)h->space.sp->get_dist_func_param();
+        LabelMask f(allowed, n_labels);
+        for (size_t r = 0; r < nq; r++) {
+            hnswlib::EpsilonSearchStopCondition<float> sc(eps, min_c, max_c);
+            auto res = h->alg->searchStopConditionClosest(q + r * dim, sc, allowed ? &f : nullptr);
+            counts[r] = (uint32_t)res.size();
+            for (size_t i = 0; i < out_stride; i++) {
+                labels[r * out_stride + i] = i < res.size() ? (uint64_t)res[i].second : UINT64_MAX;
+                dists[r * out_stride + i] = i < res.size() ? res[i].first : __builtin_inff();
+            }
+        }
+        return 0;
+    } catch (std::exception& e) {
+        g_err = e.what();
+        return -1;
+    }
+}
+
 // BFIndex.knn_query semantics (bindings.cpp:850-929 over bruteforce.h:106-125).
 REF_API int ref_bf_search(int space, size_t dim, const float* data, size_t n, const float* q, size_t nq,
                           size_t k, uint64_t* labels, float* dists) {

@@ -5,6 +5,7 @@ the hnswlib headers with setup.py's flags).  Run in a container where /root/refe
     make -C oracle ref && python tests/golden/gen_golden.py        (all fixtures)
                           python tests/golden/gen_golden.py nbb    (the deletion/filter fixture only)
                           python tests/golden/gen_golden.py bf_ops (the brute-force operations fixture)
+                          python tests/golden/gen_golden.py eps    (the epsilon-search fixture)
 
 Every fixture stores its inputs' SHA-256 (or the inputs themselves when small), the reference's
 outputs and the SHA-256 of the reference's saveIndex bytes for single-threaded builds.
@@ -174,7 +175,52 @@ def bf_ops():
     print("c1_bf_ops_golden.npz", out["file_used_sha"][:16], "count", int(hdr[2]))
 
 
+def eps():
+    """F7: searchStopConditionClosest with EpsilonSearchStopCondition on C1 (the reference's
+    epsilon_search_test.cpp setting: squared radius 1.0, min 2000, max = all elements), a
+    radius 2.0 search capped at 100 results, a filtered one (label % 3 != 0, 10% of labels
+    deleted) and self-queries at radius 1e-4.  Results are stored flat with per-query counts."""
+    shim = pyoracle.ref_shim()
+    assert shim is not None, "build the reference first: make -C oracle ref"
+    data = datasets.c1_uniform()
+    queries = datasets.c1_uniform(n=1000, seed=1)[:100]
+    n = data.shape[0]
+    h = shim.ref_hnsw_build(0, 16, data.ctypes.data, n, 16, 200, 100)
+    out = dict(data_sha=sha(data), query_sha=sha(queries))
+
+    def run(name, q, e, mn, mx, allowed=None):
+        q = np.ascontiguousarray(q, np.float32)
+        stride = mx
+        lab = np.empty((q.shape[0], stride), np.uint64)
+        dis = np.empty((q.shape[0], stride), np.float32)
+        cnt = np.empty(q.shape[0], np.uint32)
+        al = None if allowed is None else np.ascontiguousarray(allowed, np.uint8)
+        r = shim.ref_hnsw_search_epsilon(h, q.ctypes.data, q.shape[0], e, mn, mx,
+                                         None if al is None else al.ctypes.data, n, stride, lab.ctypes.data,
+                                         dis.ctypes.data, cnt.ctypes.data)
+        assert r == 0, shim.ref_last_error()
+        out[f"counts_{name}"] = cnt
+        out[f"ids_{name}"] = np.concatenate([lab[i, :cnt[i]] for i in range(q.shape[0])])
+        out[f"dists_{name}"] = np.concatenate([dis[i, :cnt[i]] for i in range(q.shape[0])])
+        out[f"params_{name}"] = np.array([e, mn, mx], np.float64)
+        print(name, "results per query", float(cnt.mean()), "max", int(cnt.max()))
+
+    run("eps1", queries, 1.0, 2000, n)
+    run("eps2_max100", queries, 2.0, 50, 100)
+    run("self", data[:300], 1e-4, 500, n)
+    for lab in range(0, n, 10):
+        assert shim.ref_hnsw_mark_deleted(h, lab) == 0
+    allowed = (np.arange(n) % 3 != 0).astype(np.uint8)  # labels = row numbers
+    run("eps1_del_f3", queries, 1.0, 300, 2000, allowed)
+    out["deleted"] = np.arange(0, n, 10, dtype=np.uint64)
+    shim.ref_hnsw_free(h)
+    np.savez_compressed(os.path.join(OUT, "c1_eps_golden.npz"), **out)
+
+
 if __name__ == "__main__":
+    if len(sys.argv) > 1 and sys.argv[1] == "eps":
+        eps()
+        sys.exit(0)
     if len(sys.argv) > 1 and sys.argv[1] == "nbb":
         nbb()
     elif len(sys.argv) > 1 and sys.argv[1] == "bf_ops":
@@ -183,3 +229,4 @@ if __name__ == "__main__":
         main()
         nbb()
         bf_ops()
+        eps()
