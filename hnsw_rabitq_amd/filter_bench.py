@@ -2,7 +2,7 @@
 the elements (label % 10 == 0), and 10% of the elements marked deleted (label % 10 == 1).
 Ground truth is the exact FP32 top-10 over the returnable rows (GPU brute force over that subset).
 Reports recall@10, kernel time and queries/s per ef; the operating point is the smallest ef with
-recall@10 >= 0.95."""
+recall@10 >= 0.95.  Then the range search (EpsilonSearchStopCondition) at three min_candidates."""
 import numpy as np
 
 FILTER_EFS = (10, 20, 40, 80, 160)
@@ -86,4 +86,54 @@ def run(H, lib, idx, gpu, data, dq, nq, stream, timer, log, k=10, steps=3):
     out["deleted10_sweep"] = pts
     out["deleted10_qps_at_095"] = op["qps"] if op else None
     out["deleted10_ef"] = op["ef"] if op else None
+    out.update(range_search(H, lib, gpu, data, dq, nq, stream, timer, log))
+    return out
+
+
+def range_search(H, lib, gpu, data, dq, nq, stream, timer, log, kmax=256, steps=3):
+    """Range search (EpsilonSearchStopCondition) with the radius at the median 10th-neighbour
+    distance: recall of the elements within the radius (exact: the FP32 brute-force top-256,
+    which holds every in-range element while fewer than 256 are in range)."""
+    from . import DeviceArray
+    n = data.shape[0]
+    bf = H.BFIndex("l2", data.shape[1])
+    bf.init_index(n)
+    bf.add_items(data, np.arange(n, dtype=np.uint64))
+    gl, gd = DeviceArray(nq * kmax * 8), DeviceArray(nq * kmax * 4)
+    lib.call("hg_bf_search_device", bf._h, dq.ptr, nq, kmax, lib.PREC_FP32, 0, gl.ptr, gd.ptr, None, stream)
+    del bf
+    gt_l = gl.to_numpy(np.uint64, (nq, kmax))
+    gt_d = gd.to_numpy(np.float32, (nq, kmax))
+    eps = float(np.median(gt_d[:, 9]))
+    in_range = gt_d <= eps
+    exact_ok = in_range[:, -1] == 0  # fewer than kmax in range: the exact set is known
+    stride = kmax
+    dl, dd = DeviceArray(nq * stride * 8), DeviceArray(nq * stride * 4)
+    dcnt, dst = DeviceArray(nq * 4), DeviceArray(4)
+    out = {"range_eps": eps, "range_sweep": []}
+    for min_c in (100, 200, 400):
+        def launch():
+            lib.call("hg_gpu_search_epsilon_device", gpu.handle, dq.ptr, nq, eps, min_c, 4 * min_c, None, stride,
+                     dl.ptr, dd.ptr, dcnt.ptr, None, dst.ptr, stream)
+        launch()
+        ms = []
+        for _ in range(steps):
+            timer.start()
+            launch()
+            ms.append(timer.stop())
+        cnt = dcnt.to_numpy(np.uint32, (nq,))
+        lab = dl.to_numpy(np.uint64, (nq, stride))
+        hit = tot = 0
+        for i in np.nonzero(exact_ok)[0]:
+            truth = gt_l[i][in_range[i]]
+            hit += len(np.intersect1d(lab[i, :min(int(cnt[i]), stride)], truth))
+            tot += len(truth)
+        rec = hit / max(tot, 1)
+        t = float(np.mean(ms))
+        status = int(dst.to_numpy(np.uint32, (1,))[0])
+        log(f"  range eps={eps:.0f} min={min_c:5d} max={4 * min_c}: recall {rec:.4f} "
+            f"({tot} in range over {int(exact_ok.sum())} queries) {t:8.2f} ms QPS={nq / (t * 1e-3):10.0f} "
+            f"results/query {cnt.mean():.1f} status={status}")
+        out["range_sweep"].append(dict(min_candidates=min_c, max_candidates=4 * min_c, recall=rec, ms=t,
+                                       qps=nq / (t * 1e-3), results_per_query=float(cnt.mean())))
     return out
