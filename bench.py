@@ -82,6 +82,7 @@ def main():
     ap.add_argument("--no-bf", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-filter", action="store_true", help="skip the filtered / deleted-element search")
+    ap.add_argument("--no-gpu-build", action="store_true", help="skip the GPU construction measurement")
     ap.add_argument("--bf-n", type=int, default=1_000_000)
     ap.add_argument("--bf-nq", type=int, default=10_000)
     ap.add_argument("--cpu-queries", type=int, default=2000)
@@ -115,6 +116,7 @@ def main():
     queries = datasets.c2_gamma(n=args.nq, seed=1)
 
     idx = H.Index("l2", D)
+    cpu_build_s = None
     if args.index_cache and os.path.exists(args.index_cache):
         log(f"loading index {args.index_cache}")
         idx.load_index(args.index_cache)
@@ -122,10 +124,12 @@ def main():
         idx.init_index(args.n, M=16, ef_construction=200, random_seed=100)
         nt = args.build_threads or os.cpu_count() or 1
         log(f"building HNSW (M=16, efC=200) with {nt} threads")
+        t_build = time.perf_counter()
         step = 100_000
         for s in range(0, args.n, step):
             idx.add_items(data[s:s + step], np.arange(s, min(args.n, s + step), dtype=np.uint64), num_threads=nt)
             log(f"  inserted {min(args.n, s + step)}")
+        cpu_build_s = time.perf_counter() - t_build
         if args.index_cache:
             idx.save_index(args.index_cache)
     info = idx._info()
@@ -260,6 +264,12 @@ def main():
     if not args.no_filter:
         from hnsw_rabitq_amd import filter_bench
         result.update(filter_bench.run(H, _lib, idx, gpu, data, dq, nq, stream, timer, log, k=K))
+
+    # construction: level 0 on the GPU beside the CPU build
+    if not args.no_gpu_build:
+        from hnsw_rabitq_amd import build_bench
+        result.update(build_bench.run(H, idx, data, queries, gt, cpu_build_s, log,
+                                      num_threads=args.build_threads))
 
     # brute force MFMA TFLOPS (1M x 768 IP GMM)
     if not args.no_bf:

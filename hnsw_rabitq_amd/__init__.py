@@ -282,6 +282,23 @@ class Index:
         _lib.call("hg_hnsw_add_items", self._h, _ptr(arr), arr.shape[0], _ptr(lab), int(nt))
         self._dirty = True
 
+    def add_items_gpu(self, data, ids=None, num_threads=-1):
+        """add_items with the level-0 graph built on the GPU (levels above 0 on the CPU)."""
+        if self._h is None:
+            raise RuntimeError("Index not initialized")
+        arr = _as_rows(data, self.dim)
+        if self.normalize:
+            arr = normalize_rows(arr)
+        lab = None
+        if ids is not None:
+            lab = np.ascontiguousarray(np.asarray(ids, dtype=np.uint64).reshape(-1))
+            if lab.shape[0] != arr.shape[0]:
+                raise RuntimeError("The input label shape %d does not match the input data vector shape %d"
+                                   % (lab.ndim, arr.shape[0]))
+        nt = self.num_threads if num_threads <= 0 else num_threads
+        _lib.call("hg_hnsw_build_gpu", self._h, _ptr(arr), arr.shape[0], _ptr(lab), int(nt), self.device)
+        self._dirty = True
+
     # -- parameters --
     def set_ef(self, ef):
         self.default_ef = int(ef)

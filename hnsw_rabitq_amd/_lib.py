@@ -41,6 +41,8 @@ SIGNATURES = {
     "hg_gpu_autotune": [_vp, _vp, _sz, _sz, _sz, _vp],
     "hg_gpu_set_plan": [_vp, _sz, _sz, ctypes.c_uint32, _i32],
     "hg_gpu_id_labels": [_vp, _vp, _sz, _vp, _vp],
+    "hg_hnsw_build_gpu": [_vp, _vp, _sz, _vp, _i32, _i32],
+    "hg_hnsw_add_items_upper": [_vp, _vp, _sz, _vp, _i32],
     "hg_bf_delete": [_vp, ctypes.c_uint64],
     "hg_bf_labels": [_vp, _vp, _sz],
     "hg_bf_save": [_vp, ctypes.c_char_p],

@@ -311,6 +311,17 @@ HG_API int hg_hnsw_add_items(void* h, const float* data, size_t rows, const uint
     });
 }
 
+// The first step of hg_hnsw_build_gpu on its own: insertion into the levels above 0 only.
+HG_API int hg_hnsw_add_items_upper(void* h, const float* data, size_t rows, const uint64_t* labels,
+                                   int num_threads) {
+    return guarded([&] {
+        require(h != nullptr, "null index");
+        require(rows == 0 || data != nullptr, "null data");
+        static_cast<hnswgpu::HnswIndex*>(h)->add_items_upper(data, rows, labels, num_threads);
+        return HG_OK;
+    });
+}
+
 HG_API int hg_hnsw_save(void* h, const char* path) {
     return guarded([&] {
         require(h && path, "null argument");
@@ -926,6 +937,142 @@ HG_API int hg_gpu_id_labels(void* gv, const uint32_t* d_ids, size_t n, uint64_t*
         auto* gi = static_cast<GpuIndex*>(gv);
         DeviceGuard dg(gi->device);
         hip_check(hg::launch_id_labels(gi->g, d_ids, n, d_labels, (hipStream_t)stream), "label gather launch");
+        return HG_OK;
+    });
+}
+
+// ---- level-0 construction on the GPU ----
+// addPoint (hnswalg.h:1152-1266) for a batch of rows: the CPU inserts them into the levels
+// above 0 (those do not depend on level 0), then the GPU builds their level-0 links in batches
+// that grow with the graph (build.hip), and the lists are written back into the CPU index.
+HG_API int hg_hnsw_build_gpu(void* hv, const float* data, size_t rows, const uint64_t* labels, int num_threads,
+                             int device) {
+    return guarded([&] {
+        require(hv != nullptr, "null index");
+        require(rows == 0 || data != nullptr, "null data");
+        auto* h = static_cast<hnswgpu::HnswIndex*>(hv);
+        require(h->num_deleted() == 0, "the GPU build needs an index without deleted elements");
+        require(h->ef_construction() >= 1 && h->ef_construction() <= 1024,
+                "the GPU build supports ef_construction up to 1024");
+        require(h->maxM0() <= 64, "M above 32 is not supported by the GPU search");
+        const size_t s0 = h->count();
+        if (rows == 0) return HG_OK;
+        h->add_items_upper(data, rows, labels, num_threads);
+        const size_t n = h->count();
+        void* gv = nullptr;
+        int rc = hg_gpu_index_create(hv, device, &gv);
+        if (rc != HG_OK) return rc;
+        std::unique_ptr<GpuIndex, int (*)(void*)> gi(static_cast<GpuIndex*>(gv), hg_gpu_index_free);
+        DeviceGuard dg(gi->device);
+        hipStream_t s = gi->stream;
+        const uint32_t efc = (uint32_t)h->ef_construction();
+        const uint32_t M = (uint32_t)h->M();
+        const uint32_t maxM0 = gi->g.maxM0;
+        const size_t bmax = 16384;
+        DevBuf ident, cand_ids, cand_d, sel, targets, offsets, incoming;
+        {
+            std::vector<uint64_t> id(n);
+            for (size_t i = 0; i < n; i++) id[i] = i;
+            ident.reserve(n * 8);
+            hip_check(hipMemcpy(ident.p, id.data(), n * 8, hipMemcpyHostToDevice), "hipMemcpy H2D");
+        }
+        cand_ids.reserve(bmax * efc * 8);
+        cand_d.reserve(bmax * efc * 4);
+        sel.reserve(bmax * M * 4);
+        targets.reserve(bmax * M * 4);
+        offsets.reserve((bmax * M + 1) * 4);
+        incoming.reserve(bmax * M * 4);
+        std::vector<int> level(n);
+        for (size_t i = 0; i < n; i++) level[i] = h->element_level((uint32_t)i);
+        // entry point of the graph of elements [0, s): the first element of the highest level
+        uint32_t entry = 0;
+        int maxl = level[0];
+        size_t seen = 1;
+        std::vector<uint32_t> selh;
+        std::vector<std::pair<uint32_t, uint32_t>> pairs;
+        std::vector<uint32_t> th, oh, ih;
+        GpuIndex::LaunchWords* lw = next_words(gi.get());
+        prepare_words(lw, bmax, false);
+        for (size_t sb = std::max<size_t>(s0, 1); sb < n;) {
+            const size_t B = std::min(std::min(bmax, std::max<size_t>(1, sb / 16)), n - sb);
+            for (; seen < sb; seen++)
+                if (level[seen] > maxl) {
+                    maxl = level[seen];
+                    entry = (uint32_t)seen;
+                }
+            // beam search of the batch over the graph of [0, sb)
+            Plan pl = make_plan(gi.get(), efc, efc, B);
+            hg::SearchParams sp = base_params(gi.get(), pl, lw);
+            sp.g.entry = entry;
+            sp.g.maxlevel = maxl;
+            sp.g.labels = (const uint64_t*)ident.p;
+            sp.queries = gi->g.vec + sb * gi->g.stride;
+            sp.q_stride = gi->g.stride;
+            sp.nq = (uint32_t)B;
+            sp.ef = efc;
+            sp.k = efc;
+            sp.k_stride = efc;
+            sp.out_labels = (uint64_t*)cand_ids.p;
+            sp.out_dists = (float*)cand_d.p;
+            sp.id_limit = (uint32_t)sb;
+            hip_check(hg::launch_search(sp, pl.blocks, s), "build search launch");
+            hg::BuildArgs ba{};
+            ba.g = gi->g;
+            ba.links0 = (uint32_t*)gi->links0.p;
+            ba.s = (uint32_t)sb;
+            ba.B = (uint32_t)B;
+            ba.efc = efc;
+            ba.M = M;
+            ba.cand_ids = (const uint64_t*)cand_ids.p;
+            ba.cand_d = (const float*)cand_d.p;
+            ba.sel = (uint32_t*)sel.p;
+            hip_check(hg::launch_build(ba, false, s), "build select launch");
+            // reverse links, grouped by the element that receives them
+            selh.resize(B * M);
+            hip_check(hipMemcpyAsync(selh.data(), sel.p, B * M * 4, hipMemcpyDeviceToHost, s), "hipMemcpy D2H");
+            hip_check(hipStreamSynchronize(s), "build");
+            pairs.clear();
+            for (size_t i = 0; i < B; i++)
+                for (size_t j = 0; j < M; j++) {
+                    const uint32_t t = selh[i * M + j];
+                    if (t == hg::SENT) break;
+                    require(t < sb + i, "cand error");
+                    pairs.emplace_back(t, (uint32_t)(sb + i));
+                }
+            std::sort(pairs.begin(), pairs.end());
+            th.clear();
+            oh.clear();
+            ih.clear();
+            for (size_t i = 0; i < pairs.size(); i++) {
+                if (i == 0 || pairs[i].first != pairs[i - 1].first) {
+                    th.push_back(pairs[i].first);
+                    oh.push_back((uint32_t)i);
+                }
+                ih.push_back(pairs[i].second);
+            }
+            oh.push_back((uint32_t)pairs.size());
+            if (!th.empty()) {
+                hip_check(hipMemcpyAsync(targets.p, th.data(), th.size() * 4, hipMemcpyHostToDevice, s), "H2D");
+                hip_check(hipMemcpyAsync(offsets.p, oh.data(), oh.size() * 4, hipMemcpyHostToDevice, s), "H2D");
+                hip_check(hipMemcpyAsync(incoming.p, ih.data(), ih.size() * 4, hipMemcpyHostToDevice, s), "H2D");
+                ba.targets = (const uint32_t*)targets.p;
+                ba.offsets = (const uint32_t*)offsets.p;
+                ba.incoming = (const uint32_t*)incoming.p;
+                ba.T = (uint32_t)th.size();
+                hip_check(hg::launch_build(ba, true, s), "build link launch");
+            }
+            sb += B;
+        }
+        // level-0 lists back into the CPU index
+        std::vector<uint32_t> l0(n * maxM0);
+        hip_check(hipMemcpyAsync(l0.data(), gi->links0.p, l0.size() * 4, hipMemcpyDeviceToHost, s), "hipMemcpy D2H");
+        hip_check(hipStreamSynchronize(s), "build");
+        for (size_t i = 0; i < n; i++) {
+            const uint32_t* r = &l0[i * maxM0];
+            size_t c = 0;
+            while (c < maxM0 && r[c] != hg::SENT) c++;
+            h->set_links0((uint32_t)i, r, c);
+        }
         return HG_OK;
     });
 }

@@ -82,6 +82,10 @@ struct SearchParams {
     float eps;
     int eps_min;
     uint32_t* counts;
+    // graph construction: the upper-level descent ignores ids >= id_limit (elements not yet in
+    // the level-0 graph; 0 = no limit); queries are rows of q_stride floats (0 = dim)
+    uint32_t id_limit;
+    uint32_t q_stride;
 };
 
 constexpr int kStopFeatures = 12;

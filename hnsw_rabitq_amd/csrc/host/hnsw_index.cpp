@@ -389,7 +389,7 @@ tableint HnswIndex::insert(const float* point, labeltype lab) {
         memset(link_lists_[cur_c], 0, size_links_per_element_ * curlevel + 1);
     }
 
-    if ((int32_t)curr_obj != -1) {
+    if ((int32_t)curr_obj != -1 && !(skip_level0_ && curlevel == 0)) {
         if (curlevel < maxlevelcopy) {
             float curdist = dist(point, vector(curr_obj));
             for (int level = maxlevelcopy; level > curlevel; level--) {
@@ -415,7 +415,7 @@ tableint HnswIndex::insert(const float* point, labeltype lab) {
         }
         Visited& vis = visited_for_thread();
         try {
-            for (int level = std::min(curlevel, maxlevelcopy); level >= 0; level--) {
+            for (int level = std::min(curlevel, maxlevelcopy); level >= (skip_level0_ ? 1 : 0); level--) {
                 MaxHeap top = search_layer(curr_obj, point, level, vis);
                 curr_obj = connect_new_element(point, cur_c, top, level);
             }
@@ -426,7 +426,7 @@ tableint HnswIndex::insert(const float* point, labeltype lab) {
         }
         std::lock_guard<std::mutex> g(visited_lock_);
         visited_pool_.push_back(&vis);
-    } else {
+    } else if ((int32_t)curr_obj == -1) {
         enterpoint_node_ = 0;
         maxlevel_ = curlevel;
     }
@@ -451,6 +451,24 @@ void HnswIndex::add_items(const float* data, size_t rows, const labeltype* label
         insert(data + row * dim_, labels ? labels[row] : base + row);
     });
     next_label_ += rows;
+}
+
+void HnswIndex::add_items_upper(const float* data, size_t rows, const labeltype* labels, int num_threads) {
+    skip_level0_ = true;
+    try {
+        add_items(data, rows, labels, num_threads);
+    } catch (...) {
+        skip_level0_ = false;
+        throw;
+    }
+    skip_level0_ = false;
+}
+
+void HnswIndex::set_links0(tableint i, const uint32_t* ids, size_t count) {
+    if (i >= cur_element_count_ || count > maxM0_) throw std::runtime_error("Bad value of sz_link_list_other");
+    uint32_t* ll = ll0_mut(i);
+    set_list_count(ll, (unsigned short)count);
+    for (size_t j = 0; j < maxM0_; j++) ll[1 + j] = j < count ? ids[j] : 0u;
 }
 
 size_t HnswIndex::serialized_size() const {

@@ -59,6 +59,12 @@ class HnswIndex {
     // hardware threads; the first point of an empty index is inserted alone first, and batches
     // of at most 4 * num_threads rows run single-threaded (bindings.cpp:251-304).
     void add_items(const float* data, size_t rows, const labeltype* labels, int num_threads);
+    // The same, linking only the levels above 0 (the level-0 lists of the new elements stay
+    // empty for a level-0 builder to fill).  The upper levels do not depend on level 0, so they
+    // are those add_items builds.
+    void add_items_upper(const float* data, size_t rows, const labeltype* labels, int num_threads);
+    // Replaces the level-0 link list of element i (count entries; the delete mark is kept).
+    void set_links0(tableint i, const uint32_t* ids, size_t count);
     void save(const std::string& path) const;
     size_t serialized_size() const;
     void serialize(char* out) const;
@@ -157,6 +163,7 @@ class HnswIndex {
     std::vector<Visited*> visited_pool_;
     labeltype next_label_ = 0;
     bool ep_added_ = false;
+    bool skip_level0_ = false;
 };
 
 }  // namespace hnswgpu

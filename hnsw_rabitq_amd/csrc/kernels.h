@@ -16,6 +16,26 @@ hipError_t launch_zero_u32(uint32_t* p, uint32_t n, hipStream_t s);  // a kernel
 hipError_t launch_copy(void* dst, const void* src, size_t bytes, hipStream_t s);
 hipError_t launch_id_labels(const DevGraph& g, const uint32_t* ids, size_t n, uint64_t* out, hipStream_t s);
 
+// ---- level-0 graph construction (build.hip) ----
+constexpr int kBuildUnionCap = 1024;  // list + arrivals re-selected at once by the link kernel
+struct BuildArgs {
+    DevGraph g;                 // vec, stride, dim, maxM0, space
+    uint32_t* links0;           // [n][maxM0], written
+    uint32_t s, B;              // batch [s, s + B)
+    uint32_t efc, M;
+    const uint64_t* cand_ids;   // [B][efc] graph-search candidates (element ids, ~0 = none)
+    const float* cand_d;        // [B][efc]
+    uint32_t* sel;              // [B][M] selected neighbours (SENT padded)
+    const uint32_t* targets;    // [T] elements that receive links
+    const uint32_t* offsets;    // [T + 1] into incoming
+    const uint32_t* incoming;   // new elements, grouped by target, ascending
+    uint32_t T;
+};
+size_t build_select_lds(uint32_t dim, uint32_t efc);
+size_t build_link_lds(uint32_t dim);
+// link = false: build_select_kernel over the batch; true: build_link_kernel over the targets
+hipError_t launch_build(const BuildArgs& a, bool link, hipStream_t s);
+
 // ---- brute force (bruteforce.hip) ----
 struct BfData {
     const float* vec;        // [n][dim] f32

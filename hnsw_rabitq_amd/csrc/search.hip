@@ -26,53 +26,9 @@
 #include "common.h"
 #include "dist.h"
 #include "kernels.h"
+#include "wave.h"
 
 namespace hg {
-
-// (da, ia) < (db, ib) in (distance, id) order; evaluated without branches.
-__device__ __forceinline__ bool key_less(float da, uint32_t ia, float db, uint32_t ib) {
-    return (int)(da < db) | ((int)(da == db) & (int)(ia < ib));
-}
-
-// Distances of the c vectors whose ids are ids[0..c) to the query in LDS; results to out[].
-// MODE 0 = generic (one lane per vector), otherwise dim % 16 == 0 with NCH chunks (0 = runtime).
-template <int SPACE, int MODE, int NCH>
-__device__ __forceinline__ void dist_batch(const DevGraph& g, const float* qs, const uint32_t* ids, int c,
-                                           float* out, int lane) {
-    if (MODE == 0) {
-        for (int base = 0; base < c; base += 64) {
-            int idx = base + lane;
-            if (idx < c) {
-                const float* x = g.vec + (size_t)ids[idx] * g.stride;
-                out[idx] = dist_generic(SPACE, qs, x, (int)g.dim);
-            }
-        }
-        return;
-    }
-    const int grp = lane >> 2, sub = lane & 3;
-    const int nch = (int)(g.dim >> 4);
-    for (int base = 0; base < c; base += 32) {
-        // two vectors per group per round: slots grp and grp + 16
-        int i0 = base + grp, i1 = base + grp + 16;
-        bool a0 = i0 < c, a1 = i1 < c;
-        uint32_t id0 = a0 ? ids[i0] : 0u, id1 = a1 ? ids[i1] : 0u;
-        float r0, r1;
-        dist_group16<SPACE, NCH>(g.vec + (size_t)id0 * g.stride, qs, nch, a0, sub, r0, lane);
-        dist_group16<SPACE, NCH>(g.vec + (size_t)id1 * g.stride, qs, nch, a1, sub, r1, lane);
-        if (sub == 0) {
-            if (a0) out[i0] = r0;
-            if (a1) out[i1] = r1;
-        }
-    }
-}
-
-// ------------------------------------------------------------------------------------------
-// Wave helpers
-// ------------------------------------------------------------------------------------------
-__device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
-__device__ __forceinline__ int lanes_below(uint64_t m, int lane) {
-    return __popcll(m & ((lane == 0) ? 0ull : (~0ull >> (64 - lane))));
-}
 
 // Visited hash: returns true when id was not present (and inserts it).
 __device__ __forceinline__ bool hash_insert(uint32_t* hash, uint32_t mask, uint32_t id) {
@@ -295,7 +251,7 @@ __global__ __launch_bounds__(64) void hnsw_search_kernel(SearchParams P) {
         const int ef_eff = (FILT || ef_want < (int)cap) ? ef_want : (int)cap;
 
         // query to LDS, hash cleared
-        const float* qg = P.queries + (size_t)qi * g.dim;
+        const float* qg = P.queries + (size_t)qi * (P.q_stride ? P.q_stride : g.dim);
         for (uint32_t i = lane; i < dimpad; i += 64) qs[i] = i < g.dim ? qg[i] : 0.f;
         for (uint32_t i = lane; i < hsize; i += 64) hash[i] = SENT;
         __syncthreads();
@@ -319,6 +275,7 @@ __global__ __launch_bounds__(64) void hnsw_search_kernel(SearchParams P) {
                 for (uint32_t b = 0; b < g.maxM; b += 64) {
                     uint32_t j = b + lane;
                     uint32_t id = j < g.maxM ? row[j] : SENT;
+                    if (P.id_limit && id >= P.id_limit) id = SENT;
                     uint64_t m = ballot(id != SENT);
                     if (id != SENT) si[cnt + lanes_below(m, lane)] = id;
                     cnt += __popcll(m);

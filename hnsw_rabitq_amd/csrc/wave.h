@@ -1,0 +1,56 @@
+// Wave-level helpers shared by the search and build kernels (gfx950, 64-lane wavefronts).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "common.h"
+#include "dist.h"
+
+namespace hg {
+
+// (da, ia) < (db, ib) in (distance, id) order; evaluated without branches.
+__device__ __forceinline__ bool key_less(float da, uint32_t ia, float db, uint32_t ib) {
+    return (int)(da < db) | ((int)(da == db) & (int)(ia < ib));
+}
+
+// Distances of the c vectors whose ids are ids[0..c) to the query in LDS; results to out[].
+// MODE 0 = generic (one lane per vector), otherwise dim % 16 == 0 with NCH chunks (0 = runtime).
+template <int SPACE, int MODE, int NCH>
+__device__ __forceinline__ void dist_batch(const DevGraph& g, const float* qs, const uint32_t* ids, int c,
+                                           float* out, int lane) {
+    if (MODE == 0) {
+        for (int base = 0; base < c; base += 64) {
+            int idx = base + lane;
+            if (idx < c) {
+                const float* x = g.vec + (size_t)ids[idx] * g.stride;
+                out[idx] = dist_generic(SPACE, qs, x, (int)g.dim);
+            }
+        }
+        return;
+    }
+    const int grp = lane >> 2, sub = lane & 3;
+    const int nch = (int)(g.dim >> 4);
+    for (int base = 0; base < c; base += 32) {
+        // two vectors per group per round: slots grp and grp + 16
+        int i0 = base + grp, i1 = base + grp + 16;
+        bool a0 = i0 < c, a1 = i1 < c;
+        uint32_t id0 = a0 ? ids[i0] : 0u, id1 = a1 ? ids[i1] : 0u;
+        float r0, r1;
+        dist_group16<SPACE, NCH>(g.vec + (size_t)id0 * g.stride, qs, nch, a0, sub, r0, lane);
+        dist_group16<SPACE, NCH>(g.vec + (size_t)id1 * g.stride, qs, nch, a1, sub, r1, lane);
+        if (sub == 0) {
+            if (a0) out[i0] = r0;
+            if (a1) out[i1] = r1;
+        }
+    }
+}
+
+// ------------------------------------------------------------------------------------------
+// Wave helpers
+// ------------------------------------------------------------------------------------------
+__device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
+__device__ __forceinline__ int lanes_below(uint64_t m, int lane) {
+    return __popcll(m & ((lane == 0) ? 0ull : (~0ull >> (64 - lane))));
+}
+
+}  // namespace hg

@@ -9,6 +9,9 @@
  *                         hnswalg.h:89-144; Index.init_index bindings.cpp:192-207
  *   hg_hnsw_add_items     Index.add_items -> ParallelFor -> addPoint  bindings.cpp:251-304,
  *                         hnswalg.h:1152-1266 (CPU: the index stays "built by addPoint")
+ *   hg_hnsw_build_gpu     Index.add_items -> addPoint hnswalg.h:1152-1266 with the level-0
+ *                         links built on the GPU in batches (a graph of the same kind, not
+ *                         the same graph; the levels above 0 are the CPU's)
  *   hg_hnsw_save / _load  saveIndex / loadIndex  hnswalg.h:685-821
  *   hg_hnsw_serialize     the same byte stream as saveIndex, into memory
  *   hg_gpu_index_create   (new) export of data_level0_memory_ / linkLists_ into the GPU layout
@@ -80,6 +83,18 @@ int hg_hnsw_mark_deleted(void* index, uint64_t label);
 int hg_hnsw_unmark_deleted(void* index, uint64_t label);
 int hg_hnsw_get_ef(void* index, size_t* ef);
 int hg_hnsw_free(void* index);
+/* add_items with the level-0 graph built on the GPU (device = HIP device): the rows are
+ * inserted into the levels above 0 on the CPU (num_threads as hg_hnsw_add_items; the upper
+ * levels do not depend on level 0), then their level-0 links are built in batches of up to
+ * 16,384 rows that grow with the graph: a beam search with ef_construction over the elements
+ * before the batch, the earlier rows of the same batch added to the candidates by brute force,
+ * the neighbour heuristic, and mutual linking with re-selection of full lists
+ * (hnswalg.h:225-305, 443-483, 506-630).  The index must hold no deleted elements;
+ * ef_construction <= 1024. */
+int hg_hnsw_build_gpu(void* index, const float* data, size_t rows, const uint64_t* labels, int num_threads,
+                      int device);
+/* Its first step alone: insert rows into the levels above 0 (level-0 lists left empty). */
+int hg_hnsw_add_items_upper(void* index, const float* data, size_t rows, const uint64_t* labels, int num_threads);
 /* Index.normalize_vector (bindings.cpp:241-248) applied row by row (the cosine space). */
 int hg_normalize_rows(const float* in, float* out, size_t rows, size_t dim);
 
