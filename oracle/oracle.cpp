@@ -15,6 +15,7 @@
 #include <stdexcept>
 #include <string>
 #include <thread>
+#include <unordered_map>
 #include <vector>
 
 namespace {
@@ -484,6 +485,181 @@ struct Searcher {
         }
     }
 
+    // searchStopConditionClosest (hnswalg.h:1326-1377) with MultiVectorSearchStopCondition
+    // (stop_condition.h:82-200): the result set is bounded by its number of distinct documents
+    // (ef_collection = max(ef_collection, num_docs)); a result is evicted (farthest first) while
+    // it holds more documents than that; the search stops at a candidate farther than the worst
+    // result once the bound is met; the farthest results are dropped until num_docs documents
+    // remain.  docs[internal id] = document.  `out` is closest first.
+    template <int MODE>
+    void run_mv(const float* q, const uint32_t* docs, size_t num_docs, size_t ef_coll, std::vector<Res>& out,
+                int64_t* ctr, uint8_t* tie, const uint8_t* allowed) {
+        out.clear();
+        int64_t H0 = 0, E0 = 0, Hu = 0, Eu = 0;
+        if (h.count == 0) return;
+        const size_t dim = h.dim;
+        const size_t efc = std::max(ef_coll, num_docs);
+        uint32_t cur = h.entry;
+        float curdist = dist(space, dim, q, h.vec(cur));
+        Eu++;
+        for (int level = h.maxlevel; level > 0; level--) {
+            bool changed = true;
+            while (changed) {
+                changed = false;
+                const uint32_t* data = h.ll(cur, level);
+                int size = Hnsw::count_of(data);
+                Hu++;
+                for (int i = 0; i < size; i++) {
+                    uint32_t cand = data[1 + i];
+                    if (cand > h.max_elements) throw std::runtime_error("cand error");
+                    float d = dist(space, dim, q, h.vec(cand));
+                    Eu++;
+                    if (d < curdist) {
+                        curdist = d;
+                        cur = cand;
+                        changed = true;
+                    }
+                }
+            }
+        }
+        next_tag();
+        std::vector<float> seen;
+        const bool track = tie != nullptr;
+        auto ok = [&](uint32_t id) { return !h.deleted(id) && (allowed == nullptr || allowed[id] != 0); };
+        const float fmax = std::numeric_limits<float>::max();
+        std::unordered_map<uint32_t, size_t> counter;
+        size_t ndocs = 0;
+        auto add_doc = [&](uint32_t id) {
+            if (counter[docs[id]]++ == 0) ndocs++;
+        };
+        auto remove_doc = [&](uint32_t doc) {
+            if (--counter[doc] == 0) ndocs--;
+        };
+        std::vector<P> res;
+        if (MODE == OR_MODE_HEAP) {
+            std::priority_queue<std::pair<float, uint32_t>> sr;  // search_results_: (dist, doc)
+            std::priority_queue<P, std::vector<P>, ByFirst> top, cand;
+            float lowerBound;
+            if (ok(cur)) {
+                float dep = dist(space, dim, q, h.vec(cur));
+                E0++;
+                if (track) seen.push_back(dep);
+                lowerBound = dep;
+                top.emplace(dep, cur);
+                add_doc(cur);
+                sr.emplace(dep, docs[cur]);
+                cand.emplace(-dep, cur);
+            } else {
+                lowerBound = fmax;
+                cand.emplace(-lowerBound, cur);
+            }
+            visited[cur] = tag;
+            while (!cand.empty()) {
+                P cp = cand.top();
+                if (-cp.first > lowerBound && ndocs == efc) break;
+                cand.pop();
+                const uint32_t* data = h.ll0(cp.second);
+                size_t size = Hnsw::count_of(data);
+                H0++;
+                for (size_t j = 1; j <= size; j++) {
+                    uint32_t cid = data[j];
+                    if (visited[cid] == tag) continue;
+                    visited[cid] = tag;
+                    float d = dist(space, dim, q, h.vec(cid));
+                    E0++;
+                    if (track) seen.push_back(d);
+                    if (ndocs < efc || lowerBound > d) {
+                        cand.emplace(-d, cid);
+                        if (ok(cid)) {
+                            top.emplace(d, cid);
+                            add_doc(cid);
+                            sr.emplace(d, docs[cid]);
+                        }
+                        while (ndocs > efc) {  // remove_point_from_result pops search_results_'s top
+                            remove_doc(docs[top.top().second]);
+                            top.pop();
+                            sr.pop();
+                        }
+                        if (!top.empty()) lowerBound = top.top().first;
+                    }
+                }
+            }
+            res.resize(top.size());
+            for (size_t i = res.size(); i-- > 0;) {
+                res[i] = top.top();
+                top.pop();
+            }
+            while (ndocs > num_docs && !res.empty()) {  // filter_results: the doc from search_results_
+                const uint32_t doc = sr.top().second;
+                remove_doc(doc);
+                sr.pop();
+                res.pop_back();
+            }
+        } else {
+            std::priority_queue<P, std::vector<P>, ByPair> top;
+            std::priority_queue<P, std::vector<P>, ByPairNeg> cand;
+            E0++;
+            if (track) seen.push_back(curdist);
+            if (ok(cur)) {
+                top.emplace(curdist, cur);
+                add_doc(cur);
+            }
+            cand.emplace(-std::numeric_limits<float>::infinity(), cur);
+            visited[cur] = tag;
+            bool first = true;
+            while (!cand.empty()) {
+                P cp = cand.top();
+                const P key = first ? P(curdist, cur) : cp;
+                first = false;
+                if (ndocs == efc && !top.empty() && top.top() < key) break;
+                cand.pop();
+                const uint32_t* data = h.ll0(cp.second);
+                size_t size = Hnsw::count_of(data);
+                H0++;
+                for (size_t j = 1; j <= size; j++) {
+                    uint32_t cid = data[j];
+                    if (visited[cid] == tag) continue;
+                    visited[cid] = tag;
+                    float d = dist(space, dim, q, h.vec(cid));
+                    E0++;
+                    if (track) seen.push_back(d);
+                    P np(d, cid);
+                    if (ndocs < efc || np < top.top()) {
+                        cand.push(np);
+                        if (ok(cid)) {
+                            top.push(np);
+                            add_doc(cid);
+                        }
+                        while (ndocs > efc) {
+                            remove_doc(docs[top.top().second]);
+                            top.pop();
+                        }
+                    }
+                }
+            }
+            res.resize(top.size());
+            for (size_t i = res.size(); i-- > 0;) {
+                res[i] = top.top();
+                top.pop();
+            }
+            while (ndocs > num_docs && !res.empty()) {
+                remove_doc(docs[res.back().second]);
+                res.pop_back();
+            }
+        }
+        for (auto& r : res) out.push_back({r.first, r.second});
+        if (ctr) {
+            ctr[0] = H0;
+            ctr[1] = E0;
+            ctr[2] = Hu;
+            ctr[3] = Eu;
+        }
+        if (tie) {
+            std::sort(seen.begin(), seen.end());
+            *tie = std::adjacent_find(seen.begin(), seen.end()) != seen.end() ? 1 : 0;
+        }
+    }
+
     // searchStopConditionClosest (hnswalg.h:1326-1377) with EpsilonSearchStopCondition
     // (stop_condition.h:207-275) driving searchBaseLayerST<false> (hnswalg.h:323-437): the result
     // set holds at most max_c allowed elements; the search stops at a candidate farther than the
@@ -782,6 +958,46 @@ int or_hnsw_search_epsilon(void* hv, int space, const float* q, size_t nq, float
         }
         // the reference returns the result vector in heap order (closest first); rows here are
         // ordered by (distance, label) so that equal distances have one order
+        std::vector<std::pair<float, uint64_t>> lab;
+        for (auto& o : out) lab.emplace_back(o.d, h.label(o.id));
+        if (mode == OR_MODE_TOTAL) std::sort(lab.begin(), lab.end());
+        counts[r] = (uint32_t)lab.size();
+        for (size_t i = 0; i < out_stride; i++) {
+            labels[r * out_stride + i] = i < lab.size() ? lab[i].second : UINT64_MAX;
+            dists[r * out_stride + i] = i < lab.size() ? lab[i].first : std::numeric_limits<float>::infinity();
+        }
+    });
+    for (auto* s : ss) delete s;
+    return failed ? -1 : 0;
+}
+
+int or_hnsw_search_multivector(void* hv, int space, const float* q, size_t nq, const uint32_t* docs,
+                               size_t num_docs, size_t ef_collection, int mode, int nthreads, const uint8_t* allowed,
+                               size_t out_stride, uint64_t* labels, float* dists, uint32_t* counts,
+                               int64_t* counters, uint8_t* tie_flags) {
+    const Hnsw& h = *(const Hnsw*)hv;
+    if (nthreads <= 0) nthreads = (int)std::thread::hardware_concurrency();
+    std::vector<Searcher*> ss(std::max(nthreads, 1), nullptr);
+    std::mutex err_mu;
+    bool failed = false;
+    parallel_for(nq, nthreads, [&](size_t r, int t) {
+        if (!ss[t]) ss[t] = new Searcher(h, space);
+        std::vector<Res> out;
+        try {
+            if (mode == OR_MODE_HEAP)
+                ss[t]->run_mv<OR_MODE_HEAP>(q + r * h.dim, docs, num_docs, ef_collection, out,
+                                            counters ? counters + 4 * r : nullptr,
+                                            tie_flags ? tie_flags + r : nullptr, allowed);
+            else
+                ss[t]->run_mv<OR_MODE_TOTAL>(q + r * h.dim, docs, num_docs, ef_collection, out,
+                                             counters ? counters + 4 * r : nullptr,
+                                             tie_flags ? tie_flags + r : nullptr, allowed);
+        } catch (std::exception& e) {
+            std::lock_guard<std::mutex> g(err_mu);
+            g_err = e.what();
+            failed = true;
+            return;
+        }
         std::vector<std::pair<float, uint64_t>> lab;
         for (auto& o : out) lab.emplace_back(o.d, h.label(o.id));
         if (mode == OR_MODE_TOTAL) std::sort(lab.begin(), lab.end());

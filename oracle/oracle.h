@@ -10,6 +10,7 @@
 //   or_hnsw_search         hnswalg.h:1269-1323 (searchKnn) + 308-440 (searchBaseLayerST<true/false>)
 //                          + bindings.cpp:612-697 (row output, closest first)
 //   or_hnsw_search_epsilon hnswalg.h:1326-1377 (searchStopConditionClosest) + stop_condition.h:207-275
+//   or_hnsw_search_multivector  the same + stop_condition.h:82-200 (MultiVectorSearchStopCondition)
 //   or_bf_search           bruteforce.h:106-125 + bindings.cpp:850-929
 //   or_mlp_*               the learned-stop network's forward/backward/Adam in FP64/FP32 scalar
 //                          form (no reference counterpart: checks the GPU training kernels)
@@ -70,6 +71,13 @@ int or_hnsw_search_allowed(void* h, int space, const float* q, size_t nq, size_t
 int or_hnsw_search_epsilon(void* h, int space, const float* q, size_t nq, float eps, size_t min_c, size_t max_c,
                            int mode, int nthreads, const uint8_t* allowed, size_t out_stride, uint64_t* labels,
                            float* dists, uint32_t* counts, int64_t* counters, uint8_t* tie_flags);
+// searchStopConditionClosest with MultiVectorSearchStopCondition (hnswalg.h:1326-1377,
+// stop_condition.h:82-200); docs[internal id] = the element's document.  Output as for
+// or_hnsw_search_epsilon.
+int or_hnsw_search_multivector(void* h, int space, const float* q, size_t nq, const uint32_t* docs,
+                               size_t num_docs, size_t ef_collection, int mode, int nthreads, const uint8_t* allowed,
+                               size_t out_stride, uint64_t* labels, float* dists, uint32_t* counts,
+                               int64_t* counters, uint8_t* tie_flags);
 // Internal id of each element's label and its delete mark, for building `allowed` masks.
 void or_hnsw_labels(void* h, uint64_t* labels_by_id, uint8_t* deleted_by_id);
 

@@ -47,6 +47,8 @@ def lib():
         L.or_hnsw_search_allowed.argtypes = [_vp, _i32, _vp, _sz, _sz, _sz, _vp, _vp, _sz, _i32, _i32, _vp, _vp,
                                              _vp, _vp, _vp]
         L.or_hnsw_labels.argtypes = [_vp, _vp, _vp]
+        L.or_hnsw_search_multivector.argtypes = [_vp, _i32, _vp, _sz, _vp, _sz, _sz, _i32, _i32, _vp, _sz, _vp, _vp,
+                                                 _vp, _vp, _vp]
         L.or_hnsw_search_epsilon.argtypes = [_vp, _i32, _vp, _sz, ctypes.c_float, _sz, _sz, _i32, _i32, _vp, _sz,
                                              _vp, _vp, _vp, _vp, _vp]
         L.or_bf_search.argtypes = [_i32, _sz, _vp, _vp, _sz, _vp, _sz, _sz, _i32, _vp, _vp]
@@ -133,6 +135,26 @@ class OracleHnsw:
             raise RuntimeError(lib().or_last_error().decode())
         return labels, dists, counts, ctr, tf
 
+    def search_multivector(self, space, queries, docs, num_docs, ef_collection, out_stride, mode=MODE_HEAP,
+                           nthreads=0, allowed=None, counters=False, ties=False):
+        """searchStopConditionClosest + MultiVectorSearchStopCondition; docs: uint32 per internal
+        id.  Returns (labels, dists, counts, counters, tie flags)."""
+        q = np.ascontiguousarray(queries, np.float32)
+        nq = q.shape[0]
+        dc = np.ascontiguousarray(docs, np.uint32)
+        labels = np.empty((nq, out_stride), np.uint64)
+        dists = np.empty((nq, out_stride), np.float32)
+        counts = np.empty(nq, np.uint32)
+        ctr = np.zeros((nq, 4), np.int64) if counters else None
+        tf = np.zeros(nq, np.uint8) if ties else None
+        al = None if allowed is None else np.ascontiguousarray(allowed, np.uint8)
+        r = lib().or_hnsw_search_multivector(self.h, space, _p(q), nq, _p(dc), num_docs, ef_collection, mode,
+                                             nthreads, _p(al), out_stride, _p(labels), _p(dists), _p(counts),
+                                             _p(ctr), _p(tf))
+        if r < 0:
+            raise RuntimeError(lib().or_last_error().decode())
+        return labels, dists, counts, ctr, tf
+
     def __del__(self):
         try:
             lib().or_hnsw_free(self.h)
@@ -214,4 +236,8 @@ def ref_shim():
     L.ref_hnsw_free.argtypes = [_vp]
     L.ref_hnsw_mark_deleted.argtypes = [_vp, ctypes.c_uint64]
     L.ref_hnsw_search_epsilon.argtypes = [_vp, _vp, _sz, ctypes.c_float, _sz, _sz, _vp, _sz, _sz, _vp, _vp, _vp]
+    L.ref_mv_build.restype = _vp
+    L.ref_mv_build.argtypes = [_sz, _vp, _vp, _sz, _sz, _sz, _sz]
+    L.ref_mv_free.argtypes = [_vp]
+    L.ref_mv_search.argtypes = [_vp, _vp, _sz, _sz, _sz, _sz, _vp, _vp, _vp]
     return L

@@ -189,6 +189,61 @@ REF_API int ref_hnsw_search_epsilon(void* hv, const float* q, size_t nq, float e
     }
 }
 
+// A multi-vector index (MultiVectorL2Space<unsigned>, stop_condition.h:19-76): each element is
+// its vector followed by its document id; built single-threaded in insertion order (labels =
+// 0..n-1).  The graph equals the plain L2 build's (the distance reads the vector only).
+namespace {
+struct RefMv {
+    hnswlib::MultiVectorL2Space<unsigned> space;
+    hnswlib::HierarchicalNSW<float>* alg = nullptr;
+    size_t dim;
+    explicit RefMv(size_t d) : space(d), dim(d) {}
+    ~RefMv() { delete alg; }
+};
+}  // namespace
+
+REF_API void* ref_mv_build(size_t dim, const float* data, const uint32_t* docs, size_t n, size_t M, size_t efc,
+                           size_t seed) {
+    try {
+        RefMv* h = new RefMv(dim);
+        h->alg = new hnswlib::HierarchicalNSW<float>(&h->space, n, M, efc, seed);
+        std::vector<char> point(h->space.get_data_size());
+        for (size_t i = 0; i < n; i++) {
+            memcpy(point.data(), data + i * dim, dim * 4);
+            h->space.set_doc_id(point.data(), docs[i]);
+            h->alg->addPoint(point.data(), (hnswlib::labeltype)i);
+        }
+        return h;
+    } catch (std::exception& e) {
+        g_err = e.what();
+        return nullptr;
+    }
+}
+
+REF_API void ref_mv_free(void* hv) { delete static_cast<RefMv*>(hv); }
+
+// searchStopConditionClosest with MultiVectorSearchStopCondition(space, num_docs, ef_collection)
+// (hnswalg.h:1326-1377, stop_condition.h:82-200); rows as ref_hnsw_search_epsilon.
+REF_API int ref_mv_search(void* hv, const float* q, size_t nq, size_t num_docs, size_t ef_collection,
+                          size_t out_stride, uint64_t* labels, float* dists, uint32_t* counts) {
+    try {
+        RefMv* h = static_cast<RefMv*>(hv);
+        for (size_t r = 0; r < nq; r++) {
+            hnswlib::MultiVectorSearchStopCondition<unsigned, float> sc(h->space, num_docs, ef_collection);
+            auto res = h->alg->searchStopConditionClosest(q + r * h->dim, sc);
+            counts[r] = (uint32_t)res.size();
+            for (size_t i = 0; i < out_stride; i++) {
+                labels[r * out_stride + i] = i < res.size() ? (uint64_t)res[i].second : UINT64_MAX;
+                dists[r * out_stride + i] = i < res.size() ? res[i].first : __builtin_inff();
+            }
+        }
+        return 0;
+    } catch (std::exception& e) {
+        g_err = e.what();
+        return -1;
+    }
+}
+
 // BFIndex.knn_query semantics (bindings.cpp:850-929 over bruteforce.h:106-125).
 REF_API int ref_bf_search(int space, size_t dim, const float* data, size_t n, const float* q, size_t nq,
                           size_t k, uint64_t* labels, float* dists) {

@@ -6,6 +6,7 @@ the hnswlib headers with setup.py's flags).  Run in a container where /root/refe
                           python tests/golden/gen_golden.py nbb    (the deletion/filter fixture only)
                           python tests/golden/gen_golden.py bf_ops (the brute-force operations fixture)
                           python tests/golden/gen_golden.py eps    (the epsilon-search fixture)
+                          python tests/golden/gen_golden.py mv     (the multi-vector fixture)
 
 Every fixture stores its inputs' SHA-256 (or the inputs themselves when small), the reference's
 outputs and the SHA-256 of the reference's saveIndex bytes for single-threaded builds.
@@ -217,7 +218,48 @@ def eps():
     np.savez_compressed(os.path.join(OUT, "c1_eps_golden.npz"), **out)
 
 
+def mv():
+    """F8: searchStopConditionClosest with MultiVectorSearchStopCondition on C1 vectors, each
+    element in one of 500 documents (document = label % 500, 20 vectors each), num_docs 10 with
+    ef_collection 15 (the reference's multivector_search_test.cpp values), and num_docs 20 with
+    ef_collection 40; 200 queries.  Results flat with per-query counts."""
+    shim = pyoracle.ref_shim()
+    assert shim is not None, "build the reference first: make -C oracle ref"
+    data = datasets.c1_uniform()
+    queries = datasets.c1_uniform(n=1000, seed=1)[:200]
+    n = data.shape[0]
+    out = dict(data_sha=sha(data), query_sha=sha(queries))
+    # scattered documents (label % 500) and spatially coherent ones (an 8 x 8 grid of the first
+    # two coordinates: ~156 nearby vectors per document)
+    scattered = (np.arange(n) % 500).astype(np.uint32)
+    grid = (np.floor(data[:, 0] * 8).astype(np.uint32) * 8 + np.floor(data[:, 1] * 8).astype(np.uint32))
+    for name, docs, nd, efc in (("d10_e15", scattered, 10, 15), ("d20_e40", scattered, 20, 40),
+                                ("grid_d3_e5", grid.astype(np.uint32), 3, 5)):
+        docs = np.ascontiguousarray(docs)
+        h = shim.ref_mv_build(16, data.ctypes.data, docs.ctypes.data, n, 16, 200, 100)
+        assert h, shim.ref_last_error()
+        out[f"docs_{name}"] = docs
+        stride = 2000
+        lab = np.empty((queries.shape[0], stride), np.uint64)
+        dis = np.empty((queries.shape[0], stride), np.float32)
+        cnt = np.empty(queries.shape[0], np.uint32)
+        q = np.ascontiguousarray(queries)
+        assert shim.ref_mv_search(h, q.ctypes.data, q.shape[0], nd, efc, stride, lab.ctypes.data, dis.ctypes.data,
+                                  cnt.ctypes.data) == 0, shim.ref_last_error()
+        assert cnt.max() < stride
+        out[f"counts_{name}"] = cnt
+        out[f"ids_{name}"] = np.concatenate([lab[i, :cnt[i]] for i in range(q.shape[0])])
+        out[f"dists_{name}"] = np.concatenate([dis[i, :cnt[i]] for i in range(q.shape[0])])
+        out[f"params_{name}"] = np.array([nd, efc], np.int64)
+        print(name, "results per query", float(cnt.mean()), "max", int(cnt.max()))
+        shim.ref_mv_free(h)
+    np.savez_compressed(os.path.join(OUT, "c1_mv_golden.npz"), **out)
+
+
 if __name__ == "__main__":
+    if len(sys.argv) > 1 and sys.argv[1] == "mv":
+        mv()
+        sys.exit(0)
     if len(sys.argv) > 1 and sys.argv[1] == "eps":
         eps()
         sys.exit(0)
@@ -230,3 +272,4 @@ if __name__ == "__main__":
         nbb()
         bf_ops()
         eps()
+        mv()
