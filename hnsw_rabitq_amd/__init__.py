@@ -133,6 +133,25 @@ class GpuHnsw:
                   _ptr(dists), _ptr(counts))
         return labels, dists, counts
 
+    def make_docs(self, labels, docs):
+        """Documents of the elements (for search_multivector): docs[i] is the document of labels[i]."""
+        return GpuDocs(self, labels, docs)
+
+    def search_multivector(self, queries, docs, num_docs, ef_collection, out_stride=None, filter=None):
+        """MultiVectorSearchStopCondition search: (labels, dists, counts)."""
+        q = np.ascontiguousarray(queries, dtype=np.float32)
+        nq = q.shape[0]
+        stride = int(out_stride or 1024)
+        if docs.owner is not self or (filter is not None and filter.owner is not self):
+            raise ValueError("the documents or filter were built for another GPU index")
+        labels = np.empty((nq, stride), dtype=np.uint64)
+        dists = np.empty((nq, stride), dtype=np.float32)
+        counts = np.empty(nq, dtype=np.uint32)
+        _lib.call("hg_gpu_search_multivector", self.handle, _ptr(q), nq, docs.handle, int(num_docs),
+                  int(ef_collection), filter.handle if filter is not None else None, stride, _ptr(labels),
+                  _ptr(dists), _ptr(counts))
+        return labels, dists, counts
+
     def labels(self):
         """External label of every internal id (getExternalLabel, hnswalg.h:185-189)."""
         n = ctypes.c_size_t()
@@ -225,6 +244,31 @@ class GpuFilter:
     def free(self):
         if self.handle is not None and self.handle.value:
             _lib.load().hg_gpu_filter_free(self.handle)
+        self.handle = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class GpuDocs:
+    """The document of every element of one GPU index (multi-vector search)."""
+
+    def __init__(self, gpu_index, labels, docs):
+        lab = np.ascontiguousarray(np.asarray(labels, dtype=np.uint64).reshape(-1))
+        dc = np.ascontiguousarray(np.asarray(docs, dtype=np.uint32).reshape(-1))
+        if lab.shape != dc.shape:
+            raise ValueError("labels and docs differ in length")
+        h = ctypes.c_void_p()
+        _lib.call("hg_gpu_docs_create", gpu_index.handle, _ptr(lab), _ptr(dc), lab.shape[0], ctypes.byref(h))
+        self.handle = h
+        self.owner = gpu_index
+
+    def free(self):
+        if self.handle is not None and self.handle.value:
+            _lib.load().hg_gpu_docs_free(self.handle)
         self.handle = None
 
     def __del__(self):
@@ -395,6 +439,39 @@ class Index:
     def make_filter(self, allowed_labels):
         """A reusable GpuFilter of the current export that allows the given labels."""
         return self.gpu().make_filter(allowed_labels)
+
+    def multivector_query(self, data, docs, num_docs, ef_collection=10, filter=None):
+        """The C++ API's searchStopConditionClosest with MultiVectorSearchStopCondition (no Python
+        binding in hnswlib): per query, the closest elements covering num_docs documents.  docs:
+        a mapping label -> document, or a sequence indexed by label.  Returns a list of
+        (labels, distances) pairs."""
+        if self._h is None:
+            raise RuntimeError("Index not initialized")
+        arr = _as_rows(data, self.dim)
+        if self.normalize:
+            arr = normalize_rows(arr)
+        if arr.shape[0] == 0:
+            return []
+        gpu = self.gpu()
+        labs = gpu.labels()
+        if isinstance(docs, dict):
+            dc = np.array([docs[int(x)] for x in labs], dtype=np.uint32)
+        else:
+            dc = np.asarray(docs, dtype=np.uint32)[labs.astype(np.int64)]
+        gd = gpu.make_docs(labs, dc)
+        gf = filter if isinstance(filter, GpuFilter) or filter is None else None
+        if filter is not None and gf is None:
+            keep = np.fromiter((bool(filter(int(x))) for x in labs), dtype=bool, count=labs.shape[0])
+            gf = gpu.make_filter(labs[keep])
+        try:
+            lab, dis, cnt = gpu.search_multivector(arr, gd, num_docs, ef_collection, filter=gf)
+        finally:
+            gd.free()
+            if gf is not None and gf is not filter:
+                gf.free()
+        if (cnt > lab.shape[1]).any():
+            raise RuntimeError("more results than the row stride; lower num_docs")
+        return [(lab[i, :cnt[i]].copy(), dis[i, :cnt[i]].copy()) for i in range(arr.shape[0])]
 
     def range_query(self, data, epsilon, min_candidates, max_candidates, filter=None):
         """The C++ API's searchStopConditionClosest with EpsilonSearchStopCondition (no Python

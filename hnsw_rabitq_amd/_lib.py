@@ -57,6 +57,11 @@ SIGNATURES = {
     "hg_gpu_search_filtered": [_vp, _vp, _sz, _sz, _sz, _vp, _vp, _vp, _vp],
     "hg_gpu_search_device_filtered": [_vp, _vp, _sz, _vp, _vp, _sz, _sz, _sz, _vp, _vp, _vp, _vp, _vp, _vp],
     "hg_gpu_search_status": [_vp, _vp, _vp],
+    "hg_gpu_docs_create": [_vp, _vp, _vp, _sz, _vp],
+    "hg_gpu_docs_free": [_vp],
+    "hg_gpu_search_multivector": [_vp, _vp, _sz, _vp, _sz, _sz, _vp, _sz, _vp, _vp, _vp],
+    "hg_gpu_search_multivector_device": [_vp, _vp, _sz, _vp, _sz, _sz, _vp, _sz, _vp, _vp, _vp, _vp, _vp,
+                                         ctypes.c_uint32, _vp],
     "hg_gpu_search_epsilon": [_vp, _vp, _sz, ctypes.c_float, _sz, _sz, _vp, _sz, _vp, _vp, _vp],
     "hg_gpu_search_epsilon_device": [_vp, _vp, _sz, ctypes.c_float, _sz, _sz, _vp, _sz, _vp, _vp, _vp, _vp, _vp,
                                      _vp],

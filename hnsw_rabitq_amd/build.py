@@ -18,7 +18,7 @@ HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 ARCH = "gfx950"
 
 HOST_SOURCES = ["host/hnsw_index.cpp"]
-HIP_SOURCES = ["search.hip", "build.hip", "bruteforce.hip", "mlp.hip", "capi.cpp", "capi_bf.cpp", "capi_stop.cpp"]
+HIP_SOURCES = ["search.hip", "build.hip", "mvsearch.hip", "bruteforce.hip", "mlp.hip", "capi.cpp", "capi_bf.cpp", "capi_stop.cpp"]
 
 GXX_FLAGS = ["-O3", "-std=c++17", "-fPIC", "-ffp-contract=off", "-march=x86-64-v3", "-pthread",
              "-fvisibility=hidden", "-Wall", "-Wno-unused-function"]

@@ -941,6 +941,172 @@ HG_API int hg_gpu_id_labels(void* gv, const uint32_t* d_ids, size_t n, uint64_t*
     });
 }
 
+// ---- multi-vector search ----
+namespace {
+// Document of every element of one export, by internal id.
+struct GpuDocs {
+    uint64_t serial = 0;
+    int device = 0;
+    size_t n = 0;
+    DevBuf docs;
+};
+
+GpuDocs* as_docs(GpuIndex* gi, void* d) {
+    require(d != nullptr, "null documents");
+    auto* gd = static_cast<GpuDocs*>(d);
+    require(gd->serial == gi->serial && gd->n == gi->host_labels.size(),
+            "the documents were built for another GPU index");
+    return gd;
+}
+
+hipError_t run_mv(GpuIndex* gi, const float* dq, size_t nq, const GpuDocs* gd, size_t num_docs, size_t ef_coll,
+                  const GpuFilter* filt, size_t out_stride, uint64_t* dl, float* dd, uint32_t* dcounts, int32_t* dc,
+                  uint32_t* status, uint32_t cap, GpuIndex::LaunchWords* lw, hipStream_t s) {
+    hg::MvParams mp{};
+    mp.g = gi->g;
+    mp.queries = dq;
+    mp.nq = (uint32_t)nq;
+    mp.work = (uint32_t*)lw->words.p + kWordWork;
+    mp.docs = (const uint32_t*)gd->docs.p;
+    mp.del_bits = gi->num_deleted > 0 ? (const uint32_t*)gi->delbits.p : nullptr;
+    mp.allow_bits = filt ? (const uint32_t*)filt->bits.p : nullptr;
+    mp.num_docs = (uint32_t)num_docs;
+    mp.ef_coll = (uint32_t)std::max(ef_coll, num_docs);  // MultiVectorSearchStopCondition's ef_collection_
+    mp.cap = cap;
+    mp.hash_bits = 11;
+    mp.dcap = (uint32_t)((mp.ef_coll + 2 + 63) / 64 * 64);
+    mp.out_labels = dl;
+    mp.out_dists = dd;
+    mp.counts = dcounts;
+    mp.k_stride = (uint32_t)out_stride;
+    mp.counters = dc;
+    mp.status = status;
+    require(hg::mv_lds_bytes(gi->g.dim, mp.cap, mp.hash_bits, mp.dcap) <= 160 * 1024,
+            "ef_collection and dimension too large for the multi-vector GPU search");
+    const size_t blocks = std::min<size_t>((size_t)gi->cus * (size_t)hg::mv_blocks_per_cu(mp), nq);
+    hipError_t e = hg::launch_zero_u32(status, 1, s);
+    if (e != hipSuccess) return e;
+    return hg::launch_mv_search(mp, (int)std::max<size_t>(blocks, 1), s);
+}
+
+void check_mv_args(size_t num_docs, size_t ef_coll, size_t out_stride) {
+    require(num_docs >= 1, "num_docs must be at least 1");
+    require(std::max(num_docs, ef_coll) <= 1024, "ef_collection above 1024 is not supported by the GPU search");
+    require(out_stride >= 1 && out_stride <= kFiltMaxCap, "out_stride must be in [1, 8192]");
+}
+}  // namespace
+
+HG_API int hg_gpu_docs_create(void* gv, const uint64_t* labels, const uint32_t* docs, size_t count, void** out) {
+    return guarded([&] {
+        require(gv && out && (count == 0 || (labels && docs)), "null argument");
+        auto* gi = static_cast<GpuIndex*>(gv);
+        std::unordered_map<uint64_t, uint32_t> doc_of;
+        doc_of.reserve(count * 2);
+        for (size_t i = 0; i < count; i++) doc_of[labels[i]] = docs[i];
+        const size_t n = gi->host_labels.size();
+        std::vector<uint32_t> by_id(n);
+        for (size_t i = 0; i < n; i++) {
+            auto it = doc_of.find(gi->host_labels[i]);
+            require(it != doc_of.end(), "an element of the index has no document");
+            by_id[i] = it->second;
+        }
+        DeviceGuard dg(gi->device);
+        std::unique_ptr<GpuDocs> d(new GpuDocs());
+        d->serial = gi->serial;
+        d->device = gi->device;
+        d->n = n;
+        d->docs.reserve(std::max<size_t>(n, 1) * 4);
+        hip_check(hipMemcpy(d->docs.p, by_id.data(), n * 4, hipMemcpyHostToDevice), "hipMemcpy H2D");
+        *out = d.release();
+        return HG_OK;
+    });
+}
+
+HG_API int hg_gpu_docs_free(void* d) {
+    if (!d) return HG_OK;
+    auto* gd = static_cast<GpuDocs*>(d);
+    int prev = -1;
+    (void)hipGetDevice(&prev);
+    (void)hipSetDevice(gd->device);
+    delete gd;
+    if (prev >= 0) (void)hipSetDevice(prev);
+    return HG_OK;
+}
+
+// searchStopConditionClosest with MultiVectorSearchStopCondition on host buffers.
+HG_API int hg_gpu_search_multivector(void* gv, const float* queries, size_t nq, void* docs, size_t num_docs,
+                                     size_t ef_collection, void* filter, size_t out_stride, uint64_t* labels,
+                                     float* dists, uint32_t* counts) {
+    return guarded([&] {
+        require(gv && labels && dists && counts, "null argument");
+        auto* gi = static_cast<GpuIndex*>(gv);
+        check_mv_args(num_docs, ef_collection, out_stride);
+        GpuDocs* gd = as_docs(gi, docs);
+        GpuFilter* gf = as_filter(gi, filter);
+        require(nq < (1ull << 31), "too many queries");
+        if (nq == 0) return HG_OK;
+        require(queries != nullptr, "null queries");
+        std::lock_guard<std::mutex> lk(gi->mu);
+        DeviceGuard dg(gi->device);
+        const size_t dim = gi->g.dim;
+        gi->q.reserve(nq * dim * 4);
+        gi->ol.reserve(nq * out_stride * 8);
+        gi->od.reserve(nq * out_stride * 4);
+        gi->cnt.reserve(nq * 4);
+        hipStream_t s = gi->stream;
+        hip_check(hipMemcpyAsync(gi->q.p, queries, nq * dim * 4, hipMemcpyHostToDevice, s), "hipMemcpy H2D");
+        uint32_t status = 0;
+        for (uint32_t cap : {2048u, kFiltMaxCap}) {  // the larger beam only if a query outgrew the first
+            GpuIndex::LaunchWords* lw = next_words(gi);
+            prepare_words(lw, nq, false);
+            uint32_t* st = (uint32_t*)lw->words.p + kWordStatus;
+            hip_check(run_mv(gi, (const float*)gi->q.p, nq, gd, num_docs, ef_collection, gf, out_stride,
+                             (uint64_t*)gi->ol.p, (float*)gi->od.p, (uint32_t*)gi->cnt.p, nullptr, st, cap, lw, s),
+                      "multi-vector search launch");
+            hip_check(hipMemcpyAsync(&status, st, 4, hipMemcpyDeviceToHost, s), "hipMemcpy D2H");
+            hip_check(hipStreamSynchronize(s), "search");
+            if (!(status & 1u)) break;
+        }
+        if (status & 1u) {
+            g_err = "multi-vector search: a query needed more than " + std::to_string(kFiltMaxCap) +
+                    " beam entries";
+            return HG_ERR_RUNTIME;
+        }
+        hip_check(hipMemcpyAsync(labels, gi->ol.p, nq * out_stride * 8, hipMemcpyDeviceToHost, s), "hipMemcpy D2H");
+        hip_check(hipMemcpyAsync(dists, gi->od.p, nq * out_stride * 4, hipMemcpyDeviceToHost, s), "hipMemcpy D2H");
+        hip_check(hipMemcpyAsync(counts, gi->cnt.p, nq * 4, hipMemcpyDeviceToHost, s), "hipMemcpy D2H");
+        hip_check(hipStreamSynchronize(s), "search");
+        return HG_OK;
+    });
+}
+
+// The same on device buffers, asynchronous on stream, at one beam capacity (cap = 0: 2048);
+// d_status bit 0 reports a query that outgrew it.
+HG_API int hg_gpu_search_multivector_device(void* gv, const float* dq, size_t nq, void* docs, size_t num_docs,
+                                            size_t ef_collection, void* filter, size_t out_stride, uint64_t* dl,
+                                            float* dd, uint32_t* d_counts, int32_t* dc, uint32_t* d_status,
+                                            uint32_t cap, void* stream) {
+    return guarded([&] {
+        require(gv && dl && dd && d_counts && d_status, "null argument");
+        auto* gi = static_cast<GpuIndex*>(gv);
+        check_mv_args(num_docs, ef_collection, out_stride);
+        GpuDocs* gd = as_docs(gi, docs);
+        GpuFilter* gf = as_filter(gi, filter);
+        if (cap == 0) cap = 2048;
+        require(cap % 64 == 0 && cap <= kFiltMaxCap, "cap must be a multiple of 64 up to 8192");
+        if (nq == 0) return HG_OK;
+        require(dq != nullptr, "null queries");
+        std::lock_guard<std::mutex> lk(gi->mu);
+        DeviceGuard dg(gi->device);
+        GpuIndex::LaunchWords* lw = next_words(gi);
+        prepare_words(lw, nq, false);
+        hip_check(run_mv(gi, dq, nq, gd, num_docs, ef_collection, gf, out_stride, dl, dd, d_counts, dc, d_status, cap,
+                         lw, (hipStream_t)stream),
+                  "multi-vector search launch");
+        return HG_OK;
+    });
+}
+
 // ---- level-0 construction on the GPU ----
 // addPoint (hnswalg.h:1152-1266) for a batch of rows: the CPU inserts them into the levels
 // above 0 (those do not depend on level 0), then the GPU builds their level-0 links in batches

@@ -88,6 +88,26 @@ struct SearchParams {
     uint32_t q_stride;
 };
 
+// Multi-vector search (MultiVectorSearchStopCondition): one wave per query; every element
+// belongs to a document docs[id]; the result set is bounded by its number of distinct documents.
+struct MvParams {
+    DevGraph g;
+    const float* queries;  // [nq][dim]
+    uint32_t nq;
+    uint32_t* work;        // query counter (zeroed by the launch)
+    const uint32_t* docs;  // [n]
+    const uint32_t* del_bits;    // optional
+    const uint32_t* allow_bits;  // optional
+    uint32_t num_docs, ef_coll;  // ef_coll >= num_docs
+    uint32_t cap, hash_bits, dcap;
+    uint64_t* out_labels;  // [nq][k_stride]
+    float* out_dists;
+    uint32_t* counts;      // [nq]
+    uint32_t k_stride;
+    int32_t* counters;     // optional [nq][4]
+    uint32_t* status;      // bit 0: a query outgrew the beam
+};
+
 constexpr int kStopFeatures = 12;
 
 }  // namespace hg

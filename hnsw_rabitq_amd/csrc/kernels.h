@@ -16,6 +16,11 @@ hipError_t launch_zero_u32(uint32_t* p, uint32_t n, hipStream_t s);  // a kernel
 hipError_t launch_copy(void* dst, const void* src, size_t bytes, hipStream_t s);
 hipError_t launch_id_labels(const DevGraph& g, const uint32_t* ids, size_t n, uint64_t* out, hipStream_t s);
 
+// ---- multi-vector search (mvsearch.hip) ----
+size_t mv_lds_bytes(uint32_t dim, uint32_t cap, uint32_t hash_bits, uint32_t dcap);
+int mv_blocks_per_cu(const MvParams& p);
+hipError_t launch_mv_search(const MvParams& p, int blocks, hipStream_t s);
+
 // ---- level-0 graph construction (build.hip) ----
 constexpr int kBuildUnionCap = 1024;  // list + arrivals re-selected at once by the link kernel
 struct BuildArgs {

@@ -53,4 +53,31 @@ __device__ __forceinline__ int lanes_below(uint64_t m, int lane) {
     return __popcll(m & ((lane == 0) ? 0ull : (~0ull >> (64 - lane))));
 }
 
+// Visited hash: returns true when id was not present (and inserts it).
+__device__ __forceinline__ bool hash_insert(uint32_t* hash, uint32_t mask, uint32_t id) {
+    uint32_t h = (id * 2654435761u) & mask;
+#pragma unroll 1
+    for (int p = 0; p < 4; p++) {
+        uint32_t slot = (h + p) & mask;
+        uint32_t old = atomicCAS(&hash[slot], SENT, id);
+        if (old == SENT) return true;
+        if (old == id) return false;
+    }
+    hash[h] = id;  // window full: overwrite (may cause one extra evaluation later)
+    return true;
+}
+
+// Number of beam entries with key < (d, id), over beam[0, n): a fixed-step lower bound (the
+// probe index is clamped, so every lane runs the same steps with no branch).
+__device__ __forceinline__ int beam_lower_bound(const float* bd, const uint32_t* bi, int n, float d, uint32_t id) {
+    int pos = 0;
+    for (int step = 1 << (31 - __clz(n > 0 ? n : 1)); step > 0; step >>= 1) {
+        const int probe = pos + step - 1;
+        const int pc = probe < n ? probe : (n > 0 ? n - 1 : 0);
+        const bool lt = (probe < n) & key_less(bd[pc], bi[pc] & ID_MASK, d, id);
+        pos += lt ? step : 0;
+    }
+    return pos;
+}
+
 }  // namespace hg

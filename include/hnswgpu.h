@@ -25,6 +25,8 @@
  *                         (extends the single ef_ of hnswalg.h:173-175)
  *   hg_gpu_search_epsilon searchStopConditionClosest(q, EpsilonSearchStopCondition, filter)
  *                         hnswalg.h:1326-1377, stop_condition.h:207-275 (range search)
+ *   hg_gpu_search_multivector  searchStopConditionClosest(q, MultiVectorSearchStopCondition)
+ *                         hnswalg.h:1326-1377, stop_condition.h:82-200 (documents of vectors)
  *   hg_bf_create / hg_bf_add   BruteforceSearch(space, maxElements) / addPoint
  *                         bruteforce.h:46-83; BFIndex.init_index/add_items bindings.cpp:785-819
  *   hg_bf_search          BFIndex.knn_query bindings.cpp:850-929 over
@@ -167,6 +169,27 @@ int hg_gpu_search_epsilon_device(void* gpu_index, const float* d_queries, size_t
                                  size_t min_candidates, size_t max_candidates, void* filter, size_t out_stride,
                                  uint64_t* d_labels, float* d_dists, uint32_t* d_counts, int32_t* d_counters,
                                  uint32_t* d_status, void* stream);
+/* Multi-vector search: searchStopConditionClosest with MultiVectorSearchStopCondition(space,
+ * num_docs, ef_collection) (hnswalg.h:1326-1377, stop_condition.h:82-200).  The reference keeps
+ * each element's document id in its vector record (MultiVectorL2Space /
+ * MultiVectorInnerProductSpace); here a documents object maps every element of the GPU index
+ * to its document, built from (label, document) pairs (every element needs one).  The result
+ * set holds at most max(ef_collection, num_docs) documents during the search and num_docs at
+ * the end; counts[q] results per query, rows as hg_gpu_search_epsilon.  The distances read the
+ * vector only, so an index built from the vectors alone is the reference's multi-vector graph.
+ * ef_collection <= 1024; the host call retries a batch at an 8192-entry beam when a query
+ * outgrows 2048 entries and then fails (HG_ERR_RUNTIME). */
+int hg_gpu_docs_create(void* gpu_index, const uint64_t* labels, const uint32_t* docs, size_t count, void** out_docs);
+int hg_gpu_docs_free(void* docs);
+int hg_gpu_search_multivector(void* gpu_index, const float* queries, size_t nq, void* docs, size_t num_docs,
+                              size_t ef_collection, void* filter, size_t out_stride, uint64_t* labels, float* dists,
+                              uint32_t* counts);
+/* The same on device buffers at one beam capacity (cap, 0 = 2048); asynchronous on stream;
+ * d_status (required) bit 0 reports a query that outgrew the beam. */
+int hg_gpu_search_multivector_device(void* gpu_index, const float* d_queries, size_t nq, void* docs,
+                                     size_t num_docs, size_t ef_collection, void* filter, size_t out_stride,
+                                     uint64_t* d_labels, float* d_dists, uint32_t* d_counts, int32_t* d_counters,
+                                     uint32_t* d_status, uint32_t cap, void* stream);
 /* Re-reads the delete marks (markDelete / unmarkDelete) of the CPU index this GPU index was
  * exported from, without a new export.  Fails if elements were added since the export. */
 int hg_gpu_refresh_deleted(void* gpu_index, void* index);
