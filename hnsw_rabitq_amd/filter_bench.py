@@ -2,7 +2,8 @@
 the elements (label % 10 == 0), and 10% of the elements marked deleted (label % 10 == 1).
 Ground truth is the exact FP32 top-10 over the returnable rows (GPU brute force over that subset).
 Reports recall@10, kernel time and queries/s per ef; the operating point is the smallest ef with
-recall@10 >= 0.95.  Then the range search (EpsilonSearchStopCondition) at three min_candidates."""
+recall@10 >= 0.95.  Then the range search (EpsilonSearchStopCondition) at three min_candidates and
+the multi-vector search (MultiVectorSearchStopCondition) at three ef_collection values."""
 import numpy as np
 
 FILTER_EFS = (10, 20, 40, 80, 160)
@@ -87,6 +88,37 @@ def run(H, lib, idx, gpu, data, dq, nq, stream, timer, log, k=10, steps=3):
     out["deleted10_qps_at_095"] = op["qps"] if op else None
     out["deleted10_ef"] = op["ef"] if op else None
     out.update(range_search(H, lib, gpu, data, dq, nq, stream, timer, log))
+    out.update(multivector(lib, gpu, dq, nq, stream, timer, log))
+    return out
+
+
+def multivector(lib, gpu, dq, nq, stream, timer, log, steps=3):
+    """Multi-vector search with documents of 10 consecutive labels (num_docs 10)."""
+    from . import DeviceArray
+    labs = gpu.labels()
+    gd = gpu.make_docs(labs, (labs // 10).astype(np.uint32))
+    stride = 128
+    dl, dd = DeviceArray(nq * stride * 8), DeviceArray(nq * stride * 4)
+    dcnt, dst = DeviceArray(nq * 4), DeviceArray(4)
+    out = {"multivector_sweep": []}
+    for efc in (10, 40, 160):
+        def launch():
+            lib.call("hg_gpu_search_multivector_device", gpu.handle, dq.ptr, nq, gd.handle, 10, efc, None, stride,
+                     dl.ptr, dd.ptr, dcnt.ptr, None, dst.ptr, 0, stream)
+        launch()
+        ms = []
+        for _ in range(steps):
+            timer.start()
+            launch()
+            ms.append(timer.stop())
+        t = float(np.mean(ms))
+        cnt = dcnt.to_numpy(np.uint32, (nq,))
+        status = int(dst.to_numpy(np.uint32, (1,))[0])
+        log(f"  multivector num_docs=10 ef_collection={efc}: {t:8.2f} ms QPS={nq / (t * 1e-3):10.0f} "
+            f"results/query {cnt.mean():.1f} status={status}")
+        out["multivector_sweep"].append(dict(ef_collection=efc, ms=t, qps=nq / (t * 1e-3),
+                                             results_per_query=float(cnt.mean())))
+    gd.free()
     return out
 
 
