@@ -92,7 +92,8 @@ int hg_hnsw_free(void* index);
  * before the batch, the earlier rows of the same batch added to the candidates by brute force,
  * the neighbour heuristic, and mutual linking with re-selection of full lists
  * (hnswalg.h:225-305, 443-483, 506-630).  The index must hold no deleted elements;
- * ef_construction <= 1024. */
+ * ef_construction <= 1024.  If the call fails after the rows were inserted above level 0, the
+ * index holds them with empty level-0 lists: discard it. */
 int hg_hnsw_build_gpu(void* index, const float* data, size_t rows, const uint64_t* labels, int num_threads,
                       int device);
 /* Its first step alone: insert rows into the levels above 0 (level-0 lists left empty). */
