@@ -93,10 +93,22 @@ class DeviceArray:
 class GpuHnsw:
     """Device copy of a CPU-built HNSW index (the exported GPU layout)."""
 
-    def __init__(self, cpu_handle, device=0):
+    def __init__(self, cpu_handle, device=0, path=None):
         h = ctypes.c_void_p()
-        _lib.call("hg_gpu_index_create", cpu_handle, device, ctypes.byref(h))
+        if path is not None:
+            _lib.call("hg_gpu_index_load", str(path).encode(), device, ctypes.byref(h))
+        else:
+            _lib.call("hg_gpu_index_create", cpu_handle, device, ctypes.byref(h))
         self.handle = h
+
+    @classmethod
+    def load(cls, path, device=0):
+        """A GPU index from a layout cache file written by save()."""
+        return cls(None, device, path=path)
+
+    def save(self, path):
+        """Write the versioned GPU-layout cache of this index."""
+        _lib.call("hg_gpu_index_save", self.handle, str(path).encode())
 
     def search(self, queries, k, ef, counters=False, filter=None):
         """filter: None or a GpuFilter of this index (non-bare-bone search)."""

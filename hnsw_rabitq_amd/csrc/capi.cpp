@@ -7,6 +7,7 @@
 #include <atomic>
 #include <cmath>
 #include <cstring>
+#include <fstream>
 #include <map>
 #include <memory>
 #include <mutex>
@@ -54,6 +55,7 @@ struct GpuIndex {
     std::mutex mu;
     // tuning: beam capacity -> (hash_bits, blocks per CU)
     std::map<uint32_t, std::pair<uint32_t, int>> tuned;
+    size_t urows = 0;  // rows of the upper-level link array
     ~GpuIndex() {
         if (stream) (void)hipStreamDestroy(stream);
     }
@@ -404,50 +406,126 @@ HG_API int hg_normalize_rows(const float* in, float* out, size_t rows, size_t di
 }
 
 // ---- GPU index ----
+namespace {
+// The GPU layout in host memory: what an export builds from the CPU index, what the layout cache
+// file holds, and what finish_export uploads.
+struct HostLayout {
+    size_t n = 0, dim = 0, stride = 0, maxM = 0, maxM0 = 0, urows = 0;
+    int maxlevel = 0, space = 0;
+    uint32_t entry = 0;
+    std::vector<float> vec;
+    std::vector<uint32_t> l0, uoff, up, del;
+    std::vector<uint64_t> labels;
+};
+
+void check_layout(const HostLayout& L) {
+    require(L.n > 0, "the index is empty");
+    require(L.n < (1ull << 30), "more than 2^30 elements are not supported by the GPU search");
+    // one wave holds a whole neighbour list: 2M <= 64 (M <= 32)
+    require(L.maxM0 <= 64 && L.maxM <= 64, "M above 32 is not supported by the GPU search");
+    require(L.dim >= 1 && L.dim <= 4096, "dim above 4096 is not supported by the GPU search");
+    require(L.stride == (L.dim + 3) / 4 * 4 && L.entry < L.n, "Index seems to be corrupted or unsupported");
+}
+
+GpuIndex* finish_export(HostLayout&& L, int device) {
+    check_layout(L);
+    int ndev = 0;
+    hip_check(hipGetDeviceCount(&ndev), "hipGetDeviceCount");
+    require(device >= 0 && device < ndev, "invalid device");
+    DeviceGuard dg(device);
+    std::unique_ptr<GpuIndex> gi(new GpuIndex());
+    gi->device = device;
+    gi->serial = ++g_export_serial;
+    hipDeviceProp_t prop;
+    hip_check(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
+    gi->cus = prop.multiProcessorCount;
+    auto upload = [&](DevBuf& b, const void* src, size_t bytes) {
+        b.reserve(std::max<size_t>(bytes, 16));
+        hip_check(hipMemcpy(b.p, src, bytes, hipMemcpyHostToDevice), "hipMemcpy H2D");
+        gi->bytes += bytes;
+    };
+    upload(gi->vec, L.vec.data(), L.vec.size() * 4);
+    upload(gi->links0, L.l0.data(), L.l0.size() * 4);
+    upload(gi->upper_off, L.uoff.data(), L.uoff.size() * 4);
+    upload(gi->upper, L.up.data(), L.up.size() * 4);
+    upload(gi->labels, L.labels.data(), L.labels.size() * 8);
+    upload(gi->delbits, L.del.data(), L.del.size() * 4);
+    for (uint32_t w : L.del) gi->num_deleted += (size_t)__builtin_popcount(w);
+    for (auto& lw : gi->ring) lw.words.reserve(kWords * 4);
+    gi->host_labels = std::move(L.labels);
+    gi->urows = L.urows;
+    hip_check(hipStreamCreateWithFlags(&gi->stream, hipStreamNonBlocking), "hipStreamCreate");
+    hg::DevGraph& g = gi->g;
+    g.vec = (const float*)gi->vec.p;
+    g.links0 = (const uint32_t*)gi->links0.p;
+    g.upper_off = (const uint32_t*)gi->upper_off.p;
+    g.upper = (const uint32_t*)gi->upper.p;
+    g.labels = (const uint64_t*)gi->labels.p;
+    g.n = (uint32_t)L.n;
+    g.dim = (uint32_t)L.dim;
+    g.stride = (uint32_t)L.stride;
+    g.maxM = (uint32_t)L.maxM;
+    g.maxM0 = (uint32_t)L.maxM0;
+    g.maxlevel = L.maxlevel;
+    g.entry = L.entry;
+    g.space = L.space;
+    return gi.release();
+}
+
+// Layout cache file: magic, version, the layout's sizes, its arrays, then an FNV-1a 64-bit hash
+// of everything before it.
+constexpr uint32_t kLayoutMagic = 0x4C474748u;  // "HGGL"
+constexpr uint32_t kLayoutVersion = 1;
+
+struct Fnv {
+    uint64_t h = 1469598103934665603ull;
+    void add(const void* p, size_t n) {
+        const unsigned char* c = (const unsigned char*)p;
+        for (size_t i = 0; i < n; i++) h = (h ^ c[i]) * 1099511628211ull;
+    }
+};
+}  // namespace
+
 HG_API int hg_gpu_index_create(void* hv, int device, void** out) {
     return guarded([&] {
         require(hv && out, "null argument");
         auto* h = static_cast<hnswgpu::HnswIndex*>(hv);
-        size_t n = h->count();
-        require(n > 0, "the index is empty");
-        require(n < (1ull << 30), "more than 2^30 elements are not supported by the GPU search");
-        int ndev = 0;
-        hip_check(hipGetDeviceCount(&ndev), "hipGetDeviceCount");
-        require(device >= 0 && device < ndev, "invalid device");
-        DeviceGuard dg(device);
-        std::unique_ptr<GpuIndex> gi(new GpuIndex());
-        gi->device = device;
-        gi->serial = ++g_export_serial;
-        hipDeviceProp_t prop;
-        hip_check(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
-        gi->cus = prop.multiProcessorCount;
-
-        const size_t dim = h->dim(), stride = (dim + 3) / 4 * 4, maxM = h->maxM(), maxM0 = h->maxM0();
-        // one wave holds a whole neighbour list: 2M <= 64 (M <= 32)
-        require(maxM0 <= 64 && maxM <= 64, "M above 32 is not supported by the GPU search");
-        require(dim <= 4096, "dim above 4096 is not supported by the GPU search");
+        HostLayout L;
+        L.n = h->count();
+        L.dim = h->dim();
+        L.stride = (L.dim + 3) / 4 * 4;
+        L.maxM = h->maxM();
+        L.maxM0 = h->maxM0();
+        L.maxlevel = h->maxlevel();
+        L.entry = h->entry();
+        L.space = h->space();
+        check_layout(L);
+        const size_t n = L.n, dim = L.dim, stride = L.stride, maxM = L.maxM, maxM0 = L.maxM0;
         // Export (integrity checked as in checkIntegrity, hnswalg.h:1380-1409: ids in range).
-        std::vector<float> vec(n * stride, 0.f);
-        std::vector<uint32_t> l0(n * maxM0, hg::SENT);
-        std::vector<uint32_t> uoff(n, hg::SENT);
-        std::vector<uint64_t> labels(n);
+        L.vec.assign(n * stride, 0.f);
+        L.l0.assign(n * maxM0, hg::SENT);
+        L.uoff.assign(n, hg::SENT);
+        L.labels.resize(n);
+        L.del.assign((n + 31) / 32, 0u);
         size_t urows = 0;
         for (size_t i = 0; i < n; i++) {
             if (h->element_level((uint32_t)i) > 0) {
-                uoff[i] = (uint32_t)urows;
+                L.uoff[i] = (uint32_t)urows;
                 urows += (size_t)h->element_level((uint32_t)i);
             }
         }
-        std::vector<uint32_t> up(std::max<size_t>(urows, 1) * maxM, hg::SENT);
+        L.urows = urows;
+        L.up.assign(std::max<size_t>(urows, 1) * maxM, hg::SENT);
         for (size_t i = 0; i < n; i++) {
-            memcpy(&vec[i * stride], h->vector((uint32_t)i), dim * 4);
-            labels[i] = h->label((uint32_t)i);
+            memcpy(&L.vec[i * stride], h->vector((uint32_t)i), dim * 4);
+            L.labels[i] = h->label((uint32_t)i);
+            if (h->is_deleted((uint32_t)i)) L.del[i >> 5] |= 1u << (i & 31);
             const uint32_t* ll = h->linklist0((uint32_t)i);
             size_t c = hnswgpu::HnswIndex::list_count(ll);
             if (c > maxM0) throw Error(HG_ERR_RUNTIME, "Bad value of sz_link_list_other");
             for (size_t j = 0; j < c; j++) {
                 if (ll[1 + j] >= n) throw Error(HG_ERR_RUNTIME, "cand error");
-                l0[i * maxM0 + j] = ll[1 + j];
+                L.l0[i * maxM0 + j] = ll[1 + j];
             }
             for (int lv = 1; lv <= h->element_level((uint32_t)i); lv++) {
                 const uint32_t* lu = h->linklist((uint32_t)i, lv);
@@ -457,48 +535,119 @@ HG_API int hg_gpu_index_create(void* hv, int device, void** out) {
                     if (lu[1 + j] >= n) throw Error(HG_ERR_RUNTIME, "cand error");
                     if (h->element_level(lu[1 + j]) < lv)
                         throw Error(HG_ERR_RUNTIME, "Trying to make a link on a non-existent level");
-                    up[(uoff[i] + (size_t)(lv - 1)) * maxM + j] = lu[1 + j];
+                    L.up[(L.uoff[i] + (size_t)(lv - 1)) * maxM + j] = lu[1 + j];
                 }
             }
         }
         require(h->element_level(h->entry()) >= h->maxlevel(), "entry point level mismatch");
-        auto upload = [&](DevBuf& b, const void* src, size_t bytes) {
-            b.reserve(std::max<size_t>(bytes, 16));
-            hip_check(hipMemcpy(b.p, src, bytes, hipMemcpyHostToDevice), "hipMemcpy H2D");
-            gi->bytes += bytes;
+        *out = finish_export(std::move(L), device);
+        return HG_OK;
+    });
+}
+
+// Saves the exported layout (the device arrays as they are, delete marks included).
+HG_API int hg_gpu_index_save(void* gv, const char* path) {
+    return guarded([&] {
+        require(gv && path, "null argument");
+        auto* gi = static_cast<GpuIndex*>(gv);
+        std::lock_guard<std::mutex> lk(gi->mu);
+        DeviceGuard dg(gi->device);
+        const hg::DevGraph& g = gi->g;
+        const size_t n = g.n;
+        HostLayout L;
+        L.vec.resize(n * g.stride);
+        L.l0.resize(n * g.maxM0);
+        L.uoff.resize(n);
+        L.up.resize(std::max<size_t>(gi->urows, 1) * g.maxM);
+        L.del.resize((n + 31) / 32);
+        hip_check(hipMemcpy(L.vec.data(), gi->vec.p, L.vec.size() * 4, hipMemcpyDeviceToHost), "D2H");
+        hip_check(hipMemcpy(L.l0.data(), gi->links0.p, L.l0.size() * 4, hipMemcpyDeviceToHost), "D2H");
+        hip_check(hipMemcpy(L.uoff.data(), gi->upper_off.p, L.uoff.size() * 4, hipMemcpyDeviceToHost), "D2H");
+        hip_check(hipMemcpy(L.up.data(), gi->upper.p, L.up.size() * 4, hipMemcpyDeviceToHost), "D2H");
+        hip_check(hipMemcpy(L.del.data(), gi->delbits.p, L.del.size() * 4, hipMemcpyDeviceToHost), "D2H");
+        std::ofstream f(path, std::ios::binary);
+        require((bool)f, std::string("Cannot open file ") + path);
+        Fnv fnv;
+        auto put = [&](const void* p, size_t bytes) {
+            f.write((const char*)p, (std::streamsize)bytes);
+            fnv.add(p, bytes);
         };
-        upload(gi->vec, vec.data(), vec.size() * 4);
-        upload(gi->links0, l0.data(), l0.size() * 4);
-        upload(gi->upper_off, uoff.data(), uoff.size() * 4);
-        upload(gi->upper, up.data(), up.size() * 4);
-        upload(gi->labels, labels.data(), labels.size() * 8);
-        {
-            std::vector<uint32_t> del((n + 31) / 32, 0u);
-            for (size_t i = 0; i < n; i++)
-                if (h->is_deleted((uint32_t)i)) {
-                    del[i >> 5] |= 1u << (i & 31);
-                    gi->num_deleted++;
-                }
-            upload(gi->delbits, del.data(), del.size() * 4);
-        }
-        for (auto& lw : gi->ring) lw.words.reserve(kWords * 4);
-        gi->host_labels = std::move(labels);
-        hip_check(hipStreamCreateWithFlags(&gi->stream, hipStreamNonBlocking), "hipStreamCreate");
-        hg::DevGraph& g = gi->g;
-        g.vec = (const float*)gi->vec.p;
-        g.links0 = (const uint32_t*)gi->links0.p;
-        g.upper_off = (const uint32_t*)gi->upper_off.p;
-        g.upper = (const uint32_t*)gi->upper.p;
-        g.labels = (const uint64_t*)gi->labels.p;
-        g.n = (uint32_t)n;
-        g.dim = (uint32_t)dim;
-        g.stride = (uint32_t)stride;
-        g.maxM = (uint32_t)maxM;
-        g.maxM0 = (uint32_t)maxM0;
-        g.maxlevel = h->maxlevel();
-        g.entry = h->entry();
-        g.space = h->space();
-        *out = gi.release();
+        const uint32_t magic = kLayoutMagic, version = kLayoutVersion;
+        const uint64_t hdr[7] = {n, g.dim, g.stride, g.maxM, g.maxM0, gi->urows, 0};
+        const int32_t hdr2[3] = {g.maxlevel, (int32_t)g.entry, g.space};
+        put(&magic, 4);
+        put(&version, 4);
+        put(hdr, sizeof(hdr));
+        put(hdr2, sizeof(hdr2));
+        put(L.vec.data(), L.vec.size() * 4);
+        put(L.l0.data(), L.l0.size() * 4);
+        put(L.uoff.data(), L.uoff.size() * 4);
+        put(L.up.data(), L.up.size() * 4);
+        put(gi->host_labels.data(), n * 8);
+        put(L.del.data(), L.del.size() * 4);
+        f.write((const char*)&fnv.h, 8);
+        require((bool)f, "write failed");
+        return HG_OK;
+    });
+}
+
+// Loads a saved layout straight to the device (no CPU index).
+HG_API int hg_gpu_index_load(const char* path, int device, void** out) {
+    return guarded([&] {
+        require(path && out, "null argument");
+        std::ifstream f(path, std::ios::binary);
+        require((bool)f, std::string("Cannot open file ") + path);
+        Fnv fnv;
+        auto get = [&](void* p, size_t bytes) {
+            f.read((char*)p, (std::streamsize)bytes);
+            require((bool)f, "Index seems to be corrupted or unsupported");
+            fnv.add(p, bytes);
+        };
+        uint32_t magic = 0, version = 0;
+        get(&magic, 4);
+        get(&version, 4);
+        require(magic == kLayoutMagic, "not a GPU layout file");
+        require(version == kLayoutVersion, "GPU layout file of another version: export the index again");
+        uint64_t hdr[7];
+        int32_t hdr2[3];
+        get(hdr, sizeof(hdr));
+        get(hdr2, sizeof(hdr2));
+        HostLayout L;
+        L.n = hdr[0];
+        L.dim = hdr[1];
+        L.stride = hdr[2];
+        L.maxM = hdr[3];
+        L.maxM0 = hdr[4];
+        L.urows = hdr[5];
+        L.maxlevel = hdr2[0];
+        L.entry = (uint32_t)hdr2[1];
+        L.space = hdr2[2];
+        check_layout(L);
+        require(L.urows <= L.n * 64 && (L.space == 0 || L.space == 1), "Index seems to be corrupted or unsupported");
+        L.vec.resize(L.n * L.stride);
+        L.l0.resize(L.n * L.maxM0);
+        L.uoff.resize(L.n);
+        L.up.resize(std::max<size_t>(L.urows, 1) * L.maxM);
+        L.labels.resize(L.n);
+        L.del.resize((L.n + 31) / 32);
+        get(L.vec.data(), L.vec.size() * 4);
+        get(L.l0.data(), L.l0.size() * 4);
+        get(L.uoff.data(), L.uoff.size() * 4);
+        get(L.up.data(), L.up.size() * 4);
+        get(L.labels.data(), L.labels.size() * 8);
+        get(L.del.data(), L.del.size() * 4);
+        uint64_t stored = 0;
+        f.read((char*)&stored, 8);
+        require((bool)f && stored == fnv.h, "Index seems to be corrupted or unsupported");
+        // the kernels index with these unchecked: ids in range, upper rows only of nodes that
+        // have some, and maxlevel rows of padding so that no level can read past the array
+        for (uint32_t v : L.l0) require(v == hg::SENT || v < L.n, "cand error");
+        for (uint32_t v : L.uoff) require(v == hg::SENT || v < L.urows, "Index seems to be corrupted or unsupported");
+        for (uint32_t v : L.up) require(v == hg::SENT || (v < L.n && L.uoff[v] != hg::SENT), "cand error");
+        require(L.maxlevel >= 0 && L.maxlevel < 64, "Index seems to be corrupted or unsupported");
+        require(L.maxlevel == 0 || L.uoff[L.entry] != hg::SENT, "entry point level mismatch");
+        L.up.resize((std::max<size_t>(L.urows, 1) + (size_t)L.maxlevel) * L.maxM, hg::SENT);
+        *out = finish_export(std::move(L), device);
         return HG_OK;
     });
 }

@@ -15,6 +15,7 @@
  *   hg_hnsw_save / _load  saveIndex / loadIndex  hnswalg.h:685-821
  *   hg_hnsw_serialize     the same byte stream as saveIndex, into memory
  *   hg_gpu_index_create   (new) export of data_level0_memory_ / linkLists_ into the GPU layout
+ *   hg_gpu_index_save / _load  (new) versioned cache of that layout
  *   hg_gpu_search         Index.knn_query bindings.cpp:612-697 over
  *                         HierarchicalNSW::searchKnn hnswalg.h:1269-1323 (bare-bone branch;
  *                         the non-bare-bone branch when the index holds deleted elements)
@@ -103,6 +104,12 @@ int hg_normalize_rows(const float* in, float* out, size_t rows, size_t dim);
 
 /* ---- GPU copy of an HNSW index ---- */
 int hg_gpu_index_create(void* index, int device, void** out_gpu_index);
+/* Versioned cache of the GPU layout (no counterpart in the reference: the exported arrays, delete
+ * marks included, with a format version and a checksum), so that a process can put an index on
+ * the device without the CPU index: hg_gpu_index_save writes it, hg_gpu_index_load checks the
+ * version, checksum and ids and uploads it.  A file of another version is refused. */
+int hg_gpu_index_save(void* gpu_index, const char* path);
+int hg_gpu_index_load(const char* path, int device, void** out_gpu_index);
 int hg_gpu_index_free(void* gpu_index);
 /* Bytes resident on the device for this index. */
 int hg_gpu_index_bytes(void* gpu_index, size_t* bytes);
