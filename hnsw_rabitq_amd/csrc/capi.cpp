@@ -1029,11 +1029,13 @@ HG_API int hg_gpu_autotune(void* gv, const float* dq, size_t nq, size_t ef, size
         float best = 1e30f;
         std::pair<uint32_t, int> best_cfg(0, 0);
         gi->tuned.erase(cap);
-        for (uint32_t bits = 9; bits <= 13; bits++) {
-            for (int per_cu : {4, 8, 12, 16, 24, 32}) {
+        for (uint32_t bits = 8; bits <= 13; bits++) {
+            int last_blocks = -1;
+            for (int per_cu : {4, 6, 8, 9, 10, 12, 16, 24, 32}) {
                 gi->tuned[cap] = std::make_pair(bits, per_cu);
                 Plan pl = make_plan(gi, ef, k, nq);
-                if (pl.blocks < std::min<size_t>(nq, (size_t)gi->cus * per_cu) && per_cu > 4) continue;
+                if (pl.blocks == last_blocks) continue;  // capped by occupancy: already timed
+                last_blocks = pl.blocks;
                 float ms_sum = 0;
                 for (int rep = 0; rep < 2; rep++) {
                     hip_check(hipEventRecord(e0, s), "hipEventRecord");
@@ -1067,7 +1069,7 @@ HG_API int hg_gpu_set_plan(void* gv, size_t ef, size_t k, uint32_t hash_bits, in
         require(gv != nullptr, "null index");
         auto* gi = static_cast<GpuIndex*>(gv);
         require(k >= 1 && ef >= 1, "ef and k must be positive");
-        require(hash_bits >= 9 && hash_bits <= 16, "hash_bits must be in [9, 16]");
+        require(hash_bits >= 8 && hash_bits <= 16, "hash_bits must be in [8, 16]");
         require(workgroups_per_cu >= 1 && workgroups_per_cu <= 32, "workgroups_per_cu must be in [1, 32]");
         std::lock_guard<std::mutex> lk(gi->mu);
         size_t ef_eff = std::max(ef, k);

@@ -201,6 +201,12 @@ __global__ __launch_bounds__(64) void hnsw_search_kernel(SearchParams P) {
     float* hbuf = feat + 16;
 
     __shared__ uint32_t s_slot;
+    // tables of 2^14 slots or more are sized to forget nothing (the distinct-evaluation count):
+    // they probe; smaller ones take one exchange per neighbour
+    const bool probe = TRACE || P.hash_bits >= 14;
+    auto visit = [&](uint32_t* h, uint32_t m, uint32_t id) {
+        return probe ? hash_insert<true>(h, m, id) : hash_insert<false>(h, m, id);
+    };
     MlpRegs mw;
     if (STOP) mlp_load(P.stop, lane, mw);
 
@@ -297,7 +303,7 @@ __global__ __launch_bounds__(64) void hnsw_search_kernel(SearchParams P) {
         if (lane == 0) {
             bd[0] = curdist;
             bi[0] = cur | ((FILT && nallowed == 0) ? DFLAG : 0u);
-            hash_insert(hash, hmask, cur);
+            visit(hash, hmask, cur);
         }
         E0 = 1;
         __syncthreads();
@@ -368,7 +374,7 @@ __global__ __launch_bounds__(64) void hnsw_search_kernel(SearchParams P) {
                 pf_row = v;
                 pf_node = node2;
                 const uint32_t id = (!upper && lr < (int)g.maxM0) ? cur : SENT;
-                const bool fresh = id != SENT && hash_insert(hash, hmask, id);
+                const bool fresh = id != SENT && visit(hash, hmask, id);
                 const uint64_t m = ballot(fresh);
                 if (fresh) si[lanes_below(m, lane)] = id;
                 cnt = __popcll(m);
@@ -377,7 +383,7 @@ __global__ __launch_bounds__(64) void hnsw_search_kernel(SearchParams P) {
                 for (uint32_t b = 0; b < g.maxM0; b += 64) {
                     uint32_t j = b + lane;
                     uint32_t id = j < g.maxM0 ? row[j] : SENT;
-                    bool fresh = id != SENT && hash_insert(hash, hmask, id);
+                    bool fresh = id != SENT && visit(hash, hmask, id);
                     uint64_t m = ballot(fresh);
                     if (fresh) si[cnt + lanes_below(m, lane)] = id;
                     cnt += __popcll(m);

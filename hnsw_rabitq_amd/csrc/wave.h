@@ -53,17 +53,24 @@ __device__ __forceinline__ int lanes_below(uint64_t m, int lane) {
     return __popcll(m & ((lane == 0) ? 0ull : (~0ull >> (64 - lane))));
 }
 
-// Visited hash: returns true when id was not present (and inserts it).
+// Visited hash.  A forgotten node costs one re-evaluation (its key is then either in the beam,
+// where the merge drops it, or no better than a bound that only tightens), never a different
+// result.  PROBE = false: one slot per id, the id replaces whatever the slot held and counts as
+// fresh unless the slot held it already (one LDS round trip).  PROBE = true: up to four probes
+// from an empty-slot insert before an overwrite, so a table larger than the visited set forgets
+// nothing (the distinct-evaluation count of the traversal).
+template <bool PROBE = false>
 __device__ __forceinline__ bool hash_insert(uint32_t* hash, uint32_t mask, uint32_t id) {
-    uint32_t h = (id * 2654435761u) & mask;
+    const uint32_t h = (id * 2654435761u) & mask;
+    if (!PROBE) return atomicExch(&hash[h], id) != id;
 #pragma unroll 1
     for (int p = 0; p < 4; p++) {
-        uint32_t slot = (h + p) & mask;
-        uint32_t old = atomicCAS(&hash[slot], SENT, id);
+        const uint32_t slot = (h + p) & mask;
+        const uint32_t old = atomicCAS(&hash[slot], SENT, id);
         if (old == SENT) return true;
         if (old == id) return false;
     }
-    hash[h] = id;  // window full: overwrite (may cause one extra evaluation later)
+    hash[h] = id;  // window full: overwrite
     return true;
 }
 
