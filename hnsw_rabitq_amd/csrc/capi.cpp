@@ -113,9 +113,9 @@ Plan make_plan(GpuIndex* gi, size_t max_ef, size_t max_k, size_t nq, bool filter
     } else if (it != gi->tuned.end()) {
         p.hash_bits = it->second.first;
     } else {
-        // visited nodes grow roughly as 16-32 per beam entry at M=16; the hash may be lossy
-        // (an overwritten slot only costs a re-evaluation), so it is sized for occupancy
-        p.hash_bits = std::min<uint32_t>(12, std::max<uint32_t>(10, ceil_log2(8 * ef_eff)));
+        // the hash is lossy (an overwritten slot only costs a re-evaluation, in the same distance
+        // round), so it is sized for occupancy: about one slot per four beam entries
+        p.hash_bits = std::min<uint32_t>(10, std::max<uint32_t>(8, ceil_log2((ef_eff + 3) / 4)));
     }
     if (filtered) {
         // a CU holds 160 KiB of LDS: shrink the (lossy) hash, then the beam, to fit
