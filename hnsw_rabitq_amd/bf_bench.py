@@ -70,13 +70,87 @@ def run(H, lib, n, nq, stream, timer, log, dim=768, k=100, c=4096):
         log(f"  {name}: GEMM {stats[0]:.2f} ms = {gemm_tf:.0f} TFLOP/s ({100 * gemm_tf / peak:.1f}% of peak); "
             f"prep {stats[1]:.2f} ms, re-rank {stats[2]:.2f} ms, end-to-end {e2e:.2f} ms; recall@100 {rec:.4f}; "
             f"rows identical to exact {same:.4f}; fp64 near-tie pairs {chk[0]:.0f}, disagreements {chk[1]:.0f}")
+    out.update(library_gemm(n, nq, dim, log))
+    return out
+
+
+def library_gemm(n, nq, dim, log):
+    """The vendor GEMM on the same Q x N x D shape (`library_gemm_run`), in a child process:
+    torch carries its own HIP runtime, which cannot share a process with this library's."""
+    import json
+    import subprocess
+    import sys
+    try:
+        r = subprocess.run([sys.executable, "-m", "hnsw_rabitq_amd.bf_bench", "--library-gemm", "--n", str(n),
+                            "--nq", str(nq), "--dim", str(dim)], capture_output=True, text=True, timeout=600)
+    except subprocess.TimeoutExpired:
+        log("  library GEMM: child process timed out")
+        return {}
+    for line in r.stderr.splitlines():
+        if "library" in line:
+            log(line.split("] ", 1)[-1])
+    if r.returncode != 0 or not r.stdout.strip():
+        log(f"  library GEMM: child process exited with {r.returncode}")
+        return {}
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
+def library_gemm_run(n, nq, dim, log, chunk=1 << 17):
+    """The vendor GEMM (torch.matmul -> hipBLASLt; torch._scaled_mm for FP8 e4m3) on the same
+    Q x N x D shape, random operands, low-precision output written in row chunks of `chunk`: a
+    plain GEMM with no selection, reported beside the distance GEMMs as the library's level on
+    this shape.  Skipped when torch is not importable."""
+    try:
+        import torch
+    except ImportError:
+        return {}
+    dev = "cuda:0"
+    g = torch.Generator(device=dev).manual_seed(0)
+    qb = torch.randn(nq, dim, device=dev, generator=g).to(torch.bfloat16)
+    xb = torch.randn(n, dim, device=dev, generator=g).to(torch.bfloat16)
+    res = {w: torch.empty(nq, w, device=dev, dtype=torch.bfloat16) for w in {min(chunk, n), n % chunk or chunk}}
+    one = torch.ones((), device=dev)
+    q8, x8 = qb.to(torch.float8_e4m3fn), xb.to(torch.float8_e4m3fn)
+
+    def bf16():
+        for c in range(0, n, chunk):
+            w = min(chunk, n - c)
+            torch.matmul(qb, xb[c:c + w].t(), out=res[w])
+
+    def fp8():
+        for c in range(0, n, chunk):
+            w = min(chunk, n - c)
+            torch._scaled_mm(q8, x8[c:c + w].t(), scale_a=one, scale_b=one, out_dtype=torch.bfloat16, out=res[w])
+
+    out = {}
+    flops = 2.0 * nq * n * dim
+    for name, fn, peak in (("bf16", bf16, 2500.0), ("fp8", fp8, 5000.0)):
+        try:
+            fn()
+            ev0, ev1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            ev0.record()
+            for _ in range(3):
+                fn()
+            ev1.record()
+            torch.cuda.synchronize()
+            ms = ev0.elapsed_time(ev1) / 3
+        except RuntimeError as e:  # e.g. no FP8 GEMM for this device in the installed library
+            log(f"  library {name} GEMM unavailable: {str(e)[:120]}")
+            continue
+        tf = flops / (ms * 1e-3) / 1e12
+        out[f"bf_library_{name}_gemm_ms"] = ms
+        out[f"bf_library_{name}_gemm_frac_peak"] = tf / peak
+        log(f"  library {name} GEMM (plain, bf16 output, no selection): {ms:.2f} ms = {tf:.0f} TFLOP/s "
+            f"({100 * tf / peak:.1f}% of peak)")
+    del qb, xb, res, q8, x8
+    torch.cuda.empty_cache()
     return out
 
 
 def main():
     """Standalone run of the brute-force measurement (one JSON line on stdout):
 
-        python -m hnsw_rabitq_amd.bf_bench [--n 1000000] [--nq 10000]
+        python -m hnsw_rabitq_amd.bf_bench [--n 1000000] [--nq 10000] [--dim 768] [--library-gemm]
     """
     import argparse
     import json
@@ -89,11 +163,17 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--n", type=int, default=1_000_000)
     ap.add_argument("--nq", type=int, default=10_000)
+    ap.add_argument("--dim", type=int, default=768)
+    ap.add_argument("--library-gemm", action="store_true", help="only time the vendor GEMM on this shape")
     args = ap.parse_args()
     t0 = time.time()
 
     def log(m):
         print(f"[bf {time.time() - t0:7.1f}s] {m}", file=sys.stderr, flush=True)
+
+    if args.library_gemm:
+        print(json.dumps(library_gemm_run(args.n, args.nq, args.dim, log)), flush=True)
+        return
 
     class Timer:
         def __init__(self, stream):
@@ -111,7 +191,7 @@ def main():
 
     stream = ctypes.c_void_p()
     _lib.call("hg_stream_create", ctypes.byref(stream))
-    out = run(H, _lib, n=args.n, nq=args.nq, stream=stream, timer=Timer(stream), log=log)
+    out = run(H, _lib, n=args.n, nq=args.nq, stream=stream, timer=Timer(stream), log=log, dim=args.dim)
     print(json.dumps(out), flush=True)
 
 

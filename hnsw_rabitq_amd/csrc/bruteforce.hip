@@ -387,13 +387,48 @@ struct LowpGemmArgs {
     uint32_t tiles_m, tiles_n;
 };
 
-// Per-row values of one workgroup tile, staged in LDS for the epilogue.
-template <int BM, int WN>
+// Per-row and per-column values of one workgroup tile, staged in LDS for the epilogue.
+template <int BM, int BN, int WN>
 struct __align__(16) EpiLds {
     float tau[BM], a_norm[BM], a_scale[BM];
+    float bsc[BN], bn[BN];
     uint32_t cnt[WN][BM];  // passes of each row in the columns of wave wn
     uint32_t base[BM];     // start of the row's reserved candidate slots
 };
+
+// Stage the tile's per-row and per-column epilogue inputs in LDS.  Called before the main loop:
+// the loads complete with the first k-slices, so their latency hides behind the fill, and no
+// register holds them across the loop.  The main loop's barriers order these writes before the
+// epilogue's reads.
+template <int PREC, bool FILTER, int BM, int BN, int WN, int NT>
+__device__ __forceinline__ void epi_stage(const LowpGemmArgs& g, uint32_t m0, uint32_t n0, int tid,
+                                          EpiLds<BM, BN, WN>& e) {
+    for (int r = tid; r < BM; r += NT) {
+        const uint32_t row = m0 + r;
+        const bool ok = row < g.nq;
+        e.tau[r] = (FILTER && ok) ? g.tau[row] : -INFINITY;
+        e.a_norm[r] = (g.space == 0 && ok) ? g.a_norm[row] : 0.f;
+        e.a_scale[r] = (PREC == 2 && ok) ? g.a_scale[row] : 1.f;
+    }
+    for (int c = tid; c < BN; c += NT) {
+        const uint32_t col = n0 + c;
+        const bool ok = col < g.n;
+        e.bsc[c] = (PREC == 2 && ok) ? g.b_scale[col] : 1.0f;
+        e.bn[c] = (g.space == 0 && ok) ? g.b_norm[col] : 0.f;
+    }
+}
+
+// Four per-row values of one 32x32 block held by a half-wave: registers 4q..4q+3 of the
+// accumulator belong to rows lb + 8q + 0..3, where lb includes the half-wave's offset of 4.
+template <typename T>
+__device__ __forceinline__ void rows4(const T* a, int lb, int q, T (&v)[4]) {
+    typedef T T4 __attribute__((ext_vector_type(4)));
+    const T4 t = *reinterpret_cast<const T4*>(a + lb + q * 8);
+    v[0] = t.x;
+    v[1] = t.y;
+    v[2] = t.z;
+    v[3] = t.w;
+}
 
 // Epilogue of the GEMM variants.  The workgroup tile is rows [m0, m0 + 64*TM) x columns
 // [n0, n0 + 32*TN*WN); wave (wm, wn) holds TM x TN 32x32 blocks in acc, so each of its 32*TM
@@ -402,49 +437,57 @@ struct __align__(16) EpiLds {
 // d < tau[row] to the row's candidate list with one device-scope atomic per (row, workgroup):
 // pass 1 counts each row's passes (ballot + popcount), one thread per row reserves the row's
 // total, pass 2 recomputes d and writes each pass at the row base + the counts of the waves
-// before it + its rank among the row's passes.  NT threads; two waves along the rows.  SPACE
-// is fixed at compile time so that the unrolled loops carry no branch.
+// before it + its rank among the row's passes.  Per-row values are read from LDS sixteen at a
+// time per block (four 16-byte reads) before the block's loop, so no LDS round trip sits
+// inside it.  NT threads; two waves along the rows.  SPACE is fixed at compile time so that
+// the unrolled loops carry no branch.
 template <int PREC, bool FILTER, int TM, int TN, int WN, int NT, int SPACE>
 __device__ __forceinline__ void gemm_epilogue_s(const LowpGemmArgs& g, f32x16_t (&acc)[TM][TN], uint32_t m0,
                                                 uint32_t n0, int wm, int wn, int lane, int tid,
-                                                EpiLds<64 * TM, WN>& e) {
+                                                EpiLds<64 * TM, 32 * TN * WN, WN>& e) {
     constexpr int BM = 64 * TM;
-    for (int r = tid; r < BM; r += NT) {
-        const uint32_t row = m0 + r;
-        const bool ok = row < g.nq;
-        e.tau[r] = (FILTER && ok) ? g.tau[row] : -INFINITY;
-        e.a_norm[r] = (SPACE == 0 && ok) ? g.a_norm[row] : 0.f;
-        e.a_scale[r] = (PREC == 2 && ok) ? g.a_scale[row] : 1.f;
-    }
     __syncthreads();
-    float bsc[TN], bn[TN];
     uint32_t col[TN];
     bool col_ok[TN];
+    float bsc[TN], bn[TN];
 #pragma unroll
     for (int j = 0; j < TN; j++) {
-        col[j] = n0 + wn * 32 * TN + j * 32 + (lane & 31);
+        const int c = wn * 32 * TN + j * 32 + (lane & 31);
+        col[j] = n0 + c;
         col_ok[j] = col[j] < g.n;
-        bsc[j] = (PREC == 2 && col_ok[j]) ? g.b_scale[col[j]] : 1.0f;
-        bn[j] = (SPACE == 0 && col_ok[j]) ? g.b_norm[col[j]] : 0.f;
+        bsc[j] = PREC == 2 ? e.bsc[c] : 1.0f;
+        bn[j] = SPACE == 0 ? e.bn[c] : 0.f;
     }
-    auto dist = [&](int i, int j, int r, int lr) {
+    auto dist = [&](int i, int j, int r, float an, float as) {
         float sv = acc[i][j][r];
-        if (PREC == 2) sv = sv * e.a_scale[lr] * bsc[j];
-        if (SPACE == 0) return e.a_norm[lr] + bn[j] - 2.0f * sv;
+        if (PREC == 2) sv = sv * as * bsc[j];
+        if (SPACE == 0) return an + bn[j] - 2.0f * sv;
         return 1.0f - sv;
     };
     const int half = lane >> 5;
+    // per-row values of rows lb + 8q + 0..3 for this half-wave
+    float an[4] = {}, as[4] = {};
+    auto load_rows = [&](int lb, int q) {
+        if (SPACE == 0) rows4(e.a_norm, lb, q, an);
+        if (PREC == 2) rows4(e.a_scale, lb, q, as);
+    };
     if (!FILTER) {
 #pragma unroll
-        for (int i = 0; i < TM; i++)
+        for (int i = 0; i < TM; i++) {
+            const int lb = wm * 32 * TM + i * 32 + half * 4;
 #pragma unroll
-            for (int r = 0; r < 16; r++) {
-                const int lr = wm * 32 * TM + i * 32 + (r >> 2) * 8 + half * 4 + (r & 3);
-                const uint32_t row = m0 + lr;
+            for (int q = 0; q < 4; q++) {
+                load_rows(lb, q);
 #pragma unroll
-                for (int j = 0; j < TN; j++)
-                    if (col_ok[j] && row < g.nq) g.out_mat[(size_t)row * g.ld_out + col[j]] = dist(i, j, r, lr);
+                for (int rr = 0; rr < 4; rr++) {
+                    const uint32_t row = m0 + lb + q * 8 + rr;
+#pragma unroll
+                    for (int j = 0; j < TN; j++)
+                        if (col_ok[j] && row < g.nq)
+                            g.out_mat[(size_t)row * g.ld_out + col[j]] = dist(i, j, q * 4 + rr, an[rr], as[rr]);
+                }
             }
+        }
         return;
     }
     // pass 1: per-row pass counts of this wave (each row belongs to one half-wave, so lanes 0
@@ -454,21 +497,23 @@ __device__ __forceinline__ void gemm_epilogue_s(const LowpGemmArgs& g, f32x16_t 
 #pragma unroll
     for (int j = 0; j < TN; j++) colm[j] = __builtin_amdgcn_ballot_w64(col_ok[j]);
 #pragma unroll
-    for (int i = 0; i < TM; i++)
+    for (int i = 0; i < TM; i++) {
+        const int lb = wm * 32 * TM + i * 32 + half * 4;
 #pragma unroll
-        for (int rq = 0; rq < 4; rq++) {
+        for (int q = 0; q < 4; q++) {
             // rows lr0 + rr (half-wave 0) and lr0 + 4 + rr (half-wave 1), rr = 0..3
-            const int lr0 = wm * 32 * TM + i * 32 + rq * 8;
-            const float4 t4 = *reinterpret_cast<const float4*>(&e.tau[lr0 + half * 4]);
-            const float tq[4] = {t4.x, t4.y, t4.z, t4.w};
+            const int lr0 = wm * 32 * TM + i * 32 + q * 8;
+            float tq[4];
+            rows4(e.tau, lb, q, tq);
+            load_rows(lb, q);
             uint32_t clo[4], chi[4];
 #pragma unroll
             for (int rr = 0; rr < 4; rr++) {
-                const int r = rq * 4 + rr, lr = lr0 + half * 4 + rr;
+                const int r = q * 4 + rr;
                 uint32_t lo = 0, hi = 0;
 #pragma unroll
                 for (int j = 0; j < TN; j++) {
-                    const uint64_t m = __builtin_amdgcn_ballot_w64(dist(i, j, r, lr) < tq[rr]) & colm[j];
+                    const uint64_t m = __builtin_amdgcn_ballot_w64(dist(i, j, r, an[rr], as[rr]) < tq[rr]) & colm[j];
                     lo += (uint32_t)__popcll(m & 0xFFFFFFFFull);
                     hi += (uint32_t)__popcll(m >> 32);
                 }
@@ -481,6 +526,7 @@ __device__ __forceinline__ void gemm_epilogue_s(const LowpGemmArgs& g, f32x16_t 
                 *reinterpret_cast<uint4*>(&e.cnt[wn][lr0 + 4]) = make_uint4(chi[0], chi[1], chi[2], chi[3]);
             }
         }
+    }
     __syncthreads();
     // one reservation per row
     for (int r = tid; r < BM; r += NT) {
@@ -490,46 +536,63 @@ __device__ __forceinline__ void gemm_epilogue_s(const LowpGemmArgs& g, f32x16_t 
         if (c) e.base[r] = atomicAdd(&g.cand_cnt[m0 + r], c);
     }
     __syncthreads();
-    // pass 2: write the candidates into the reserved slots
+    // pass 2: write the candidates into the reserved slots; the per-row values of a group of
+    // four pairs load together before the group
     const uint32_t below = (lane & 31) ? (1u << (lane & 31)) - 1u : 0u;
 #pragma unroll
-    for (int i = 0; i < TM; i++)
+    for (int i = 0; i < TM; i++) {
+        const int lb = wm * 32 * TM + i * 32 + half * 4;
 #pragma unroll
-        for (int r = 0; r < 16; r++) {
-            if (!((hit >> (i * 16 + r)) & 1ull)) continue;  // no pass in this pair: scalar skip
-            const int lr = wm * 32 * TM + i * 32 + (r >> 2) * 8 + half * 4 + (r & 3);
-            const uint32_t row = m0 + lr;
-            const float t = e.tau[lr];
-            float d[TN];
-            bool pass[TN];
-            uint32_t hm[TN];
+        for (int q = 0; q < 4; q++) {
+            if (!((hit >> (i * 16 + q * 4)) & 0xFull)) continue;  // no pass in this group: scalar skip
+            float tq[4];
+            uint32_t oq[4];  // row base + the passes of the waves before this one
+            rows4(e.tau, lb, q, tq);
+            rows4(e.base, lb, q, oq);
 #pragma unroll
-            for (int j = 0; j < TN; j++) {
-                d[j] = dist(i, j, r, lr);
-                pass[j] = (int)col_ok[j] & (int)(d[j] < t);
-                hm[j] = (uint32_t)(__ballot(pass[j]) >> (32 * half));
-            }
-            uint32_t off = e.base[lr];
+            for (int w = 0; w < WN - 1; w++)
+                if (w < wn) {
+                    uint32_t cq[4];
+                    rows4(e.cnt[w], lb, q, cq);
 #pragma unroll
-            for (int w = 0; w < WN - 1; w++) off += w < wn ? e.cnt[w][lr] : 0u;
-#pragma unroll
-            for (int j = 0; j < TN; j++) {
-                if (pass[j]) {
-                    const uint32_t slot = off + (uint32_t)__popc(hm[j] & below);
-                    if (slot < g.cap) {
-                        g.cand_rows[(size_t)row * g.cap + slot] = col[j];
-                        g.cand_d[(size_t)row * g.cap + slot] = d[j];
-                    }
+                    for (int rr = 0; rr < 4; rr++) oq[rr] += cq[rr];
                 }
-                off += (uint32_t)__popc(hm[j]);
+            load_rows(lb, q);
+#pragma unroll
+            for (int rr = 0; rr < 4; rr++) {
+                const int r = q * 4 + rr;
+                if (!((hit >> (i * 16 + r)) & 1ull)) continue;  // no pass in this pair: scalar skip
+                const uint32_t row = m0 + lb + q * 8 + rr;
+                float d[TN];
+                bool pass[TN];
+                uint32_t hm[TN];
+#pragma unroll
+                for (int j = 0; j < TN; j++) {
+                    d[j] = dist(i, j, r, an[rr], as[rr]);
+                    pass[j] = (int)col_ok[j] & (int)(d[j] < tq[rr]);
+                    hm[j] = (uint32_t)(__ballot(pass[j]) >> (32 * half));
+                }
+                uint32_t off = oq[rr];
+#pragma unroll
+                for (int j = 0; j < TN; j++) {
+                    if (pass[j]) {
+                        const uint32_t slot = off + (uint32_t)__popc(hm[j] & below);
+                        if (slot < g.cap) {
+                            g.cand_rows[(size_t)row * g.cap + slot] = col[j];
+                            g.cand_d[(size_t)row * g.cap + slot] = d[j];
+                        }
+                    }
+                    off += (uint32_t)__popc(hm[j]);
+                }
             }
         }
+    }
 }
 
 template <int PREC, bool FILTER, int TM, int TN, int WN, int NT>
 __device__ __forceinline__ void gemm_epilogue(const LowpGemmArgs& g, f32x16_t (&acc)[TM][TN], uint32_t m0,
                                               uint32_t n0, int wm, int wn, int lane, int tid,
-                                              EpiLds<64 * TM, WN>& e) {
+                                              EpiLds<64 * TM, 32 * TN * WN, WN>& e) {
     if (g.space == 0)
         gemm_epilogue_s<PREC, FILTER, TM, TN, WN, NT, 0>(g, acc, m0, n0, wm, wn, lane, tid, e);
     else
@@ -545,6 +608,8 @@ __global__ __launch_bounds__(256, 2) void lowp_gemm_kernel(LowpGemmArgs g) {
     uint32_t tm, tn;
     if (!TileOrder::make(g.tiles_m, g.tiles_n).map(blockIdx.x, tm, tn)) return;
     const uint32_t m0 = tm * GM, n0 = tn * GN;
+    __shared__ EpiLds<64 * 2, 128, 2> epi;
+    epi_stage<PREC, FILTER, 128, 128, 2, 256>(g, m0, n0, tid, epi);
     const size_t row_bytes = (size_t)g.dim * (PREC == 1 ? 2 : 1);
 
     f32x16_t acc[2][2];
@@ -630,7 +695,6 @@ __global__ __launch_bounds__(256, 2) void lowp_gemm_kernel(LowpGemmArgs g) {
         }
     }
 
-    __shared__ EpiLds<64 * 2, 2> epi;
     gemm_epilogue<PREC, FILTER, 2, 2, 2, 256>(g, acc, m0, n0, wm, wn, lane, tid, epi);
 }
 
@@ -649,6 +713,8 @@ __global__ __launch_bounds__(256, 2) void lowp_gemm_wide_kernel(LowpGemmArgs g) 
     uint32_t tm, tn;
     if (!TileOrder::make((g.nq + BM - 1) / BM, (g.n + BN - 1) / BN).map(blockIdx.x, tm, tn)) return;
     const uint32_t m0 = tm * BM, n0 = tn * BN;
+    __shared__ EpiLds<BM, BN, 2> epi;
+    epi_stage<PREC, FILTER, BM, BN, 2, 256>(g, m0, n0, tid, epi);
     const size_t row_bytes = (size_t)g.dim * (PREC == 1 ? 2 : 1);
 
     f32x16_t acc[TM][TN];
@@ -738,7 +804,6 @@ __global__ __launch_bounds__(256, 2) void lowp_gemm_wide_kernel(LowpGemmArgs g) 
         }
     }
 
-    __shared__ EpiLds<64 * TM, 2> epi;
     gemm_epilogue<PREC, FILTER, TM, TN, 2, 256>(g, acc, m0, n0, wm, wn, lane, tid, epi);
 }
 
@@ -934,6 +999,8 @@ __global__ __launch_bounds__(128 * WNW, (WNW == 2 && TN == 2) ? 2 : 1) void lowp
     uint32_t tm, tn;
     if (!TileOrder::make((g.nq + BM - 1) / BM, (g.n + BN - 1) / BN).map(blockIdx.x, tm, tn)) return;
     const uint32_t m0 = tm * BM, n0 = tn * BN;
+    __shared__ EpiLds<BM, BN, WNW> epi;
+    epi_stage<PREC, FILTER, BM, BN, WNW, NT>(g, m0, n0, tid, epi);
     const size_t row_bytes = (size_t)g.dim * (PREC == 1 ? 2 : 1);
     const int nst = (int)(row_bytes / LKB);
     constexpr int PER_STAGE = APW + BPW;  // direct-to-LDS loads per wave per slice
@@ -968,7 +1035,6 @@ __global__ __launch_bounds__(128 * WNW, (WNW == 2 && TN == 2) ? 2 : 1) void lowp
         if (st + 2 < nst) step(st + 2, A2, B2, A1, B1);
     }
 
-    __shared__ EpiLds<64 * TM, WNW> epi;
     gemm_epilogue<PREC, FILTER, TM, TN, WNW, NT>(g, acc, m0, n0, wm, wn, lane, tid, epi);
 }
 
