@@ -30,7 +30,9 @@ sys.path.insert(0, ROOT)
 PEAK_HBM_GBS = 8000.0
 PEAK_BF16_TFLOPS = 2500.0
 PEAK_FP8_TFLOPS = 5000.0
-EF_SWEEP = [10, 20, 40, 80, 160, 320, 640, 960, 1280, 1600, 1920, 2560]
+# steps of 64 between 1280 and 1600, where recall@10 crosses 0.95 on this workload, so that the
+# operating point is not taken up to a coarse grid value
+EF_SWEEP = [10, 20, 40, 80, 160, 320, 640, 960, 1280, 1344, 1408, 1472, 1536, 1600, 1920, 2560]
 T0 = time.time()
 
 
@@ -306,13 +308,14 @@ def main():
     value = max(plain_qps, learned_qps)
     result["value_method"] = "learned" if learned_qps > plain_qps else "plain"
     workload = f"hnsw_l2_gamma_{args.n}x{D}_q{nq}_M16_efC200_k10"
-    traffic = None
+    traffic = traffic_plan = None
     tpath = os.path.join(ROOT, "profiles", "search_traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
             rec = json.load(f)
         if rec.get("workload") == workload and rec.get("ef") == ef_op:
             traffic = rec.get("traffic_bytes_per_launch")
+            traffic_plan = rec.get("plan")
     line = {
         "metric": "QPS@recall10=0.95, 1M×128 L2: plain vs learned-stop GPU; brute-force TFLOPS",
         "value": value,
@@ -329,7 +332,8 @@ def main():
         "config": {"workload": workload,
                    "n": args.n, "dim": D, "queries": nq, "ef_sweep": EF_SWEEP, "k": K},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
-                     "frac": achieved / PEAK_HBM_GBS, "traffic": traffic,
+                     "frac": achieved / PEAK_HBM_GBS, "traffic": traffic, "traffic_plan": traffic_plan,
+                     "plan": {"hash_slots": plan["hash_slots"], "workgroups": plan["workgroups"]},
                      "kernel": "hnsw_search_kernel", "bytes_per_launch": bytes_launch, "kernel_ms": ev_ms},
         "cpu_baseline": cpu,
         "sweep": [{k2: (round(v, 5) if isinstance(v, float) else v) for k2, v in s.items()} for s in sweep],
