@@ -119,15 +119,15 @@ Plan make_plan(GpuIndex* gi, size_t max_ef, size_t max_k, size_t nq, bool filter
     }
     if (filtered) {
         // a CU holds 160 KiB of LDS: shrink the (lossy) hash, then the beam, to fit
-        while (hg::search_lds_bytes(gi->g.dim, p.cap, p.hash_bits) > 160 * 1024 && p.hash_bits > 9)
+        while (hg::search_lds_bytes(gi->g.dim, p.cap, p.hash_bits, false) > 160 * 1024 && p.hash_bits > 9)
             p.hash_bits--;
-        while (hg::search_lds_bytes(gi->g.dim, p.cap, p.hash_bits) > 160 * 1024 && p.cap > 64) p.cap -= 64;
+        while (hg::search_lds_bytes(gi->g.dim, p.cap, p.hash_bits, false) > 160 * 1024 && p.cap > 64) p.cap -= 64;
         require(p.cap >= ef_eff, "ef and dimension too large for the filtered GPU search");
         // the largest beam that keeps the same number of workgroups per CU (a free margin)
-        const size_t per = (size_t)(160 * 1024) / hg::search_lds_bytes(gi->g.dim, p.cap, p.hash_bits);
+        const size_t per = (size_t)(160 * 1024) / hg::search_lds_bytes(gi->g.dim, p.cap, p.hash_bits, false);
         if (per >= 1) {
             const size_t budget = (size_t)(160 * 1024) / per - 512;
-            while (p.cap + 64 <= kFiltMaxCap && hg::search_lds_bytes(gi->g.dim, p.cap + 64, p.hash_bits) <= budget)
+            while (p.cap + 64 <= kFiltMaxCap && hg::search_lds_bytes(gi->g.dim, p.cap + 64, p.hash_bits, false) <= budget)
                 p.cap += 64;
         }
     }
@@ -135,7 +135,7 @@ Plan make_plan(GpuIndex* gi, size_t max_ef, size_t max_k, size_t nq, bool filter
     sp.g = gi->g;
     sp.cap = p.cap;
     sp.hash_bits = p.hash_bits;
-    p.lds = hg::search_lds_bytes(gi->g.dim, p.cap, p.hash_bits);
+    p.lds = hg::search_lds_bytes(gi->g.dim, p.cap, p.hash_bits, false);
     int per_cu = hg::search_blocks_per_cu(sp, filtered);
     if (it != gi->tuned.end() && it->second.second > 0) per_cu = std::min(per_cu, it->second.second);
     size_t blocks = (size_t)gi->cus * (size_t)per_cu;
@@ -1029,7 +1029,7 @@ HG_API int hg_gpu_autotune(void* gv, const float* dq, size_t nq, size_t ef, size
         float best = 1e30f;
         std::pair<uint32_t, int> best_cfg(0, 0);
         gi->tuned.erase(cap);
-        for (uint32_t bits = 8; bits <= 13; bits++) {
+        for (uint32_t bits = 7; bits <= 13; bits++) {
             int last_blocks = -1;
             for (int per_cu : {4, 6, 8, 9, 10, 12, 16, 24, 32}) {
                 gi->tuned[cap] = std::make_pair(bits, per_cu);
@@ -1069,12 +1069,12 @@ HG_API int hg_gpu_set_plan(void* gv, size_t ef, size_t k, uint32_t hash_bits, in
         require(gv != nullptr, "null index");
         auto* gi = static_cast<GpuIndex*>(gv);
         require(k >= 1 && ef >= 1, "ef and k must be positive");
-        require(hash_bits >= 8 && hash_bits <= 16, "hash_bits must be in [8, 16]");
+        require(hash_bits >= 7 && hash_bits <= 16, "hash_bits must be in [7, 16]");
         require(workgroups_per_cu >= 1 && workgroups_per_cu <= 32, "workgroups_per_cu must be in [1, 32]");
         std::lock_guard<std::mutex> lk(gi->mu);
         size_t ef_eff = std::max(ef, k);
         uint32_t cap = (uint32_t)((ef_eff + 63) / 64 * 64);
-        require(hg::search_lds_bytes(gi->g.dim, cap, hash_bits) <= 160 * 1024,
+        require(hg::search_lds_bytes(gi->g.dim, cap, hash_bits, true) <= 160 * 1024,
                 "beam and visited hash exceed the 160 KiB of LDS of a CU");
         gi->tuned[cap] = std::make_pair(hash_bits, workgroups_per_cu);
         return HG_OK;

@@ -197,7 +197,7 @@ __global__ __launch_bounds__(64) void hnsw_search_kernel(SearchParams P) {
     uint32_t* hash = bi + cap;
     float* sd = reinterpret_cast<float*>(hash + hsize);
     uint32_t* si = reinterpret_cast<uint32_t*>(sd + 64);
-    float* feat = reinterpret_cast<float*>(si + 64);  // kStopFeatures + 2*64 floats (STOP/TRACE)
+    float* feat = reinterpret_cast<float*>(si + 64);  // kStopFeatures + 2*64 floats, allocated for STOP/TRACE only
     float* hbuf = feat + 16;
 
     __shared__ uint32_t s_slot;
@@ -730,9 +730,10 @@ hipError_t launch_zero_u32(uint32_t* p, uint32_t n, hipStream_t s) {
 // ------------------------------------------------------------------------------------------
 // Host-side launch
 // ------------------------------------------------------------------------------------------
-size_t search_lds_bytes(uint32_t dim, uint32_t cap, uint32_t hash_bits) {
+size_t search_lds_bytes(uint32_t dim, uint32_t cap, uint32_t hash_bits, bool stop_area) {
     uint32_t dimpad = (dim + 3u) & ~3u;
-    return (size_t)dimpad * 4 + (size_t)cap * 8 + ((size_t)1 << hash_bits) * 4 + 64 * 8 + (16 + 128) * 4;
+    return (size_t)dimpad * 4 + (size_t)cap * 8 + ((size_t)1 << hash_bits) * 4 + 64 * 8 +
+           (stop_area ? (16 + 128) * 4 : 0);
 }
 
 template <int SPACE, int MODE, int NCH, bool STOP, bool TRACE, bool FILT = false>
@@ -760,11 +761,12 @@ static hipError_t launch_dim(const SearchParams& p, int blocks, size_t lds, hipS
 }
 
 hipError_t launch_search(const SearchParams& p, int blocks, hipStream_t s) {
-    size_t lds = search_lds_bytes(p.g.dim, p.cap, p.hash_bits);
+    bool stop = p.use_stop != 0, trace = p.trace_feat != nullptr;
+    const bool filt = p.del_bits || p.allow_bits || p.eps_mode;
+    size_t lds = search_lds_bytes(p.g.dim, p.cap, p.hash_bits, !filt && (stop || trace));
     uint32_t* w = p.work;
     hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, s, w, 1u);
-    bool stop = p.use_stop != 0, trace = p.trace_feat != nullptr;
-    if (p.del_bits || p.allow_bits || p.eps_mode)
+    if (filt)
         return p.g.space == 0 ? launch_dim<0, false, false, true>(p, blocks, lds, s)
                               : launch_dim<1, false, false, true>(p, blocks, lds, s);
     if (p.g.space == 0) {
@@ -778,7 +780,7 @@ hipError_t launch_search(const SearchParams& p, int blocks, hipStream_t s) {
 }
 
 int search_blocks_per_cu(const SearchParams& p, bool filtered) {
-    size_t lds = search_lds_bytes(p.g.dim, p.cap, p.hash_bits);
+    size_t lds = search_lds_bytes(p.g.dim, p.cap, p.hash_bits, false);
     int nb = 0;
     hipError_t e;
     if (filtered && p.g.dim == 128 && p.g.space == 0)

@@ -209,12 +209,13 @@ int hg_gpu_index_labels(void* gpu_index, uint64_t* labels, size_t capacity, size
  * out[0] = waves per workgroup, out[1] = visited-hash slots per wave, out[2] = beam capacity,
  * out[3] = LDS bytes per workgroup, out[4] = workgroups launched. */
 int hg_gpu_search_plan(void* gpu_index, size_t max_ef, size_t max_k, size_t nq, int32_t* out);
-/* Re-tune: time candidate (waves, hash) pairs on the given device-resident queries and keep the
- * fastest for this (max ef, max k) bucket. */
+/* Re-tune: time candidate (visited-hash slots 2^7..2^13, workgroups per CU) pairs on the given
+ * device-resident queries and keep the fastest for this (max ef, max k) bucket. */
 int hg_gpu_autotune(void* gpu_index, const float* d_queries, size_t nq, size_t ef, size_t k, void* stream);
 /* Stores a launch plan (visited-hash slots = 1 << hash_bits, persistent workgroups per CU) for
  * the beam capacity of (ef, k), in place of an autotuned one.  No counterpart in the reference:
- * a tuning knob of the GPU search for callers that already know their plan. */
+ * a tuning knob of the GPU search for callers that already know their plan.  hash_bits in
+ * [7, 16], workgroups_per_cu in [1, 32]. */
 int hg_gpu_set_plan(void* gpu_index, size_t ef, size_t k, uint32_t hash_bits, int workgroups_per_cu);
 
 /* ---- Brute force ---- */
