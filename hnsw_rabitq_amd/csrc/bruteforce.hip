@@ -396,25 +396,38 @@ struct __align__(16) EpiLds {
     uint32_t base[BM];     // start of the row's reserved candidate slots
 };
 
-// Stage the tile's per-row and per-column epilogue inputs in LDS.  Called before the main loop:
-// the loads complete with the first k-slices, so their latency hides behind the fill, and no
-// register holds them across the loop.  The main loop's barriers order these writes before the
-// epilogue's reads.
-template <int PREC, bool FILTER, int BM, int BN, int WN, int NT>
-__device__ __forceinline__ void epi_stage(const LowpGemmArgs& g, uint32_t m0, uint32_t n0, int tid,
-                                          EpiLds<BM, BN, WN>& e) {
-    for (int r = tid; r < BM; r += NT) {
-        const uint32_t row = m0 + r;
-        const bool ok = row < g.nq;
-        e.tau[r] = (FILTER && ok) ? g.tau[row] : -INFINITY;
-        e.a_norm[r] = (g.space == 0 && ok) ? g.a_norm[row] : 0.f;
-        e.a_scale[r] = (PREC == 2 && ok) ? g.a_scale[row] : 1.f;
+// The tile's per-row and per-column epilogue inputs: thread t loads those of row m0 + t and of
+// column n0 + t.  Issued before the first k-slices (so they are the oldest loads in flight and
+// the main loop's first wait covers them) and written to LDS after that wait, so their latency
+// hides behind the pipeline fill and the registers are free again for the loop.
+struct EpiVals {
+    float tau, a_norm, a_scale, bsc, bn;
+};
+
+template <int PREC, bool FILTER, int BM, int BN, int NT>
+__device__ __forceinline__ EpiVals epi_load(const LowpGemmArgs& g, uint32_t m0, uint32_t n0, int tid) {
+    static_assert(BM <= NT && BN <= NT, "one row and one column of the tile per thread");
+    EpiVals v;
+    const uint32_t row = m0 + tid, col = n0 + tid;
+    const bool rok = tid < BM && row < g.nq, cok = tid < BN && col < g.n;
+    v.tau = (FILTER && rok) ? g.tau[row] : -INFINITY;
+    v.a_norm = (g.space == 0 && rok) ? g.a_norm[row] : 0.f;
+    v.a_scale = (PREC == 2 && rok) ? g.a_scale[row] : 1.f;
+    v.bsc = (PREC == 2 && cok) ? g.b_scale[col] : 1.0f;
+    v.bn = (g.space == 0 && cok) ? g.b_norm[col] : 0.f;
+    return v;
+}
+
+template <int BM, int BN, int WN>
+__device__ __forceinline__ void epi_store(const EpiVals& v, int tid, EpiLds<BM, BN, WN>& e) {
+    if (tid < BM) {
+        e.tau[tid] = v.tau;
+        e.a_norm[tid] = v.a_norm;
+        e.a_scale[tid] = v.a_scale;
     }
-    for (int c = tid; c < BN; c += NT) {
-        const uint32_t col = n0 + c;
-        const bool ok = col < g.n;
-        e.bsc[c] = (PREC == 2 && ok) ? g.b_scale[col] : 1.0f;
-        e.bn[c] = (g.space == 0 && ok) ? g.b_norm[col] : 0.f;
+    if (tid < BN) {
+        e.bsc[tid] = v.bsc;
+        e.bn[tid] = v.bn;
     }
 }
 
@@ -609,7 +622,7 @@ __global__ __launch_bounds__(256, 2) void lowp_gemm_kernel(LowpGemmArgs g) {
     if (!TileOrder::make(g.tiles_m, g.tiles_n).map(blockIdx.x, tm, tn)) return;
     const uint32_t m0 = tm * GM, n0 = tn * GN;
     __shared__ EpiLds<64 * 2, 128, 2> epi;
-    epi_stage<PREC, FILTER, 128, 128, 2, 256>(g, m0, n0, tid, epi);
+    epi_store(epi_load<PREC, FILTER, 128, 128, 256>(g, m0, n0, tid), tid, epi);
     const size_t row_bytes = (size_t)g.dim * (PREC == 1 ? 2 : 1);
 
     f32x16_t acc[2][2];
@@ -714,7 +727,7 @@ __global__ __launch_bounds__(256, 2) void lowp_gemm_wide_kernel(LowpGemmArgs g) 
     if (!TileOrder::make((g.nq + BM - 1) / BM, (g.n + BN - 1) / BN).map(blockIdx.x, tm, tn)) return;
     const uint32_t m0 = tm * BM, n0 = tn * BN;
     __shared__ EpiLds<BM, BN, 2> epi;
-    epi_stage<PREC, FILTER, BM, BN, 2, 256>(g, m0, n0, tid, epi);
+    epi_store(epi_load<PREC, FILTER, BM, BN, 256>(g, m0, n0, tid), tid, epi);
     const size_t row_bytes = (size_t)g.dim * (PREC == 1 ? 2 : 1);
 
     f32x16_t acc[TM][TN];
@@ -1000,7 +1013,7 @@ __global__ __launch_bounds__(128 * WNW, (WNW == 2 && TN == 2) ? 2 : 1) void lowp
     if (!TileOrder::make((g.nq + BM - 1) / BM, (g.n + BN - 1) / BN).map(blockIdx.x, tm, tn)) return;
     const uint32_t m0 = tm * BM, n0 = tn * BN;
     __shared__ EpiLds<BM, BN, WNW> epi;
-    epi_stage<PREC, FILTER, BM, BN, WNW, NT>(g, m0, n0, tid, epi);
+    const EpiVals ev = epi_load<PREC, FILTER, BM, BN, NT>(g, m0, n0, tid);
     const size_t row_bytes = (size_t)g.dim * (PREC == 1 ? 2 : 1);
     const int nst = (int)(row_bytes / LKB);
     constexpr int PER_STAGE = APW + BPW;  // direct-to-LDS loads per wave per slice
@@ -1029,10 +1042,14 @@ __global__ __launch_bounds__(128 * WNW, (WNW == 2 && TN == 2) ? 2 : 1) void lowp
         else
             gemm_lds_mfma<PREC, TM, TN>((uint32_t)(uintptr_t)rA, (uint32_t)(uintptr_t)rB, acc, wm, wn, lane);
     };
-    for (int st = 0; st < nst; st += 3) {
-        step(st, A0, B0, A2, B2);
-        if (st + 1 < nst) step(st + 1, A1, B1, A0, B0);
-        if (st + 2 < nst) step(st + 2, A2, B2, A1, B1);
+    // nst >= 1 (dim is a multiple of 64): the first step, then the epilogue inputs (complete by its
+    // wait) to LDS, then the ring continues from slot 1
+    step(0, A0, B0, A2, B2);
+    epi_store(ev, tid, epi);
+    for (int st = 1; st < nst; st += 3) {
+        step(st, A1, B1, A0, B0);
+        if (st + 1 < nst) step(st + 1, A2, B2, A1, B1);
+        if (st + 2 < nst) step(st + 2, A0, B0, A2, B2);
     }
 
     gemm_epilogue<PREC, FILTER, TM, TN, WNW, NT>(g, acc, m0, n0, wm, wn, lane, tid, epi);
