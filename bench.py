@@ -302,6 +302,30 @@ def main():
                    sample=f"{sample.shape[0]} of the {nq} queries at ef={ef_op}, k=10, median of 3 after warm-up; "
                           f"{model}")
         log(f"CPU oracle port: {cpu_qps:.0f} QPS on {nthreads} threads")
+        # full-size parity on the same sample: the GPU rows at the operating point against the
+        # oracle in total (distance, id) order (every row) and in the reference's heap order
+        # (rows whose traversal meets no distance tie, where the two orders cannot differ)
+        ns = sample.shape[0]
+        dqs = DeviceArray.from_numpy(np.ascontiguousarray(sample, np.float32))
+        sl, sd = DeviceArray(ns * K * 8), DeviceArray(ns * K * 4)
+        gpu.search_device(dqs, ns, K, ef_op, sl, sd, stream=stream)
+        _lib.call("hg_stream_synchronize", stream)
+        g_lab, g_dis = sl.to_numpy(np.uint64, (ns, K)), sd.to_numpy(np.float32, (ns, K)).view(np.uint32)
+        t_lab, t_dis, _, _, _ = o.search(0, sample, K, ef_op, mode=pyoracle.MODE_TOTAL, nthreads=nthreads)
+        h_lab, h_dis, _, tf, _ = o.search(0, sample, K, ef_op, mode=pyoracle.MODE_HEAP, nthreads=nthreads,
+                                          ties=True)
+        same_t = (g_lab == t_lab).all(1) & (g_dis == t_dis.view(np.uint32)).all(1)
+        same_h = (g_lab == h_lab).all(1) & (g_dis == h_dis.view(np.uint32)).all(1)
+        free = tf == 0
+        parity = dict(queries=ns, ef=ef_op, rows_identical_total_order=float(same_t.mean()),
+                      rows_identical_heap_order=float(same_h.mean()), tie_free_queries=int(free.sum()),
+                      tie_free_rows_identical_heap_order=float(same_h[free].mean()) if free.any() else None)
+        result["parity_full_size"] = parity
+        log(f"parity at full size ({ns} queries, ef={ef_op}): rows identical to the oracle in total order "
+            f"{parity['rows_identical_total_order']:.4f}, in the reference's heap order "
+            f"{parity['rows_identical_heap_order']:.4f}; tie-free queries {parity['tie_free_queries']}, identical "
+            f"to the heap-order oracle {parity['tie_free_rows_identical_heap_order']}")
+        dqs.free(), sl.free(), sd.free()
 
     # headline: the faster of plain and learned at recall@10 >= 0.95 (plain is the fallback)
     learned_qps = result.get("learned_qps_at_095") or 0.0
