@@ -10,7 +10,7 @@ import numpy as np
 from . import datasets
 
 
-def run(H, lib, n, nq, stream, timer, log, dim=768, k=100, c=4096):
+def run(H, lib, n, nq, stream, timer, log, dim=768, k=100, c=4096, library=True):
     from . import DeviceArray
     log(f"brute force: generating {n} x {dim} GMM + {nq} queries")
     data, q = datasets.c3_gmm(n=n, nq=nq, dim=dim)
@@ -70,7 +70,8 @@ def run(H, lib, n, nq, stream, timer, log, dim=768, k=100, c=4096):
         log(f"  {name}: GEMM {stats[0]:.2f} ms = {gemm_tf:.0f} TFLOP/s ({100 * gemm_tf / peak:.1f}% of peak); "
             f"prep {stats[1]:.2f} ms, re-rank {stats[2]:.2f} ms, end-to-end {e2e:.2f} ms; recall@100 {rec:.4f}; "
             f"rows identical to exact {same:.4f}; fp64 near-tie pairs {chk[0]:.0f}, disagreements {chk[1]:.0f}")
-    out.update(library_gemm(n, nq, dim, log))
+    if library:
+        out.update(library_gemm(n, nq, dim, log))
     return out
 
 
@@ -150,7 +151,7 @@ def library_gemm_run(n, nq, dim, log, chunk=1 << 17):
 def main():
     """Standalone run of the brute-force measurement (one JSON line on stdout):
 
-        python -m hnsw_rabitq_amd.bf_bench [--n 1000000] [--nq 10000] [--dim 768] [--library-gemm]
+        python -m hnsw_rabitq_amd.bf_bench [--n 1000000] [--nq 10000] [--dim 768] [--library-gemm | --no-library]
     """
     import argparse
     import json
@@ -165,6 +166,8 @@ def main():
     ap.add_argument("--nq", type=int, default=10_000)
     ap.add_argument("--dim", type=int, default=768)
     ap.add_argument("--library-gemm", action="store_true", help="only time the vendor GEMM on this shape")
+    ap.add_argument("--no-library", action="store_true",
+                    help="skip the vendor GEMM line (its child process would share a profiler's output)")
     args = ap.parse_args()
     t0 = time.time()
 
@@ -191,7 +194,8 @@ def main():
 
     stream = ctypes.c_void_p()
     _lib.call("hg_stream_create", ctypes.byref(stream))
-    out = run(H, _lib, n=args.n, nq=args.nq, stream=stream, timer=Timer(stream), log=log, dim=args.dim)
+    out = run(H, _lib, n=args.n, nq=args.nq, stream=stream, timer=Timer(stream), log=log, dim=args.dim,
+              library=not args.no_library)
     print(json.dumps(out), flush=True)
 
 
