@@ -96,8 +96,9 @@ def main():
     args = ap.parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus != 1 or world != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path (queries shard with no "
-              "exchange; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU: there is no multi-GPU path (each GPU would hold a replica "
+              "of the index and take a slice of the queries, with no exchange). Run with --gpus 1.",
+              file=sys.stderr)
         sys.exit(2)
 
     import hnsw_rabitq_amd as H
