@@ -347,7 +347,7 @@ struct TileOrder {
     }
     __host__ __device__ uint32_t grid() const { return 8u * ((groups + 7) / 8) * G * tiles_n; }
     // false when this workgroup has no tile
-    __device__ bool map(uint32_t b, uint32_t& tm, uint32_t& tn) const {
+    __device__ __forceinline__ bool map(uint32_t b, uint32_t& tm, uint32_t& tn) const {
         const uint32_t x = b % 8, local = b / 8;
         const uint32_t per_group = G * tiles_n;
         const uint32_t g = x + 8 * (local / per_group);
@@ -394,6 +394,7 @@ struct __align__(16) EpiLds {
     float bsc[BN], bn[BN];
     uint32_t cnt[WN][BM];  // passes of each row in the columns of wave wn
     uint32_t base[BM];     // start of the row's reserved candidate slots
+    uint32_t ovf;          // a wave's pass list outgrew its LDS space: the tile takes pass 2
 };
 
 // The tile's per-row and per-column epilogue inputs: thread t loads those of row m0 + t and of
@@ -429,6 +430,7 @@ __device__ __forceinline__ void epi_store(const EpiVals& v, int tid, EpiLds<BM, 
         e.bsc[tid] = v.bsc;
         e.bn[tid] = v.bn;
     }
+    if (tid == 0) e.ovf = 0u;
 }
 
 // Four per-row values of one 32x32 block held by a half-wave: registers 4q..4q+3 of the
@@ -457,7 +459,7 @@ __device__ __forceinline__ void rows4(const T* a, int lb, int q, T (&v)[4]) {
 template <int PREC, bool FILTER, int TM, int TN, int WN, int NT, int SPACE>
 __device__ __forceinline__ void gemm_epilogue_s(const LowpGemmArgs& g, f32x16_t (&acc)[TM][TN], uint32_t m0,
                                                 uint32_t n0, int wm, int wn, int lane, int tid,
-                                                EpiLds<64 * TM, 32 * TN * WN, WN>& e) {
+                                                EpiLds<64 * TM, 32 * TN * WN, WN>& e, uint2* cbuf, uint32_t capw) {
     constexpr int BM = 64 * TM;
     __syncthreads();
     uint32_t col[TN];
@@ -473,6 +475,9 @@ __device__ __forceinline__ void gemm_epilogue_s(const LowpGemmArgs& g, f32x16_t 
     }
     auto dist = [&](int i, int j, int r, float an, float as) {
         float sv = acc[i][j][r];
+        // opaque to the compiler, so that a pass recomputes its distances instead of keeping
+        // another pass's 16 * TM * TN values live across the barriers
+        __asm__ volatile("" : "+v"(sv));
         if (PREC == 2) sv = sv * as * bsc[j];
         if (SPACE == 0) return an + bn[j] - 2.0f * sv;
         return 1.0f - sv;
@@ -509,6 +514,11 @@ __device__ __forceinline__ void gemm_epilogue_s(const LowpGemmArgs& g, f32x16_t 
     uint64_t hit = 0, colm[TN];
 #pragma unroll
     for (int j = 0; j < TN; j++) colm[j] = __builtin_amdgcn_ballot_w64(col_ok[j]);
+    // each wave also lists its passes in its own part of the idle operand LDS as (row << 16 |
+    // column within the tile, distance bits); wcnt is wave-uniform
+    uint2* wl = cbuf + (size_t)(tid >> 6) * capw;
+    uint32_t wcnt = 0;
+    const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
 #pragma unroll
     for (int i = 0; i < TM; i++) {
         const int lb = wm * 32 * TM + i * 32 + half * 4;
@@ -526,9 +536,18 @@ __device__ __forceinline__ void gemm_epilogue_s(const LowpGemmArgs& g, f32x16_t 
                 uint32_t lo = 0, hi = 0;
 #pragma unroll
                 for (int j = 0; j < TN; j++) {
-                    const uint64_t m = __builtin_amdgcn_ballot_w64(dist(i, j, r, an[rr], as[rr]) < tq[rr]) & colm[j];
+                    const float dv = dist(i, j, r, an[rr], as[rr]);
+                    const uint64_t m = __builtin_amdgcn_ballot_w64(dv < tq[rr]) & colm[j];
                     lo += (uint32_t)__popcll(m & 0xFFFFFFFFull);
                     hi += (uint32_t)__popcll(m >> 32);
+                    if (m) {  // wave-uniform
+                        const uint32_t pos = wcnt + (uint32_t)__popcll(m & lt);
+                        if (((m >> lane) & 1ull) && pos < capw)
+                            wl[pos] = make_uint2(((uint32_t)(lb + q * 8 + rr) << 16) |
+                                                     (uint32_t)(wn * 32 * TN + j * 32 + (lane & 31)),
+                                                 __float_as_uint(dv));
+                        wcnt += (uint32_t)__popcll(m);
+                    }
                 }
                 clo[rr] = lo;
                 chi[rr] = hi;
@@ -540,6 +559,7 @@ __device__ __forceinline__ void gemm_epilogue_s(const LowpGemmArgs& g, f32x16_t 
             }
         }
     }
+    if (wcnt > capw && lane == 0) e.ovf = 1u;
     __syncthreads();
     // one reservation per row
     for (int r = tid; r < BM; r += NT) {
@@ -549,8 +569,24 @@ __device__ __forceinline__ void gemm_epilogue_s(const LowpGemmArgs& g, f32x16_t 
         if (c) e.base[r] = atomicAdd(&g.cand_cnt[m0 + r], c);
     }
     __syncthreads();
-    // pass 2: write the candidates into the reserved slots; the per-row values of a group of
-    // four pairs load together before the group
+    if (e.ovf == 0u) {
+        // every wave writes its list out, 64 candidates per store; a row's slots are handed out
+        // by an LDS counter that starts at the row's reserved base (the order of candidates
+        // inside a row's list carries no meaning)
+        for (uint32_t idx = lane; idx < wcnt; idx += 64) {
+            const uint2 v = wl[idx];
+            const uint32_t lr = v.x >> 16;
+            const uint32_t slot = atomicAdd(&e.base[lr], 1u);
+            if (slot < g.cap) {
+                const size_t o = (size_t)(m0 + lr) * g.cap + slot;
+                g.cand_rows[o] = n0 + (v.x & 0xFFFFu);
+                g.cand_d[o] = __uint_as_float(v.y);
+            }
+        }
+        return;
+    }
+    // pass 2 (a list overflowed): recompute and write the candidates into the reserved slots;
+    // the per-row values of a group of four pairs load together before the group
     const uint32_t below = (lane & 31) ? (1u << (lane & 31)) - 1u : 0u;
 #pragma unroll
     for (int i = 0; i < TM; i++) {
@@ -605,11 +641,12 @@ __device__ __forceinline__ void gemm_epilogue_s(const LowpGemmArgs& g, f32x16_t 
 template <int PREC, bool FILTER, int TM, int TN, int WN, int NT>
 __device__ __forceinline__ void gemm_epilogue(const LowpGemmArgs& g, f32x16_t (&acc)[TM][TN], uint32_t m0,
                                               uint32_t n0, int wm, int wn, int lane, int tid,
-                                              EpiLds<64 * TM, 32 * TN * WN, WN>& e) {
+                                              EpiLds<64 * TM, 32 * TN * WN, WN>& e, uint2* cbuf,
+                                              uint32_t capw) {
     if (g.space == 0)
-        gemm_epilogue_s<PREC, FILTER, TM, TN, WN, NT, 0>(g, acc, m0, n0, wm, wn, lane, tid, e);
+        gemm_epilogue_s<PREC, FILTER, TM, TN, WN, NT, 0>(g, acc, m0, n0, wm, wn, lane, tid, e, cbuf, capw);
     else
-        gemm_epilogue_s<PREC, FILTER, TM, TN, WN, NT, 1>(g, acc, m0, n0, wm, wn, lane, tid, e);
+        gemm_epilogue_s<PREC, FILTER, TM, TN, WN, NT, 1>(g, acc, m0, n0, wm, wn, lane, tid, e, cbuf, capw);
 }
 
 template <int PREC, bool FILTER>
@@ -708,7 +745,9 @@ __global__ __launch_bounds__(256, 2) void lowp_gemm_kernel(LowpGemmArgs g) {
         }
     }
 
-    gemm_epilogue<PREC, FILTER, 2, 2, 2, 256>(g, acc, m0, n0, wm, wn, lane, tid, epi);
+    // the A operand's LDS (36 KiB) holds the pass lists once the main loop is done
+    gemm_epilogue<PREC, FILTER, 2, 2, 2, 256>(g, acc, m0, n0, wm, wn, lane, tid, epi, reinterpret_cast<uint2*>(&As[0][0][0]),
+                                              (uint32_t)(sizeof(As) / 8 / 4));
 }
 
 // Larger tiles: each wave owns (32*TM) x (32*TN) outputs (TM x TN MFMA blocks), the workgroup
@@ -817,7 +856,9 @@ __global__ __launch_bounds__(256, 2) void lowp_gemm_wide_kernel(LowpGemmArgs g) 
         }
     }
 
-    gemm_epilogue<PREC, FILTER, TM, TN, 2, 256>(g, acc, m0, n0, wm, wn, lane, tid, epi);
+    // the A operand's LDS holds the pass lists once the main loop is done
+    gemm_epilogue<PREC, FILTER, TM, TN, 2, 256>(g, acc, m0, n0, wm, wn, lane, tid, epi, reinterpret_cast<uint2*>(&As[0][0]),
+                                                (uint32_t)(sizeof(As) / 8 / 4));
 }
 
 // Direct-to-LDS pipeline: rows are fetched with global_load_lds_dwordx4 (64 lanes x 16 bytes
@@ -1052,7 +1093,9 @@ __global__ __launch_bounds__(128 * WNW, (WNW == 2 && TN == 2) ? 2 : 1) void lowp
         if (st + 2 < nst) step(st + 2, A0, B0, A2, B2);
     }
 
-    gemm_epilogue<PREC, FILTER, TM, TN, WNW, NT>(g, acc, m0, n0, wm, wn, lane, tid, epi);
+    // the first A slice's LDS holds the pass lists once the main loop is done
+    gemm_epilogue<PREC, FILTER, TM, TN, WNW, NT>(g, acc, m0, n0, wm, wn, lane, tid, epi, reinterpret_cast<uint2*>(sA0),
+                                                 (uint32_t)(sizeof(sA0) / 8 / NW));
 }
 
 // Copy every `step`-th row of a low-precision matrix (and its scale / norm) to a compact sample.

@@ -118,6 +118,33 @@ def test_bf_lowp_ties_at_rerank_cut(hg, prec):
     assert stats[3] == 32
 
 
+@pytest.mark.parametrize("prec", ["bf16", "fp8"])
+@pytest.mark.parametrize("n,c", [(60_000, 512), (6_000, 4096)])
+def test_bf_lowp_candidate_list_paths(hg, prec, n, c):
+    """The GEMM epilogue keeps each wave's passes in an LDS list and writes them out after the
+    reservation; a tile whose list outgrows its space recomputes them instead.  About 1% of the
+    rows pass at (60,000, 512) (lists only) and nearly all at (6,000, 4,096) (every tile takes the
+    recompute): both give the exact path's lists."""
+    from hnsw_rabitq_amd import datasets, DeviceArray, _lib
+    import ctypes
+    print(f"bf lowp list paths prec={prec} n={n} c={c}", flush=True)
+    data, q = datasets.c3_gmm(n=n, nq=200, dim=192, n_centres=32)
+    b = _bf(hg, "ip", data)
+    k = 20
+    el, ed = b.knn_query(q, k=k)
+    dq = DeviceArray.from_numpy(q)
+    dl = DeviceArray(q.shape[0] * k * 8)
+    dd = DeviceArray(q.shape[0] * k * 4)
+    stats = np.zeros(4, np.float64)
+    P = {"bf16": _lib.PREC_BF16, "fp8": _lib.PREC_FP8}[prec]
+    _lib.call("hg_bf_search_device", b._h, dq.ptr, q.shape[0], k, P, c, dl.ptr, dd.ptr,
+              ctypes.c_void_p(stats.ctypes.data), None)
+    _lib.call("hg_device_synchronize")
+    np.testing.assert_array_equal(dl.to_numpy(np.uint64, (q.shape[0], k)), el)
+    np.testing.assert_array_equal(dd.to_numpy(np.float32, (q.shape[0], k)).view(np.uint32), ed.view(np.uint32))
+    assert stats[3] == 0  # no query fell back to the exact path
+
+
 @pytest.mark.parametrize("variant", [0, 1, 2, 3, 4])
 def test_bf_lowp_gemm_variants_agree(hg, variant):
     """Every GEMM tile configuration gives the exact path's lists (BF16 and FP8, inner product,
