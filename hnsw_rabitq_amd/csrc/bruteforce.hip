@@ -392,7 +392,7 @@ template <int BM, int BN, int WN>
 struct __align__(16) EpiLds {
     float tau[BM], a_norm[BM], a_scale[BM];
     float bsc[BN], bn[BN];
-    uint32_t cnt[WN][BM];  // passes of each row in the columns of wave wn
+    uint32_t cnt[BM];      // passes of each row in the tile (LDS adds, cleared before the main loop)
     uint32_t base[BM];     // start of the row's reserved candidate slots
     uint32_t ovf;          // a wave's pass list outgrew its LDS space: the tile takes pass 2
 };
@@ -430,6 +430,7 @@ __device__ __forceinline__ void epi_store(const EpiVals& v, int tid, EpiLds<BM, 
         e.bsc[tid] = v.bsc;
         e.bn[tid] = v.bn;
     }
+    if (tid < BM) e.cnt[tid] = 0u;
     if (tid == 0) e.ovf = 0u;
 }
 
@@ -450,12 +451,12 @@ __device__ __forceinline__ void rows4(const T* a, int lb, int q, T (&v)[4]) {
 // rows sits in one (block row i, register r, half-wave h) and the WN waves wn share a row.
 // Sample mode writes every approximate distance.  Filter mode appends the columns with
 // d < tau[row] to the row's candidate list with one device-scope atomic per (row, workgroup):
-// pass 1 counts each row's passes (ballot + popcount), one thread per row reserves the row's
-// total, pass 2 recomputes d and writes each pass at the row base + the counts of the waves
-// before it + its rank among the row's passes.  Per-row values are read from LDS sixteen at a
-// time per block (four 16-byte reads) before the block's loop, so no LDS round trip sits
-// inside it.  NT threads; two waves along the rows.  SPACE is fixed at compile time so that
-// the unrolled loops carry no branch.
+// pass 1 counts each row's passes (LDS adds) and lists each wave's passes in LDS, one thread
+// per row reserves the row's total, and each wave then writes its list out (a list that
+// outgrows its LDS space sends the tile to pass 2, which recomputes the passes instead).
+// Per-row values are read from LDS four rows at a time (one 16-byte read) before each group
+// of four pairs.  NT threads; two waves along the rows.  SPACE is fixed at compile time so
+// that the unrolled loops carry no branch.
 template <int PREC, bool FILTER, int TM, int TN, int WN, int NT, int SPACE>
 __device__ __forceinline__ void gemm_epilogue_s(const LowpGemmArgs& g, f32x16_t (&acc)[TM][TN], uint32_t m0,
                                                 uint32_t n0, int wm, int wn, int lane, int tid,
@@ -508,14 +509,13 @@ __device__ __forceinline__ void gemm_epilogue_s(const LowpGemmArgs& g, f32x16_t 
         }
         return;
     }
-    // pass 1: per-row pass counts of this wave (each row belongs to one half-wave, so lanes 0
-    // and 32 store them without atomics); `hit` marks the (i, r) pairs with any pass (the
-    // counts are wave-uniform, so the mask lives in scalar registers)
+    // pass 1: each passing lane adds one to its row's count in LDS (no return value is used,
+    // so the adds do not wait); `hit` marks the (i, r) pairs with any pass (wave-uniform, in
+    // scalar registers).  Each wave also lists its passes in its own part of the idle operand
+    // LDS as (row << 16 | column within the tile, distance bits); wcnt is wave-uniform.
     uint64_t hit = 0, colm[TN];
 #pragma unroll
     for (int j = 0; j < TN; j++) colm[j] = __builtin_amdgcn_ballot_w64(col_ok[j]);
-    // each wave also lists its passes in its own part of the idle operand LDS as (row << 16 |
-    // column within the tile, distance bits); wcnt is wave-uniform
     uint2* wl = cbuf + (size_t)(tid >> 6) * capw;
     uint32_t wcnt = 0;
     const uint64_t lt = lane ? (~0ull >> (64 - lane)) : 0ull;
@@ -524,38 +524,29 @@ __device__ __forceinline__ void gemm_epilogue_s(const LowpGemmArgs& g, f32x16_t 
         const int lb = wm * 32 * TM + i * 32 + half * 4;
 #pragma unroll
         for (int q = 0; q < 4; q++) {
-            // rows lr0 + rr (half-wave 0) and lr0 + 4 + rr (half-wave 1), rr = 0..3
-            const int lr0 = wm * 32 * TM + i * 32 + q * 8;
             float tq[4];
             rows4(e.tau, lb, q, tq);
             load_rows(lb, q);
-            uint32_t clo[4], chi[4];
 #pragma unroll
             for (int rr = 0; rr < 4; rr++) {
                 const int r = q * 4 + rr;
-                uint32_t lo = 0, hi = 0;
+                const uint32_t lr = (uint32_t)(lb + q * 8 + rr);
 #pragma unroll
                 for (int j = 0; j < TN; j++) {
                     const float dv = dist(i, j, r, an[rr], as[rr]);
                     const uint64_t m = __builtin_amdgcn_ballot_w64(dv < tq[rr]) & colm[j];
-                    lo += (uint32_t)__popcll(m & 0xFFFFFFFFull);
-                    hi += (uint32_t)__popcll(m >> 32);
                     if (m) {  // wave-uniform
-                        const uint32_t pos = wcnt + (uint32_t)__popcll(m & lt);
-                        if (((m >> lane) & 1ull) && pos < capw)
-                            wl[pos] = make_uint2(((uint32_t)(lb + q * 8 + rr) << 16) |
-                                                     (uint32_t)(wn * 32 * TN + j * 32 + (lane & 31)),
-                                                 __float_as_uint(dv));
+                        hit |= 1ull << (i * 16 + r);
+                        if ((m >> lane) & 1ull) {
+                            atomicAdd(&e.cnt[lr], 1u);
+                            const uint32_t pos = wcnt + (uint32_t)__popcll(m & lt);
+                            if (pos < capw)
+                                wl[pos] = make_uint2((lr << 16) | (uint32_t)(wn * 32 * TN + j * 32 + (lane & 31)),
+                                                     __float_as_uint(dv));
+                        }
                         wcnt += (uint32_t)__popcll(m);
                     }
                 }
-                clo[rr] = lo;
-                chi[rr] = hi;
-                if (lo + hi) hit |= 1ull << (i * 16 + r);
-            }
-            if (lane == 0) {
-                *reinterpret_cast<uint4*>(&e.cnt[wn][lr0]) = make_uint4(clo[0], clo[1], clo[2], clo[3]);
-                *reinterpret_cast<uint4*>(&e.cnt[wn][lr0 + 4]) = make_uint4(chi[0], chi[1], chi[2], chi[3]);
             }
         }
     }
@@ -563,16 +554,14 @@ __device__ __forceinline__ void gemm_epilogue_s(const LowpGemmArgs& g, f32x16_t 
     __syncthreads();
     // one reservation per row
     for (int r = tid; r < BM; r += NT) {
-        uint32_t c = 0;
-#pragma unroll
-        for (int w = 0; w < WN; w++) c += e.cnt[w][r];
+        const uint32_t c = e.cnt[r];
         if (c) e.base[r] = atomicAdd(&g.cand_cnt[m0 + r], c);
     }
     __syncthreads();
+    // a row's slots are handed out by an LDS counter that starts at the row's reserved base (the
+    // order of candidates inside a row's list carries no meaning)
     if (e.ovf == 0u) {
-        // every wave writes its list out, 64 candidates per store; a row's slots are handed out
-        // by an LDS counter that starts at the row's reserved base (the order of candidates
-        // inside a row's list carries no meaning)
+        // every wave writes its list out, 64 candidates per store
         for (uint32_t idx = lane; idx < wcnt; idx += 64) {
             const uint2 v = wl[idx];
             const uint32_t lr = v.x >> 16;
@@ -585,9 +574,7 @@ __device__ __forceinline__ void gemm_epilogue_s(const LowpGemmArgs& g, f32x16_t 
         }
         return;
     }
-    // pass 2 (a list overflowed): recompute and write the candidates into the reserved slots;
-    // the per-row values of a group of four pairs load together before the group
-    const uint32_t below = (lane & 31) ? (1u << (lane & 31)) - 1u : 0u;
+    // pass 2 (a list overflowed): recompute the marked pairs and write their passes
 #pragma unroll
     for (int i = 0; i < TM; i++) {
         const int lb = wm * 32 * TM + i * 32 + half * 4;
@@ -595,43 +582,23 @@ __device__ __forceinline__ void gemm_epilogue_s(const LowpGemmArgs& g, f32x16_t 
         for (int q = 0; q < 4; q++) {
             if (!((hit >> (i * 16 + q * 4)) & 0xFull)) continue;  // no pass in this group: scalar skip
             float tq[4];
-            uint32_t oq[4];  // row base + the passes of the waves before this one
             rows4(e.tau, lb, q, tq);
-            rows4(e.base, lb, q, oq);
-#pragma unroll
-            for (int w = 0; w < WN - 1; w++)
-                if (w < wn) {
-                    uint32_t cq[4];
-                    rows4(e.cnt[w], lb, q, cq);
-#pragma unroll
-                    for (int rr = 0; rr < 4; rr++) oq[rr] += cq[rr];
-                }
             load_rows(lb, q);
 #pragma unroll
             for (int rr = 0; rr < 4; rr++) {
                 const int r = q * 4 + rr;
                 if (!((hit >> (i * 16 + r)) & 1ull)) continue;  // no pass in this pair: scalar skip
-                const uint32_t row = m0 + lb + q * 8 + rr;
-                float d[TN];
-                bool pass[TN];
-                uint32_t hm[TN];
+                const uint32_t lr = (uint32_t)(lb + q * 8 + rr);
 #pragma unroll
                 for (int j = 0; j < TN; j++) {
-                    d[j] = dist(i, j, r, an[rr], as[rr]);
-                    pass[j] = (int)col_ok[j] & (int)(d[j] < tq[rr]);
-                    hm[j] = (uint32_t)(__ballot(pass[j]) >> (32 * half));
-                }
-                uint32_t off = oq[rr];
-#pragma unroll
-                for (int j = 0; j < TN; j++) {
-                    if (pass[j]) {
-                        const uint32_t slot = off + (uint32_t)__popc(hm[j] & below);
+                    const float dv = dist(i, j, r, an[rr], as[rr]);
+                    if (col_ok[j] && dv < tq[rr]) {
+                        const uint32_t slot = atomicAdd(&e.base[lr], 1u);
                         if (slot < g.cap) {
-                            g.cand_rows[(size_t)row * g.cap + slot] = col[j];
-                            g.cand_d[(size_t)row * g.cap + slot] = d[j];
+                            g.cand_rows[(size_t)(m0 + lr) * g.cap + slot] = col[j];
+                            g.cand_d[(size_t)(m0 + lr) * g.cap + slot] = dv;
                         }
                     }
-                    off += (uint32_t)__popc(hm[j]);
                 }
             }
         }
