@@ -17,6 +17,8 @@
 // Distances are the reference's FP32 values (the query kernel's distance routines); equal
 // distances order by element id.  The result is a valid HNSW level-0 graph of the same kind as
 // the reference's, not the same graph: insertion order within a batch differs.
+#include <hipcub/hipcub.hpp>
+
 #include "common.h"
 #include "dist.h"
 #include "kernels.h"
@@ -178,8 +180,10 @@ __global__ __launch_bounds__(64) void build_link_kernel(BuildArgs A) {
     float* sd = reinterpret_cast<float*>(S + 64);  // [64]
 
     const uint32_t t = blockIdx.x;
-    const uint32_t n = A.targets[t];
-    const uint32_t b0 = A.offsets[t], b1 = A.offsets[t + 1];
+    const uint32_t T = *A.n_targets;
+    if (t >= T) return;
+    const uint32_t b0 = A.starts[t], b1 = t + 1 < T ? A.starts[t + 1] : *A.n_links;
+    const uint32_t n = A.keys[b0];
     const int maxM0 = (int)g.maxM0;
     uint32_t* row = A.links0 + (size_t)n * maxM0;
     const uint32_t v = lane < maxM0 ? row[lane] : SENT;
@@ -220,6 +224,113 @@ __global__ __launch_bounds__(64) void build_link_kernel(BuildArgs A) {
     if (lane < maxM0) row[lane] = lane < c ? cur[lane] : SENT;
 }
 
+namespace {
+
+// Link pairs of the batch: key = receiving element, value = new element (s + i).  Slots after a
+// row's first SENT are unused and take the key s + B, above every receiver (receivers of row i
+// are elements < s + i), so they sort last.
+__global__ void link_pairs_kernel(const uint32_t* __restrict__ sel, uint32_t B, uint32_t M, uint32_t s,
+                                  uint32_t* __restrict__ keys, uint32_t* __restrict__ vals, uint32_t* err) {
+    const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= B * M) return;
+    const uint32_t i = idx / M, j = idx % M;
+    bool ok = true;
+    for (uint32_t q = 0; q <= j; q++) ok = ok && sel[i * M + q] != SENT;
+    const uint32_t t = sel[idx];
+    if (ok && t >= s + i) {  // not an earlier element: reported, and never used as a receiver
+        atomicOr(err, 1u);
+        ok = false;
+    }
+    keys[idx] = ok ? t : s + B;
+    vals[idx] = s + i;
+}
+
+// After the sort: flags[idx] = 1 at the first link of each receiver; *n_links = number of links.
+__global__ void link_runs_kernel(const uint32_t* __restrict__ keys, uint32_t N, uint32_t key_unused,
+                                 uint32_t* __restrict__ flags, uint32_t* n_links) {
+    const uint32_t idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= N) return;
+    const uint32_t k = keys[idx];
+    const bool valid = k < key_unused;
+    flags[idx] = (valid && (idx == 0 || keys[idx - 1] != k)) ? 1u : 0u;
+    if (valid && (idx + 1 == N || keys[idx + 1] >= key_unused)) *n_links = idx + 1;
+}
+
+__global__ void zero_words_kernel(uint32_t* p, uint32_t n) {
+    if (threadIdx.x < n) p[threadIdx.x] = 0u;
+}
+
+struct GroupScratch {
+    uint32_t *keys_in, *vals_in, *keys, *vals, *flags, *starts, *words;  // words: n_targets, n_links
+    void* temp;
+    size_t temp_bytes;
+};
+
+size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+size_t cub_temp_bytes(size_t max_links) {
+    size_t a = 0, b = 0;
+    uint32_t* np = nullptr;
+    (void)hipcub::DeviceRadixSort::SortPairs(nullptr, a, np, np, np, np, (int)max_links, 0, 32);
+    (void)hipcub::DeviceSelect::Flagged(nullptr, b, hipcub::CountingInputIterator<uint32_t>(0), np, np, np,
+                                        (int)max_links);
+    return a > b ? a : b;
+}
+
+GroupScratch carve(void* base, size_t max_links) {
+    GroupScratch g;
+    char* p = (char*)base;
+    auto take = [&](size_t bytes) {
+        char* r = p;
+        p += align256(bytes);
+        return r;
+    };
+    g.keys_in = (uint32_t*)take(max_links * 4);
+    g.vals_in = (uint32_t*)take(max_links * 4);
+    g.keys = (uint32_t*)take(max_links * 4);
+    g.vals = (uint32_t*)take(max_links * 4);
+    g.flags = (uint32_t*)take(max_links * 4);
+    g.starts = (uint32_t*)take(max_links * 4);
+    g.words = (uint32_t*)take(16);
+    g.temp_bytes = cub_temp_bytes(max_links);
+    g.temp = take(g.temp_bytes);
+    return g;
+}
+}  // namespace
+
+size_t build_group_scratch_bytes(size_t max_links) {
+    return 6 * align256(max_links * 4) + align256(16) + align256(cub_temp_bytes(max_links)) + 256;
+}
+
+hipError_t build_group_links(BuildArgs& a, void* scratch, size_t scratch_bytes, uint32_t* err, hipStream_t s) {
+    const size_t N = (size_t)a.B * a.M;
+    if (N == 0 || build_group_scratch_bytes(N) > scratch_bytes) return hipErrorInvalidValue;
+    GroupScratch g = carve(scratch, N);
+    const uint32_t key_unused = a.s + a.B;
+    int end_bit = 1;
+    while (end_bit < 32 && (1ull << end_bit) <= key_unused) end_bit++;
+    const uint32_t blocks = (uint32_t)((N + 255) / 256);
+    hipLaunchKernelGGL(zero_words_kernel, dim3(1), dim3(64), 0, s, g.words, 4u);
+    hipLaunchKernelGGL(link_pairs_kernel, dim3(blocks), dim3(256), 0, s, a.sel, a.B, a.M, a.s, g.keys_in, g.vals_in,
+                       err);
+    size_t tb = g.temp_bytes;
+    hipError_t e = hipcub::DeviceRadixSort::SortPairs(g.temp, tb, g.keys_in, g.keys, g.vals_in, g.vals, (int)N, 0,
+                                                      end_bit, s);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(link_runs_kernel, dim3(blocks), dim3(256), 0, s, g.keys, (uint32_t)N, key_unused, g.flags,
+                       g.words + 1);
+    tb = g.temp_bytes;
+    e = hipcub::DeviceSelect::Flagged(g.temp, tb, hipcub::CountingInputIterator<uint32_t>(0), g.flags, g.starts,
+                                      g.words, (int)N, s);
+    if (e != hipSuccess) return e;
+    a.keys = g.keys;
+    a.incoming = g.vals;
+    a.starts = g.starts;
+    a.n_targets = g.words;
+    a.n_links = g.words + 1;
+    return hipGetLastError();
+}
+
 size_t build_select_lds(uint32_t dim, uint32_t efc) {
     const size_t dimpad = (dim + 3u) & ~3u;
     const size_t capL = (efc + 63u) & ~63u;
@@ -239,7 +350,7 @@ static hipError_t launch_build_t(const BuildArgs& a, bool link, hipStream_t s) {
         hipError_t e = hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
     }
-    const uint32_t grid = link ? a.T : a.B;
+    const uint32_t grid = link ? a.B * a.M : a.B;  // link: receivers beyond *n_targets exit
     if (grid == 0) return hipSuccess;
     hipLaunchKernelGGL(k, dim3(grid), dim3(64), lds, s, a);
     return hipGetLastError();

@@ -1286,7 +1286,7 @@ HG_API int hg_hnsw_build_gpu(void* hv, const float* data, size_t rows, const uin
         const uint32_t M = (uint32_t)h->M();
         const uint32_t maxM0 = gi->g.maxM0;
         const size_t bmax = 16384;
-        DevBuf ident, cand_ids, cand_d, sel, targets, offsets, incoming;
+        DevBuf ident, cand_ids, cand_d, sel, group, err;
         {
             std::vector<uint64_t> id(n);
             for (size_t i = 0; i < n; i++) id[i] = i;
@@ -1296,18 +1296,16 @@ HG_API int hg_hnsw_build_gpu(void* hv, const float* data, size_t rows, const uin
         cand_ids.reserve(bmax * efc * 8);
         cand_d.reserve(bmax * efc * 4);
         sel.reserve(bmax * M * 4);
-        targets.reserve(bmax * M * 4);
-        offsets.reserve((bmax * M + 1) * 4);
-        incoming.reserve(bmax * M * 4);
+        const size_t group_bytes = hg::build_group_scratch_bytes(bmax * M);
+        group.reserve(group_bytes);
+        err.reserve(256);
+        hip_check(hg::launch_zero_u32((uint32_t*)err.p, 1, s), "zero launch");
         std::vector<int> level(n);
         for (size_t i = 0; i < n; i++) level[i] = h->element_level((uint32_t)i);
         // entry point of the graph of elements [0, s): the first element of the highest level
         uint32_t entry = 0;
         int maxl = level[0];
         size_t seen = 1;
-        std::vector<uint32_t> selh;
-        std::vector<std::pair<uint32_t, uint32_t>> pairs;
-        std::vector<uint32_t> th, oh, ih;
         GpuIndex::LaunchWords* lw = next_words(gi.get());
         prepare_words(lw, bmax, false);
         for (size_t sb = std::max<size_t>(s0, 1); sb < n;) {
@@ -1344,42 +1342,16 @@ HG_API int hg_hnsw_build_gpu(void* hv, const float* data, size_t rows, const uin
             ba.cand_d = (const float*)cand_d.p;
             ba.sel = (uint32_t*)sel.p;
             hip_check(hg::launch_build(ba, false, s), "build select launch");
-            // reverse links, grouped by the element that receives them
-            selh.resize(B * M);
-            hip_check(hipMemcpyAsync(selh.data(), sel.p, B * M * 4, hipMemcpyDeviceToHost, s), "hipMemcpy D2H");
-            hip_check(hipStreamSynchronize(s), "build");
-            pairs.clear();
-            for (size_t i = 0; i < B; i++)
-                for (size_t j = 0; j < M; j++) {
-                    const uint32_t t = selh[i * M + j];
-                    if (t == hg::SENT) break;
-                    require(t < sb + i, "cand error");
-                    pairs.emplace_back(t, (uint32_t)(sb + i));
-                }
-            std::sort(pairs.begin(), pairs.end());
-            th.clear();
-            oh.clear();
-            ih.clear();
-            for (size_t i = 0; i < pairs.size(); i++) {
-                if (i == 0 || pairs[i].first != pairs[i - 1].first) {
-                    th.push_back(pairs[i].first);
-                    oh.push_back((uint32_t)i);
-                }
-                ih.push_back(pairs[i].second);
-            }
-            oh.push_back((uint32_t)pairs.size());
-            if (!th.empty()) {
-                hip_check(hipMemcpyAsync(targets.p, th.data(), th.size() * 4, hipMemcpyHostToDevice, s), "H2D");
-                hip_check(hipMemcpyAsync(offsets.p, oh.data(), oh.size() * 4, hipMemcpyHostToDevice, s), "H2D");
-                hip_check(hipMemcpyAsync(incoming.p, ih.data(), ih.size() * 4, hipMemcpyHostToDevice, s), "H2D");
-                ba.targets = (const uint32_t*)targets.p;
-                ba.offsets = (const uint32_t*)offsets.p;
-                ba.incoming = (const uint32_t*)incoming.p;
-                ba.T = (uint32_t)th.size();
-                hip_check(hg::launch_build(ba, true, s), "build link launch");
-            }
+            // reverse links, grouped on the device by the element that receives them (a stable
+            // sort: new elements in ascending order within a receiver); no host round trip
+            hip_check(hg::build_group_links(ba, group.p, group_bytes, (uint32_t*)err.p, s), "build link grouping");
+            hip_check(hg::launch_build(ba, true, s), "build link launch");
             sb += B;
         }
+        uint32_t bad = 0;
+        hip_check(hipMemcpyAsync(&bad, err.p, 4, hipMemcpyDeviceToHost, s), "hipMemcpy D2H");
+        hip_check(hipStreamSynchronize(s), "build");
+        require(bad == 0, "cand error");
         // level-0 lists back into the CPU index
         std::vector<uint32_t> l0(n * maxM0);
         hip_check(hipMemcpyAsync(l0.data(), gi->links0.p, l0.size() * 4, hipMemcpyDeviceToHost, s), "hipMemcpy D2H");

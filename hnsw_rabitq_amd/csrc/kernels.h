@@ -33,15 +33,25 @@ struct BuildArgs {
     const uint64_t* cand_ids;   // [B][efc] graph-search candidates (element ids, ~0 = none)
     const float* cand_d;        // [B][efc]
     uint32_t* sel;              // [B][M] selected neighbours (SENT padded)
-    const uint32_t* targets;    // [T] elements that receive links
-    const uint32_t* offsets;    // [T + 1] into incoming
-    const uint32_t* incoming;   // new elements, grouped by target, ascending
-    uint32_t T;
+    // reverse links of the batch, sorted by the element that receives them (all on the device)
+    const uint32_t* keys;       // [B*M] receiving element of each link, ascending (then unused slots)
+    const uint32_t* incoming;   // [B*M] new element of each link, ascending within a receiver
+    const uint32_t* starts;     // [*n_targets] first link of each receiver
+    const uint32_t* n_targets;  // number of receivers (device word)
+    const uint32_t* n_links;    // number of links (device word)
 };
 size_t build_select_lds(uint32_t dim, uint32_t efc);
 size_t build_link_lds(uint32_t dim);
-// link = false: build_select_kernel over the batch; true: build_link_kernel over the targets
+// link = false: build_select_kernel over the batch; true: build_link_kernel over the receivers
+// (launched with B*M workgroups, those beyond *n_targets exit at once)
 hipError_t launch_build(const BuildArgs& a, bool link, hipStream_t s);
+// Scratch of build_group_links for batches of up to max_links links.
+size_t build_group_scratch_bytes(size_t max_links);
+// Reverse links of the batch [s, s + B) from sel: pairs (receiver, new element), a stable radix
+// sort by receiver, and the first link of each receiver; fills keys / incoming / starts /
+// n_targets / n_links of a (all device memory inside `scratch`, no host synchronisation).
+// Sets *err to 1 when a selected neighbour is not an earlier element.
+hipError_t build_group_links(BuildArgs& a, void* scratch, size_t scratch_bytes, uint32_t* err, hipStream_t s);
 
 // ---- brute force (bruteforce.hip) ----
 struct BfData {
