@@ -42,6 +42,7 @@ SIGNATURES = {
     "hg_gpu_set_plan": [_vp, _sz, _sz, ctypes.c_uint32, _i32],
     "hg_gpu_id_labels": [_vp, _vp, _sz, _vp, _vp],
     "hg_hnsw_build_gpu": [_vp, _vp, _sz, _vp, _i32, _i32],
+    "hg_hnsw_build_gpu_upper_min_rows": [_sz],
     "hg_gpu_index_save": [_vp, ctypes.c_char_p],
     "hg_gpu_index_load": [ctypes.c_char_p, _i32, _vp],
     "hg_hnsw_add_items_upper": [_vp, _vp, _sz, _vp, _i32],

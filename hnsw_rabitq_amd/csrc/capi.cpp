@@ -436,9 +436,9 @@ GpuIndex* finish_export(HostLayout&& L, int device) {
     std::unique_ptr<GpuIndex> gi(new GpuIndex());
     gi->device = device;
     gi->serial = ++g_export_serial;
-    hipDeviceProp_t prop;
-    hip_check(hipGetDeviceProperties(&prop, device), "hipGetDeviceProperties");
-    gi->cus = prop.multiProcessorCount;
+    // one attribute (hipGetDeviceProperties fills every field and costs a large fraction of a second)
+    hip_check(hipDeviceGetAttribute(&gi->cus, hipDeviceAttributeMultiprocessorCount, device),
+              "hipDeviceGetAttribute");
     auto upload = [&](DevBuf& b, const void* src, size_t bytes) {
         b.reserve(std::max<size_t>(bytes, 16));
         hip_check(hipMemcpy(b.p, src, bytes, hipMemcpyHostToDevice), "hipMemcpy H2D");
@@ -1262,23 +1262,16 @@ HG_API int hg_gpu_search_multivector_device(void* gv, const float* dq, size_t nq
 // addPoint (hnswalg.h:1152-1266) for a batch of rows: the CPU inserts them into the levels
 // above 0 (those do not depend on level 0), then the GPU builds their level-0 links in batches
 // that grow with the graph (build.hip), and the lists are written back into the CPU index.
-HG_API int hg_hnsw_build_gpu(void* hv, const float* data, size_t rows, const uint64_t* labels, int num_threads,
-                             int device) {
-    return guarded([&] {
-        require(hv != nullptr, "null index");
-        require(rows == 0 || data != nullptr, "null data");
-        auto* h = static_cast<hnswgpu::HnswIndex*>(hv);
-        require(h->num_deleted() == 0, "the GPU build needs an index without deleted elements");
-        require(h->ef_construction() >= 1 && h->ef_construction() <= 1024,
-                "the GPU build supports ef_construction up to 1024");
-        require(h->maxM0() <= 64, "M above 32 is not supported by the GPU search");
-        const size_t s0 = h->count();
-        if (rows == 0) return HG_OK;
-        h->add_items_upper(data, rows, labels, num_threads);
+// Fresh builds of at least this many rows also build the levels above 0 on the GPU.
+static size_t g_gpu_upper_min_rows = 250000;
+
+// Level-0 lists of the elements [s0, count) of h (their upper levels already linked) on the GPU.
+static void gpu_build_level0(hnswgpu::HnswIndex* h, size_t s0, int device) {
+    {
         const size_t n = h->count();
         void* gv = nullptr;
-        int rc = hg_gpu_index_create(hv, device, &gv);
-        if (rc != HG_OK) return rc;
+        int rc = hg_gpu_index_create(h, device, &gv);
+        if (rc != HG_OK) throw Error(rc, hg_last_error());
         std::unique_ptr<GpuIndex, int (*)(void*)> gi(static_cast<GpuIndex*>(gv), hg_gpu_index_free);
         DeviceGuard dg(gi->device);
         hipStream_t s = gi->stream;
@@ -1309,7 +1302,9 @@ HG_API int hg_hnsw_build_gpu(void* hv, const float* data, size_t rows, const uin
         GpuIndex::LaunchWords* lw = next_words(gi.get());
         prepare_words(lw, bmax, false);
         for (size_t sb = std::max<size_t>(s0, 1); sb < n;) {
-            const size_t B = std::min(std::min(bmax, std::max<size_t>(1, sb / 16)), n - sb);
+            // at least 256 rows: a small batch costs one beam search's latency whatever its size,
+            // and the rows of a batch see one another exactly (brute force in the select kernel)
+            const size_t B = std::min(std::min(bmax, std::max<size_t>(256, sb / 16)), n - sb);
             for (; seen < sb; seen++)
                 if (level[seen] > maxl) {
                     maxl = level[seen];
@@ -1362,6 +1357,81 @@ HG_API int hg_hnsw_build_gpu(void* hv, const float* data, size_t rows, const uin
             while (c < maxM0 && r[c] != hg::SENT) c++;
             h->set_links0((uint32_t)i, r, c);
         }
+    }
+}
+
+// Rows into h with every level built on the GPU where it pays.  The elements of level >= 1 form
+// an HNSW of their own one level down, whose level 0 follows the reference's upper-level rules
+// when its lists hold maxM = M links: it is built by this same function (recursively, down to
+// a size the CPU inserts quickly), its lists are copied back as levels >= 1, and then level 0 is
+// built over the whole index.  levels: optional given levels of the rows.
+static void build_levels_gpu(hnswgpu::HnswIndex* h, const float* data, size_t rows, const uint64_t* labels,
+                             const int* levels, int num_threads, int device) {
+    const size_t s0 = h->count();
+    if (s0 != 0 || rows < g_gpu_upper_min_rows) {
+        h->add_items_upper(data, rows, labels, num_threads, levels);
+        gpu_build_level0(h, s0, device);
+        return;
+    }
+    std::vector<int> lv = levels ? std::vector<int>(levels, levels + rows) : h->draw_levels(rows);
+    h->register_items(data, rows, labels, lv.data(), num_threads);
+    const size_t n = h->count();
+    std::vector<uint32_t> up;  // elements of level >= 1, in id order
+    int top = 0;
+    for (size_t i = 0; i < n; i++) {
+        const int l = h->element_level((uint32_t)i);
+        if (l >= 1) up.push_back((uint32_t)i);
+        top = std::max(top, l);
+    }
+    if (up.empty()) {
+        h->set_entry(0, 0);
+    } else {
+        const size_t m = up.size(), dim = h->dim();
+        hnswgpu::HnswIndex sub(h->space(), dim, m, h->M(), h->ef_construction(), 100, h->maxM());
+        std::vector<float> sv(m * dim);
+        std::vector<int> sl(m);
+        for (size_t k = 0; k < m; k++) {
+            memcpy(&sv[k * dim], h->vector(up[k]), dim * sizeof(float));
+            sl[k] = h->element_level(up[k]) - 1;
+        }
+        build_levels_gpu(&sub, sv.data(), m, nullptr, sl.data(), num_threads, device);
+        // element ids of the sub-index are its insertion order (labels 0..m-1 in row order, ids
+        // assigned as the rows register); map them through the labels
+        std::vector<uint32_t> pos(m);
+        for (size_t k = 0; k < m; k++) pos[k] = up[(size_t)sub.label((uint32_t)k)];
+        std::vector<uint32_t> ids(std::max(h->maxM(), h->maxM0()));
+        for (size_t k = 0; k < m; k++) {
+            const uint32_t e = pos[k];
+            const int le = h->element_level(e);
+            for (int l = 1; l <= le; l++) {
+                const uint32_t* src = l == 1 ? sub.linklist0((uint32_t)k) : sub.linklist((uint32_t)k, l - 1);
+                const size_t c = hnswgpu::HnswIndex::list_count(src);
+                for (size_t j = 0; j < c; j++) ids[j] = pos[src[1 + j]];
+                h->set_links(e, l, ids.data(), c);
+            }
+        }
+        h->set_entry(pos[sub.entry()], top);
+    }
+    gpu_build_level0(h, 0, device);
+}
+
+HG_API int hg_hnsw_build_gpu_upper_min_rows(size_t rows) {
+    g_gpu_upper_min_rows = rows;
+    return HG_OK;
+}
+
+HG_API int hg_hnsw_build_gpu(void* hv, const float* data, size_t rows, const uint64_t* labels, int num_threads,
+                             int device) {
+    return guarded([&] {
+        require(hv != nullptr, "null index");
+        require(rows == 0 || data != nullptr, "null data");
+        auto* h = static_cast<hnswgpu::HnswIndex*>(hv);
+        require(h->num_deleted() == 0, "the GPU build needs an index without deleted elements");
+        require(h->ef_construction() >= 1 && h->ef_construction() <= 1024,
+                "the GPU build supports ef_construction up to 1024");
+        require(h->maxM0() <= 64, "M above 32 is not supported by the GPU search");
+        if (rows == 0) return HG_OK;
+        build_levels_gpu(h, data, rows, labels, nullptr, num_threads, device);
         return HG_OK;
     });
 }

@@ -61,7 +61,8 @@ void parallel_for(size_t start, size_t end, int num_threads, F fn) {
 
 }  // namespace
 
-HnswIndex::HnswIndex(int space, size_t dim, size_t max_elements, size_t M, size_t ef_construction, size_t random_seed)
+HnswIndex::HnswIndex(int space, size_t dim, size_t max_elements, size_t M, size_t ef_construction, size_t random_seed,
+                     size_t maxM0)
     : space_(space),
       dim_(dim),
       fn_(distance_fn(space)),
@@ -76,7 +77,7 @@ HnswIndex::HnswIndex(int space, size_t dim, size_t max_elements, size_t M, size_
         M_ = 10000;
     }
     maxM_ = M_;
-    maxM0_ = M_ * 2;
+    maxM0_ = maxM0 ? maxM0 : M_ * 2;
     ef_construction_ = std::max(ef_construction, M_);
     ef_ = 10;
     level_generator_.seed(random_seed);
@@ -355,7 +356,7 @@ tableint HnswIndex::connect_new_element(const float* point, tableint cur_c, MaxH
     return next_closest;
 }
 
-tableint HnswIndex::insert(const float* point, labeltype lab) {
+tableint HnswIndex::insert(const float* point, labeltype lab, int level) {
     tableint cur_c;
     {
         std::lock_guard<std::mutex> g(label_lookup_lock_);
@@ -368,18 +369,14 @@ tableint HnswIndex::insert(const float* point, labeltype lab) {
         label_lookup_[lab] = cur_c;
     }
     std::unique_lock<std::mutex> lock_el(link_list_locks_[cur_c]);
-    int curlevel;
-    {
+    int curlevel = level;
+    if (curlevel < 0) {
         std::lock_guard<std::mutex> g(visited_lock_);  // the generator is shared by all inserters
         curlevel = random_level(mult_);
     }
     element_levels_[cur_c] = curlevel;
 
-    std::unique_lock<std::mutex> templock(global_);
-    int maxlevelcopy = maxlevel_;
-    if (curlevel <= maxlevelcopy) templock.unlock();
-    tableint curr_obj = enterpoint_node_;
-
+    // the element's own record (private to this inserter: its lock is held)
     memset(level0_ + cur_c * size_data_per_element_ + offset_level0_, 0, size_data_per_element_);
     memcpy(level0_ + cur_c * size_data_per_element_ + label_offset_, &lab, sizeof(labeltype));
     memcpy(level0_ + cur_c * size_data_per_element_ + offset_data_, point, dim_ * sizeof(float));
@@ -388,6 +385,11 @@ tableint HnswIndex::insert(const float* point, labeltype lab) {
         if (!link_lists_[cur_c]) throw std::runtime_error("Not enough memory: addPoint failed to allocate linklist");
         memset(link_lists_[cur_c], 0, size_links_per_element_ * curlevel + 1);
     }
+
+    std::unique_lock<std::mutex> templock(global_);
+    int maxlevelcopy = maxlevel_;
+    if (curlevel <= maxlevelcopy) templock.unlock();
+    tableint curr_obj = enterpoint_node_;
 
     if ((int32_t)curr_obj != -1 && !(skip_level0_ && curlevel == 0)) {
         if (curlevel < maxlevelcopy) {
@@ -437,31 +439,96 @@ tableint HnswIndex::insert(const float* point, labeltype lab) {
     return cur_c;
 }
 
-void HnswIndex::add_items(const float* data, size_t rows, const labeltype* labels, int num_threads) {
+void HnswIndex::add_items(const float* data, size_t rows, const labeltype* labels, int num_threads,
+                          const int* levels) {
     if (num_threads <= 0) num_threads = (int)std::thread::hardware_concurrency();
     if (rows <= (size_t)num_threads * 4) num_threads = 1;
     size_t start = 0;
     labeltype base = next_label_;
     if (!ep_added_ && rows > 0) {
-        insert(data, labels ? labels[0] : base);
+        insert(data, labels ? labels[0] : base, levels ? levels[0] : -1);
         start = 1;
         ep_added_ = true;
     }
     parallel_for(start, rows, num_threads, [&](size_t row) {
-        insert(data + row * dim_, labels ? labels[row] : base + row);
+        insert(data + row * dim_, labels ? labels[row] : base + row, levels ? levels[row] : -1);
     });
     next_label_ += rows;
 }
 
-void HnswIndex::add_items_upper(const float* data, size_t rows, const labeltype* labels, int num_threads) {
+void HnswIndex::add_items_upper(const float* data, size_t rows, const labeltype* labels, int num_threads,
+                                const int* levels) {
     skip_level0_ = true;
     try {
-        add_items(data, rows, labels, num_threads);
+        add_items(data, rows, labels, num_threads, levels);
     } catch (...) {
         skip_level0_ = false;
         throw;
     }
     skip_level0_ = false;
+}
+
+void HnswIndex::register_items(const float* data, size_t rows, const labeltype* labels, const int* levels,
+                               int num_threads) {
+    if (num_threads <= 0) num_threads = (int)std::thread::hardware_concurrency();
+    const size_t s0 = cur_element_count_;
+    if (rows > max_elements_ - std::min(s0, max_elements_))
+        throw std::runtime_error("The number of elements exceeds the specified limit");
+    const labeltype base = next_label_;
+    // labels first, one thread (the table is reserved once): a label already present throws
+    // before anything is stored
+    label_lookup_.reserve(s0 + rows);
+    for (size_t r = 0; r < rows; r++) {
+        const labeltype lab = labels ? labels[r] : base + r;
+        if (!label_lookup_.emplace(lab, (tableint)(s0 + r)).second) {
+            for (size_t q = 0; q < r; q++) label_lookup_.erase(labels ? labels[q] : base + q);
+            throw std::runtime_error("Replacing the vector of an existing label is not supported by this index");
+        }
+    }
+    // element s0 + r is row r: its record, level and empty lists (no other thread reads them),
+    // in chunks of 4,096 rows
+    const size_t chunk = 4096;
+    parallel_for(0, (rows + chunk - 1) / chunk, num_threads, [&](size_t ch) {
+        for (size_t r = ch * chunk; r < std::min(rows, (ch + 1) * chunk); r++) {
+            const tableint c = (tableint)(s0 + r);
+            const labeltype lab = labels ? labels[r] : base + r;
+            const int l = levels[r];
+            element_levels_[c] = l;
+            char* rec = level0_ + c * size_data_per_element_;
+            memset(rec + offset_level0_, 0, size_data_per_element_);
+            memcpy(rec + label_offset_, &lab, sizeof(labeltype));
+            memcpy(rec + offset_data_, data + r * dim_, dim_ * sizeof(float));
+            if (l) {
+                link_lists_[c] = (char*)malloc(size_links_per_element_ * l + 1);
+                if (!link_lists_[c]) throw std::runtime_error("Not enough memory: addPoint failed to allocate linklist");
+                memset(link_lists_[c], 0, size_links_per_element_ * l + 1);
+            }
+        }
+    });
+    cur_element_count_ = s0 + rows;
+    next_label_ += rows;
+    if (rows > 0) ep_added_ = true;
+}
+
+std::vector<int> HnswIndex::draw_levels(size_t rows) {
+    std::vector<int> lv(rows);
+    std::lock_guard<std::mutex> g(visited_lock_);
+    for (auto& l : lv) l = random_level(mult_);
+    return lv;
+}
+
+void HnswIndex::set_links(tableint i, int level, const uint32_t* ids, size_t count) {
+    if (i >= cur_element_count_ || level < 1 || level > element_levels_[i] || count > maxM_)
+        throw std::runtime_error("Bad value of sz_link_list_other");
+    uint32_t* ll = ll_mut(i, level);
+    set_list_count(ll, (unsigned short)count);
+    for (size_t j = 0; j < maxM_; j++) ll[1 + j] = j < count ? ids[j] : 0u;
+}
+
+void HnswIndex::set_entry(tableint entry, int maxlevel) {
+    std::lock_guard<std::mutex> g(global_);
+    enterpoint_node_ = entry;
+    maxlevel_ = maxlevel;
 }
 
 void HnswIndex::set_links0(tableint i, const uint32_t* ids, size_t count) {

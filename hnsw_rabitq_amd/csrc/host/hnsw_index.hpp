@@ -46,7 +46,10 @@ struct HostGraphView {
 
 class HnswIndex {
  public:
-    HnswIndex(int space, size_t dim, size_t max_elements, size_t M, size_t ef_construction, size_t random_seed);
+    // maxM0 = 0: 2 * M, as the reference; another value gives a graph whose level 0 follows the
+    // rules of the reference's upper levels when maxM0 = M (used to build those levels)
+    HnswIndex(int space, size_t dim, size_t max_elements, size_t M, size_t ef_construction, size_t random_seed,
+              size_t maxM0 = 0);
     // Load from a reference-format file (hnswalg.h:715-821).  max_elements == 0 keeps the
     // stored capacity, as the reference does.
     HnswIndex(int space, size_t dim, const std::string& path, size_t max_elements);
@@ -58,11 +61,23 @@ class HnswIndex {
     // Insert rows; labels may be null (then next_label_ + row).  num_threads <= 0 uses all
     // hardware threads; the first point of an empty index is inserted alone first, and batches
     // of at most 4 * num_threads rows run single-threaded (bindings.cpp:251-304).
-    void add_items(const float* data, size_t rows, const labeltype* labels, int num_threads);
+    // levels: optional level of each row (else drawn from the index's generator).
+    void add_items(const float* data, size_t rows, const labeltype* labels, int num_threads,
+                   const int* levels = nullptr);
     // The same, linking only the levels above 0 (the level-0 lists of the new elements stay
     // empty for a level-0 builder to fill).  The upper levels do not depend on level 0, so they
     // are those add_items builds.
-    void add_items_upper(const float* data, size_t rows, const labeltype* labels, int num_threads);
+    void add_items_upper(const float* data, size_t rows, const labeltype* labels, int num_threads,
+                         const int* levels = nullptr);
+    // Rows stored with the given levels and no links at any level (empty lists; the entry point
+    // and maximum level are left to set_entry), for a builder that fills every level.
+    void register_items(const float* data, size_t rows, const labeltype* labels, const int* levels,
+                        int num_threads);
+    // The levels of the next `rows` insertions, drawn as a one-thread add_items would draw them.
+    std::vector<int> draw_levels(size_t rows);
+    // Replaces the level-`level` (>= 1) link list of element i.
+    void set_links(tableint i, int level, const uint32_t* ids, size_t count);
+    void set_entry(tableint entry, int maxlevel);
     // Replaces the level-0 link list of element i (count entries; the delete mark is kept).
     void set_links0(tableint i, const uint32_t* ids, size_t count);
     void save(const std::string& path) const;
@@ -129,7 +144,7 @@ class HnswIndex {
 
     void init_layout();
     int random_level(double reverse_size);
-    tableint insert(const float* point, labeltype label);
+    tableint insert(const float* point, labeltype label, int level = -1);
     MaxHeap search_layer(tableint ep, const float* point, int layer, Visited& vis);
     void select_neighbors(MaxHeap& top, size_t M);
     tableint connect_new_element(const float* point, tableint cur_c, MaxHeap& top, int level);

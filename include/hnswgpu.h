@@ -86,17 +86,21 @@ int hg_hnsw_mark_deleted(void* index, uint64_t label);
 int hg_hnsw_unmark_deleted(void* index, uint64_t label);
 int hg_hnsw_get_ef(void* index, size_t* ef);
 int hg_hnsw_free(void* index);
-/* add_items with the level-0 graph built on the GPU (device = HIP device): the rows are
- * inserted into the levels above 0 on the CPU (num_threads as hg_hnsw_add_items; the upper
- * levels do not depend on level 0), then their level-0 links are built in batches of up to
- * 16,384 rows that grow with the graph: a beam search with ef_construction over the elements
- * before the batch, the earlier rows of the same batch added to the candidates by brute force,
- * the neighbour heuristic, and mutual linking with re-selection of full lists
- * (hnswalg.h:225-305, 443-483, 506-630).  The index must hold no deleted elements;
- * ef_construction <= 1024.  If the call fails after the rows were inserted above level 0, the
- * index holds them with empty level-0 lists: discard it. */
+/* add_items with the graph built on the GPU (device = HIP device).  Level 0 is built in batches
+ * of up to 16,384 rows that grow with the graph: a beam search with ef_construction over the
+ * elements before the batch, the earlier rows of the same batch added to the candidates by brute
+ * force, the neighbour heuristic, and mutual linking with re-selection of full lists
+ * (hnswalg.h:225-305, 443-483, 506-630).  The levels above 0 do not depend on level 0: they are
+ * inserted on the CPU (num_threads as hg_hnsw_add_items), or, for a build of an empty index of
+ * at least hg_hnsw_build_gpu_upper_min_rows rows, built the same way on the GPU as the level 0
+ * of the HNSW that the elements of level >= 1 form one level down (lists of maxM = M links, the
+ * reference's rule for those levels).  The index must hold no deleted elements;
+ * ef_construction <= 1024.  If the call fails part way, discard the index. */
 int hg_hnsw_build_gpu(void* index, const float* data, size_t rows, const uint64_t* labels, int num_threads,
                       int device);
+/* Row count from which hg_hnsw_build_gpu builds the levels above 0 on the GPU too (default
+ * 250,000).  No counterpart in the reference: a tuning knob of the GPU build. */
+int hg_hnsw_build_gpu_upper_min_rows(size_t rows);
 /* Its first step alone: insert rows into the levels above 0 (level-0 lists left empty). */
 int hg_hnsw_add_items_upper(void* index, const float* data, size_t rows, const uint64_t* labels, int num_threads);
 /* Index.normalize_vector (bindings.cpp:241-248) applied row by row (the cosine space). */
