@@ -29,7 +29,7 @@ def main():
     ap.add_argument("--reps", type=int, default=5)
     ap.add_argument("--recall-sample", type=int, default=2000)
     ap.add_argument("--index-cache", default="")
-    ap.add_argument("--gpu-build", action="store_true", help="build level 0 on the GPU (add_items_gpu)")
+    ap.add_argument("--gpu-build", action="store_true", help="build the graph on the GPU (add_items_gpu)")
     args = ap.parse_args()
     import hnsw_rabitq_amd as H
     from hnsw_rabitq_amd import DeviceArray, _lib, datasets
@@ -53,7 +53,7 @@ def main():
         idx.init_index(args.n, M=16, ef_construction=200, random_seed=100)
         if args.gpu_build:
             idx.add_items_gpu(data, np.arange(args.n, dtype=np.uint64))
-            log(f"  inserted {args.n} (level 0 built on the GPU)")
+            log(f"  inserted {args.n} (graph built on the GPU)")
         else:
             step = 500_000
             for s in range(0, args.n, step):
@@ -103,7 +103,7 @@ def main():
     t = float(np.median(ms))
     out = dict(workload=f"hnsw_ip_gmm_{args.n}x{args.dim}_q{nq}_ef16-512_k1-10-100", qps=nq / (t * 1e-3),
                ms_per_batch=t, graph_replays=args.reps, plan=plan, recall_by_k=rec, build_s=build_s,
-               build="gpu level 0" if args.gpu_build else "cpu",
+               build="gpu" if args.gpu_build else "cpu",
                ef_mean=float(ef.mean()), k_counts={str(kv): int((k == kv).sum()) for kv in sorted(set(k.tolist()))})
     _lib.call("hg_graph_free", graph)
     print(json.dumps(out), flush=True)
