@@ -23,91 +23,113 @@
 namespace hg {
 
 // ------------------------------------------------------------------------------------------
-// Exact distance tile, dim % 16 == 0.  32 queries x 32 rows per workgroup of 256 threads; each
-// thread owns 2 queries x 2 rows and keeps the 16 reference accumulator lanes per pair, so the
-// dimension can be streamed through LDS in chunks of 64 coordinates without changing the order
-// of any accumulation: acc[l] takes coordinates l, l+16, l+32, ... in increasing order.
+// Exact distance tile, dim % 16 == 0.  32 queries x 32 rows per workgroup of 256 threads.  The
+// reference keeps 16 accumulator lanes per (query, row) pair, acc[l] taking coordinates l, l+16,
+// l+32, ... in increasing order; here the 16 lanes of a pair sit in 16 threads: thread (group g,
+// lane l) keeps lane l of 8 queries x 8 rows (queries 8*(g/4) + i, rows g%4 + 4*j), so one
+// coordinate of 8 queries and 8 rows (16 LDS reads) feeds 64 FMAs, and the dimension streams
+// through LDS in chunks of 64 coordinates without changing the order of any accumulation.  At
+// the end the 16 lanes of each pair are summed in the reference's order through LDS.
 // ------------------------------------------------------------------------------------------
-constexpr int TQ = 32, TR = 32, TC = 64, TPAD = 4;
+constexpr int TQ = 32, TR = 32, TC = 64;
+constexpr int TS = 80;  // LDS row stride (floats): rows one apart fall 16 banks apart
 
 template <int SPACE>
 __global__ __launch_bounds__(256) void bf_dist_tile_kernel(const float* __restrict__ Q, uint32_t nq,
                                                            const float* __restrict__ X, uint32_t n, uint32_t dim,
                                                            float* __restrict__ out, size_t ld_out) {
-    __shared__ __align__(16) float Qs[TQ][TC + TPAD];
-    __shared__ __align__(16) float Xs[TR][TC + TPAD];
-    const int tid = threadIdx.x;
-    const uint32_t q0 = blockIdx.y * TQ, r0 = blockIdx.x * TR;
-    const int tq = (tid >> 4) * 2, tr = (tid & 15) * 2;
-    float acc[4][16];
+    __shared__ __align__(16) float Qs[TQ][TS];
+    __shared__ __align__(16) float Xs[TR][TS];
+    const int tid = threadIdx.x, l = tid & 15, grp = tid >> 4;
+    const int qb = (grp >> 2) * 8, rb = grp & 3;
+    // query tiles vary fastest, so the tiles that share a row tile run together and the rows
+    // come from HBM about once
+    const uint32_t qt = (nq + TQ - 1) / TQ;
+    const uint32_t q0 = (blockIdx.x % qt) * TQ, r0 = (blockIdx.x / qt) * TR;
+    float acc[8][8];
 #pragma unroll
-    for (int p = 0; p < 4; p++)
+    for (int i = 0; i < 8; i++)
 #pragma unroll
-        for (int l = 0; l < 16; l++) acc[p][l] = 0.f;
+        for (int j = 0; j < 8; j++) acc[i][j] = 0.f;
 
+    // this thread's two float4 granules of the query and of the row chunk (32 x 64 floats each),
+    // the next chunk's loaded into registers while the current one is used
+    static_assert(TQ * (TC / 4) == 512 && TR == TQ, "two granules per thread");
+    float4 pq[2], px[2];
+    auto gload = [&](uint32_t c0) {
+        const uint32_t cw = dim - c0 < (uint32_t)TC ? dim - c0 : (uint32_t)TC;
+#pragma unroll
+        for (int k = 0; k < 2; k++) {
+            const int i = tid + 256 * k, rr = i / (TC / 4), cc = (i % (TC / 4)) * 4;
+            pq[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+            px[k] = make_float4(0.f, 0.f, 0.f, 0.f);
+            if (q0 + rr < nq && (uint32_t)cc < cw)
+                pq[k] = *reinterpret_cast<const float4*>(Q + (size_t)(q0 + rr) * dim + c0 + cc);
+            if (r0 + rr < n && (uint32_t)cc < cw)
+                px[k] = *reinterpret_cast<const float4*>(X + (size_t)(r0 + rr) * dim + c0 + cc);
+        }
+    };
+    gload(0);
     for (uint32_t c0 = 0; c0 < dim; c0 += TC) {
         const uint32_t cw = dim - c0 < (uint32_t)TC ? dim - c0 : (uint32_t)TC;  // multiple of 16
-        // load 32 x cw of queries and rows (float4 granules)
-        for (int i = tid; i < TQ * (TC / 4); i += 256) {
-            int rr = i / (TC / 4), cc = (i % (TC / 4)) * 4;
-            float4 v = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (q0 + rr < nq && (uint32_t)cc < cw)
-                v = *reinterpret_cast<const float4*>(Q + (size_t)(q0 + rr) * dim + c0 + cc);
-            *reinterpret_cast<float4*>(&Qs[rr][cc]) = v;
-            float4 w = make_float4(0.f, 0.f, 0.f, 0.f);
-            if (r0 + rr < n && (uint32_t)cc < cw)
-                w = *reinterpret_cast<const float4*>(X + (size_t)(r0 + rr) * dim + c0 + cc);
-            *reinterpret_cast<float4*>(&Xs[rr][cc]) = w;
+#pragma unroll
+        for (int k = 0; k < 2; k++) {
+            const int i = tid + 256 * k, rr = i / (TC / 4), cc = (i % (TC / 4)) * 4;
+            *reinterpret_cast<float4*>(&Qs[rr][cc]) = pq[k];
+            *reinterpret_cast<float4*>(&Xs[rr][cc]) = px[k];
         }
         __syncthreads();
+        if (c0 + TC < dim) gload(c0 + TC);
         for (uint32_t c = 0; c < cw; c += 16) {
+            float qv[8], xv[8];
 #pragma unroll
-            for (int l4 = 0; l4 < 16; l4 += 4) {
-                float4 qa = *reinterpret_cast<const float4*>(&Qs[tq][c + l4]);
-                float4 qb = *reinterpret_cast<const float4*>(&Qs[tq + 1][c + l4]);
-                float4 xa = *reinterpret_cast<const float4*>(&Xs[tr][c + l4]);
-                float4 xb = *reinterpret_cast<const float4*>(&Xs[tr + 1][c + l4]);
-                const float qv[2][4] = {{qa.x, qa.y, qa.z, qa.w}, {qb.x, qb.y, qb.z, qb.w}};
-                const float xv[2][4] = {{xa.x, xa.y, xa.z, xa.w}, {xb.x, xb.y, xb.z, xb.w}};
+            for (int i = 0; i < 8; i++) qv[i] = Qs[qb + i][c + l];
 #pragma unroll
-                for (int a = 0; a < 2; a++)
+            for (int j = 0; j < 8; j++) xv[j] = Xs[rb + 4 * j][c + l];
 #pragma unroll
-                    for (int b = 0; b < 2; b++)
+            for (int i = 0; i < 8; i++)
 #pragma unroll
-                        for (int j = 0; j < 4; j++) {
-                            if (SPACE == 0) {
-                                float d = qv[a][j] - xv[b][j];
-                                acc[a * 2 + b][l4 + j] = fma_(d, d, acc[a * 2 + b][l4 + j]);
-                            } else {
-                                acc[a * 2 + b][l4 + j] = fma_(qv[a][j], xv[b][j], acc[a * 2 + b][l4 + j]);
-                            }
-                        }
-            }
+                for (int j = 0; j < 8; j++) {
+                    if (SPACE == 0) {
+                        const float d = qv[i] - xv[j];
+                        acc[i][j] = fma_(d, d, acc[i][j]);
+                    } else {
+                        acc[i][j] = fma_(qv[i], xv[j], acc[i][j]);
+                    }
+                }
         }
         __syncthreads();
     }
+    // the 16 lanes of each pair, one query index i at a time: lane values to LDS (pair p = 8g + j
+    // at p * 17, padded against bank conflicts), then one thread per pair sums them
+    float* R = &Qs[0][0];
 #pragma unroll
-    for (int a = 0; a < 2; a++)
+    for (int i = 0; i < 8; i++) {
 #pragma unroll
-        for (int b = 0; b < 2; b++) {
-            const float* A = acc[a * 2 + b];
+        for (int j = 0; j < 8; j++) R[(grp * 8 + j) * 17 + l] = acc[i][j];
+        __syncthreads();
+        if (tid < 128) {
+            const float* A = R + tid * 17;
             float r;
             if (SPACE == 0) {
                 r = A[0];
 #pragma unroll
-                for (int l = 1; l < 16; l++) r = r + A[l];
+                for (int m = 1; m < 16; m++) r = r + A[m];
             } else {
                 float h8[8], h4[4];
 #pragma unroll
-                for (int i = 0; i < 8; i++) h8[i] = A[i] + A[i + 8];
+                for (int m = 0; m < 8; m++) h8[m] = A[m] + A[m + 8];
 #pragma unroll
-                for (int i = 0; i < 4; i++) h4[i] = h8[i] + h8[i + 4];
+                for (int m = 0; m < 4; m++) h4[m] = h8[m] + h8[m + 4];
                 float s0 = h4[0] + h4[2], s1 = h4[1] + h4[3];
                 r = 1.0f - (s0 + s1);
             }
-            uint32_t qq = q0 + tq + a, rr = r0 + tr + b;
+            const int g = tid >> 3, j = tid & 7;
+            const uint32_t qq = q0 + (g >> 2) * 8 + i, rr = r0 + (g & 3) + 4 * j;
             if (qq < nq && rr < n) out[(size_t)qq * ld_out + rr] = r;
         }
+        __syncthreads();
+    }
 }
 
 // Any dimension: one thread per (query, row) pair, the reference kernel for the dimension class.
@@ -228,9 +250,10 @@ __global__ __launch_bounds__(SEL_THREADS) void bf_select_kernel(const float* __r
 
 size_t bf_exact_scratch_bytes(uint32_t nq, uint32_t k, uint32_t n) {
     (void)k;
-    // a chunk of queries x n distances, at most 1 GiB (at least one query row)
+    // a chunk of queries x n distances, at most 4 GiB (at least one query row): at 1M rows, 1,024
+    // queries per pass, so that the selection has four workgroups per CU
     size_t per_q = (size_t)n * 4;
-    size_t qc = std::max<size_t>(1, std::min<size_t>(nq, ((size_t)1 << 30) / std::max<size_t>(per_q, 1)));
+    size_t qc = std::max<size_t>(1, std::min<size_t>(nq, ((size_t)1 << 32) / std::max<size_t>(per_q, 1)));
     qc = (qc + TQ - 1) / TQ * TQ;
     return qc * per_q;
 }
@@ -244,7 +267,7 @@ hipError_t launch_bf_exact(const BfData& d, const float* q, uint32_t nq, uint32_
         uint32_t cq = nq - q0 < qc ? nq - q0 : qc;
         const float* qp = q + (size_t)q0 * d.dim;
         if (d.dim % 16 == 0) {
-            dim3 grid((d.n + TR - 1) / TR, (cq + TQ - 1) / TQ);
+            dim3 grid((uint32_t)(((size_t)d.n + TR - 1) / TR * ((cq + TQ - 1) / TQ)));
             if (d.space == 0)
                 hipLaunchKernelGGL(bf_dist_tile_kernel<0>, grid, dim3(256), 0, s, qp, cq, d.vec, d.n, d.dim, mat,
                                    (size_t)d.n);
