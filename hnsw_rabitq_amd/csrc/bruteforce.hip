@@ -1196,9 +1196,12 @@ __global__ __launch_bounds__(RR_THREADS) void rerank_kernel(const float* __restr
     extern __shared__ __align__(16) unsigned char smem[];
     uint32_t cpow = 2;  // >= 2 keeps the query copy 16-byte aligned
     while (cpow < c) cpow <<= 1;
+    // the tie labels are resolved before the first candidate is compacted, so both share the
+    // first region (max(cpow, RR_TIES) 8-byte slots); the query follows
     SelEntry* buf = reinterpret_cast<SelEntry*>(smem);    // cpow entries
-    float* qs = reinterpret_cast<float*>(buf + cpow);      // dim floats
-    uint64_t* ties = reinterpret_cast<uint64_t*>(qs + ((dim + 1) & ~1u));  // RR_TIES labels
+    uint64_t* ties = reinterpret_cast<uint64_t*>(smem);   // RR_TIES labels
+    const uint32_t slots = cpow > (uint32_t)RR_TIES ? cpow : (uint32_t)RR_TIES;
+    float* qs = reinterpret_cast<float*>(buf + slots);     // dim floats
     __shared__ uint32_t hist[256];
     __shared__ uint32_t prefix, want, nsel, ntie, tfill, bad, cut_on;
     __shared__ uint64_t cut_label;
@@ -1516,7 +1519,7 @@ hipError_t launch_bf_lowp(const BfData& d, int precision, const float* q, uint32
     (void)hipEventRecord(ev[2], s);
     uint32_t cpow = 2;
     while (cpow < c) cpow <<= 1;
-    size_t lds = (size_t)cpow * sizeof(SelEntry) + (size_t)((dim + 1) & ~1u) * 4 + (size_t)RR_TIES * 8;
+    size_t lds = (size_t)std::max<uint32_t>(cpow, RR_TIES) * 8 + (size_t)((dim + 1) & ~1u) * 4;
     if (d.space == 0) {
         (void)hipFuncSetAttribute((const void*)rerank_kernel<0>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         hipLaunchKernelGGL(rerank_kernel<0>, dim3(nq), dim3(RR_THREADS), lds, s, q, d.vec, d.labels, dim, crow, cd,
