@@ -208,15 +208,35 @@ __global__ __launch_bounds__(SEL_THREADS) void bf_select_kernel(const float* __r
         thr = SelEntry{INFINITY, SENT};
     }
     __syncthreads();
+    // this thread's values of a chunk (SEL_CHUNK / SEL_THREADS of them), the next chunk's loaded
+    // while the current one is filtered, so the stream keeps two chunks of loads in flight
+    constexpr int PER = SEL_CHUNK / SEL_THREADS;
+    static_assert(SEL_CHUNK % SEL_THREADS == 0, "whole chunks per thread");
+    float dv[PER];
+    auto load = [&](uint32_t base) {
+#pragma unroll
+        for (int u = 0; u < PER; u++) {
+            const uint32_t i = base + threadIdx.x + (uint32_t)u * SEL_THREADS;
+            dv[u] = i < n ? row[i] : INFINITY;
+        }
+    };
+    load(0);
     for (uint32_t base = 0; base < n; base += SEL_CHUNK) {
         SelEntry t = thr;
-        for (uint32_t i = base + threadIdx.x; i < base + SEL_CHUNK && i < n; i += SEL_THREADS) {
+        float cur[PER];
+#pragma unroll
+        for (int u = 0; u < PER; u++) cur[u] = dv[u];
+        if (base + SEL_CHUNK < n) load(base + SEL_CHUNK);
+#pragma unroll
+        for (int u = 0; u < PER; u++) {
+            const uint32_t i = base + threadIdx.x + (uint32_t)u * SEL_THREADS;
+            if (i >= n) continue;
             uint32_t r = rmap ? rmap[i] : i;
             if (r == SENT) continue;
             // the filter of BruteforceSearch::searchKnn (bruteforce.h:115-117): a row whose label
             // is not allowed never enters the result
             if (allow && ((allow[r >> 5] >> (r & 31u)) & 1u) == 0u) continue;
-            SelEntry e{row[i], r};
+            SelEntry e{cur[u], r};
             if (sel_less(e, t, labels) || t.r == SENT) {
                 int pos = atomicAdd(&cnt, 1);
                 buf[pos] = e;
