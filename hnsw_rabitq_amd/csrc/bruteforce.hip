@@ -274,7 +274,7 @@ size_t bf_exact_scratch_bytes(uint32_t nq, uint32_t k, uint32_t n) {
     // queries per pass, so that the selection has four workgroups per CU
     size_t per_q = (size_t)n * 4;
     size_t qc = std::max<size_t>(1, std::min<size_t>(nq, ((size_t)1 << 32) / std::max<size_t>(per_q, 1)));
-    qc = (qc + TQ - 1) / TQ * TQ;
+    if (qc >= (size_t)TQ) qc = qc / TQ * TQ;  // whole query tiles, within the cap
     return qc * per_q;
 }
 
