@@ -1150,6 +1150,8 @@ __global__ void fill_u32_kernel(uint32_t* __restrict__ p, uint32_t n, uint32_t v
 
 // tau[q] = the rank-th smallest value of row q of the sample matrix (radix select, 4 x 8 bits),
 // nudged up by one ulp so that exactly the sample values below it and at it pass `d < tau`.
+constexpr uint32_t KTH_MAX = 16384;  // sample rows (the sample step keeps it at most this)
+
 __global__ __launch_bounds__(256) void kth_kernel(const float* __restrict__ mat, size_t ld, uint32_t s,
                                                   uint32_t rank, float* __restrict__ tau) {
     __shared__ uint32_t hist[256];
@@ -1159,15 +1161,40 @@ __global__ __launch_bounds__(256) void kth_kernel(const float* __restrict__ mat,
         prefix = 0;
         want = rank;  // 1-based rank among the remaining candidates
     }
+    // the row's keys, read once: element threadIdx.x + 256 u in register u
+    constexpr int PER = KTH_MAX / 256;
+    uint32_t keys[PER];
+#pragma unroll
+    for (int u = 0; u < PER; u++) {
+        const uint32_t i = threadIdx.x + 256u * (uint32_t)u;
+        keys[u] = i < s ? f2key(row[i]) : 0u;
+    }
     __syncthreads();
+    const uint32_t lane = threadIdx.x & 63u;
     for (int shift = 24; shift >= 0; shift -= 8) {
         hist[threadIdx.x] = 0;
         __syncthreads();
         const uint32_t pre = prefix;
         const uint32_t mask_hi = shift == 24 ? 0u : (0xFFFFFFFFu << (shift + 8));
-        for (uint32_t i = threadIdx.x; i < s; i += 256) {
-            uint32_t k = f2key(row[i]);
-            if ((k & mask_hi) == pre) atomicAdd(&hist[(k >> shift) & 255u], 1u);
+        // most keys share a few buckets (the high bytes of similar distances): lanes whose bucket
+        // is the wave's first active lane's add together with one atomic, the others one each
+#pragma unroll
+        for (int u = 0; u < PER; u++) {
+            const uint32_t i = threadIdx.x + 256u * (uint32_t)u;
+            const uint32_t k = keys[u];
+            const bool in = i < s && (k & mask_hi) == pre;
+            const uint32_t bkt = (k >> shift) & 255u;
+            const uint64_t act = __ballot(in);
+            if (act) {
+                const uint32_t lead = (uint32_t)__builtin_amdgcn_readlane((int)bkt, __ffsll((unsigned long long)act) - 1);
+                const uint64_t same = __ballot(in && bkt == lead);
+                if (in && bkt == lead) {
+                    if (lane == (uint32_t)(__ffsll((unsigned long long)same) - 1))
+                        atomicAdd(&hist[bkt], (uint32_t)__popcll(same));
+                } else if (in) {
+                    atomicAdd(&hist[bkt], 1u);
+                }
+            }
         }
         __syncthreads();
         if (threadIdx.x == 0) {
@@ -1522,6 +1549,7 @@ hipError_t launch_bf_lowp(const BfData& d, int precision, const float* q, uint32
     // ~1.2 c candidates expected (buffer holds 4c); never below k + 1 sample ranks
     double want = 1.2 * (double)c * (double)srows / (double)n;
     uint32_t rank = (uint32_t)std::min<double>((double)srows, std::max<double>((double)k + 1, std::ceil(want)));
+    if (srows > KTH_MAX) return hipErrorInvalidValue;  // the sample step keeps srows <= 16384
     hipLaunchKernelGGL(kth_kernel, dim3(nq), dim3(256), 0, s, smat, (size_t)srows, srows, rank, tau);
     hipLaunchKernelGGL(fill_u32_kernel, dim3((2 * nq + 255) / 256), dim3(256), 0, s, ccnt, 2 * nq, 0u);
     (void)hipEventRecord(ev[1], s);
