@@ -570,6 +570,28 @@ class BFIndex:
         self._cur_l = 0
         self.num_threads = os.cpu_count() or 1
         self.device = 0
+        self.precision = "fp32"
+        self.rerank_c = 4096
+
+    def set_precision(self, precision, rerank_c=4096):
+        """Distance precision of knn_query: "fp32" (exact scan), "bf16" or "fp8" (MFMA distance
+        GEMM, rerank_c candidates re-ranked in exact FP32, every query the error bound cannot
+        certify answered by the exact scan: the lists equal the "fp32" ones).  No counterpart in
+        the reference; the reduced precisions need dim % 64 == 0."""
+        if precision not in _PRECISIONS:
+            raise ValueError("precision must be one of " + ", ".join(_PRECISIONS))
+        if precision != "fp32" and self.dim % 64 != 0:
+            raise ValueError("the bf16 / fp8 distance GEMM needs dim to be a multiple of 64")
+        self.precision = precision
+        self.rerank_c = int(rerank_c)
+
+    def lowp_report(self):
+        """What the last bf16 / fp8 search did besides the re-rank (hg_bf_lowp_report)."""
+        out = np.zeros(8, np.float64)
+        _lib.call("hg_bf_lowp_report", self._h, _ptr(out))
+        keys = ("exact_overflow", "exact_uncertified", "near_ties", "near_tie_disagree", "boundary_near_ties",
+                "boundary_disagree", "max_bound", "certified")
+        return {k: (float(v) if k == "max_bound" else int(v)) for k, v in zip(keys, out)}
 
     def init_index(self, max_elements):
         if self._h is not None:
@@ -591,7 +613,17 @@ class BFIndex:
         else:
             # default labels continue a counter of every row ever passed in (bindings.cpp:812-822)
             lab = np.arange(self._cur_l, self._cur_l + arr.shape[0], dtype=np.uint64)
-        _lib.call("hg_bf_add", self._h, _ptr(arr), arr.shape[0], _ptr(lab))
+        try:
+            _lib.call("hg_bf_add", self._h, _ptr(arr), arr.shape[0], _ptr(lab))
+        except RuntimeError:
+            # the rows placed before the index filled up stay added (bindings.cpp:811-822);
+            # the default-label counter does not advance, as in the reference
+            n = self.get_current_count()
+            labs = np.empty(n, dtype=np.uint64)
+            if n:
+                _lib.call("hg_bf_labels", self._h, _ptr(labs), n)
+            self._labels = set(int(x) for x in labs)
+            raise
         self._cur_l += arr.shape[0]
         self._labels.update(int(x) for x in lab)
 
@@ -604,7 +636,9 @@ class BFIndex:
         self.num_threads = int(n)
 
     def knn_query(self, data, k=1, num_threads=-1, filter=None):
-        """filter: a callable label -> bool, as in hnswlib (evaluated here once per element)."""
+        """filter: a callable label -> bool, as in hnswlib (evaluated here once per element).
+        Runs at the precision set by set_precision; a filtered query always takes the exact
+        FP32 scan."""
         if filter is not None and not callable(filter):
             raise TypeError("filter must be a callable taking a label")
         arr = _as_rows(data, self.dim)
@@ -617,7 +651,10 @@ class BFIndex:
         if nq == 0:
             return labels, dists
         lib = _lib.load()
-        if filter is None:
+        if filter is None and self.precision != "fp32":
+            code = lib.hg_bf_search_precision(self._h, _ptr(arr), nq, k, _PRECISIONS[self.precision], self.rerank_c,
+                                              _ptr(labels), _ptr(dists))
+        elif filter is None:
             code = lib.hg_bf_search(self._h, _ptr(arr), nq, k, _ptr(labels), _ptr(dists))
         else:
             labs = np.array(list(self._labels), dtype=np.uint64)

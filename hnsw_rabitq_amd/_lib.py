@@ -79,6 +79,8 @@ SIGNATURES = {
     "hg_bf_autotune": [_vp, _vp, _sz, _sz, _i32, _sz, _vp, _vp, _vp, _vp, _vp],
     "hg_bf_gemm_variant": [_i32],
     "hg_bf_fp64_check": [_vp, _vp, _sz, _sz, _vp, _vp, ctypes.c_double, _vp, _vp],
+    "hg_bf_search_precision": [_vp, _vp, _sz, _sz, _i32, _sz, _vp, _vp],
+    "hg_bf_lowp_report": [_vp, _vp],
     "hg_stop_model_create": [_i32, _i32, _i32, _pvp],
     "hg_stop_model_free": [_vp],
     "hg_stop_model_param_count": [_vp, ctypes.POINTER(_sz)],

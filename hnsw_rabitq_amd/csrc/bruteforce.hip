@@ -279,9 +279,15 @@ size_t bf_exact_scratch_bytes(uint32_t nq, uint32_t k, uint32_t n) {
 }
 
 hipError_t launch_bf_exact(const BfData& d, const float* q, uint32_t nq, uint32_t k, uint64_t* labels, float* dists,
-                           void* scratch, hipStream_t s) {
+                           void* scratch, hipStream_t s, size_t scratch_cap) {
     size_t scratch_bytes = bf_exact_scratch_bytes(nq, k, d.n);
     uint32_t qc = (uint32_t)(scratch_bytes / ((size_t)d.n * 4));
+    if (scratch_cap && scratch_cap < scratch_bytes) {
+        // a smaller scratch: as many queries per chunk as fit, whole query tiles when possible
+        qc = (uint32_t)std::min<size_t>(qc, scratch_cap / ((size_t)d.n * 4));
+        if (qc == 0) return hipErrorInvalidValue;
+        if (qc >= (uint32_t)TQ) qc = qc / TQ * TQ;
+    }
     float* mat = (float*)scratch;
     for (uint32_t q0 = 0; q0 < nq; q0 += qc) {
         uint32_t cq = nq - q0 < qc ? nq - q0 : qc;
@@ -372,6 +378,89 @@ __global__ __launch_bounds__(256) void convert_rows_kernel(const float* __restri
         for (uint32_t i = threadIdx.x; i < dim; i += 256) out_fp8[(size_t)row * dim + i] = f32_to_e4m3(x[i] * inv);
         if (threadIdx.x == 0) scale[row] = sc;
     }
+}
+
+// OCP e4m3 code -> its value (the codes the converter writes: no NaN).
+__device__ __forceinline__ double e4m3_value(uint8_t c) {
+    const int e = (c >> 3) & 15, m = c & 7;
+    const double v = e == 0 ? (double)m * 0x1p-9 : (1.0 + (double)m * 0.125) * ldexp(1.0, e - 7);
+    return (c & 0x80) ? -v : v;
+}
+
+// Quantisation statistics of rows (FP64, rounded up to FP32): the norm of the FP32 row, the
+// norm of its reduced-precision copy (value = code x scale) and the norm of the difference.
+// With `each` the three values go to row r of out ([n][3]); otherwise each is folded into
+// out[0..2] with an integer max (non-negative floats order as their bits).
+__global__ __launch_bounds__(256) void quant_stats_kernel(const float* __restrict__ X, uint32_t n, uint32_t dim,
+                                                          const uint16_t* __restrict__ x_bf16,
+                                                          const uint8_t* __restrict__ x_fp8,
+                                                          const float* __restrict__ scale, int each,
+                                                          uint32_t* __restrict__ out) {
+    const uint32_t row = blockIdx.x;
+    if (row >= n) return;
+    const float* x = X + (size_t)row * dim;
+    const double sc = x_fp8 ? (double)scale[row] : 1.0;
+    double a = 0, b = 0, c = 0;
+    for (uint32_t i = threadIdx.x; i < dim; i += 256) {
+        const double v = x[i];
+        const double t = x_bf16 ? (double)__uint_as_float((uint32_t)x_bf16[(size_t)row * dim + i] << 16)
+                                : sc * e4m3_value(x_fp8[(size_t)row * dim + i]);
+        a += v * v;
+        b += t * t;
+        c += (v - t) * (v - t);
+    }
+    __shared__ double red[3][256];
+    red[0][threadIdx.x] = a;
+    red[1][threadIdx.x] = b;
+    red[2][threadIdx.x] = c;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s)
+            for (int j = 0; j < 3; j++) red[j][threadIdx.x] += red[j][threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x < 3) {
+        // the FP64 sums are accurate to ~1e-13 relative: a 1e-6 margin and rounding up keep
+        // the FP32 value an upper bound
+        const float v = __double2float_ru(sqrt(red[threadIdx.x][0]) * (1.0 + 1e-6));
+        if (each)
+            out[(size_t)row * 3 + threadIdx.x] = __float_as_uint(v);
+        else
+            atomicMax(&out[threadIdx.x], __float_as_uint(v));
+    }
+}
+
+// Per-query bound E on |approximate distance - exact FP32 distance| over every row, from the
+// query's statistics (qs[q] = ‖q‖, ‖q~‖, ‖q - q~‖) and the rows' maxima (xs = max ‖x‖, max
+// ‖x~‖, max ‖x - x~‖), with u = 2^-24 and g = D u / (1 - D u) (the bound of an FP32 sum of D
+// terms in any order):
+//   dot part   |fl(q~.x~) - q.x| <= (2g + 3u)(1 + 3u) ‖q~‖‖x~‖ + ‖q‖ ‖x - x~‖ + ‖q - q~‖ ‖x~‖
+//              (q.x - q~.x~ = q.(x - x~) + (q - q~).x~; the MFMA sum, whose rounding mode is not
+//              assumed, so 2u per addition, and the two scale products)
+//   IP         E = dot + g ‖q‖‖x‖ + u (1 + ‖q‖‖x‖) + u (1 + ‖q~‖‖x~‖)   (1 - s, both sides)
+//   L2         E = 2 dot + g (‖q‖² + ‖x‖²) [the FP32 norms] + 2u (‖q‖² + ‖x‖² + ‖q~‖‖x~‖)
+//              [the two roundings of a = (nq + nx) - 2 s] + g' (‖q‖ + ‖x‖)² [the exact form's
+//              differences and sum, g' for D + 2 terms]
+// A row that the candidate pass left out has approximate distance >= A (the cut); its exact
+// distance is then >= A - E, so when A - E exceeds the k-th exact distance found, no left-out
+// row can be among the k smallest and the list equals the exact search's.
+__global__ void lowp_bound_kernel(uint32_t nq, uint32_t dim, int space, const uint32_t* __restrict__ qs,
+                                  const uint32_t* __restrict__ xs, float* __restrict__ bound) {
+    const uint32_t q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= nq) return;
+    const double u = 0x1p-24;
+    const double g = (double)dim * u / (1.0 - (double)dim * u);
+    const double g2 = (double)(dim + 2) * u / (1.0 - (double)(dim + 2) * u);
+    const double qn = __uint_as_float(qs[(size_t)q * 3]), qt = __uint_as_float(qs[(size_t)q * 3 + 1]),
+                 qe = __uint_as_float(qs[(size_t)q * 3 + 2]);
+    const double xn = __uint_as_float(xs[0]), xt = __uint_as_float(xs[1]), xe = __uint_as_float(xs[2]);
+    const double dot = (2 * g + 3 * u) * (1 + 3 * u) * qt * xt + qn * xe + qe * xt;
+    double e;
+    if (space == 0)
+        e = 2 * dot + g * (qn * qn + xn * xn) + 2 * u * (qn * qn + xn * xn + qt * xt) + g2 * (qn + xn) * (qn + xn);
+    else
+        e = dot + g * qn * xn + u * (1 + qn * xn) + u * (1 + qt * xt);
+    bound[q] = __double2float_ru(e * 1.001);
 }
 
 // Tile order shared by the GEMM kernels.  Workgroup b runs on XCD b % 8.  The query tiles are
@@ -1230,7 +1319,13 @@ constexpr int RR_TIES = 2048;  // candidates tied at the c-th approximate distan
 // so LDS holds only the c chosen rows; their exact FP32 distances (reference lane order) are
 // computed four lanes per row and the k smallest by (distance, label) are written.  A query
 // whose candidate list overflowed, came up short, or has more than RR_TIES ties at the cut is
-// flagged for the exact path.
+// flagged (1) for the exact path, and so is (2) a query whose list the error bound cannot
+// certify: the rows left out have approximate distance >= A (the c-th candidate's, or tau when
+// every candidate was taken), so when A - bound[q] <= the k-th exact distance a left-out row
+// might belong to the k smallest.  tie_ctr (optional): FP64 check of the near-tie decisions --
+// adjacent pairs of the re-ranked order, up to the pair at the k-th boundary, whose FP32
+// distances differ by at most rel_tol (relative) are recomputed in FP64; counters [0] near-tie
+// pairs, [1] pairs whose FP64 order differs, [2] and [3] the same for the boundary pair.
 template <int SPACE>
 __global__ __launch_bounds__(RR_THREADS) void rerank_kernel(const float* __restrict__ Q, const float* __restrict__ X,
                                                             const uint64_t* __restrict__ labels, uint32_t dim,
@@ -1239,7 +1334,10 @@ __global__ __launch_bounds__(RR_THREADS) void rerank_kernel(const float* __restr
                                                             const uint32_t* __restrict__ cand_cnt, uint32_t cap,
                                                             uint32_t c, uint32_t k, uint64_t* __restrict__ out_l,
                                                             float* __restrict__ out_d, size_t ld_out,
-                                                            uint32_t* __restrict__ overflow) {
+                                                            uint32_t* __restrict__ overflow, uint32_t n_rows,
+                                                            const float* __restrict__ tau,
+                                                            const float* __restrict__ bound,
+                                                            unsigned long long* __restrict__ tie_ctr, double rel_tol) {
     extern __shared__ __align__(16) unsigned char smem[];
     uint32_t cpow = 2;  // >= 2 keeps the query copy 16-byte aligned
     while (cpow < c) cpow <<= 1;
@@ -1263,6 +1361,8 @@ __global__ __launch_bounds__(RR_THREADS) void rerank_kernel(const float* __restr
     for (uint32_t i = threadIdx.x; i < dim; i += RR_THREADS) qs[i] = Q[(size_t)q * dim + i];
     const bool all = cnt <= c;
     const uint32_t m = all ? cnt : c;
+    // smallest approximate distance a left-out row can have (every row taken: none is left out)
+    float acut = all ? (cnt >= n_rows ? INFINITY : tau[q]) : 0.f;
     if (threadIdx.x == 0) {
         prefix = 0;
         want = c;
@@ -1305,6 +1405,7 @@ __global__ __launch_bounds__(RR_THREADS) void rerank_kernel(const float* __restr
         }
         const uint32_t kc = prefix;
         const uint32_t nt = ntie;
+        acut = key2f(kc);
         if (nt > 0) {
             // more entries carry the cut key than are taken: keep the `want` smallest labels
             // (the order the exact lists use among equal distances)
@@ -1388,6 +1489,43 @@ __global__ __launch_bounds__(RR_THREADS) void rerank_kernel(const float* __restr
     }
     __syncthreads();
     sel_sort<RR_THREADS>(buf, (int)m, labels);
+    // certification (uniform over the workgroup: every thread reads the same values)
+    if (bound && !((double)acut - (double)bound[q] > (double)buf[k - 1].d)) {
+        if (threadIdx.x == 0) overflow[q] = 2;
+        return;
+    }
+    if (tie_ctr) {
+        for (uint32_t i = threadIdx.x; i < k && i + 1 < m; i += RR_THREADS) {
+            const double a = buf[i].d, b = buf[i + 1].d;
+            if (b - a > rel_tol * fabs(a)) continue;
+            const float* xa = X + (size_t)buf[i].r * dim;
+            const float* xb = X + (size_t)buf[i + 1].r * dim;
+            double sa = 0, sb = 0;
+            for (uint32_t j = 0; j < dim; j++) {
+                const double qj = qs[j];
+                if (SPACE == 0) {
+                    const double ta = qj - (double)xa[j], tb = qj - (double)xb[j];
+                    sa = fma(ta, ta, sa);
+                    sb = fma(tb, tb, sb);
+                } else {
+                    sa = fma(qj, (double)xa[j], sa);
+                    sb = fma(qj, (double)xb[j], sb);
+                }
+            }
+            if (SPACE != 0) {
+                sa = 1.0 - sa;
+                sb = 1.0 - sb;
+            }
+            // the FP32 order puts a before b; FP64 disagrees when it puts b first
+            const bool dis = sa > sb || (sa == sb && labels[buf[i].r] > labels[buf[i + 1].r]);
+            atomicAdd(&tie_ctr[0], 1ull);
+            if (dis) atomicAdd(&tie_ctr[1], 1ull);
+            if (i == k - 1) {
+                atomicAdd(&tie_ctr[2], 1ull);
+                if (dis) atomicAdd(&tie_ctr[3], 1ull);
+            }
+        }
+    }
     uint64_t* ol = out_l + (size_t)q * ld_out;
     float* od = out_d + (size_t)q * ld_out;
     for (uint32_t i = threadIdx.x; i < k; i += RR_THREADS) {
@@ -1396,19 +1534,24 @@ __global__ __launch_bounds__(RR_THREADS) void rerank_kernel(const float* __restr
     }
 }
 
+// Sample step of the threshold pass: at most 16,384 sample rows.
+static inline uint32_t sample_step(uint32_t n) { return n > 16384 ? (n + 16383) / 16384 : 1; }
+
 size_t bf_lowp_scratch_bytes(uint32_t nq, uint32_t dim, uint32_t c, uint32_t n) {
     (void)c;
-    const uint32_t step = n > 16384 ? (n + 16383) / 16384 : 1;
+    const uint32_t step = sample_step(n);
     const uint32_t s = (n + step - 1) / step;
     size_t b = 0;
-    b += (size_t)nq * dim;            // queries low precision (bf16 needs 2x: counted below)
-    b += (size_t)nq * dim;
-    b += (size_t)nq * 4 * 3;          // a_scale, a_norm, tau
+    b += (size_t)nq * dim * 2;        // queries in low precision (bf16: 2 bytes)
+    b += (size_t)nq * 4 * 4;          // a_scale, a_norm, tau, bound
+    b += (size_t)nq * 12;             // query quantisation statistics
     b += (size_t)s * dim * 2 + (size_t)s * 8;  // sample rows + scale + norm
-    b += (size_t)nq * s * 4;          // sample matrix
-    b += (size_t)nq * RR_CAP * 8;     // candidates
-    b += (size_t)nq * 8;              // counts + overflow flags
+    // sample matrix; afterwards the exact path's distance chunk for the queries it answers
+    b += std::max<size_t>((size_t)nq * s, n) * 4;
+    b += (size_t)nq * RR_CAP * 8;     // candidates; afterwards the exact path's queries and rows
+    b += (size_t)nq * 8;              // counts + flags
     b += (size_t)n * 4;               // row norms
+    b += 64;                          // FP64 near-tie counters
     return b + 16 * 256;
 }
 
@@ -1418,6 +1561,14 @@ hipError_t launch_bf_convert(const BfData& d, uint16_t* vec_bf16, uint8_t* vec_f
     if (d.n == 0) return hipSuccess;
     hipLaunchKernelGGL(convert_rows_kernel, dim3(d.n), dim3(256), 0, s, d.vec, d.n, d.dim, vec_bf16, vec_fp8,
                        fp8_scale);
+    return hipGetLastError();
+}
+
+hipError_t launch_quant_stats(const BfData& d, int precision, uint32_t* out, hipStream_t s) {
+    if (d.n == 0) return hipSuccess;
+    hipLaunchKernelGGL(quant_stats_kernel, dim3(d.n), dim3(256), 0, s, d.vec, d.n, d.dim,
+                       precision == 1 ? d.vec_bf16 : nullptr, precision == 2 ? d.vec_fp8 : nullptr, d.fp8_scale, 0,
+                       out);
     return hipGetLastError();
 }
 
@@ -1490,12 +1641,46 @@ static void launch_gemm(LowpGemmArgs ga, int precision, bool filter, int variant
     }
 }
 
+// Rows r of the exact path's output go back to row idx[r] of the caller's lists.
+__global__ void gather_queries_kernel(const float* __restrict__ q, uint32_t dim, const uint32_t* __restrict__ idx,
+                                      uint32_t nf, float* __restrict__ out) {
+    const uint32_t r = blockIdx.x;
+    if (r >= nf) return;
+    for (uint32_t i = threadIdx.x; i < dim; i += blockDim.x) out[(size_t)r * dim + i] = q[(size_t)idx[r] * dim + i];
+}
+
+__global__ void scatter_rows_kernel(const uint64_t* __restrict__ sl, const float* __restrict__ sd, uint32_t k,
+                                    const uint32_t* __restrict__ idx, uint32_t nf, uint64_t* __restrict__ labels,
+                                    float* __restrict__ dists) {
+    const uint32_t r = blockIdx.x;
+    if (r >= nf) return;
+    for (uint32_t i = threadIdx.x; i < k; i += blockDim.x) {
+        labels[(size_t)idx[r] * k + i] = sl[(size_t)r * k + i];
+        dists[(size_t)idx[r] * k + i] = sd[(size_t)r * k + i];
+    }
+}
+
+namespace {
+struct Events {
+    hipEvent_t ev[4] = {};
+    Events() {
+        for (auto& e : ev) (void)hipEventCreate(&e);
+    }
+    ~Events() {
+        for (auto& e : ev)
+            if (e) (void)hipEventDestroy(e);
+    }
+};
+}  // namespace
+
 hipError_t launch_bf_lowp(const BfData& d, int precision, const float* q, uint32_t nq, uint32_t k, uint32_t c,
-                          uint64_t* labels, float* dists, void* scratch, hipStream_t s, double* stats, int variant) {
+                          uint64_t* labels, float* dists, void* scratch, hipStream_t s, double* stats, int variant,
+                          const uint32_t* xstats, double tie_rel, LowpReport* rep) {
     const uint32_t n = d.n, dim = d.dim;
     const size_t elem = precision == 1 ? 2 : 1;
-    const uint32_t step = n > 16384 ? (n + 16383) / 16384 : 1;
+    const uint32_t step = sample_step(n);
     const uint32_t srows = (n + step - 1) / step;
+    if (srows > KTH_MAX) return hipErrorInvalidValue;  // the sample step keeps srows <= 16384
     char* p = (char*)scratch;
     auto take = [&](size_t bytes) {
         char* r = p;
@@ -1506,26 +1691,36 @@ hipError_t launch_bf_lowp(const BfData& d, int precision, const float* q, uint32
     float* qsc = (float*)take((size_t)nq * 4);
     float* qnm = (float*)take((size_t)nq * 4);
     float* tau = (float*)take((size_t)nq * 4);
+    float* bound = (float*)take((size_t)nq * 4);
+    uint32_t* qst = (uint32_t*)take((size_t)nq * 12);
     uint8_t* slp = (uint8_t*)take((size_t)srows * dim * elem);
     float* ssc = (float*)take((size_t)srows * 4);
     float* snm = (float*)take((size_t)srows * 4);
-    float* smat = (float*)take((size_t)nq * srows * 4);
+    const size_t smat_bytes = std::max<size_t>((size_t)nq * srows, n) * 4;
+    float* smat = (float*)take(smat_bytes);
     uint32_t* crow = (uint32_t*)take((size_t)nq * RR_CAP * 4);
     float* cd = (float*)take((size_t)nq * RR_CAP * 4);
-    uint32_t* ccnt = (uint32_t*)take((size_t)nq * 8);  // counts, then overflow flags
+    uint32_t* ccnt = (uint32_t*)take((size_t)nq * 8);  // counts, then flags
     uint32_t* ovf = ccnt + nq;
     float* xnm = (float*)take((size_t)n * 4);
+    unsigned long long* tie_ctr = (unsigned long long*)take(64);
     const uint8_t* xlp = precision == 1 ? (const uint8_t*)d.vec_bf16 : d.vec_fp8;
-    hipEvent_t ev[4];
-    for (auto& e : ev) (void)hipEventCreate(&e);
-    (void)hipEventRecord(ev[0], s);
+    Events ev;
+    (void)hipEventRecord(ev.ev[0], s);
 
-    // queries to low precision; norms for the L2 form
+    // queries to low precision; norms for the L2 form; the error bound of each query
     hipLaunchKernelGGL(convert_rows_kernel, dim3(nq), dim3(256), 0, s, q, nq, dim,
                        precision == 1 ? (uint16_t*)qlp : nullptr, precision == 2 ? qlp : nullptr, qsc);
     if (d.space == 0) {
         hipLaunchKernelGGL(row_norms_kernel, dim3((nq + 255) / 256), dim3(256), 0, s, q, nq, dim, qnm);
         hipLaunchKernelGGL(row_norms_kernel, dim3((n + 255) / 256), dim3(256), 0, s, d.vec, n, dim, xnm);
+    }
+    if (xstats) {
+        hipLaunchKernelGGL(quant_stats_kernel, dim3(nq), dim3(256), 0, s, q, nq, dim,
+                           precision == 1 ? (const uint16_t*)qlp : nullptr, precision == 2 ? (const uint8_t*)qlp : nullptr,
+                           (const float*)qsc, 1, qst);
+        hipLaunchKernelGGL(lowp_bound_kernel, dim3((nq + 255) / 256), dim3(256), 0, s, nq, dim, (int)d.space,
+                           (const uint32_t*)qst, xstats, bound);
     }
     // sample pass -> per-query threshold
     hipLaunchKernelGGL(gather_rows_kernel, dim3(srows), dim3(64), 0, s, xlp, (size_t)dim * elem, step, srows, slp,
@@ -1549,10 +1744,10 @@ hipError_t launch_bf_lowp(const BfData& d, int precision, const float* q, uint32
     // ~1.2 c candidates expected (buffer holds 4c); never below k + 1 sample ranks
     double want = 1.2 * (double)c * (double)srows / (double)n;
     uint32_t rank = (uint32_t)std::min<double>((double)srows, std::max<double>((double)k + 1, std::ceil(want)));
-    if (srows > KTH_MAX) return hipErrorInvalidValue;  // the sample step keeps srows <= 16384
     hipLaunchKernelGGL(kth_kernel, dim3(nq), dim3(256), 0, s, smat, (size_t)srows, srows, rank, tau);
     hipLaunchKernelGGL(fill_u32_kernel, dim3((2 * nq + 255) / 256), dim3(256), 0, s, ccnt, 2 * nq, 0u);
-    (void)hipEventRecord(ev[1], s);
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, s, (uint32_t*)tie_ctr, 16u, 0u);
+    (void)hipEventRecord(ev.ev[1], s);
     // filter pass over every row
     ga.B = xlp;
     ga.b_scale = d.fp8_scale;
@@ -1564,64 +1759,126 @@ hipError_t launch_bf_lowp(const BfData& d, int precision, const float* q, uint32
     ga.cand_cnt = ccnt;
     ga.cap = RR_CAP;
     launch_gemm(ga, precision, true, variant, s);
-    (void)hipEventRecord(ev[2], s);
+    (void)hipEventRecord(ev.ev[2], s);
     uint32_t cpow = 2;
     while (cpow < c) cpow <<= 1;
     size_t lds = (size_t)std::max<uint32_t>(cpow, RR_TIES) * 8 + (size_t)((dim + 1) & ~1u) * 4;
+    const float* bnd = xstats ? bound : nullptr;
+    unsigned long long* tc = tie_rel > 0 ? tie_ctr : nullptr;
     if (d.space == 0) {
         (void)hipFuncSetAttribute((const void*)rerank_kernel<0>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         hipLaunchKernelGGL(rerank_kernel<0>, dim3(nq), dim3(RR_THREADS), lds, s, q, d.vec, d.labels, dim, crow, cd,
-                           ccnt, (uint32_t)RR_CAP, c, k, labels, dists, (size_t)k, ovf);
+                           ccnt, (uint32_t)RR_CAP, c, k, labels, dists, (size_t)k, ovf, n, (const float*)tau, bnd, tc,
+                           tie_rel);
     } else {
         (void)hipFuncSetAttribute((const void*)rerank_kernel<1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         hipLaunchKernelGGL(rerank_kernel<1>, dim3(nq), dim3(RR_THREADS), lds, s, q, d.vec, d.labels, dim, crow, cd,
-                           ccnt, (uint32_t)RR_CAP, c, k, labels, dists, (size_t)k, ovf);
+                           ccnt, (uint32_t)RR_CAP, c, k, labels, dists, (size_t)k, ovf, n, (const float*)tau, bnd, tc,
+                           tie_rel);
     }
-    (void)hipEventRecord(ev[3], s);
+    (void)hipEventRecord(ev.ev[3], s);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    // queries whose candidate pass overflowed (or came up short) take the exact path
+    // queries the re-rank flagged take the exact path, all in one batch
     std::vector<uint32_t> flags(nq);
+    unsigned long long ties[4] = {0, 0, 0, 0};
+    float bmax = 0.f;
     e = hipMemcpyAsync(flags.data(), ovf, (size_t)nq * 4, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess && tc) e = hipMemcpyAsync(ties, tie_ctr, sizeof(ties), hipMemcpyDeviceToHost, s);
+    std::vector<float> bounds(xstats && rep ? nq : 0);
+    if (e == hipSuccess && !bounds.empty()) e = hipMemcpyAsync(bounds.data(), bound, (size_t)nq * 4, hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess) e = hipStreamSynchronize(s);
     if (e != hipSuccess) return e;
-    e = hipStreamSynchronize(s);
-    if (e != hipSuccess) return e;
-    uint32_t retried = 0;
+    for (float v : bounds) bmax = std::max(bmax, v);
+    std::vector<uint32_t> idx;
+    uint32_t n_ovf = 0, n_unc = 0;
     for (uint32_t i = 0; i < nq; i++) {
         if (!flags[i]) continue;
-        retried++;
-        // the sample matrix is no longer needed: reuse it as exact scratch for one query
-        size_t need = bf_exact_scratch_bytes(1, k, n);
-        if (need > (size_t)nq * srows * 4) return hipErrorInvalidValue;
-        e = launch_bf_exact(d, q + (size_t)i * dim, 1, k, labels + (size_t)i * k, dists + (size_t)i * k, smat, s);
+        idx.push_back(i);
+        if (flags[i] == 2)
+            n_unc++;
+        else
+            n_ovf++;
+    }
+    if (!idx.empty()) {
+        // the candidate region is free now: the list of queries, their vectors and the exact
+        // rows (nf * (4 + 4 dim + 12 k) bytes <= nq * 8 * RR_CAP for dim <= 8192, k <= 2048);
+        // the sample matrix region holds the distance chunk (at least one query row of n)
+        const uint32_t nf = (uint32_t)idx.size();
+        char* cp = (char*)crow;
+        uint32_t* didx = (uint32_t*)cp;
+        float* fq = (float*)(cp + align_up((size_t)nf * 4));
+        uint64_t* fl = (uint64_t*)((char*)fq + align_up((size_t)nf * dim * 4));
+        float* fd = (float*)((char*)fl + align_up((size_t)nf * k * 8));
+        if ((char*)fd + (size_t)nf * k * 4 > cp + (size_t)nq * RR_CAP * 8) return hipErrorInvalidValue;
+        e = hipMemcpyAsync(didx, idx.data(), (size_t)nf * 4, hipMemcpyHostToDevice, s);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(gather_queries_kernel, dim3(nf), dim3(256), 0, s, q, dim, (const uint32_t*)didx, nf, fq);
+        e = launch_bf_exact(d, fq, nf, k, fl, fd, smat, s, smat_bytes);
+        if (e != hipSuccess) return e;
+        hipLaunchKernelGGL(scatter_rows_kernel, dim3(nf), dim3(256), 0, s, (const uint64_t*)fl, (const float*)fd, k,
+                           (const uint32_t*)didx, nf, labels, dists);
+        e = hipGetLastError();
+        if (e != hipSuccess) return e;
+        // the host list must outlive the copy
+        e = hipStreamSynchronize(s);
         if (e != hipSuccess) return e;
     }
     if (stats) {
         float t0 = 0, t1 = 0, t2 = 0;
-        (void)hipEventSynchronize(ev[3]);
-        (void)hipEventElapsedTime(&t0, ev[0], ev[1]);
-        (void)hipEventElapsedTime(&t1, ev[1], ev[2]);
-        (void)hipEventElapsedTime(&t2, ev[2], ev[3]);
+        (void)hipEventSynchronize(ev.ev[3]);
+        (void)hipEventElapsedTime(&t0, ev.ev[0], ev.ev[1]);
+        (void)hipEventElapsedTime(&t1, ev.ev[1], ev.ev[2]);
+        (void)hipEventElapsedTime(&t2, ev.ev[2], ev.ev[3]);
         stats[0] = t1;  // filter GEMM over all rows
         stats[1] = t0;  // conversion + sample GEMM + thresholds
         stats[2] = t2;  // re-rank
-        stats[3] = retried;
+        stats[3] = (double)idx.size();
     }
-    for (auto& ev_ : ev) (void)hipEventDestroy(ev_);
+    if (rep) {
+        rep->exact_overflow = n_ovf;
+        rep->exact_uncertified = n_unc;
+        rep->near_ties = ties[0];
+        rep->near_tie_disagree = ties[1];
+        rep->boundary_near_ties = ties[2];
+        rep->boundary_disagree = ties[3];
+        rep->max_bound = bmax;
+        rep->certified = xstats != nullptr;
+    }
     return hipGetLastError();
 }
 
-// FP64 check of the final lists (see hg_bf_fp64_check).
+// FP64 check of the final lists (see hg_bf_fp64_check).  The row of each listed label is
+// found by binary search in (sorted_labels, sorted_rows), the index's labels in ascending order;
+// a label the index does not hold counts in counters[2] and is skipped.
+__device__ __forceinline__ int64_t row_of_label(const uint64_t* sl, const uint32_t* sr, uint32_t n, uint64_t label) {
+    uint32_t lo = 0, hi = n;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (sl[mid] < label)
+            lo = mid + 1;
+        else
+            hi = mid;
+    }
+    return (lo < n && sl[lo] == label) ? (int64_t)sr[lo] : -1;
+}
+
 __global__ void fp64_check_kernel(int space, const float* __restrict__ Q, const float* __restrict__ X,
-                                  const uint64_t* __restrict__ row_of_label_is_label, uint32_t dim, uint32_t nq,
-                                  uint32_t k, const uint64_t* __restrict__ labels, const float* __restrict__ dists,
-                                  double rel_tol, unsigned long long* counters, double* maxes) {
+                                  const uint64_t* __restrict__ sorted_labels, const uint32_t* __restrict__ sorted_rows,
+                                  uint32_t n, uint32_t dim, uint32_t nq, uint32_t k, const uint64_t* __restrict__ labels,
+                                  const float* __restrict__ dists, double rel_tol, unsigned long long* counters,
+                                  double* maxes) {
     uint32_t q = blockIdx.x;
     if (q >= nq) return;
-    (void)row_of_label_is_label;
     __shared__ double d64[2048];
     for (uint32_t i = threadIdx.x; i < k; i += blockDim.x) {
-        uint64_t r = labels[(size_t)q * k + i];
+        const uint64_t li = labels[(size_t)q * k + i];
+        const int64_t r = li == ~0ull ? -1 : row_of_label(sorted_labels, sorted_rows, n, li);
+        if (r < 0) {
+            if (li != ~0ull) atomicAdd(&counters[2], 1ull);  // an empty slot is no error
+            d64[i] = NAN;
+            continue;
+        }
         double s = 0;
         const float* a = Q + (size_t)q * dim;
         const float* x = X + (size_t)r * dim;
@@ -1647,26 +1904,28 @@ __global__ void fp64_check_kernel(int space, const float* __restrict__ Q, const 
         float a = dists[(size_t)q * k + i], b = dists[(size_t)q * k + i + 1];
         if ((double)b - (double)a <= rel_tol * fabs((double)a)) {
             atomicAdd(&counters[0], 1ull);
-            if (d64[i] > d64[i + 1]) atomicAdd(&counters[1], 1ull);
+            // the lists order equal FP32 distances by label; FP64 disagrees when it puts the
+            // later entry strictly first, or ties them against that label order
+            const uint64_t la = labels[(size_t)q * k + i], lb = labels[(size_t)q * k + i + 1];
+            if (d64[i] > d64[i + 1] || (d64[i] == d64[i + 1] && la > lb)) atomicAdd(&counters[1], 1ull);
         }
     }
 }
 
 hipError_t launch_fp64_check(const BfData& d, const float* q, uint32_t nq, uint32_t k, const uint64_t* labels,
-                             const float* dists, double rel_tol, double* out_host, void* scratch, hipStream_t s) {
+                             const float* dists, double rel_tol, const uint64_t* sorted_labels,
+                             const uint32_t* sorted_rows, double* out_host, void* scratch, hipStream_t s) {
     if (k > 2048) return hipErrorInvalidValue;
-    // labels are row indices for this check (the caller's rows carry label == row)
-    unsigned long long* ctr = (unsigned long long*)scratch;
-    double* mx = (double*)((char*)scratch + 16);
-    hipError_t e = hipMemsetAsync(scratch, 0, 32, s);
+    unsigned long long* ctr = (unsigned long long*)scratch;  // [0] pairs, [1] disagreements, [2] unknown labels
+    double* mx = (double*)((char*)scratch + 32);
+    hipLaunchKernelGGL(fill_u32_kernel, dim3(1), dim3(64), 0, s, (uint32_t*)scratch, 12u, 0u);
+    hipLaunchKernelGGL(fp64_check_kernel, dim3(nq), dim3(256), 0, s, (int)d.space, q, d.vec, sorted_labels,
+                       sorted_rows, d.n, d.dim, nq, k, labels, dists, rel_tol, ctr, mx);
+    hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(fp64_check_kernel, dim3(nq), dim3(256), 0, s, (int)d.space, q, d.vec, d.labels, d.dim, nq, k,
-                       labels, dists, rel_tol, ctr, mx);
-    e = hipGetLastError();
-    if (e != hipSuccess) return e;
-    unsigned long long hc[2];
+    unsigned long long hc[3];
     double hm[2];
-    e = hipMemcpyAsync(hc, ctr, 16, hipMemcpyDeviceToHost, s);
+    e = hipMemcpyAsync(hc, ctr, 24, hipMemcpyDeviceToHost, s);
     if (e != hipSuccess) return e;
     e = hipMemcpyAsync(hm, mx, 16, hipMemcpyDeviceToHost, s);
     if (e != hipSuccess) return e;
@@ -1676,6 +1935,7 @@ hipError_t launch_fp64_check(const BfData& d, const float* q, uint32_t nq, uint3
     out_host[1] = (double)hc[1];
     out_host[2] = hm[0];
     out_host[3] = hm[1];
+    out_host[4] = (double)hc[2];
     return hipSuccess;
 }
 

@@ -28,6 +28,12 @@ namespace {
 // ------------------------------------------------------------------------------------------
 // GPU copy of an HNSW graph.
 // ------------------------------------------------------------------------------------------
+struct GpuIndex;
+// Live exported GPU indexes and their export serials (a captured graph checks that its index
+// still exists before a replay).
+std::mutex g_live_mu;
+std::unordered_map<const GpuIndex*, uint64_t> g_live;
+
 struct GpuIndex {
     uint64_t serial = 0;  // unique per export (filters name the export they were built for)
     int device = 0;
@@ -37,12 +43,22 @@ struct GpuIndex {
     // Per-launch device words (status, work counters, overflow-list counts; layout at kWords)
     // and ovf = two overflow lists of [nq] u32.  Launches take the next of kRing sets, so up to
     // kRing launches on different streams may be in flight on one index.
+    // A set's event is recorded after the launch that used it; the next launch that takes the
+    // set waits for it on its own stream (so a work counter is never cleared under a running
+    // launch), and hg_gpu_search_status waits for it before reading the words.
     struct LaunchWords {
         DevBuf words, ovf;
+        hipEvent_t done = nullptr;
+        bool pending = false;
+        ~LaunchWords() {
+            if (done) (void)hipEventDestroy(done);
+        }
     };
     static constexpr int kRing = 16;
     LaunchWords ring[kRing];
     unsigned next_ring = 0, last_ring = 0;
+    // changes whenever the delete marks are re-read (a captured search graph checks it)
+    uint64_t del_version = 0;
     // non-bare-bone search: delete marks as a bitmap over internal ids, and the host labels
     // (filters are built from label lists)
     DevBuf delbits;
@@ -57,6 +73,10 @@ struct GpuIndex {
     std::map<uint32_t, std::pair<uint32_t, int>> tuned;
     size_t urows = 0;  // rows of the upper-level link array
     ~GpuIndex() {
+        {
+            std::lock_guard<std::mutex> lk(g_live_mu);
+            g_live.erase(this);
+        }
         if (stream) (void)hipStreamDestroy(stream);
     }
 };
@@ -153,10 +173,23 @@ hg::SearchParams base_params(GpuIndex* gi, const Plan& pl, GpuIndex::LaunchWords
     return sp;
 }
 
-// The next set of launch words of the ring (host state under the index mutex).
-GpuIndex::LaunchWords* next_words(GpuIndex* gi) {
+// The next set of launch words of the ring (host state under the index mutex), for a launch
+// on stream s: the launch waits on the device for the set's previous launch to end.
+GpuIndex::LaunchWords* next_words(GpuIndex* gi, hipStream_t s) {
     gi->last_ring = gi->next_ring++ % GpuIndex::kRing;
-    return &gi->ring[gi->last_ring];
+    GpuIndex::LaunchWords* lw = &gi->ring[gi->last_ring];
+    if (lw->pending) {
+        hip_check(hipStreamWaitEvent(s, lw->done, 0), "hipStreamWaitEvent");
+        lw->pending = false;
+    }
+    return lw;
+}
+
+// Marks the end of the launch that used lw on stream s.
+void words_done(GpuIndex::LaunchWords* lw, hipStream_t s) {
+    if (!lw->done) hip_check(hipEventCreateWithFlags(&lw->done, hipEventDisableTiming), "hipEventCreate");
+    hip_check(hipEventRecord(lw->done, s), "hipEventRecord");
+    lw->pending = true;
 }
 
 // Fraction of the elements a query may return (not deleted, allowed by the filter), used to
@@ -179,9 +212,13 @@ struct EpsArgs {
 
 // Reserves the launch words (and, for the non-bare-bone search, the overflow list) that a
 // launch of nq queries needs; done before a graph capture, where allocation is not allowed.
+// A set whose buffers grow is first waited for (its old buffers are freed).
 void prepare_words(GpuIndex::LaunchWords* lw, size_t nq, bool filtered) {
+    const size_t ovf = filtered ? std::max<size_t>(nq, 1) * 8 : 0;
+    if (lw->done && (lw->words.bytes < kWords * 4 || lw->ovf.bytes < ovf))
+        hip_check(hipEventSynchronize(lw->done), "hipEventSynchronize");
     lw->words.reserve(kWords * 4);
-    if (filtered) lw->ovf.reserve(std::max<size_t>(nq, 1) * 8);
+    if (filtered) lw->ovf.reserve(ovf);
 }
 
 // The non-bare-bone branch runs when the index holds deleted elements or a filter is given
@@ -469,6 +506,10 @@ GpuIndex* finish_export(HostLayout&& L, int device) {
     g.maxlevel = L.maxlevel;
     g.entry = L.entry;
     g.space = L.space;
+    {
+        std::lock_guard<std::mutex> lk(g_live_mu);
+        g_live[gi.get()] = gi->serial;
+    }
     return gi.release();
 }
 
@@ -522,7 +563,7 @@ HG_API int hg_gpu_index_create(void* hv, int device, void** out) {
             if (h->is_deleted((uint32_t)i)) L.del[i >> 5] |= 1u << (i & 31);
             const uint32_t* ll = h->linklist0((uint32_t)i);
             size_t c = hnswgpu::HnswIndex::list_count(ll);
-            if (c > maxM0) throw Error(HG_ERR_RUNTIME, "Bad value of sz_link_list_other");
+            if (c > maxM0) throw Error(HG_ERR_RUNTIME, "a link list holds more entries than its capacity");
             for (size_t j = 0; j < c; j++) {
                 if (ll[1 + j] >= n) throw Error(HG_ERR_RUNTIME, "cand error");
                 L.l0[i * maxM0 + j] = ll[1 + j];
@@ -530,11 +571,11 @@ HG_API int hg_gpu_index_create(void* hv, int device, void** out) {
             for (int lv = 1; lv <= h->element_level((uint32_t)i); lv++) {
                 const uint32_t* lu = h->linklist((uint32_t)i, lv);
                 size_t cu = hnswgpu::HnswIndex::list_count(lu);
-                if (cu > maxM) throw Error(HG_ERR_RUNTIME, "Bad value of sz_link_list_other");
+                if (cu > maxM) throw Error(HG_ERR_RUNTIME, "a link list holds more entries than its capacity");
                 for (size_t j = 0; j < cu; j++) {
                     if (lu[1 + j] >= n) throw Error(HG_ERR_RUNTIME, "cand error");
                     if (h->element_level(lu[1 + j]) < lv)
-                        throw Error(HG_ERR_RUNTIME, "Trying to make a link on a non-existent level");
+                        throw Error(HG_ERR_RUNTIME, "a link on a level above the element's own");
                     L.up[(L.uoff[i] + (size_t)(lv - 1)) * maxM + j] = lu[1 + j];
                 }
             }
@@ -689,11 +730,12 @@ int search_host(GpuIndex* gi, const float* queries, size_t nq, size_t k, size_t 
     hipStream_t s = gi->stream;
     const bool filtered = gi->num_deleted > 0 || filt != nullptr;
     hip_check(hipMemcpyAsync(gi->q.p, queries, nq * dim * 4, hipMemcpyHostToDevice, s), "hipMemcpy H2D");
-    GpuIndex::LaunchWords* lw = next_words(gi);
+    GpuIndex::LaunchWords* lw = next_words(gi, s);
     prepare_words(lw, nq, filtered);
     hip_check(run_search(gi, (const float*)gi->q.p, nq, nullptr, nullptr, ef, k, k, (uint64_t*)gi->ol.p,
                          (float*)gi->od.p, counters ? (int32_t*)gi->ctr.p : nullptr, nullptr, s, lw, filt),
               "search launch");
+    words_done(lw, s);
     if (filtered) {
         uint32_t status = 0;
         hip_check(hipMemcpyAsync(&status, (uint32_t*)lw->words.p + kWordStatus, 4, hipMemcpyDeviceToHost, s),
@@ -767,12 +809,13 @@ HG_API int hg_gpu_search_epsilon(void* gv, const float* queries, size_t nq, floa
         gi->cnt.reserve(nq * 4);
         hipStream_t s = gi->stream;
         hip_check(hipMemcpyAsync(gi->q.p, queries, nq * dim * 4, hipMemcpyHostToDevice, s), "hipMemcpy H2D");
-        GpuIndex::LaunchWords* lw = next_words(gi);
+        GpuIndex::LaunchWords* lw = next_words(gi, s);
         prepare_words(lw, nq, true);
         EpsArgs ea{epsilon, min_candidates, max_candidates, (uint32_t*)gi->cnt.p};
         hip_check(run_search(gi, (const float*)gi->q.p, nq, nullptr, nullptr, max_candidates, out_stride, out_stride,
                              (uint64_t*)gi->ol.p, (float*)gi->od.p, nullptr, nullptr, s, lw, gf, nullptr, &ea),
                   "search launch");
+        words_done(lw, s);
         uint32_t status = 0;
         hip_check(hipMemcpyAsync(&status, (uint32_t*)lw->words.p + kWordStatus, 4, hipMemcpyDeviceToHost, s),
                   "hipMemcpy D2H");
@@ -804,12 +847,13 @@ HG_API int hg_gpu_search_epsilon_device(void* gv, const float* dq, size_t nq, fl
         require(dq != nullptr, "null queries");
         std::lock_guard<std::mutex> lk(gi->mu);
         DeviceGuard dg(gi->device);
-        GpuIndex::LaunchWords* lw = next_words(gi);
+        GpuIndex::LaunchWords* lw = next_words(gi, (hipStream_t)stream);
         prepare_words(lw, nq, true);
         EpsArgs ea{epsilon, min_candidates, max_candidates, d_counts};
         hip_check(run_search(gi, dq, nq, nullptr, nullptr, max_candidates, out_stride, out_stride, dl, dd, dc, nullptr,
                              (hipStream_t)stream, lw, gf, d_status, &ea),
                   "search launch");
+        words_done(lw, (hipStream_t)stream);
         return HG_OK;
     });
 }
@@ -863,9 +907,14 @@ HG_API int hg_gpu_search_status(void* gv, uint32_t* status, uint32_t* second_pas
     return guarded([&] {
         require(gv != nullptr, "null argument");
         auto* gi = static_cast<GpuIndex*>(gv);
+        std::lock_guard<std::mutex> lk(gi->mu);
         DeviceGuard dg(gi->device);
         uint32_t w[kWords] = {};
-        hip_check(hipMemcpy(w, gi->ring[gi->last_ring].words.p, sizeof(w), hipMemcpyDeviceToHost), "hipMemcpy D2H");
+        GpuIndex::LaunchWords& lw = gi->ring[gi->last_ring];
+        if (!lw.words.p) return HG_OK;  // no search yet
+        // the words of the last launch, read once that launch has ended
+        if (lw.pending) hip_check(hipEventSynchronize(lw.done), "hipEventSynchronize");
+        hip_check(hipMemcpy(w, lw.words.p, sizeof(w), hipMemcpyDeviceToHost), "hipMemcpy D2H");
         if (status) *status = w[kWordStatus];
         if (second_pass_queries) *second_pass_queries = w[kWordCount];
         return HG_OK;
@@ -893,6 +942,7 @@ HG_API int hg_gpu_refresh_deleted(void* gv, void* cpu) {
         DeviceGuard dg(gi->device);
         hip_check(hipMemcpy(gi->delbits.p, del.data(), del.size() * 4, hipMemcpyHostToDevice), "hipMemcpy H2D");
         gi->num_deleted = nd;
+        gi->del_version++;
         return HG_OK;
     });
 }
@@ -922,11 +972,12 @@ HG_API int hg_gpu_search_device(void* gv, const float* dq, size_t nq, const int3
         check_stop_args(gi, nullptr, stop);
         DeviceGuard dg(gi->device);
         std::lock_guard<std::mutex> lk(gi->mu);
-        GpuIndex::LaunchWords* lw = next_words(gi);
+        GpuIndex::LaunchWords* lw = next_words(gi, (hipStream_t)stream);
         prepare_words(lw, nq, is_filtered(gi, nullptr));
         hip_check(run_search(gi, dq, nq, def, dk, ef, k, k_stride, dl, dd, dc, static_cast<StopModel*>(stop),
                              (hipStream_t)stream, lw),
                   "search launch");
+        words_done(lw, (hipStream_t)stream);
         return HG_OK;
     });
 }
@@ -943,11 +994,12 @@ HG_API int hg_gpu_search_device_filtered(void* gv, const float* dq, size_t nq, c
         GpuFilter* gf = as_filter(gi, filter);
         DeviceGuard dg(gi->device);
         std::lock_guard<std::mutex> lk(gi->mu);
-        GpuIndex::LaunchWords* lw = next_words(gi);
+        GpuIndex::LaunchWords* lw = next_words(gi, (hipStream_t)stream);
         prepare_words(lw, nq, is_filtered(gi, gf));
         hip_check(run_search(gi, dq, nq, def, dk, ef, k, k_stride, dl, dd, dc, nullptr, (hipStream_t)stream, lw, gf,
                              d_status),
                   "search launch");
+        words_done(lw, (hipStream_t)stream);
         return HG_OK;
     });
 }
@@ -971,7 +1023,7 @@ HG_API int hg_gpu_collect_traces(void* gv, const float* dq, size_t nq, size_t ef
         ol.reserve(nq * k * 8);
         od.reserve(nq * k * 4);
         Plan pl = make_plan(gi, ef, k, nq);
-        GpuIndex::LaunchWords* lw = next_words(gi);
+        GpuIndex::LaunchWords* lw = next_words(gi, s);
         prepare_words(lw, nq, false);
         hg::SearchParams sp = base_params(gi, pl, lw);
         sp.queries = dq;
@@ -988,6 +1040,7 @@ HG_API int hg_gpu_collect_traces(void* gv, const float* dq, size_t nq, size_t ef
         sp.trace_period = period;
         sp.trace_max_rec = max_records;
         hip_check(hg::launch_search(sp, pl.blocks, s), "trace launch");
+        words_done(lw, s);
         hip_check(hipStreamSynchronize(s), "trace search");
         return HG_OK;
     });
@@ -1039,11 +1092,12 @@ HG_API int hg_gpu_autotune(void* gv, const float* dq, size_t nq, size_t ef, size
                 float ms_sum = 0;
                 for (int rep = 0; rep < 2; rep++) {
                     hip_check(hipEventRecord(e0, s), "hipEventRecord");
-                    GpuIndex::LaunchWords* lw = next_words(gi);
+                    GpuIndex::LaunchWords* lw = next_words(gi, s);
                     prepare_words(lw, nq, false);
                     hip_check(run_search(gi, dq, nq, nullptr, nullptr, ef, k, k, (uint64_t*)ol.p, (float*)od.p,
                                          nullptr, nullptr, s, lw),
                               "search launch");
+                    words_done(lw, s);
                     hip_check(hipEventRecord(e1, s), "hipEventRecord");
                     hip_check(hipEventSynchronize(e1), "hipEventSynchronize");
                     float ms = 0;
@@ -1208,12 +1262,13 @@ HG_API int hg_gpu_search_multivector(void* gv, const float* queries, size_t nq, 
         hip_check(hipMemcpyAsync(gi->q.p, queries, nq * dim * 4, hipMemcpyHostToDevice, s), "hipMemcpy H2D");
         uint32_t status = 0;
         for (uint32_t cap : {2048u, kFiltMaxCap}) {  // the larger beam only if a query outgrew the first
-            GpuIndex::LaunchWords* lw = next_words(gi);
+            GpuIndex::LaunchWords* lw = next_words(gi, s);
             prepare_words(lw, nq, false);
             uint32_t* st = (uint32_t*)lw->words.p + kWordStatus;
             hip_check(run_mv(gi, (const float*)gi->q.p, nq, gd, num_docs, ef_collection, gf, out_stride,
                              (uint64_t*)gi->ol.p, (float*)gi->od.p, (uint32_t*)gi->cnt.p, nullptr, st, cap, lw, s),
                       "multi-vector search launch");
+            words_done(lw, s);
             hip_check(hipMemcpyAsync(&status, st, 4, hipMemcpyDeviceToHost, s), "hipMemcpy D2H");
             hip_check(hipStreamSynchronize(s), "search");
             if (!(status & 1u)) break;
@@ -1249,11 +1304,12 @@ HG_API int hg_gpu_search_multivector_device(void* gv, const float* dq, size_t nq
         require(dq != nullptr, "null queries");
         std::lock_guard<std::mutex> lk(gi->mu);
         DeviceGuard dg(gi->device);
-        GpuIndex::LaunchWords* lw = next_words(gi);
+        GpuIndex::LaunchWords* lw = next_words(gi, (hipStream_t)stream);
         prepare_words(lw, nq, false);
         hip_check(run_mv(gi, dq, nq, gd, num_docs, ef_collection, gf, out_stride, dl, dd, d_counts, dc, d_status, cap,
                          lw, (hipStream_t)stream),
                   "multi-vector search launch");
+        words_done(lw, (hipStream_t)stream);
         return HG_OK;
     });
 }
@@ -1299,7 +1355,7 @@ static void gpu_build_level0(hnswgpu::HnswIndex* h, size_t s0, int device) {
         uint32_t entry = 0;
         int maxl = level[0];
         size_t seen = 1;
-        GpuIndex::LaunchWords* lw = next_words(gi.get());
+        GpuIndex::LaunchWords* lw = next_words(gi.get(), s);
         prepare_words(lw, bmax, false);
         for (size_t sb = std::max<size_t>(s0, 1); sb < n;) {
             // at least 256 rows: a small batch costs one beam search's latency whatever its size,
@@ -1437,10 +1493,16 @@ HG_API int hg_hnsw_build_gpu(void* hv, const float* data, size_t rows, const uin
 }
 
 // ---- hipGraph of a device search ----
+// A captured graph replays the kernels chosen at capture (bare-bone or non-bare-bone, from the
+// delete marks then) over that export's buffers.  It records which index, export and delete-mark
+// version it was captured from; a replay after the index was freed or its delete marks re-read
+// fails instead of replaying a stale choice.
 struct GraphHandle {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     GpuIndex::LaunchWords lw;  // the graph's own launch words (replays reuse them)
+    GpuIndex* gi = nullptr;
+    uint64_t serial = 0, del_version = 0;
     ~GraphHandle() {
         if (exec) (void)hipGraphExecDestroy(exec);
         if (graph) (void)hipGraphDestroy(graph);
@@ -1483,15 +1545,28 @@ HG_API int hg_graph_capture_search(void* gv, const float* dq, size_t nq, const i
         hip_check(ce, "hipStreamEndCapture");
         gh->graph = graph;
         hip_check(hipGraphInstantiate(&gh->exec, graph, nullptr, nullptr, 0), "hipGraphInstantiate");
+        gh->gi = gi;
+        gh->serial = gi->serial;
+        gh->del_version = gi->del_version;
         *out_graph = gh.release();
         return HG_OK;
     });
 }
 
-HG_API int hg_graph_launch(void* gh, void* stream) {
+HG_API int hg_graph_launch(void* ghv, void* stream) {
     return guarded([&] {
-        require(gh != nullptr, "null graph");
-        hip_check(hipGraphLaunch(static_cast<GraphHandle*>(gh)->exec, (hipStream_t)stream), "hipGraphLaunch");
+        require(ghv != nullptr, "null graph");
+        auto* gh = static_cast<GraphHandle*>(ghv);
+        {
+            std::lock_guard<std::mutex> lk(g_live_mu);
+            auto it = g_live.find(gh->gi);
+            if (it == g_live.end() || it->second != gh->serial)
+                throw Error(HG_ERR_RUNTIME, "the GPU index of this graph was freed; capture the search again");
+        }
+        if (gh->gi->del_version != gh->del_version)
+            throw Error(HG_ERR_RUNTIME,
+                        "the delete marks changed since this graph was captured; capture the search again");
+        hip_check(hipGraphLaunch(gh->exec, (hipStream_t)stream), "hipGraphLaunch");
         return HG_OK;
     });
 }

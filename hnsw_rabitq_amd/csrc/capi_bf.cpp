@@ -25,6 +25,12 @@ struct BfIndex {
     size_t dim = 0, max_elements = 0, count = 0;
     DevBuf vec, labels, vec_lp, scale, scratch, cand;
     DevBuf q, ol, od;
+    DevBuf xstat;  // quantisation statistics of the rows' reduced-precision copy (3 floats)
+    // labels ascending with their rows (the FP64 report maps labels to rows), for row version
+    // sorted_version
+    DevBuf sorted_labels, sorted_rows;
+    uint64_t sorted_version = ~0ull;
+    hg::LowpReport last;  // what the last reduced-precision search did besides the re-rank
     int lp_kind = 0;  // precision of vec_lp (0 = not built)
     int tuned[3] = {-1, -1, -1};  // GEMM variant chosen by the autotuner, per precision
     size_t lp_count = 0;
@@ -94,6 +100,29 @@ void ensure_lowp(BfIndex* b, int precision, hipStream_t s) {
               "convert launch");
     b->lp_kind = precision;
     b->lp_count = b->count;
+    // the maxima the certification bound needs (row norms and quantisation error)
+    b->xstat.reserve(64);
+    hip_check(hg::launch_zero_u32((uint32_t*)b->xstat.p, 4, s), "zero launch");
+    hip_check(hg::launch_quant_stats(b->data(), precision, (uint32_t*)b->xstat.p, s), "quantisation statistics");
+}
+
+// Reduced-precision search with certification and the FP64 near-tie check; the report is kept
+// for hg_bf_lowp_report.
+void lowp_search(BfIndex* b, const float* dq, size_t nq, size_t k, int precision, size_t rerank_c, uint64_t* dl,
+                 float* dd, double* stats, hipStream_t s, int variant) {
+    require(b->dim % 64 == 0, "the reduced-precision GEMM needs dim to be a multiple of 64");
+    require(b->dim <= 8192, "the reduced-precision path supports dim up to 8192");
+    size_t c = std::max(rerank_c, k);
+    require(c <= 4096, "rerank_c above 4096 is not supported");
+    c = std::min(c, b->count);
+    ensure_lowp(b, precision, s);
+    size_t sb = hg::bf_lowp_scratch_bytes((uint32_t)nq, (uint32_t)b->dim, (uint32_t)c, (uint32_t)b->count);
+    b->scratch.reserve(sb);
+    hg::LowpReport rep;
+    hip_check(hg::launch_bf_lowp(b->data(), precision, dq, (uint32_t)nq, (uint32_t)k, (uint32_t)c, dl, dd,
+                                 b->scratch.p, s, stats, variant, (const uint32_t*)b->xstat.p, 1e-6, &rep),
+              "reduced-precision brute-force launch");
+    b->last = rep;
 }
 
 }  // namespace
@@ -122,39 +151,47 @@ HG_API int hg_bf_create(int space, size_t dim, size_t max_elements, int device, 
     });
 }
 
-// BruteforceSearch::addPoint (bruteforce.h:64-83): an existing label overwrites its row,
-// a new label takes the next row; capacity errors use the reference's message.
+// BruteforceSearch::addPoint (bruteforce.h:64-83) row by row: an existing label overwrites its
+// row, a new label takes the next row.  When a new label finds the index full, the rows placed
+// before it stay added (as the reference's loop over addPoint leaves them,
+// bindings.cpp:811-822) and the call fails with the reference's message.
 HG_API int hg_bf_add(void* bv, const float* data, size_t rows, const uint64_t* labels) {
     return guarded([&] {
         require(bv != nullptr, "null index");
         require(rows == 0 || data != nullptr, "null data");
         auto* b = static_cast<BfIndex*>(bv);
         DeviceGuard dg(b->device);
+        // rows [0, placed) get a row each; the label map changes only for them
         std::vector<size_t> target(rows);
         std::vector<uint64_t> lab(rows);
-        size_t count = b->count;
-        for (size_t r = 0; r < rows; r++) {
-            uint64_t l = labels ? labels[r] : (uint64_t)(b->count + r);
+        size_t count = b->count, placed = 0;
+        bool full = false;
+        for (; placed < rows; placed++) {
+            const uint64_t l = labels ? labels[placed] : (uint64_t)(b->count + placed);
             auto it = b->label_row.find(l);
             if (it != b->label_row.end()) {
-                target[r] = it->second;
+                target[placed] = it->second;
             } else {
-                if (count >= b->max_elements)
-                    throw Error(HG_ERR_RUNTIME, "The number of elements exceeds the specified limit\n");
-                target[r] = count;
+                if (count >= b->max_elements) {
+                    full = true;
+                    break;
+                }
+                target[placed] = count;
                 b->label_row[l] = count;
                 count++;
             }
-            lab[r] = l;
+            lab[placed] = l;
         }
         b->host_labels.resize(count);
-        for (size_t r2 = 0; r2 < rows; r2++) b->host_labels[target[r2]] = lab[r2];
+        for (size_t r2 = 0; r2 < placed; r2++) b->host_labels[target[r2]] = lab[r2];
+        b->count = count;
         b->version++;
+        b->lp_count = (size_t)-1;  // reduced-precision copy is stale
         // contiguous runs are copied in one transfer
         size_t r = 0;
-        while (r < rows) {
+        while (r < placed) {
             size_t e = r + 1;
-            while (e < rows && target[e] == target[e - 1] + 1) e++;
+            while (e < placed && target[e] == target[e - 1] + 1) e++;
             hip_check(hipMemcpy((char*)b->vec.p + target[r] * b->dim * 4, data + r * b->dim, (e - r) * b->dim * 4,
                                 hipMemcpyHostToDevice),
                       "hipMemcpy H2D");
@@ -163,8 +200,7 @@ HG_API int hg_bf_add(void* bv, const float* data, size_t rows, const uint64_t* l
                       "hipMemcpy H2D");
             r = e;
         }
-        b->count = count;
-        b->lp_count = (size_t)-1;  // reduced-precision copy is stale
+        if (full) throw Error(HG_ERR_RUNTIME, "The number of elements exceeds the specified limit\n");
         return HG_OK;
     });
 }
@@ -241,17 +277,55 @@ HG_API int hg_bf_search_device(void* bv, const float* dq, size_t nq, size_t k, i
             if (stats) std::fill(stats, stats + 4, 0.0);
             return HG_OK;
         }
-        require(b->dim % 64 == 0, "the reduced-precision GEMM needs dim to be a multiple of 64");
-        require(b->dim <= 8192, "the reduced-precision path supports dim up to 8192");
-        size_t c = std::max(rerank_c, k);
-        require(c <= 4096, "rerank_c above 4096 is not supported");
-        c = std::min(c, b->count);
-        ensure_lowp(b, precision, s);
-        size_t sb = hg::bf_lowp_scratch_bytes((uint32_t)nq, (uint32_t)b->dim, (uint32_t)c, (uint32_t)b->count);
-        b->scratch.reserve(sb);
-        hip_check(hg::launch_bf_lowp(b->data(), precision, dq, (uint32_t)nq, (uint32_t)k, (uint32_t)c, dl, dd,
-                                     b->scratch.p, s, stats, b->tuned[precision]),
-                  "reduced-precision brute-force launch");
+        lowp_search(b, dq, nq, k, precision, rerank_c, dl, dd, stats, s, b->tuned[precision]);
+        return HG_OK;
+    });
+}
+
+// BFIndex.knn_query on host buffers at a chosen precision: FP32 is hg_bf_search; BF16 / FP8 run
+// the reduced-precision GEMM, re-rank rerank_c candidates in exact FP32 and answer every query
+// the error bound cannot certify with the exact path, so the lists equal hg_bf_search's.
+HG_API int hg_bf_search_precision(void* bv, const float* queries, size_t nq, size_t k, int precision,
+                                  size_t rerank_c, uint64_t* labels, float* dists) {
+    if (precision == HG_PREC_FP32) return hg_bf_search(bv, queries, nq, k, labels, dists);
+    return guarded([&] {
+        require(bv && labels && dists, "null argument");
+        auto* b = static_cast<BfIndex*>(bv);
+        require(k >= 1 && k <= 2048, "k must be in [1, 2048]");
+        require(precision == HG_PREC_BF16 || precision == HG_PREC_FP8, "unknown precision");
+        if (nq == 0) return HG_OK;
+        require(queries != nullptr, "null queries");
+        if (k > b->count) {
+            g_err = "Cannot return the results in a contiguous 2D array. There are not enough elements.";
+            return HG_ERR_SHORT;
+        }
+        DeviceGuard dg(b->device);
+        hipStream_t s = b->stream;
+        b->q.reserve(nq * b->dim * 4);
+        b->ol.reserve(nq * k * 8);
+        b->od.reserve(nq * k * 4);
+        hip_check(hipMemcpyAsync(b->q.p, queries, nq * b->dim * 4, hipMemcpyHostToDevice, s), "hipMemcpy H2D");
+        lowp_search(b, (const float*)b->q.p, nq, k, precision, rerank_c, (uint64_t*)b->ol.p, (float*)b->od.p, nullptr,
+                    s, b->tuned[precision]);
+        hip_check(hipMemcpyAsync(labels, b->ol.p, nq * k * 8, hipMemcpyDeviceToHost, s), "hipMemcpy D2H");
+        hip_check(hipMemcpyAsync(dists, b->od.p, nq * k * 4, hipMemcpyDeviceToHost, s), "hipMemcpy D2H");
+        hip_check(hipStreamSynchronize(s), "brute force");
+        return HG_OK;
+    });
+}
+
+HG_API int hg_bf_lowp_report(void* bv, double* out) {
+    return guarded([&] {
+        require(bv && out, "null argument");
+        const hg::LowpReport& r = static_cast<BfIndex*>(bv)->last;
+        out[0] = r.exact_overflow;
+        out[1] = r.exact_uncertified;
+        out[2] = (double)r.near_ties;
+        out[3] = (double)r.near_tie_disagree;
+        out[4] = (double)r.boundary_near_ties;
+        out[5] = (double)r.boundary_disagree;
+        out[6] = r.max_bound;
+        out[7] = r.certified ? 1.0 : 0.0;
         return HG_OK;
     });
 }
@@ -261,11 +335,37 @@ HG_API int hg_bf_fp64_check(void* bv, const float* dq, size_t nq, size_t k, cons
     return guarded([&] {
         require(bv && dq && dl && dd && out, "null argument");
         auto* b = static_cast<BfIndex*>(bv);
+        require(k >= 1 && k <= 2048, "k must be in [1, 2048]");
+        if (nq == 0) {
+            std::fill(out, out + 4, 0.0);
+            return HG_OK;
+        }
         DeviceGuard dg(b->device);
-        b->cand.reserve(std::max<size_t>(nq * k * 8, 64));
-        hip_check(hg::launch_fp64_check(b->data(), dq, (uint32_t)nq, (uint32_t)k, dl, dd, rel_tol, out, b->cand.p,
-                                        (hipStream_t)stream),
+        hipStream_t s = (hipStream_t)stream;
+        if (b->sorted_version != b->version) {
+            std::vector<std::pair<uint64_t, uint32_t>> lr(b->count);
+            for (size_t r = 0; r < b->count; r++) lr[r] = std::make_pair(b->host_labels[r], (uint32_t)r);
+            std::sort(lr.begin(), lr.end());
+            std::vector<uint64_t> sl(b->count);
+            std::vector<uint32_t> sr(b->count);
+            for (size_t i = 0; i < lr.size(); i++) {
+                sl[i] = lr[i].first;
+                sr[i] = lr[i].second;
+            }
+            b->sorted_labels.reserve(std::max<size_t>(sl.size() * 8, 8));
+            b->sorted_rows.reserve(std::max<size_t>(sr.size() * 4, 4));
+            hip_check(hipMemcpy(b->sorted_labels.p, sl.data(), sl.size() * 8, hipMemcpyHostToDevice), "hipMemcpy H2D");
+            hip_check(hipMemcpy(b->sorted_rows.p, sr.data(), sr.size() * 4, hipMemcpyHostToDevice), "hipMemcpy H2D");
+            b->sorted_version = b->version;
+        }
+        b->cand.reserve(64);
+        double o[5];
+        hip_check(hg::launch_fp64_check(b->data(), dq, (uint32_t)nq, (uint32_t)k, dl, dd, rel_tol,
+                                        (const uint64_t*)b->sorted_labels.p, (const uint32_t*)b->sorted_rows.p, o,
+                                        b->cand.p, s),
                   "fp64 check launch");
+        require(o[4] == 0.0, "a listed label is not in the index");
+        std::copy(o, o + 4, out);
         return HG_OK;
     });
 }
@@ -283,18 +383,12 @@ HG_API int hg_bf_autotune(void* bv, const float* dq, size_t nq, size_t k, int pr
         require(b->dim % 64 == 0 && b->dim <= 8192, "the reduced-precision GEMM needs dim % 64 == 0, dim <= 8192");
         DeviceGuard dg(b->device);
         hipStream_t s = (hipStream_t)stream;
-        size_t c = std::min(std::max(rerank_c, k), b->count);
-        require(c <= 4096, "rerank_c above 4096 is not supported");
-        ensure_lowp(b, precision, s);
-        b->scratch.reserve(hg::bf_lowp_scratch_bytes((uint32_t)nq, (uint32_t)b->dim, (uint32_t)c, (uint32_t)b->count));
+        require(std::max(rerank_c, k) <= 4096, "rerank_c above 4096 is not supported");
         double best = 1e30;
         int best_v = -1;
         for (int v = 0; v < hg::kGemmVariants; v++) {
             double st[4] = {0, 0, 0, 0};
-            for (int rep = 0; rep < 2; rep++)
-                hip_check(hg::launch_bf_lowp(b->data(), precision, dq, (uint32_t)nq, (uint32_t)k, (uint32_t)c, dl, dd,
-                                             b->scratch.p, s, st, v),
-                          "reduced-precision brute-force launch");
+            for (int rep = 0; rep < 2; rep++) lowp_search(b, dq, nq, k, precision, rerank_c, dl, dd, st, s, v);
             if (times_ms) times_ms[v] = st[0];
             if (st[0] < best) {
                 best = st[0];

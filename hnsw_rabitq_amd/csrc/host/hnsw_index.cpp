@@ -161,7 +161,7 @@ HnswIndex::HnswIndex(int space, size_t dim, const std::string& path, size_t max_
     if (size_per != size_data_per_element_ || offset_level0 != 0)
         throw std::runtime_error("Index seems to be corrupted or unsupported");
     level0_ = (char*)malloc(std::max<size_t>(max_elements_, 1) * size_data_per_element_);
-    if (!level0_) throw std::runtime_error("Not enough memory: loadIndex failed to allocate level0");
+    if (!level0_) throw std::runtime_error("out of host memory for the level-0 records of the loaded index");
     rd(level0_, cur_count * size_data_per_element_);
     link_lists_.assign(max_elements_, nullptr);
     element_levels_.assign(max_elements_, 0);
@@ -175,7 +175,7 @@ HnswIndex::HnswIndex(int space, size_t dim, const std::string& path, size_t max_
         } else {
             element_levels_[i] = (int)(sz / size_links_per_element_);
             link_lists_[i] = (char*)malloc(sz);
-            if (!link_lists_[i]) throw std::runtime_error("Not enough memory: loadIndex failed to allocate linklist");
+            if (!link_lists_[i]) throw std::runtime_error("out of host memory for an upper-level link list of the loaded index");
             rd(link_lists_[i], sz);
         }
     }
@@ -306,7 +306,7 @@ tableint HnswIndex::connect_new_element(const float* point, tableint cur_c, MaxH
     (void)point;
     size_t mcurmax = level ? maxM_ : maxM0_;
     select_neighbors(top, M_);
-    if (top.size() > M_) throw std::runtime_error("Should be not be more than M_ candidates returned by the heuristic");
+    if (top.size() > M_) throw std::runtime_error("internal: the neighbour heuristic kept more than M links");
     std::vector<tableint> selected;
     selected.reserve(M_);
     while (!top.empty()) {
@@ -316,13 +316,13 @@ tableint HnswIndex::connect_new_element(const float* point, tableint cur_c, MaxH
     tableint next_closest = selected.back();
     {
         uint32_t* ll_cur = level == 0 ? ll0_mut(cur_c) : ll_mut(cur_c, level);
-        if (*ll_cur) throw std::runtime_error("The newly inserted element should have blank link list");
+        if (*ll_cur) throw std::runtime_error("internal: a new element's link list is not empty");
         set_list_count(ll_cur, (unsigned short)selected.size());
         tableint* data = ll_cur + 1;
         for (size_t i = 0; i < selected.size(); i++) {
-            if (data[i]) throw std::runtime_error("Possible memory corruption");
+            if (data[i]) throw std::runtime_error("internal: a new element's link slot is not empty");
             if (level > element_levels_[selected[i]])
-                throw std::runtime_error("Trying to make a link on a non-existent level");
+                throw std::runtime_error("internal: a link on a level above the element's own");
             data[i] = selected[i];
         }
     }
@@ -331,9 +331,9 @@ tableint HnswIndex::connect_new_element(const float* point, tableint cur_c, MaxH
         std::unique_lock<std::mutex> lock(link_list_locks_[other]);
         uint32_t* ll_other = level == 0 ? ll0_mut(other) : ll_mut(other, level);
         size_t sz = list_count(ll_other);
-        if (sz > mcurmax) throw std::runtime_error("Bad value of sz_link_list_other");
-        if (other == cur_c) throw std::runtime_error("Trying to connect an element to itself");
-        if (level > element_levels_[other]) throw std::runtime_error("Trying to make a link on a non-existent level");
+        if (sz > mcurmax) throw std::runtime_error("internal: a link list holds more entries than its capacity");
+        if (other == cur_c) throw std::runtime_error("internal: an element selected itself as a neighbour");
+        if (level > element_levels_[other]) throw std::runtime_error("internal: a link on a level above the element's own");
         tableint* data = ll_other + 1;
         if (sz < mcurmax) {
             data[sz] = cur_c;
@@ -382,7 +382,7 @@ tableint HnswIndex::insert(const float* point, labeltype lab, int level) {
     memcpy(level0_ + cur_c * size_data_per_element_ + offset_data_, point, dim_ * sizeof(float));
     if (curlevel) {
         link_lists_[cur_c] = (char*)malloc(size_links_per_element_ * curlevel + 1);
-        if (!link_lists_[cur_c]) throw std::runtime_error("Not enough memory: addPoint failed to allocate linklist");
+        if (!link_lists_[cur_c]) throw std::runtime_error("out of host memory for an upper-level link list");
         memset(link_lists_[cur_c], 0, size_links_per_element_ * curlevel + 1);
     }
 
@@ -500,7 +500,7 @@ void HnswIndex::register_items(const float* data, size_t rows, const labeltype* 
             memcpy(rec + offset_data_, data + r * dim_, dim_ * sizeof(float));
             if (l) {
                 link_lists_[c] = (char*)malloc(size_links_per_element_ * l + 1);
-                if (!link_lists_[c]) throw std::runtime_error("Not enough memory: addPoint failed to allocate linklist");
+                if (!link_lists_[c]) throw std::runtime_error("out of host memory for an upper-level link list");
                 memset(link_lists_[c], 0, size_links_per_element_ * l + 1);
             }
         }
@@ -519,7 +519,7 @@ std::vector<int> HnswIndex::draw_levels(size_t rows) {
 
 void HnswIndex::set_links(tableint i, int level, const uint32_t* ids, size_t count) {
     if (i >= cur_element_count_ || level < 1 || level > element_levels_[i] || count > maxM_)
-        throw std::runtime_error("Bad value of sz_link_list_other");
+        throw std::runtime_error("internal: a link list holds more entries than its capacity");
     uint32_t* ll = ll_mut(i, level);
     set_list_count(ll, (unsigned short)count);
     for (size_t j = 0; j < maxM_; j++) ll[1 + j] = j < count ? ids[j] : 0u;
@@ -532,7 +532,7 @@ void HnswIndex::set_entry(tableint entry, int maxlevel) {
 }
 
 void HnswIndex::set_links0(tableint i, const uint32_t* ids, size_t count) {
-    if (i >= cur_element_count_ || count > maxM0_) throw std::runtime_error("Bad value of sz_link_list_other");
+    if (i >= cur_element_count_ || count > maxM0_) throw std::runtime_error("internal: a link list holds more entries than its capacity");
     uint32_t* ll = ll0_mut(i);
     set_list_count(ll, (unsigned short)count);
     for (size_t j = 0; j < maxM0_; j++) ll[1 + j] = j < count ? ids[j] : 0u;

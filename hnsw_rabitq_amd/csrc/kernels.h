@@ -69,20 +69,41 @@ struct BfData {
 // Exact FP32 search: k smallest (distance, label) pairs per query, rows closest first.
 // Needs scratch of bf_exact_scratch_bytes().
 size_t bf_exact_scratch_bytes(uint32_t nq, uint32_t k, uint32_t n);
+// scratch_cap (bytes, 0 = bf_exact_scratch_bytes) bounds the distance chunk: fewer queries per
+// pass when the scratch is smaller (at least one row of n distances).
 hipError_t launch_bf_exact(const BfData& d, const float* q, uint32_t nq, uint32_t k, uint64_t* labels, float* dists,
-                           void* scratch, hipStream_t s);
+                           void* scratch, hipStream_t s, size_t scratch_cap = 0);
 
 // Reduced-precision candidate pass + FP32 re-rank.
 size_t bf_lowp_scratch_bytes(uint32_t nq, uint32_t dim, uint32_t c, uint32_t n);
 hipError_t launch_bf_convert(const BfData& d, uint16_t* vec_bf16, uint8_t* vec_fp8, float* fp8_scale, float* norms,
                              hipStream_t s);
+// What one reduced-precision search did besides the re-rank.
+struct LowpReport {
+    uint32_t exact_overflow = 0;     // queries answered by the exact path: candidate list overflowed,
+                                     // came up short or had too many ties at the cut
+    uint32_t exact_uncertified = 0;  // ... the error bound could not certify the re-ranked list
+    uint64_t near_ties = 0, near_tie_disagree = 0;            // FP64 check of adjacent near-tie pairs
+    uint64_t boundary_near_ties = 0, boundary_disagree = 0;   // the same at the k-th boundary
+    float max_bound = 0.f;           // largest per-query error bound
+    bool certified = false;          // the bound was applied
+};
+// Quantisation statistics of the stored rows (3 uint32: float bits of max |x|, max |x~|,
+// max |x - x~|), folded in by quant_stats_kernel into out (cleared by the caller).
+hipError_t launch_quant_stats(const BfData& d, int precision, uint32_t* out, hipStream_t s);
+// xstats (may be null: no certification) = launch_quant_stats's output for these rows;
+// tie_rel > 0 runs the FP64 near-tie check in the re-rank.
 hipError_t launch_bf_lowp(const BfData& d, int precision, const float* q, uint32_t nq, uint32_t k, uint32_t c,
                           uint64_t* labels, float* dists, void* scratch, hipStream_t s, double* stats,
-                          int variant = -1);
+                          int variant = -1, const uint32_t* xstats = nullptr, double tie_rel = 0.0,
+                          LowpReport* rep = nullptr);
 constexpr int kGemmVariants = 5;
 void set_gemm_variant(int v);
+// FP64 report of final lists: labels are mapped to rows through (sorted_labels, sorted_rows), the
+// index's labels ascending and their rows; out_host[5]; scratch >= 64 bytes.
 hipError_t launch_fp64_check(const BfData& d, const float* q, uint32_t nq, uint32_t k, const uint64_t* labels,
-                             const float* dists, double rel_tol, double* out_host, void* scratch, hipStream_t s);
+                             const float* dists, double rel_tol, const uint64_t* sorted_labels,
+                             const uint32_t* sorted_rows, double* out_host, void* scratch, hipStream_t s);
 
 // ---- learned stop (mlp.hip) ----
 hipError_t launch_mlp_forward(const float* params, int in_dim, int hidden, const float* x, uint32_t n, float* logits,

@@ -34,6 +34,8 @@
  *                         BruteforceSearch::searchKnn bruteforce.h:106-125
  *   hg_bf_search_device   the same, device-resident, with reduced-precision distance GEMM
  *                         (BF16 / FP8) and an exact FP32 re-rank of the top-c candidates
+ *   hg_bf_search_precision  BFIndex.knn_query bindings.cpp:850-929 with the certified
+ *                         reduced-precision search (same lists as bruteforce.h:106-125)
  *   hg_bf_search_filtered BFIndex.knn_query(filter=) -> searchKnn(q, k, isIdAllowed)
  *                         bruteforce.h:106-125 (exact FP32)
  *   hg_bf_delete          BFIndex.delete_vector bindings.cpp:829-831 -> removePoint
@@ -159,8 +161,8 @@ int hg_gpu_search_device_filtered(void* gpu_index, const float* d_queries, size_
                                   const int32_t* d_k_per_query, size_t ef, size_t k, size_t k_stride,
                                   uint64_t* d_labels, float* d_dists, int32_t* d_counters, void* filter,
                                   uint32_t* d_status, void* stream);
-/* After the last search launched on this index (synchronise its stream first): the index's own
- * status word (see above) and the number of queries that needed the second, largest-beam pass.
+/* The index's own status word (see above) and the number of queries that needed the second,
+ * largest-beam pass, for the last search launched on this index; waits for that launch to end.
  * Either pointer may be NULL. */
 int hg_gpu_search_status(void* gpu_index, uint32_t* status, uint32_t* second_pass_queries);
 /* Range search: searchStopConditionClosest with EpsilonSearchStopCondition(epsilon,
@@ -231,14 +233,38 @@ int hg_bf_free(void* bf);
 /* Exact FP32 BFIndex.knn_query on host buffers (closest first, ties by label). */
 int hg_bf_search(void* bf, const float* queries, size_t nq, size_t k, uint64_t* labels, float* dists);
 /* Device-resident search.  precision = HG_PREC_*; rerank_c (<= 4096) = candidates kept per
- * query by the reduced-precision pass and re-ranked in exact FP32 (ignored for FP32).  stats (optional, host, 4 doubles): GEMM ms,
- * select ms, re-rank ms, queries whose candidate pass had to be retried. */
+ * query by the reduced-precision pass and re-ranked in exact FP32 (ignored for FP32).
+ * Reduced precision is certified per query: with B a bound on |approximate - exact FP32
+ * distance| over all rows (from the query's and the rows' norms and quantisation errors and
+ * the FP32 rounding of both forms), a query whose rows left out of the re-rank all have
+ * approximate distance >= A, where A - B exceeds the k-th exact distance found, has the exact
+ * search's list; every other query (and one whose candidate list overflowed) is answered by the
+ * exact path.  The lists therefore always equal BFIndex.knn_query's (bruteforce.h:106-125).
+ * Adjacent near-tie pairs of the re-ranked order (FP32 distances within 1e-6 relative, up to
+ * the pair at the k-th boundary) are recomputed in FP64 (hg_bf_lowp_report).  stats (optional,
+ * host, 4 doubles): GEMM ms, select ms, re-rank ms, queries answered by the exact path.
+ * Synchronises the stream (the exact-path queries are chosen on the host). */
 int hg_bf_search_device(void* bf, const float* d_queries, size_t nq, size_t k, int precision, size_t rerank_c,
                         uint64_t* d_labels, float* d_dists, double* stats, void* stream);
-/* FP64 near-tie check on the final lists: for each query, pairs of neighbouring results whose
- * FP32 distances differ by at most rel_tol (relative) are re-evaluated in FP64.  out[0] =
- * near-tie pairs checked, out[1] = pairs whose FP64 order disagrees with the FP32 order,
- * out[2] = max |d32 - d64| over all returned results, out[3] = max relative error. */
+/* BFIndex.knn_query (bindings.cpp:850-929) on host buffers at a chosen precision: HG_PREC_FP32
+ * is hg_bf_search; HG_PREC_BF16 / HG_PREC_FP8 run the certified reduced-precision search above
+ * (dim % 64 == 0), whose lists equal hg_bf_search's. */
+int hg_bf_search_precision(void* bf, const float* queries, size_t nq, size_t k, int precision, size_t rerank_c,
+                           uint64_t* labels, float* dists);
+/* What the last reduced-precision search on this index did besides the re-rank: out[0] queries
+ * answered by the exact path because the candidate list overflowed, came up short or had too
+ * many ties at the cut, out[1] queries answered by the exact path because the bound could not
+ * certify them, out[2] near-tie pairs checked in FP64, out[3] those whose FP64 order differs from
+ * the FP32 order (the reference's order is the FP32 one: the lists keep it), out[4] / out[5] the
+ * same two counts for the pair at the k-th boundary, out[6] the largest per-query bound, out[7]
+ * 1 when the bound was applied.  No counterpart in the reference. */
+int hg_bf_lowp_report(void* bf, double* out);
+/* FP64 report on final lists (any precision): every listed (query, label) distance is recomputed
+ * in FP64; pairs of neighbouring results whose FP32 distances differ by at most rel_tol
+ * (relative) are near ties.  out[0] = near-tie pairs, out[1] = those whose FP64 order disagrees
+ * with the list's (distance, label) order, out[2] = max |d32 - d64| over all returned results,
+ * out[3] = max relative error.  Labels are mapped to rows through the index's label table; a
+ * label the index does not hold is an error. */
 int hg_bf_fp64_check(void* bf, const float* d_queries, size_t nq, size_t k, const uint64_t* d_labels,
                      const float* d_dists, double rel_tol, double* out, void* stream);
 
@@ -332,7 +358,11 @@ int hg_timer_create(void** timer);
 int hg_timer_destroy(void* timer);
 int hg_timer_start(void* timer, void* stream);
 int hg_timer_stop(void* timer, void* stream, float* ms);
-/* hipGraph capture of a device search: capture once, replay many times. */
+/* hipGraph capture of a device search: capture once, replay many times.  The graph keeps the
+ * kernels chosen at capture (the bare-bone search, or the non-bare-bone one when the index held
+ * deleted elements then), so hg_graph_launch fails (HG_ERR_RUNTIME) once the GPU index was freed
+ * or its delete marks were re-read (hg_gpu_refresh_deleted): capture the search again.
+ * Replays of one graph share its launch words: do not overlap them. */
 int hg_graph_capture_search(void* gpu_index, const float* d_queries, size_t nq, const int32_t* d_ef_per_query,
                             const int32_t* d_k_per_query, size_t ef, size_t k, size_t k_stride, uint64_t* d_labels,
                             float* d_dists, int32_t* d_counters, void* stop_model, void* stream, void** out_graph);

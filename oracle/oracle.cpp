@@ -282,9 +282,12 @@ struct Searcher {
         bool operator()(const P& a, const P& b) const { return b < a; }
     };
 
+    // hop_cap > 0: the bare-bone search also stops once it has expanded hop_cap level-0 nodes
+    // (a stop condition whose should_stop_search fires after that many hops; hnswlib.h:134-150,
+    // hnswalg.h:343-355)
     template <int MODE>
     void run(const float* q, size_t k, size_t ef, std::vector<Res>& out, int64_t* ctr, uint8_t* tie,
-             const uint8_t* allowed = nullptr) {
+             const uint8_t* allowed = nullptr, int64_t hop_cap = 0) {
         out.clear();
         int64_t H0 = 0, E0 = 0, Hu = 0, Eu = 0;
         if (h.count == 0) return;
@@ -411,7 +414,7 @@ struct Searcher {
             visited[cur] = tag;
             while (!cand.empty()) {
                 P cp = cand.top();
-                if (-cp.first > lowerBound) break;
+                if (-cp.first > lowerBound || (hop_cap > 0 && H0 >= hop_cap)) break;
                 cand.pop();
                 const uint32_t* data = h.ll0(cp.second);
                 size_t size = Hnsw::count_of(data);
@@ -447,7 +450,7 @@ struct Searcher {
             visited[cur] = tag;
             while (!cand.empty()) {
                 P cp = cand.top();
-                if (top.top() < cp) break;
+                if (top.top() < cp || (hop_cap > 0 && H0 >= hop_cap)) break;
                 cand.pop();
                 const uint32_t* data = h.ll0(cp.second);
                 size_t size = Hnsw::count_of(data);
@@ -886,10 +889,30 @@ int or_hnsw_search(void* hv, int space, const float* q, size_t nq, size_t k, siz
                                   nullptr, labels, dists, counters, tie_flags);
 }
 
+static int search_rows(void* hv, int space, const float* q, size_t nq, size_t k, size_t ef,
+                       const int32_t* ef_per_query, const int32_t* k_per_query, size_t k_stride, int mode,
+                       int nthreads, const uint8_t* allowed, const int32_t* hop_caps, uint64_t* labels,
+                       float* dists, int64_t* counters, uint8_t* tie_flags);
+
 int or_hnsw_search_allowed(void* hv, int space, const float* q, size_t nq, size_t k, size_t ef,
                            const int32_t* ef_per_query, const int32_t* k_per_query, size_t k_stride, int mode,
                            int nthreads, const uint8_t* allowed, uint64_t* labels, float* dists, int64_t* counters,
                            uint8_t* tie_flags) {
+    return search_rows(hv, space, q, nq, k, ef, ef_per_query, k_per_query, k_stride, mode, nthreads, allowed, nullptr,
+                       labels, dists, counters, tie_flags);
+}
+
+int or_hnsw_search_hopcap(void* hv, int space, const float* q, size_t nq, size_t k, size_t ef,
+                          const int32_t* hop_caps, size_t k_stride, int mode, int nthreads, uint64_t* labels,
+                          float* dists, int64_t* counters, uint8_t* tie_flags) {
+    return search_rows(hv, space, q, nq, k, ef, nullptr, nullptr, k_stride, mode, nthreads, nullptr, hop_caps, labels,
+                       dists, counters, tie_flags);
+}
+
+static int search_rows(void* hv, int space, const float* q, size_t nq, size_t k, size_t ef,
+                       const int32_t* ef_per_query, const int32_t* k_per_query, size_t k_stride, int mode,
+                       int nthreads, const uint8_t* allowed, const int32_t* hop_caps, uint64_t* labels,
+                       float* dists, int64_t* counters, uint8_t* tie_flags) {
     const Hnsw& h = *(const Hnsw*)hv;
     if (k_stride == 0) k_stride = k;
     if (nthreads <= 0) nthreads = (int)std::thread::hardware_concurrency();
@@ -903,12 +926,13 @@ int or_hnsw_search_allowed(void* hv, int space, const float* q, size_t nq, size_
         size_t efq = ef_per_query ? (size_t)ef_per_query[r] : ef;
         std::vector<Res> out;
         try {
+            const int64_t cap = hop_caps ? (int64_t)hop_caps[r] : 0;
             if (mode == OR_MODE_HEAP)
                 ss[t]->run<OR_MODE_HEAP>(q + r * h.dim, kq, efq, out, counters ? counters + 4 * r : nullptr,
-                                         tie_flags ? tie_flags + r : nullptr, allowed);
+                                         tie_flags ? tie_flags + r : nullptr, allowed, cap);
             else
                 ss[t]->run<OR_MODE_TOTAL>(q + r * h.dim, kq, efq, out, counters ? counters + 4 * r : nullptr,
-                                          tie_flags ? tie_flags + r : nullptr, allowed);
+                                          tie_flags ? tie_flags + r : nullptr, allowed, cap);
         } catch (std::exception& e) {
             std::lock_guard<std::mutex> g(err_mu);
             g_err = e.what();

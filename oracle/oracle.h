@@ -9,6 +9,8 @@
 //   or_hnsw_load           hnswalg.h:715-821 (on-disk format; header field order 685-712)
 //   or_hnsw_search         hnswalg.h:1269-1323 (searchKnn) + 308-440 (searchBaseLayerST<true/false>)
 //                          + bindings.cpp:612-697 (row output, closest first)
+//   or_hnsw_search_hopcap  the same with a stop after a given number of level-0 hops
+//                          (hnswalg.h:343-355 + hnswlib.h:134-150, the stop-condition slot)
 //   or_hnsw_search_epsilon hnswalg.h:1326-1377 (searchStopConditionClosest) + stop_condition.h:207-275
 //   or_hnsw_search_multivector  the same + stop_condition.h:82-200 (MultiVectorSearchStopCondition)
 //   or_bf_search           bruteforce.h:106-125 + bindings.cpp:850-929
@@ -64,6 +66,13 @@ int or_hnsw_search_allowed(void* h, int space, const float* q, size_t nq, size_t
                            const int32_t* ef_per_query, const int32_t* k_per_query, size_t k_stride, int mode,
                            int nthreads, const uint8_t* allowed, uint64_t* labels, float* dists, int64_t* counters,
                            uint8_t* tie_flags);
+// The bare-bone search with a per-query hop cap: query r also stops once it has expanded
+// hop_caps[r] level-0 nodes (hop_caps[r] <= 0: no cap).  This is searchBaseLayerST driven by a
+// stop condition that ends the search after a given number of hops (hnswalg.h:343-355 with
+// hnswlib.h:134-150), the shape of a learned stop that fired after that many expansions.
+int or_hnsw_search_hopcap(void* h, int space, const float* q, size_t nq, size_t k, size_t ef,
+                          const int32_t* hop_caps, size_t k_stride, int mode, int nthreads, uint64_t* labels,
+                          float* dists, int64_t* counters, uint8_t* tie_flags);
 // searchStopConditionClosest with EpsilonSearchStopCondition (hnswalg.h:1326-1377,
 // stop_condition.h:207-275): per query, counts[q] results (distance <= eps, at most max_c),
 // closest first, in rows of out_stride (the first out_stride of them).  Mode HEAP keeps the

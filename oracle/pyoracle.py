@@ -47,6 +47,7 @@ def lib():
         L.or_hnsw_search_allowed.argtypes = [_vp, _i32, _vp, _sz, _sz, _sz, _vp, _vp, _sz, _i32, _i32, _vp, _vp,
                                              _vp, _vp, _vp]
         L.or_hnsw_labels.argtypes = [_vp, _vp, _vp]
+        L.or_hnsw_search_hopcap.argtypes = [_vp, _i32, _vp, _sz, _sz, _sz, _vp, _sz, _i32, _i32, _vp, _vp, _vp, _vp]
         L.or_hnsw_search_multivector.argtypes = [_vp, _i32, _vp, _sz, _vp, _sz, _sz, _i32, _i32, _vp, _sz, _vp, _vp,
                                                  _vp, _vp, _vp]
         L.or_hnsw_search_epsilon.argtypes = [_vp, _i32, _vp, _sz, ctypes.c_float, _sz, _sz, _i32, _i32, _vp, _sz,
@@ -116,6 +117,22 @@ class OracleHnsw:
         if r < 0:
             raise RuntimeError(lib().or_last_error().decode())
         return labels, dists, ctr, tf, r
+
+    def search_hopcap(self, space, queries, k, ef, hop_caps, mode=MODE_TOTAL, nthreads=0, counters=False):
+        """The bare-bone search stopped after hop_caps[q] level-0 expansions (<= 0: no cap):
+        (labels, dists, counters, short rows)."""
+        q = np.ascontiguousarray(queries, np.float32)
+        nq = q.shape[0]
+        caps = np.ascontiguousarray(hop_caps, np.int32)
+        assert caps.shape == (nq,)
+        labels = np.empty((nq, k), np.uint64)
+        dists = np.empty((nq, k), np.float32)
+        ctr = np.zeros((nq, 4), np.int64) if counters else None
+        r = lib().or_hnsw_search_hopcap(self.h, space, _p(q), nq, k, ef, _p(caps), k, mode, nthreads, _p(labels),
+                                        _p(dists), _p(ctr), None)
+        if r < 0:
+            raise RuntimeError(lib().or_last_error().decode())
+        return labels, dists, ctr, r
 
     def search_epsilon(self, space, queries, eps, min_c, max_c, out_stride, mode=MODE_HEAP, nthreads=0,
                        allowed=None, counters=False, ties=False):

@@ -91,3 +91,44 @@ def test_autotune_keeps_results(hg):
     assert plan["hash_slots"] >= 2 * k and plan["beam_cap"] >= ef
     gpu.search_device(dq, nq, k, ef, l0, d0)
     np.testing.assert_array_equal(l0.to_numpy(np.uint64, (nq, k)), a)
+
+
+def test_graph_replay_refuses_stale_index(hg):
+    """A captured search keeps the kernels chosen at capture: after the delete marks are re-read
+    (mark_deleted, then the index refreshes them) or the GPU index is freed, a replay fails
+    instead of returning deleted labels or reading freed buffers."""
+    from hnsw_rabitq_amd import DeviceArray, HnswGpuError, _lib
+    rng = np.random.default_rng(2)
+    data = rng.random((3000, 16), dtype=np.float32)
+    p = hg.Index("l2", 16)
+    p.init_index(3000, M=16, ef_construction=100)
+    p.add_items(data)
+    q = data[:64] + 0.001
+    gpu = p.gpu()
+    stream = ctypes.c_void_p()
+    _lib.call("hg_stream_create", ctypes.byref(stream))
+    dq = DeviceArray.from_numpy(q)
+    gl, gd = DeviceArray(64 * 5 * 8), DeviceArray(64 * 5 * 4)
+    graph = ctypes.c_void_p()
+    _lib.call("hg_graph_capture_search", gpu.handle, dq.ptr, 64, None, None, 40, 5, 5, gl.ptr, gd.ptr, None, None,
+              stream, ctypes.byref(graph))
+    _lib.call("hg_graph_launch", graph, stream)
+    _lib.call("hg_stream_synchronize", stream)
+    assert gl.to_numpy(np.uint64, (64, 5))[0, 0] == 0
+    p.mark_deleted(0)
+    assert p.gpu() is gpu  # delete marks re-read in place
+    with pytest.raises(HnswGpuError, match="delete marks changed"):
+        _lib.call("hg_graph_launch", graph, stream)
+    _lib.call("hg_graph_free", graph)
+    graph = ctypes.c_void_p()
+    _lib.call("hg_graph_capture_search", gpu.handle, dq.ptr, 64, None, None, 40, 5, 5, gl.ptr, gd.ptr, None, None,
+              stream, ctypes.byref(graph))
+    _lib.call("hg_graph_launch", graph, stream)
+    _lib.call("hg_stream_synchronize", stream)
+    assert 0 not in gl.to_numpy(np.uint64, (64, 5))  # captured with the mark: the filtered kernel
+    p._gpu.free()
+    p._gpu = None
+    with pytest.raises(HnswGpuError, match="was freed"):
+        _lib.call("hg_graph_launch", graph, stream)
+    _lib.call("hg_graph_free", graph)
+    _lib.call("hg_stream_destroy", stream)

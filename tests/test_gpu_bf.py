@@ -110,12 +110,16 @@ def test_bf_lowp_ties_at_rerank_cut(hg, prec):
     _lib.call("hg_device_synchronize")
     ll = dl.to_numpy(np.uint64, (q.shape[0], k))
     ld = dd.to_numpy(np.float32, (q.shape[0], k))
-    print(f"stats {stats}", flush=True)
+    rep = b.lowp_report()
+    print(f"stats {stats} report {rep}", flush=True)
     np.testing.assert_array_equal(ll, el)
     np.testing.assert_array_equal(ld.view(np.uint32), ed.view(np.uint32))
     # the first group's lists are its 100 smallest labels
     np.testing.assert_array_equal(ll[0], np.sort(labels[:1500])[:k])
-    assert stats[3] == 32
+    # the second group's ties outnumber what the re-rank resolves; the first group's copies left
+    # out at the cut have the k-th distance itself, which the bound cannot rule out
+    assert rep["exact_overflow"] == 32
+    assert rep["exact_overflow"] + rep["exact_uncertified"] == stats[3]
 
 
 @pytest.mark.parametrize("prec", ["bf16", "fp8"])
@@ -142,7 +146,9 @@ def test_bf_lowp_candidate_list_paths(hg, prec, n, c):
     _lib.call("hg_device_synchronize")
     np.testing.assert_array_equal(dl.to_numpy(np.uint64, (q.shape[0], k)), el)
     np.testing.assert_array_equal(dd.to_numpy(np.float32, (q.shape[0], k)).view(np.uint32), ed.view(np.uint32))
-    assert stats[3] == 0  # no query fell back to the exact path
+    rep = b.lowp_report()
+    print(f"report {rep}", flush=True)
+    assert rep["exact_overflow"] == 0  # no candidate list overflowed
 
 
 @pytest.mark.parametrize("variant", [0, 1, 2, 3, 4])
@@ -195,3 +201,178 @@ def test_bf_gemm_autotune(hg):
     _lib.call("hg_device_synchronize")
     np.testing.assert_array_equal(dl.to_numpy(np.uint64, (q.shape[0], k)), el)
     np.testing.assert_array_equal(dd.to_numpy(np.float32, (q.shape[0], k)).view(np.uint32), ed.view(np.uint32))
+
+
+def _lowp_device(b, q, k, prec, c):
+    from hnsw_rabitq_amd import DeviceArray, _lib
+    import ctypes
+    dq = DeviceArray.from_numpy(q)
+    dl = DeviceArray(q.shape[0] * k * 8)
+    dd = DeviceArray(q.shape[0] * k * 4)
+    stats = np.zeros(4, np.float64)
+    P = {"bf16": _lib.PREC_BF16, "fp8": _lib.PREC_FP8}[prec]
+    _lib.call("hg_bf_search_device", b._h, dq.ptr, q.shape[0], k, P, c, dl.ptr, dd.ptr,
+              ctypes.c_void_p(stats.ctypes.data), None)
+    _lib.call("hg_device_synchronize")
+    return dq, dl, dd, dl.to_numpy(np.uint64, (q.shape[0], k)), dd.to_numpy(np.float32, (q.shape[0], k)), stats
+
+
+@pytest.mark.parametrize("prec", ["bf16", "fp8"])
+def test_bf_lowp_c3_workload_matches_oracle(hg, orc, prec):
+    """The 1M x 768 mixed-precision workload's generator at 60,000 rows (1,024 centres,
+    sigma 0.35, unit norm, inner product), 200 queries, c = 4,096, k = 100: BF16 / FP8 GEMM, FP32
+    re-rank and certification give the oracle's exact lists (bruteforce.h:106-125) bit for bit.
+    Queries the bound cannot certify (FP8's bound is wider than the gap between the 100th and the
+    4,096th distance at this row count) take the exact path; the report counts them."""
+    from hnsw_rabitq_amd import datasets
+    print(f"case=c3 reduced 60000x768 prec={prec}", flush=True)
+    data, q = datasets.c3_gmm(n=60_000, nq=200, dim=768, n_centres=1024)
+    b = _bf(hg, "ip", data)
+    k = 100
+    _, _, _, ll, ld, stats = _lowp_device(b, q, k, prec, 4096)
+    rep = b.lowp_report()
+    print(f"stats {stats} report {rep}", flush=True)
+    ol, od = orc.bf_search(1, data, q, k)
+    np.testing.assert_array_equal(ll, ol)
+    np.testing.assert_array_equal(ld.view(np.uint32), od.view(np.uint32))
+    assert rep["certified"] == 1 and rep["exact_overflow"] == 0
+    assert rep["exact_uncertified"] == stats[3]
+    if prec == "bf16":
+        assert rep["exact_uncertified"] == 0  # the BF16 bound (~3e-3) is far below the gap
+    # the bound is an upper bound on the approximate error: never below the BF16 / FP8
+    # quantisation error of a unit vector
+    assert rep["max_bound"] >= (2.0 ** -9 if prec == "bf16" else 2.0 ** -5)
+    # the host-buffer entry point (BFIndex.knn_query at this precision) gives the same lists
+    b.set_precision(prec, 4096)
+    hl, hd = b.knn_query(q, k=k)
+    np.testing.assert_array_equal(hl, ol)
+    np.testing.assert_array_equal(hd.view(np.uint32), od.view(np.uint32))
+
+
+def test_bf_lowp_uncertified_queries_take_exact_path(hg, orc):
+    """c = k: the re-rank cut sits at the k-th candidate, so the bound cannot certify a list and
+    every query is answered by the exact path -- still the oracle's lists."""
+    from hnsw_rabitq_amd import datasets
+    data, q = datasets.c3_gmm(n=20_000, nq=50, dim=128, n_centres=16)
+    b = _bf(hg, "l2", data * 2.0)
+    _, _, _, ll, ld, stats = _lowp_device(b, q * 2.0, 10, "fp8", 10)
+    rep = b.lowp_report()
+    print(f"stats {stats} report {rep}", flush=True)
+    ol, od = orc.bf_search(0, data * 2.0, q * 2.0, 10)
+    np.testing.assert_array_equal(ll, ol)
+    np.testing.assert_array_equal(ld.view(np.uint32), od.view(np.uint32))
+    assert rep["exact_uncertified"] + rep["exact_overflow"] == 50 and stats[3] == 50
+
+
+def _near_tie_data(space, dim, seed=11):
+    """Rows in twins that differ by one FP32 ulp in one coordinate; queries next to twins: the
+    twins' FP32 distances differ by ~1e-8 relative or tie, and their FP64 order can disagree."""
+    from hnsw_rabitq_amd import datasets
+    data, _ = datasets.c3_gmm(n=8_000, nq=1, dim=dim, n_centres=8, seed=seed)
+    rng = np.random.default_rng(seed)
+    twins = data[:4000].copy()
+    j = rng.integers(0, dim, 4000)
+    up = rng.random(4000) < 0.5
+    twins[np.arange(4000), j] = np.where(up, np.nextafter(twins[np.arange(4000), j], np.float32(np.inf)),
+                                         np.nextafter(twins[np.arange(4000), j], np.float32(-np.inf)))
+    data[4000:] = twins
+    qi = rng.integers(0, 4000, 120)
+    q = (data[qi] + 0.02 * rng.standard_normal((120, dim))).astype(np.float32)
+    if space == "ip":
+        q /= np.linalg.norm(q, axis=1, keepdims=True)
+    return data, q
+
+
+def _expected_near_ties(orc, sp, data, q, lab, dis, rel):
+    """(near-tie pairs, FP64 order disagreements, max |d32 - d64|) of closest-first lists whose
+    labels are row numbers, from the oracle's FP64 distances."""
+    nq, k = lab.shape
+    qi = np.repeat(np.arange(nq), k)
+    d64 = orc.fp64_pairs(sp, q, data, qi, lab.reshape(-1).astype(np.int64)).reshape(nq, k)
+    a, b = dis[:, :-1].astype(np.float64), dis[:, 1:].astype(np.float64)
+    near = (b - a) <= rel * np.abs(a)
+    dis64 = (d64[:, :-1] > d64[:, 1:]) | ((d64[:, :-1] == d64[:, 1:]) & (lab[:, :-1] > lab[:, 1:]))
+    return int(near.sum()), int((near & dis64).sum()), float(np.abs(dis.astype(np.float64) - d64).max())
+
+
+@pytest.mark.parametrize("space", ["l2", "ip"])
+def test_fp64_check_on_planted_near_ties(hg, orc, space):
+    """hg_bf_fp64_check against the oracle's FP64 distances (or_fp64_pairs) on lists full of
+    planted near ties: the same near-tie pairs, the same FP64 order disagreements, the same max
+    error; labels differ from rows (a shuffled label table) to exercise the label -> row map."""
+    from hnsw_rabitq_amd import DeviceArray, _lib
+    import ctypes
+    sp = 0 if space == "l2" else 1
+    data, q = _near_tie_data(space, 64)
+    rng = np.random.default_rng(3)
+    labels = (rng.permutation(data.shape[0]) * 3 + 5).astype(np.uint64)
+    b = _bf(hg, space, data, labels)
+    k = 16
+    lab, dis = b.knn_query(q, k=k)
+    rows = np.argsort(labels)[np.searchsorted(np.sort(labels), lab)]
+    want = _expected_near_ties(orc, sp, data, q, rows, dis, 1e-6)
+    print(f"expected near ties {want}", flush=True)
+    assert want[0] > 50, "the twins must produce near ties"
+    dq, dl, dd = DeviceArray.from_numpy(q), DeviceArray.from_numpy(lab), DeviceArray.from_numpy(dis)
+    out = np.zeros(4, np.float64)
+    _lib.call("hg_bf_fp64_check", b._h, dq.ptr, q.shape[0], k, dl.ptr, dd.ptr, ctypes.c_double(1e-6),
+              ctypes.c_void_p(out.ctypes.data), None)
+    print(f"fp64 check {out}", flush=True)
+    assert int(out[0]) == want[0] and int(out[1]) == want[1]
+    assert out[2] == pytest.approx(want[2], rel=1e-9, abs=1e-15)
+    # a label the index does not hold is an error
+    bad = lab.copy()
+    bad[0, 0] = 4
+    dbad = DeviceArray.from_numpy(bad)
+    with pytest.raises(RuntimeError, match="not in the index"):
+        _lib.call("hg_bf_fp64_check", b._h, dq.ptr, q.shape[0], k, dbad.ptr, dd.ptr, ctypes.c_double(1e-6),
+                  ctypes.c_void_p(out.ctypes.data), None)
+
+
+def test_rerank_fp64_near_tie_check(hg, orc):
+    """The FP64 check inside the re-rank (BF16 path) counts the near-tie pairs of the returned
+    lists and their FP64 order disagreements as the oracle's FP64 distances do (pairs inside the
+    list; the pair at the k-th boundary is counted apart)."""
+    data, q = _near_tie_data("ip", 128, seed=12)
+    b = _bf(hg, "ip", data)
+    k = 16
+    _, _, _, ll, ld, _ = _lowp_device(b, q, k, "bf16", 256)
+    rep = b.lowp_report()
+    print(f"report {rep}", flush=True)
+    ol, od = orc.bf_search(1, data, q, k)
+    np.testing.assert_array_equal(ll, ol)
+    assert rep["exact_overflow"] == 0 and rep["exact_uncertified"] == 0
+    want = _expected_near_ties(orc, 1, data, q, ll.astype(np.int64), ld, 1e-6)
+    assert want[0] > 50
+    assert rep["near_ties"] - rep["boundary_near_ties"] == want[0]
+    assert rep["near_tie_disagree"] - rep["boundary_disagree"] == want[1]
+    assert rep["boundary_near_ties"] <= q.shape[0]
+
+
+def test_bf_add_over_capacity_keeps_placed_rows(hg, orc):
+    """addPoint row by row (bruteforce.h:64-83): rows placed before the index filled up stay
+    added; re-adding and deleting those labels afterwards works on the same rows."""
+    rng = np.random.default_rng(4)
+    data = rng.standard_normal((130, 16)).astype(np.float32)
+    b = hg.BFIndex("l2", 16)
+    b.init_index(100)
+    b.add_items(data[:90], ids=np.arange(90))
+    with pytest.raises(RuntimeError, match="exceeds the specified limit"):
+        b.add_items(data[90:120], ids=np.arange(90, 120))
+    assert b.get_current_count() == 100
+    # re-add labels 90..99 (now existing: rows overwritten in place) and delete two of them
+    b.add_items(data[120:130], ids=np.arange(90, 100))
+    b.delete_vector(95)
+    b.delete_vector(3)
+    assert b.get_current_count() == 98
+    # the reference's rows after the same operations
+    rows = {i: data[i] for i in range(90)}
+    rows.update({90 + i: data[120 + i] for i in range(10)})
+    del rows[95], rows[3]
+    labs = np.array(sorted(rows), np.uint64)
+    mat = np.stack([rows[int(l)] for l in labs])
+    q = rng.standard_normal((20, 16)).astype(np.float32)
+    lab, dis = b.knn_query(q, k=7)
+    ol, od = orc.bf_search(0, mat, q, 7, labels=labs)
+    np.testing.assert_array_equal(lab, ol)
+    np.testing.assert_array_equal(dis.view(np.uint32), od.view(np.uint32))

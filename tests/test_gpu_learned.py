@@ -119,3 +119,74 @@ def test_traces_record_features_and_topk(hg):
     assert np.isfinite(f[np.arange(64)[None, :] < r[:, None]]).all()
     # feature 0 is the query norm
     np.testing.assert_allclose(f[:, 0, 0], np.linalg.norm(queries[:nq], axis=1), rtol=1e-5)
+
+
+def _trained_model(gpu, ef, k, period, seed=7, n_train=4000):
+    """A network trained (GPU forward/backward/Adam) on traces of the plain search over training
+    queries of the reduced 1M x 128 generator (20,000 x 128 integer Gamma, separate seed)."""
+    from hnsw_rabitq_amd import DeviceArray, datasets, learned_stop
+    tq = datasets.c2_gamma(n=n_train, seed=seed)
+    dtq = DeviceArray.from_numpy(tq)
+    f, t, r, fin = learned_stop.collect_traces(gpu, dtq, n_train, ef, k, period=period, max_records=128)
+    eq, valid = learned_stop.label_records(t, r, fin)
+    X, Y = f[valid], eq[valid].astype(np.float32)
+    mean, std = X.mean(0), X.std(0) + 1e-6
+    m = learned_stop.StopModel()
+    m.init_params(0)
+    m.set_norm(mean, 1.0 / std)
+    learned_stop.train(m, (X - mean) / std, Y, steps=400, batch=2048, lr=2e-3)
+    return m, mean.astype(np.float32), (1.0 / std).astype(np.float32), float(Y.mean())
+
+
+def test_learned_stop_matches_hop_capped_oracle(hg, orc):
+    """The learned search's rows equal the oracle's total-order search stopped after the same
+    number of level-0 expansions (the stop-condition slot, hnswlib.h:134-150, hnswalg.h:343-355),
+    and the in-kernel network's stop decisions equal those of the oracle's FP64 network on the
+    same traversal features (queries whose logit comes within 0.01 of the threshold excepted: the
+    kernel's FP32 arithmetic differs from the oracle's FP64 by ~1e-5 there)."""
+    from hnsw_rabitq_amd import DeviceArray, learned_stop
+    p, space, data, queries = built_index("c2s")
+    gpu = p.gpu()
+    nq, k, ef, period, thr = queries.shape[0], 10, 160, 8, 0.9
+    print(f"case=c2s learned stop ef={ef} period={period}", flush=True)
+    m, mean, inv_std, pos = _trained_model(gpu, ef, k, period)
+    m.config(period=period, threshold=thr, min_hops=period)
+    dq = DeviceArray.from_numpy(queries)
+    dl, dd, dc = DeviceArray(nq * k * 8), DeviceArray(nq * k * 4), DeviceArray(nq * 16)
+    gpu.search_device(dq, nq, k, ef, dl, dd, d_counters=dc)
+    plain_h0 = dc.to_numpy(np.int32, (nq, 4))[:, 0]
+    gpu.search_device(dq, nq, k, ef, dl, dd, d_counters=dc, stop_model=m.h)
+    lab = dl.to_numpy(np.uint64, (nq, k))
+    dis = dd.to_numpy(np.float32, (nq, k))
+    ctr = dc.to_numpy(np.int32, (nq, 4))
+    stopped = ctr[:, 0] < plain_h0
+    print(f"positive records {pos:.3f}; stopped early {stopped.sum()} of {nq}; mean hops "
+          f"{ctr[:, 0].mean():.1f} vs {plain_h0.mean():.1f}", flush=True)
+    assert stopped.sum() >= nq // 10, "the trained network should stop a share of the queries early"
+    o = orc.OracleHnsw(blob=p.serialize())
+    ol, od, oc, short = o.search_hopcap(0, queries, k, ef, ctr[:, 0], mode=orc.MODE_TOTAL, counters=True)
+    assert short == 0
+    np.testing.assert_array_equal(lab, ol)
+    np.testing.assert_array_equal(dis.view(np.uint32), od.view(np.uint32))
+    np.testing.assert_array_equal(ctr[:, 0], oc[:, 0])
+    np.testing.assert_array_equal(ctr[:, 2:], oc[:, 2:])
+    # stop decisions: the oracle network (FP64, the kernel's BF16-rounded weights) on the traced
+    # features, normalised as the kernel does (FP32)
+    f, _, r, _ = learned_stop.collect_traces(gpu, dq, nq, ef, k, period=period, max_records=512)
+    assert (r < 512).all()
+    fn = ((f - mean[None, None, :]) * inv_std[None, None, :]).astype(np.float32)
+    eff = effective(m.get_params(), 12, 64)
+    logits = orc.mlp_forward(eff, 12, 64, fn.reshape(-1, 12)).reshape(nq, 512).astype(np.float64)
+    lthr = np.log(thr / (1 - thr))
+    checked = 0
+    for q in range(nq):
+        fire = np.nonzero((logits[q, :r[q]] >= lthr))[0]
+        want = (fire[0] + 1) * period if fire.size else plain_h0[q]
+        upto = fire[0] + 1 if fire.size else r[q]
+        if np.abs(logits[q, :upto] - lthr).min(initial=1.0) < 0.01:
+            continue
+        checked += 1
+        assert ctr[q, 0] == want, f"query {q}: kernel stopped at {ctr[q, 0]}, oracle network at {want}"
+    print(f"stop decisions checked against the oracle network: {checked} of {nq}", flush=True)
+    assert checked >= nq * 0.9
+    m.free()
