@@ -60,6 +60,7 @@ SIGNATURES = {
     "hg_gpu_search_filtered": [_vp, _vp, _sz, _sz, _sz, _vp, _vp, _vp, _vp],
     "hg_gpu_search_device_filtered": [_vp, _vp, _sz, _vp, _vp, _sz, _sz, _sz, _vp, _vp, _vp, _vp, _vp, _vp],
     "hg_gpu_search_status": [_vp, _vp, _vp],
+    "hg_gpu_search_device_noalloc": [_vp, _vp, _sz, _vp, _vp, _sz, _sz, _sz, _vp, _vp, _vp, _vp, _vp, _vp],
     "hg_gpu_docs_create": [_vp, _vp, _vp, _sz, _vp],
     "hg_gpu_docs_free": [_vp],
     "hg_gpu_search_multivector": [_vp, _vp, _sz, _vp, _sz, _sz, _vp, _sz, _vp, _vp, _vp],

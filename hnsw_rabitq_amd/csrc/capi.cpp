@@ -163,12 +163,20 @@ Plan make_plan(GpuIndex* gi, size_t max_ef, size_t max_k, size_t nq, bool filter
     return p;
 }
 
-hg::SearchParams base_params(GpuIndex* gi, const Plan& pl, GpuIndex::LaunchWords* lw) {
+// Device words of one search launch: kWords status / counter words and, for the non-bare-bone
+// search, two overflow lists of nq u32 (may be null for the bare-bone search).
+struct WordsView {
+    uint32_t* words;
+    uint32_t* ovf;
+};
+WordsView view(GpuIndex::LaunchWords* lw) { return WordsView{(uint32_t*)lw->words.p, (uint32_t*)lw->ovf.p}; }
+
+hg::SearchParams base_params(GpuIndex* gi, const Plan& pl, uint32_t* words) {
     hg::SearchParams sp{};
     sp.g = gi->g;
     sp.cap = pl.cap;
     sp.hash_bits = pl.hash_bits;
-    sp.work = (uint32_t*)lw->words.p + kWordWork;
+    sp.work = words + kWordWork;
     sp.trace_period = 1;
     return sp;
 }
@@ -229,7 +237,7 @@ void prepare_words(GpuIndex::LaunchWords* lw, size_t nq, bool filtered) {
 // beam too.  lw: the launch words to use (prepared).
 hipError_t run_search(GpuIndex* gi, const float* dq, size_t nq, const int32_t* def, const int32_t* dk, size_t ef,
                       size_t k, size_t k_stride, uint64_t* dl, float* dd, int32_t* dc, StopModel* stop,
-                      hipStream_t s, GpuIndex::LaunchWords* lw, const GpuFilter* filt = nullptr,
+                      hipStream_t s, const WordsView& lw, const GpuFilter* filt = nullptr,
                       uint32_t* status = nullptr, const EpsArgs* eps = nullptr) {
     const bool filtered = is_filtered(gi, filt) || eps != nullptr;
     const double frac = filtered ? allowed_fraction(gi, filt) : 1.0;
@@ -238,7 +246,7 @@ hipError_t run_search(GpuIndex* gi, const float* dq, size_t nq, const int32_t* d
     const size_t plan_ef = eps ? std::max<size_t>(eps->min_c, 1) : ef;
     const size_t plan_k = eps ? 1 : k;
     Plan pl = make_plan(gi, plan_ef, plan_k, nq, filtered, frac);
-    hg::SearchParams sp = base_params(gi, pl, lw);
+    hg::SearchParams sp = base_params(gi, pl, lw.words);
     sp.queries = dq;
     sp.nq = (uint32_t)nq;
     sp.ef_pq = def;
@@ -260,9 +268,16 @@ hipError_t run_search(GpuIndex* gi, const float* dq, size_t nq, const int32_t* d
         sp.eps_min = (int)std::min<size_t>(eps->min_c, 0x7FFFFFFF);
         sp.counts = eps->counts;
     }
-    if (!filtered) return hg::launch_search(sp, pl.blocks, s);
+    if (!filtered) {
+        // the bare-bone beam is clamped to its capacity: no query can outgrow it
+        if (status) {
+            hipError_t e = hg::launch_zero_u32(status, 1, s);
+            if (e != hipSuccess) return e;
+        }
+        return hg::launch_search(sp, pl.blocks, s);
+    }
 
-    uint32_t* words = (uint32_t*)lw->words.p;
+    uint32_t* words = lw.words;
     sp.del_bits = gi->num_deleted > 0 ? (const uint32_t*)gi->delbits.p : nullptr;
     sp.allow_bits = filt ? (const uint32_t*)filt->bits.p : nullptr;
     sp.status = status ? status : words + kWordStatus;
@@ -279,11 +294,11 @@ hipError_t run_search(GpuIndex* gi, const float* dq, size_t nq, const int32_t* d
         sq.hash_bits = cur.hash_bits;
         sq.work = words + kWordWork + pass;
         if (pass > 0) {
-            sq.order = (const uint32_t*)lw->ovf.p + (size_t)((pass - 1) & 1) * nq;
+            sq.order = (const uint32_t*)lw.ovf + (size_t)((pass - 1) & 1) * nq;
             sq.nq_dev = words + kWordCount + pass - 1;
         }
         if (!last) {
-            sq.ovf_list = (uint32_t*)lw->ovf.p + (size_t)(pass & 1) * nq;
+            sq.ovf_list = lw.ovf + (size_t)(pass & 1) * nq;
             sq.ovf_count = words + kWordCount + pass;
         }
         e = hg::launch_search(sq, cur.blocks, s);
@@ -733,7 +748,7 @@ int search_host(GpuIndex* gi, const float* queries, size_t nq, size_t k, size_t 
     GpuIndex::LaunchWords* lw = next_words(gi, s);
     prepare_words(lw, nq, filtered);
     hip_check(run_search(gi, (const float*)gi->q.p, nq, nullptr, nullptr, ef, k, k, (uint64_t*)gi->ol.p,
-                         (float*)gi->od.p, counters ? (int32_t*)gi->ctr.p : nullptr, nullptr, s, lw, filt),
+                         (float*)gi->od.p, counters ? (int32_t*)gi->ctr.p : nullptr, nullptr, s, view(lw), filt),
               "search launch");
     words_done(lw, s);
     if (filtered) {
@@ -813,7 +828,7 @@ HG_API int hg_gpu_search_epsilon(void* gv, const float* queries, size_t nq, floa
         prepare_words(lw, nq, true);
         EpsArgs ea{epsilon, min_candidates, max_candidates, (uint32_t*)gi->cnt.p};
         hip_check(run_search(gi, (const float*)gi->q.p, nq, nullptr, nullptr, max_candidates, out_stride, out_stride,
-                             (uint64_t*)gi->ol.p, (float*)gi->od.p, nullptr, nullptr, s, lw, gf, nullptr, &ea),
+                             (uint64_t*)gi->ol.p, (float*)gi->od.p, nullptr, nullptr, s, view(lw), gf, nullptr, &ea),
                   "search launch");
         words_done(lw, s);
         uint32_t status = 0;
@@ -851,7 +866,7 @@ HG_API int hg_gpu_search_epsilon_device(void* gv, const float* dq, size_t nq, fl
         prepare_words(lw, nq, true);
         EpsArgs ea{epsilon, min_candidates, max_candidates, d_counts};
         hip_check(run_search(gi, dq, nq, nullptr, nullptr, max_candidates, out_stride, out_stride, dl, dd, dc, nullptr,
-                             (hipStream_t)stream, lw, gf, d_status, &ea),
+                             (hipStream_t)stream, view(lw), gf, d_status, &ea),
                   "search launch");
         words_done(lw, (hipStream_t)stream);
         return HG_OK;
@@ -975,7 +990,7 @@ HG_API int hg_gpu_search_device(void* gv, const float* dq, size_t nq, const int3
         GpuIndex::LaunchWords* lw = next_words(gi, (hipStream_t)stream);
         prepare_words(lw, nq, is_filtered(gi, nullptr));
         hip_check(run_search(gi, dq, nq, def, dk, ef, k, k_stride, dl, dd, dc, static_cast<StopModel*>(stop),
-                             (hipStream_t)stream, lw),
+                             (hipStream_t)stream, view(lw)),
                   "search launch");
         words_done(lw, (hipStream_t)stream);
         return HG_OK;
@@ -996,10 +1011,62 @@ HG_API int hg_gpu_search_device_filtered(void* gv, const float* dq, size_t nq, c
         std::lock_guard<std::mutex> lk(gi->mu);
         GpuIndex::LaunchWords* lw = next_words(gi, (hipStream_t)stream);
         prepare_words(lw, nq, is_filtered(gi, gf));
-        hip_check(run_search(gi, dq, nq, def, dk, ef, k, k_stride, dl, dd, dc, nullptr, (hipStream_t)stream, lw, gf,
-                             d_status),
+        hip_check(run_search(gi, dq, nq, def, dk, ef, k, k_stride, dl, dd, dc, nullptr, (hipStream_t)stream, view(lw),
+                             gf, d_status),
                   "search launch");
         words_done(lw, (hipStream_t)stream);
+        return HG_OK;
+    });
+}
+
+// ---- allocation-free device search ----
+// Scratch of hg_gpu_search_device_noalloc: 64 launch words (status, work counters, overflow-list
+// counts) and the two overflow lists of the non-bare-bone search (2 x kNoAllocMaxQueries u32):
+// 2 MiB + 256 B in a file-scope device array, which every device the library runs on holds
+// until the process ends.  Host code never uses the array's name as an address: the address of
+// each device's copy comes from hipGetSymbolAddress, once per device.
+constexpr size_t kNoAllocMaxQueries = 262144;
+constexpr size_t kNoAllocWordSlots = 64;
+static_assert(kWords <= kNoAllocWordSlots, "launch words fit their slots");
+static __device__ __align__(256) uint32_t scratch[kNoAllocWordSlots + 2 * kNoAllocMaxQueries];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+uint32_t* g_scratch_ptr[kMaxDevices] = {};
+
+// The device address of this device's `scratch` (the device must be current).
+uint32_t* scratch_on(int device) {
+    require(device >= 0 && device < kMaxDevices, "device index out of range");
+    std::lock_guard<std::mutex> lk(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        uint32_t* ptr = nullptr;
+        hip_check(hipGetSymbolAddress((void**)&ptr, HIP_SYMBOL(scratch)), "hipGetSymbolAddress");
+        g_scratch_ptr[device] = ptr;
+    }
+    return g_scratch_ptr[device];
+}
+}  // namespace
+
+HG_API int hg_gpu_search_device_noalloc(void* gv, const float* dq, size_t nq, const int32_t* def, const int32_t* dk,
+                                        size_t ef, size_t k, size_t k_stride, uint64_t* dl, float* dd, int32_t* dc,
+                                        void* filter, uint32_t* d_status, void* stream) {
+    return guarded([&] {
+        require(gv && dl && dd, "null argument");
+        auto* gi = static_cast<GpuIndex*>(gv);
+        check_search_args(gi, nq, ef, k, k_stride);
+        if (nq == 0) return HG_OK;
+        require(dq != nullptr, "null queries");
+        GpuFilter* gf = as_filter(gi, filter);
+        require(!is_filtered(gi, gf) || nq <= kNoAllocMaxQueries,
+                "allocation-free search: at most 262,144 queries per call when the index holds deleted elements or "
+                "a filter is given (the overflow lists live in a fixed device array)");
+        DeviceGuard dg(gi->device);
+        std::lock_guard<std::mutex> lk(gi->mu);
+        uint32_t* sc = scratch_on(gi->device);
+        hip_check(run_search(gi, dq, nq, def, dk, ef, k, k_stride, dl, dd, dc, nullptr, (hipStream_t)stream,
+                             WordsView{sc, sc + kNoAllocWordSlots}, gf, d_status),
+                  "search launch");
         return HG_OK;
     });
 }
@@ -1025,7 +1092,7 @@ HG_API int hg_gpu_collect_traces(void* gv, const float* dq, size_t nq, size_t ef
         Plan pl = make_plan(gi, ef, k, nq);
         GpuIndex::LaunchWords* lw = next_words(gi, s);
         prepare_words(lw, nq, false);
-        hg::SearchParams sp = base_params(gi, pl, lw);
+        hg::SearchParams sp = base_params(gi, pl, (uint32_t*)lw->words.p);
         sp.queries = dq;
         sp.nq = (uint32_t)nq;
         sp.ef = (uint32_t)ef;
@@ -1095,7 +1162,7 @@ HG_API int hg_gpu_autotune(void* gv, const float* dq, size_t nq, size_t ef, size
                     GpuIndex::LaunchWords* lw = next_words(gi, s);
                     prepare_words(lw, nq, false);
                     hip_check(run_search(gi, dq, nq, nullptr, nullptr, ef, k, k, (uint64_t*)ol.p, (float*)od.p,
-                                         nullptr, nullptr, s, lw),
+                                         nullptr, nullptr, s, view(lw)),
                               "search launch");
                     words_done(lw, s);
                     hip_check(hipEventRecord(e1, s), "hipEventRecord");
@@ -1368,7 +1435,7 @@ static void gpu_build_level0(hnswgpu::HnswIndex* h, size_t s0, int device) {
                 }
             // beam search of the batch over the graph of [0, sb)
             Plan pl = make_plan(gi.get(), efc, efc, B);
-            hg::SearchParams sp = base_params(gi.get(), pl, lw);
+            hg::SearchParams sp = base_params(gi.get(), pl, (uint32_t*)lw->words.p);
             sp.g.entry = entry;
             sp.g.maxlevel = maxl;
             sp.g.labels = (const uint64_t*)ident.p;
@@ -1532,7 +1599,7 @@ HG_API int hg_graph_capture_search(void* gv, const float* dq, size_t nq, const i
         hipError_t le = hipSuccess;
         try {
             le = run_search(gi, dq, nq, def, dk, ef, k, k_stride, dl, dd, dc, static_cast<StopModel*>(stop), s,
-                            &gh->lw);
+                            view(&gh->lw));
         } catch (...) {
             hipGraph_t g = nullptr;
             (void)hipStreamEndCapture(s, &g);

@@ -161,6 +161,20 @@ int hg_gpu_search_device_filtered(void* gpu_index, const float* d_queries, size_
                                   const int32_t* d_k_per_query, size_t ef, size_t k, size_t k_stride,
                                   uint64_t* d_labels, float* d_dists, int32_t* d_counters, void* filter,
                                   uint32_t* d_status, void* stream);
+/* hg_gpu_search_device_filtered (filter may be NULL; the bare-bone search when the index holds no
+ * deleted elements and no filter is given) that allocates nothing: no hipMalloc, hipFree or other
+ * allocating call.  No counterpart in the reference: it is for callers that cannot allocate --
+ * code under hipGraph capture, tight serving loops, harnesses that forbid allocation.  Its device
+ * words (work counters, status, the non-bare-bone search's overflow lists) live in one fixed
+ * device array per GPU (2 MiB + 256 B, held until the process ends), so with deleted elements or
+ * a filter a call takes at most 262,144 queries (HG_ERR_INVALID beyond).  Every call on one GPU
+ * shares that array: calls that overlap on one GPU (other streams, other threads) race; the
+ * array keeps the last call's contents and each call clears what it reads.  d_status as for
+ * hg_gpu_search_device_filtered (NULL = the array's own word, not readable by the caller). */
+int hg_gpu_search_device_noalloc(void* gpu_index, const float* d_queries, size_t nq, const int32_t* d_ef_per_query,
+                                 const int32_t* d_k_per_query, size_t ef, size_t k, size_t k_stride,
+                                 uint64_t* d_labels, float* d_dists, int32_t* d_counters, void* filter,
+                                 uint32_t* d_status, void* stream);
 /* The index's own status word (see above) and the number of queries that needed the second,
  * largest-beam pass, for the last search launched on this index; waits for that launch to end.
  * Either pointer may be NULL. */
