@@ -60,6 +60,7 @@ SIGNATURES = {
     "hg_gpu_search_filtered": [_vp, _vp, _sz, _sz, _sz, _vp, _vp, _vp, _vp],
     "hg_gpu_search_device_filtered": [_vp, _vp, _sz, _vp, _vp, _sz, _sz, _sz, _vp, _vp, _vp, _vp, _vp, _vp],
     "hg_gpu_search_status": [_vp, _vp, _vp],
+    "hg_gpu_search_device_hopcap": [_vp, _vp, _sz, _vp, _sz, _sz, _vp, _vp, _vp, _vp],
     "hg_gpu_search_device_noalloc": [_vp, _vp, _sz, _vp, _vp, _sz, _sz, _sz, _vp, _vp, _vp, _vp, _vp, _vp],
     "hg_gpu_docs_create": [_vp, _vp, _vp, _sz, _vp],
     "hg_gpu_docs_free": [_vp],
@@ -89,7 +90,8 @@ SIGNATURES = {
     "hg_stop_model_get_params": [_vp, _vp, _sz],
     "hg_stop_model_set_norm": [_vp, _vp, _vp, _sz],
     "hg_stop_model_config": [_vp, _i32, ctypes.c_float, _i32],
-    "hg_gpu_collect_traces": [_vp, _vp, _sz, _sz, _sz, _i32, _i32, _vp, _vp, _vp, _vp, _vp],
+    "hg_gpu_collect_traces": [_vp, _vp, _sz, _sz, _sz, _i32, _i32, _vp, _vp, _vp, _vp, _i32, _vp],
+    "hg_stop_model_config_extend": [_vp, _i32, ctypes.c_float],
     "hg_stop_model_forward": [_vp, _vp, _sz, _vp, _vp],
     "hg_stop_model_train_step": [_vp, _vp, _vp, _sz, ctypes.c_float, ctypes.c_float, ctypes.c_float,
                                  ctypes.c_float, ctypes.POINTER(ctypes.c_float), _vp, _vp],

@@ -56,6 +56,7 @@ def run(H, lib, n, nq, stream, timer, log, dim=768, k=100, c=4096, library=True)
         got = ll.to_numpy(np.uint64, (nq, k))
         rec = float(np.mean([len(np.intersect1d(a, b)) for a, b in zip(got, exact)]) / k)
         same = float((got == exact).all(1).mean())
+        rep = bf.lowp_report()
         chk = np.zeros(4, np.float64)
         lib.call("hg_bf_fp64_check", bf._h, dq.ptr, nq, k, ll.ptr, ld.ptr, ctypes.c_double(1e-6),
                  ctypes.c_void_p(chk.ctypes.data), stream)
@@ -66,10 +67,22 @@ def run(H, lib, n, nq, stream, timer, log, dim=768, k=100, c=4096, library=True)
                     f"bf_{name}_e2e_tflops": flops / (e2e * 1e-3) / 1e12, f"bf_{name}_recall100": rec,
                     f"bf_{name}_rows_identical": same, f"bf_{name}_retried_queries": stats[3],
                     f"bf_{name}_fp64_neartie_pairs": chk[0], f"bf_{name}_fp64_order_disagree": chk[1],
-                    f"bf_{name}_fp64_max_abs_err": chk[2], f"bf_{name}_fp64_max_rel_err": chk[3]})
+                    f"bf_{name}_fp64_max_abs_err": chk[2], f"bf_{name}_fp64_max_rel_err": chk[3],
+                    f"bf_{name}_exact_path_overflow": rep["exact_overflow"],
+                    f"bf_{name}_exact_path_uncertified": rep["exact_uncertified"],
+                    f"bf_{name}_certified_queries": nq - rep["exact_overflow"] - rep["exact_uncertified"],
+                    f"bf_{name}_error_bound_max": rep["max_bound"],
+                    f"bf_{name}_rerank_neartie_pairs": rep["near_ties"],
+                    f"bf_{name}_rerank_neartie_fp64_disagree": rep["near_tie_disagree"],
+                    f"bf_{name}_boundary_neartie_pairs": rep["boundary_near_ties"],
+                    f"bf_{name}_boundary_fp64_disagree": rep["boundary_disagree"]})
         log(f"  {name}: GEMM {stats[0]:.2f} ms = {gemm_tf:.0f} TFLOP/s ({100 * gemm_tf / peak:.1f}% of peak); "
             f"prep {stats[1]:.2f} ms, re-rank {stats[2]:.2f} ms, end-to-end {e2e:.2f} ms; recall@100 {rec:.4f}; "
-            f"rows identical to exact {same:.4f}; fp64 near-tie pairs {chk[0]:.0f}, disagreements {chk[1]:.0f}")
+            f"rows identical to exact {same:.4f}; fp64 near-tie pairs {chk[0]:.0f}, disagreements {chk[1]:.0f}; "
+            f"certified {nq - rep['exact_overflow'] - rep['exact_uncertified']} of {nq} (bound <= "
+            f"{rep['max_bound']:.4f}), exact path {rep['exact_overflow']} overflow + {rep['exact_uncertified']} "
+            f"uncertified; k-th boundary near ties {rep['boundary_near_ties']}, FP64 disagreements "
+            f"{rep['boundary_disagree']}")
     if library:
         out.update(library_gemm(n, nq, dim, log))
     return out

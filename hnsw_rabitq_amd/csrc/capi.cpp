@@ -139,15 +139,15 @@ Plan make_plan(GpuIndex* gi, size_t max_ef, size_t max_k, size_t nq, bool filter
     }
     if (filtered) {
         // a CU holds 160 KiB of LDS: shrink the (lossy) hash, then the beam, to fit
-        while (hg::search_lds_bytes(gi->g.dim, p.cap, p.hash_bits, false) > 160 * 1024 && p.hash_bits > 9)
+        while (hg::search_lds_bytes(gi->g.dim, p.cap, p.hash_bits) > 160 * 1024 && p.hash_bits > 9)
             p.hash_bits--;
-        while (hg::search_lds_bytes(gi->g.dim, p.cap, p.hash_bits, false) > 160 * 1024 && p.cap > 64) p.cap -= 64;
+        while (hg::search_lds_bytes(gi->g.dim, p.cap, p.hash_bits) > 160 * 1024 && p.cap > 64) p.cap -= 64;
         require(p.cap >= ef_eff, "ef and dimension too large for the filtered GPU search");
         // the largest beam that keeps the same number of workgroups per CU (a free margin)
-        const size_t per = (size_t)(160 * 1024) / hg::search_lds_bytes(gi->g.dim, p.cap, p.hash_bits, false);
+        const size_t per = (size_t)(160 * 1024) / hg::search_lds_bytes(gi->g.dim, p.cap, p.hash_bits);
         if (per >= 1) {
             const size_t budget = (size_t)(160 * 1024) / per - 512;
-            while (p.cap + 64 <= kFiltMaxCap && hg::search_lds_bytes(gi->g.dim, p.cap + 64, p.hash_bits, false) <= budget)
+            while (p.cap + 64 <= kFiltMaxCap && hg::search_lds_bytes(gi->g.dim, p.cap + 64, p.hash_bits) <= budget)
                 p.cap += 64;
         }
     }
@@ -155,7 +155,7 @@ Plan make_plan(GpuIndex* gi, size_t max_ef, size_t max_k, size_t nq, bool filter
     sp.g = gi->g;
     sp.cap = p.cap;
     sp.hash_bits = p.hash_bits;
-    p.lds = hg::search_lds_bytes(gi->g.dim, p.cap, p.hash_bits, false);
+    p.lds = hg::search_lds_bytes(gi->g.dim, p.cap, p.hash_bits);
     int per_cu = hg::search_blocks_per_cu(sp, filtered);
     if (it != gi->tuned.end() && it->second.second > 0) per_cu = std::min(per_cu, it->second.second);
     size_t blocks = (size_t)gi->cus * (size_t)per_cu;
@@ -274,7 +274,29 @@ hipError_t run_search(GpuIndex* gi, const float* dq, size_t nq, const int32_t* d
             hipError_t e = hg::launch_zero_u32(status, 1, s);
             if (e != hipSuccess) return e;
         }
-        return hg::launch_search(sp, pl.blocks, s);
+        if (!(stop && stop->ext_ef > (int)std::max(ef, k))) return hg::launch_search(sp, pl.blocks, s);
+        // extended learned search: the first pass lists the queries whose network is not
+        // confident at the end of their search; the second answers them at ext_ef (the plain
+        // fixed-ef search, with the same stop checks), reading the list's count on the device
+        hipError_t e = hg::launch_zero_u32(lw.words, kWords, s);
+        if (e != hipSuccess) return e;
+        sp.ext_list = lw.ovf;
+        sp.ext_count = lw.words + kWordCount;
+        e = hg::launch_search(sp, pl.blocks, s);
+        if (e != hipSuccess) return e;
+        const size_t ext = (size_t)stop->ext_ef;
+        Plan p2 = make_plan(gi, ext, k, nq);
+        hg::SearchParams s2 = sp;
+        s2.cap = p2.cap;
+        s2.hash_bits = p2.hash_bits;
+        s2.work = lw.words + kWordWork + 1;
+        s2.ext_list = nullptr;
+        s2.ext_count = nullptr;
+        s2.order = lw.ovf;
+        s2.nq_dev = lw.words + kWordCount;
+        s2.ef_pq = nullptr;
+        s2.ef = (uint32_t)ext;
+        return hg::launch_search(s2, p2.blocks, s);
     }
 
     uint32_t* words = lw.words;
@@ -306,6 +328,11 @@ hipError_t run_search(GpuIndex* gi, const float* dq, size_t nq, const int32_t* d
         const uint32_t next = std::min<uint32_t>(kFiltMaxCap, (uint32_t)(cur.cap * 3 / 2 + 63) / 64 * 64);
         cur = make_plan(gi, plan_ef, plan_k, nq, true, frac, next);
     }
+}
+
+// The stop model asks for a second pass at a larger ef (the overflow list holds its queries).
+bool extends(void* stop, size_t ef, size_t k) {
+    return stop && static_cast<StopModel*>(stop)->ext_ef > (int)std::max(ef, k);
 }
 
 void check_stop_args(const GpuIndex* gi, const GpuFilter* filt, void* stop) {
@@ -988,7 +1015,7 @@ HG_API int hg_gpu_search_device(void* gv, const float* dq, size_t nq, const int3
         DeviceGuard dg(gi->device);
         std::lock_guard<std::mutex> lk(gi->mu);
         GpuIndex::LaunchWords* lw = next_words(gi, (hipStream_t)stream);
-        prepare_words(lw, nq, is_filtered(gi, nullptr));
+        prepare_words(lw, nq, is_filtered(gi, nullptr) || extends(stop, ef, k));
         hip_check(run_search(gi, dq, nq, def, dk, ef, k, k_stride, dl, dd, dc, static_cast<StopModel*>(stop),
                              (hipStream_t)stream, view(lw)),
                   "search launch");
@@ -1015,6 +1042,41 @@ HG_API int hg_gpu_search_device_filtered(void* gv, const float* dq, size_t nq, c
                              gf, d_status),
                   "search launch");
         words_done(lw, (hipStream_t)stream);
+        return HG_OK;
+    });
+}
+
+// The bare-bone device search with a per-query cap on level-0 expansions (d_hop_cap, device
+// int32 [nq]; <= 0 = no cap): the search driven by a stop condition that ends it after a given
+// number of hops (hnswalg.h:343-355, hnswlib.h:134-150).  It replays a learned search's stopping
+// points without the network, so the two can be compared hop for hop.
+HG_API int hg_gpu_search_device_hopcap(void* gv, const float* dq, size_t nq, const int32_t* d_hop_cap, size_t ef,
+                                       size_t k, uint64_t* dl, float* dd, int32_t* dc, void* stream) {
+    return guarded([&] {
+        require(gv && dl && dd && d_hop_cap, "null argument");
+        auto* gi = static_cast<GpuIndex*>(gv);
+        check_search_args(gi, nq, ef, k, k);
+        require(!is_filtered(gi, nullptr), "the hop-capped search is the bare-bone search: the index has deleted elements");
+        if (nq == 0) return HG_OK;
+        require(dq != nullptr, "null queries");
+        DeviceGuard dg(gi->device);
+        std::lock_guard<std::mutex> lk(gi->mu);
+        hipStream_t s = (hipStream_t)stream;
+        GpuIndex::LaunchWords* lw = next_words(gi, s);
+        prepare_words(lw, nq, false);
+        Plan pl = make_plan(gi, ef, k, nq);
+        hg::SearchParams sp = base_params(gi, pl, (uint32_t*)lw->words.p);
+        sp.queries = dq;
+        sp.nq = (uint32_t)nq;
+        sp.ef = (uint32_t)ef;
+        sp.k = (uint32_t)k;
+        sp.k_stride = (uint32_t)k;
+        sp.out_labels = dl;
+        sp.out_dists = dd;
+        sp.counters = dc;
+        sp.hop_cap = d_hop_cap;
+        hip_check(hg::launch_search(sp, pl.blocks, s), "search launch");
+        words_done(lw, s);
         return HG_OK;
     });
 }
@@ -1075,7 +1137,7 @@ HG_API int hg_gpu_search_device_noalloc(void* gv, const float* dq, size_t nq, co
 // expansions the 12 traversal features and the current top-k ids.
 HG_API int hg_gpu_collect_traces(void* gv, const float* dq, size_t nq, size_t ef, size_t k, int period,
                                  int max_records, float* d_features, uint32_t* d_topk, int32_t* d_nrec,
-                                 uint32_t* d_final, void* stream) {
+                                 uint32_t* d_final, int end_record, void* stream) {
     return guarded([&] {
         require(gv && dq && d_features && d_topk && d_nrec && d_final, "null argument");
         require(period >= 1 && max_records >= 1, "period and max_records must be positive");
@@ -1106,6 +1168,7 @@ HG_API int hg_gpu_collect_traces(void* gv, const float* dq, size_t nq, size_t ef
         sp.trace_final = d_final;
         sp.trace_period = period;
         sp.trace_max_rec = max_records;
+        sp.trace_end = end_record ? 1 : 0;
         hip_check(hg::launch_search(sp, pl.blocks, s), "trace launch");
         words_done(lw, s);
         hip_check(hipStreamSynchronize(s), "trace search");
@@ -1195,7 +1258,7 @@ HG_API int hg_gpu_set_plan(void* gv, size_t ef, size_t k, uint32_t hash_bits, in
         std::lock_guard<std::mutex> lk(gi->mu);
         size_t ef_eff = std::max(ef, k);
         uint32_t cap = (uint32_t)((ef_eff + 63) / 64 * 64);
-        require(hg::search_lds_bytes(gi->g.dim, cap, hash_bits, true) <= 160 * 1024,
+        require(hg::search_lds_bytes(gi->g.dim, cap, hash_bits) <= 160 * 1024,
                 "beam and visited hash exceed the 160 KiB of LDS of a CU");
         gi->tuned[cap] = std::make_pair(hash_bits, workgroups_per_cu);
         return HG_OK;
@@ -1592,7 +1655,7 @@ HG_API int hg_graph_capture_search(void* gv, const float* dq, size_t nq, const i
         std::unique_ptr<GraphHandle> gh(new GraphHandle());
         // allocate and resolve kernel attributes outside the capture
         const bool filtered = is_filtered(gi, nullptr);
-        prepare_words(&gh->lw, nq, filtered);
+        prepare_words(&gh->lw, nq, filtered || extends(stop, ef, k));
         Plan pl = make_plan(gi, ef, k, nq, filtered, allowed_fraction(gi, nullptr));
         (void)pl;
         hip_check(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal), "hipStreamBeginCapture");

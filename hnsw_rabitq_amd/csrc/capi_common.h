@@ -85,6 +85,10 @@ struct StopModel {
     int step = 0;
     int period = 16, min_hops = 16;
     float threshold = 0.9f;
+    // extension: at the end of a search, a query whose confidence is below ext_threshold is
+    // answered again at ext_ef (0 = no extension)
+    int ext_ef = 0;
+    float ext_threshold = 0.5f;
 };
 
 inline size_t mlp_param_count(int in, int h) { return (size_t)h * in + h + (size_t)h * h + h + h + 1; }
@@ -100,6 +104,8 @@ inline hg::DevStopModel dev_stop(StopModel* m) {
     d.min_hops = m->min_hops;
     float t = std::min(std::max(m->threshold, 1e-6f), 1.0f - 1e-6f);
     d.logit_threshold = std::log(t / (1.0f - t));
+    float te = std::min(std::max(m->ext_threshold, 1e-6f), 1.0f - 1e-6f);
+    d.ext_logit_threshold = m->ext_ef > 0 ? std::log(te / (1.0f - te)) : -INFINITY;
     return d;
 }
 

@@ -100,6 +100,21 @@ HG_API int hg_stop_model_config(void* mv, int period, float threshold, int min_h
     });
 }
 
+// Extension of the learned search: where a query's search ends by its own rule at the given ef,
+// the network is asked whether the top-k has converged; below ext_threshold the query is answered
+// again by the search at ext_ef (ext_ef = 0 turns the extension off).
+HG_API int hg_stop_model_config_extend(void* mv, int ext_ef, float ext_threshold) {
+    return guarded([&] {
+        require(mv != nullptr, "null model");
+        require(ext_ef >= 0 && ext_ef <= 4096, "ext_ef must be in [0, 4096]");
+        require(ext_threshold > 0.f && ext_threshold < 1.f, "ext_threshold must be in (0, 1)");
+        auto* m = static_cast<StopModel*>(mv);
+        m->ext_ef = ext_ef;
+        m->ext_threshold = ext_threshold;
+        return HG_OK;
+    });
+}
+
 HG_API int hg_stop_model_forward(void* mv, const float* dx, size_t n, float* dlogits, void* stream) {
     return guarded([&] {
         require(mv && (n == 0 || (dx && dlogits)), "null argument");

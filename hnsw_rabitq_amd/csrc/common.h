@@ -38,6 +38,7 @@ struct DevStopModel {
     int32_t period;             // evaluate every `period` level-0 expansions
     int32_t min_hops;           // never stop before this many expansions
     float logit_threshold;      // stop when logit >= threshold (= logit(confidence))
+    float ext_logit_threshold;  // at the search's own end: extend when logit < this
 };
 
 struct SearchParams {
@@ -86,6 +87,16 @@ struct SearchParams {
     // the level-0 graph; 0 = no limit); queries are rows of q_stride floats (0 = dim)
     uint32_t id_limit;
     uint32_t q_stride;
+    // optional per-query cap on level-0 expansions: query qi stops once it has expanded
+    // hop_cap[qi] nodes (a value <= 0 means no cap)
+    const int32_t* hop_cap;
+    // learned stop, first pass of an extended search: a query whose search ends by its own rule
+    // while the network is not confident its top-k has converged is appended to ext_list
+    // (counter ext_count) with no output; a later launch at the larger ef answers it
+    uint32_t* ext_list;
+    uint32_t* ext_count;
+    // trace collection: also record the features where the search ends by its own rule
+    int32_t trace_end;
 };
 
 // Multi-vector search (MultiVectorSearchStopCondition): one wave per query; every element

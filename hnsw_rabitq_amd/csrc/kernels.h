@@ -9,9 +9,9 @@
 namespace hg {
 
 // ---- graph search (search.hip) ----
-// LDS of one search wave: query, beam, visited hash, stage, and (stop_area) the feature and
-// hidden-activation scratch of the stop-model and trace variants.
-size_t search_lds_bytes(uint32_t dim, uint32_t cap, uint32_t hash_bits, bool stop_area);
+// LDS of one search wave: query, beam, visited hash and the neighbour stage (which the stop-model
+// and trace variants also use for the features and the network's activations).
+size_t search_lds_bytes(uint32_t dim, uint32_t cap, uint32_t hash_bits);
 int search_blocks_per_cu(const SearchParams& p, bool filtered = false);
 hipError_t launch_search(const SearchParams& p, int blocks, hipStream_t s);
 hipError_t launch_zero_u32(uint32_t* p, uint32_t n, hipStream_t s);  // a kernel (graph-safe), not a memset

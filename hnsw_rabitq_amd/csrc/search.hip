@@ -81,86 +81,53 @@ __device__ __forceinline__ float gelu_tanh(float x) {
     return 0.5f * x * (1.0f + tanhf(c * (x + 0.044715f * x * x * x)));
 }
 
-// MLP logit for one feature vector held in LDS (feat[in_dim]); hidden activations in LDS
-// scratch (hbuf[2*hidden]).  Weights are read as BF16-rounded values of the FP32 master
-// copy, accumulation in FP32.  Lanes own hidden units (hidden <= 64).
+// FP32 -> the nearest BF16 value (ties to even), as FP32: the network's weights as the search
+// kernel and the training kernels use them.
 __device__ __forceinline__ float bf16_round(float x) {
     uint32_t u = __float_as_uint(x);
     u += 0x7FFFu + ((u >> 16) & 1u);
     return __uint_as_float(u & 0xFFFF0000u);
 }
 
-// The network's BF16 weights held in registers for the life of a persistent wave: lane j keeps
-// hidden unit j's rows of W1 (kStopFeatures values) and W2 (64 values, two per register) and
-// its biases and output weight.  Loaded once; an evaluation then reads only LDS.
-struct MlpRegs {
-    uint32_t w1[kStopFeatures / 2];
-    uint32_t w2[32];
-    float b1, b2, w3, b3;
-};
-
-__device__ __forceinline__ uint32_t bf16_bits(float x) { return __float_as_uint(bf16_round(x)) >> 16; }
-__device__ __forceinline__ float bf16_lo(uint32_t p) { return __uint_as_float(p << 16); }
-__device__ __forceinline__ float bf16_hi(uint32_t p) { return __uint_as_float(p & 0xFFFF0000u); }
-
-__device__ __forceinline__ void mlp_load(const DevStopModel& m, int lane, MlpRegs& w) {
+// Logit of the network for the (normalised) features in LDS: lane j computes hidden unit j, its
+// weight rows read from the FP32 master copy (cached; about 20 KB shared by every wave) and
+// rounded to BF16 as they arrive, accumulation in FP32 -- the same operation order as the
+// training forward pass (sequential FMAs over inputs, output accumulated in unit order).  The
+// weights are not held in registers across the search: one evaluation every `period`
+// expansions does not pay for the ~80 VGPRs (203 against 121 for the whole kernel, and two waves
+// per SIMD fewer) that would take.  (A transposed copy read with coalesced loads took 144 VGPRs
+// and was slower.)
+__device__ __noinline__ float mlp_logit(const DevStopModel& m, const float* feat, float* hbuf, int lane) {
     const int H = m.hidden;
     const float* W1 = m.params;
     const float* b1 = W1 + H * kStopFeatures;
     const float* W2 = b1 + H;
     const float* b2 = W2 + H * H;
     const float* w3 = b2 + H;
-    const bool u = lane < H;
-#pragma unroll
-    for (int i = 0; i < kStopFeatures / 2; i++) {
-        const float a = u ? W1[lane * kStopFeatures + 2 * i] : 0.f;
-        const float b = u ? W1[lane * kStopFeatures + 2 * i + 1] : 0.f;
-        w.w1[i] = bf16_bits(a) | (bf16_bits(b) << 16);
-    }
-#pragma unroll
-    for (int i = 0; i < 32; i++) {
-        const float a = u && 2 * i < H ? W2[lane * H + 2 * i] : 0.f;
-        const float b = u && 2 * i + 1 < H ? W2[lane * H + 2 * i + 1] : 0.f;
-        w.w2[i] = bf16_bits(a) | (bf16_bits(b) << 16);
-    }
-    w.b1 = u ? b1[lane] : 0.f;
-    w.b2 = u ? b2[lane] : 0.f;
-    w.w3 = u ? bf16_round(w3[lane]) : 0.f;
-    w.b3 = w3[H];
-}
-
-// Logit of the network for the (normalised) features in LDS; the same operation order as the
-// training forward pass (sequential FMAs over inputs, output accumulated in unit order).
-__device__ float mlp_logit(const DevStopModel& m, const MlpRegs& w, const float* feat, float* hbuf, int lane) {
-    const int H = m.hidden;
     if (lane < H) {
-        float s = w.b1;
+        float s = b1[lane];
 #pragma unroll
-        for (int i = 0; i < kStopFeatures / 2; i++) {
-            s = fma_(bf16_lo(w.w1[i]), feat[2 * i], s);
-            s = fma_(bf16_hi(w.w1[i]), feat[2 * i + 1], s);
-        }
+        for (int i = 0; i < kStopFeatures; i++) s = fma_(bf16_round(W1[lane * kStopFeatures + i]), feat[i], s);
         hbuf[lane] = gelu_tanh(s);
     }
     __syncthreads();
     float h2 = 0.f;
     if (lane < H) {
-        float s = w.b2;
-#pragma unroll
-        for (int i = 0; i < 32; i++) {
-            if (2 * i < H) s = fma_(bf16_lo(w.w2[i]), hbuf[2 * i], s);
-            if (2 * i + 1 < H) s = fma_(bf16_hi(w.w2[i]), hbuf[2 * i + 1], s);
-        }
+        float s = b2[lane];
+        const float* row = W2 + lane * H;
+#pragma unroll 8
+        for (int i = 0; i < H; i++) s = fma_(bf16_round(row[i]), hbuf[i], s);
         h2 = gelu_tanh(s);
     }
     __syncthreads();
     if (lane < H) hbuf[lane] = h2;
     __syncthreads();
+    const float w3l = lane < H ? bf16_round(w3[lane]) : 0.f;
     float o = 0.f;
     for (int j = 0; j < H; j++)
-        o = fma_(hbuf[j], __int_as_float(__builtin_amdgcn_readlane(__float_as_int(w.w3), j)), o);
+        o = fma_(hbuf[j], __int_as_float(__builtin_amdgcn_readlane(__float_as_int(w3l), j)), o);
     __syncthreads();
-    return o + w.b3;
+    return o + w3[H];
 }
 
 // ------------------------------------------------------------------------------------------
@@ -197,8 +164,10 @@ __global__ __launch_bounds__(64) void hnsw_search_kernel(SearchParams P) {
     uint32_t* hash = bi + cap;
     float* sd = reinterpret_cast<float*>(hash + hsize);
     uint32_t* si = reinterpret_cast<uint32_t*>(sd + 64);
-    float* feat = reinterpret_cast<float*>(si + 64);  // kStopFeatures + 2*64 floats, allocated for STOP/TRACE only
-    float* hbuf = feat + 16;
+    // STOP / TRACE: the features (16 floats) and the network's hidden activations (64 floats)
+    // use the stage, which holds nothing live between expansions
+    float* feat = sd;
+    float* hbuf = sd + 16;
 
     __shared__ uint32_t s_slot;
     // tables of 2^14 slots or more are sized to forget nothing (the distinct-evaluation count):
@@ -207,8 +176,6 @@ __global__ __launch_bounds__(64) void hnsw_search_kernel(SearchParams P) {
     auto visit = [&](uint32_t* h, uint32_t m, uint32_t id) {
         return probe ? hash_insert<true>(h, m, id) : hash_insert<false>(h, m, id);
     };
-    MlpRegs mw;
-    if (STOP) mlp_load(P.stop, lane, mw);
 
     const uint32_t nq_all = P.nq_dev ? *P.nq_dev : P.nq;
     for (;;) {
@@ -323,6 +290,28 @@ __global__ __launch_bounds__(64) void hnsw_search_kernel(SearchParams P) {
         }
         int nrec = 0;
         bool stopped = false;
+        // the 12 traversal features of the current state into feat[] (STOP / TRACE)
+        auto make_features = [&]() {
+            int kk = kq < n ? kq : n;
+            float d1 = bd[0], dk = bd[kk - 1], dlast = bd[n - 1];
+            // the next expansion: first unexpanded entry (the search's stop rule ends the loop
+            // when there is none)
+            float dnext = dlast;
+            int pnext = n;
+            for (int b = scan_lo; b < n; b += 64) {
+                int i = b + lane;
+                uint64_t m = ballot(i < n && (bi[i] & XFLAG) == 0);
+                if (m) {
+                    pnext = b + __ffsll((unsigned long long)m) - 1;
+                    dnext = bd[pnext];
+                    break;
+                }
+            }
+            const int crowd = beam_lower_bound(bd, bi, n, 1.2f * dk, 0u);
+            if (lane < kStopFeatures)
+                feat[lane] = stop_feature(lane, st, d1, dk, dlast, dnext, pnext, kq, H0, ef_eff, crowd);
+        };
+        const int hcap = (P.hop_cap && P.hop_cap[qi] > 0) ? P.hop_cap[qi] : 0x7FFFFFFF;
         bool ovf = false;  // FILT: the beam overflowed (the result may differ from the reference's)
         uint32_t pf_node = SENT, pf_row = SENT;
         for (;;) {
@@ -341,7 +330,7 @@ __global__ __launch_bounds__(64) void hnsw_search_kernel(SearchParams P) {
                     break;
                 }
             }
-            if (p < 0) break;
+            if (p < 0 || H0 >= hcap) break;
             const uint32_t bip = bi[p];
             if (FILT && P.eps_mode) {
                 // EpsilonSearchStopCondition::should_stop_search (stop_condition.h:232-243); a
@@ -545,25 +534,7 @@ __global__ __launch_bounds__(64) void hnsw_search_kernel(SearchParams P) {
             if (STOP || TRACE) {
                 const int period = STOP ? P.stop.period : P.trace_period;
                 if (H0 % period == 0) {
-                    int kk = kq < n ? kq : n;
-                    float d1 = bd[0], dk = bd[kk - 1], dlast = bd[n - 1];
-                    // next candidate: first unexpanded from scan_lo (approximate: scan_lo entry)
-                    // the next expansion: first unexpanded entry (the search's stop rule ends
-                    // the loop when there is none)
-                    float dnext = dlast;
-                    int pnext = n;
-                    for (int b = scan_lo; b < n; b += 64) {
-                        int i = b + lane;
-                        uint64_t m = ballot(i < n && (bi[i] & XFLAG) == 0);
-                        if (m) {
-                            pnext = b + __ffsll((unsigned long long)m) - 1;
-                            dnext = bd[pnext];
-                            break;
-                        }
-                    }
-                    const int crowd = beam_lower_bound(bd, bi, n, 1.2f * dk, 0u);
-                    if (lane < kStopFeatures)
-                        feat[lane] = stop_feature(lane, st, d1, dk, dlast, dnext, pnext, kq, H0, ef_eff, crowd);
+                    make_features();
                     st.prev_pos[2] = st.prev_pos[1];
                     st.prev_pos[1] = st.prev_pos[0];
                     st.prev_pos[0] = st.win_pos < ef_eff ? st.win_pos : ef_eff;
@@ -580,7 +551,7 @@ __global__ __launch_bounds__(64) void hnsw_search_kernel(SearchParams P) {
                         if (lane < kStopFeatures)
                             feat[lane] = (feat[lane] - P.stop.mean[lane]) * P.stop.inv_std[lane];
                         __syncthreads();
-                        float logit = mlp_logit(P.stop, mw, feat, hbuf, lane);
+                        float logit = mlp_logit(P.stop, feat, hbuf, lane);
                         if (logit >= P.stop.logit_threshold) stopped = true;
                     }
                     __syncthreads();
@@ -592,6 +563,34 @@ __global__ __launch_bounds__(64) void hnsw_search_kernel(SearchParams P) {
             if (FILT && ovf && P.ovf_list) break;
         }
 
+        // where the search ended by its own rule (no stop, no hop cap): a trace records the
+        // state; the first pass of an extended learned search asks the network whether the
+        // top-k has converged and, when it is not confident, lists the query for the larger ef
+        if ((STOP && P.ext_list) || (TRACE && P.trace_end)) {
+            if (!stopped && H0 < hcap) {
+                make_features();
+                __syncthreads();
+                if (TRACE && nrec < P.trace_max_rec) {
+                    float* fo = P.trace_feat + ((size_t)qi * P.trace_max_rec + nrec) * kStopFeatures;
+                    if (lane < kStopFeatures) fo[lane] = feat[lane];
+                    uint32_t* to = P.trace_topk + ((size_t)qi * P.trace_max_rec + nrec) * P.k_stride;
+                    for (int i = lane; i < kq; i += 64) to[i] = i < n ? (bi[i] & ID_MASK) : SENT;
+                    nrec++;
+                }
+                if (STOP) {
+                    if (lane < kStopFeatures)
+                        feat[lane] = (feat[lane] - P.stop.mean[lane]) * P.stop.inv_std[lane];
+                    __syncthreads();
+                    const float logit = mlp_logit(P.stop, feat, hbuf, lane);
+                    if (logit < P.stop.ext_logit_threshold) {
+                        if (lane == 0) P.ext_list[atomicAdd(P.ext_count, 1u)] = qi;
+                        __syncthreads();
+                        continue;
+                    }
+                }
+                __syncthreads();
+            }
+        }
         if (FILT && ovf && P.ovf_list) {
             if (lane == 0) P.ovf_list[atomicAdd(P.ovf_count, 1u)] = qi;
             __syncthreads();
@@ -730,10 +729,9 @@ hipError_t launch_zero_u32(uint32_t* p, uint32_t n, hipStream_t s) {
 // ------------------------------------------------------------------------------------------
 // Host-side launch
 // ------------------------------------------------------------------------------------------
-size_t search_lds_bytes(uint32_t dim, uint32_t cap, uint32_t hash_bits, bool stop_area) {
+size_t search_lds_bytes(uint32_t dim, uint32_t cap, uint32_t hash_bits) {
     uint32_t dimpad = (dim + 3u) & ~3u;
-    return (size_t)dimpad * 4 + (size_t)cap * 8 + ((size_t)1 << hash_bits) * 4 + 64 * 8 +
-           (stop_area ? (16 + 128) * 4 : 0);
+    return (size_t)dimpad * 4 + (size_t)cap * 8 + ((size_t)1 << hash_bits) * 4 + 64 * 8;
 }
 
 template <int SPACE, int MODE, int NCH, bool STOP, bool TRACE, bool FILT = false>
@@ -763,7 +761,7 @@ static hipError_t launch_dim(const SearchParams& p, int blocks, size_t lds, hipS
 hipError_t launch_search(const SearchParams& p, int blocks, hipStream_t s) {
     bool stop = p.use_stop != 0, trace = p.trace_feat != nullptr;
     const bool filt = p.del_bits || p.allow_bits || p.eps_mode;
-    size_t lds = search_lds_bytes(p.g.dim, p.cap, p.hash_bits, !filt && (stop || trace));
+    size_t lds = search_lds_bytes(p.g.dim, p.cap, p.hash_bits);
     uint32_t* w = p.work;
     hipLaunchKernelGGL(zero_u32_kernel, dim3(1), dim3(64), 0, s, w, 1u);
     if (filt)
@@ -780,7 +778,7 @@ hipError_t launch_search(const SearchParams& p, int blocks, hipStream_t s) {
 }
 
 int search_blocks_per_cu(const SearchParams& p, bool filtered) {
-    size_t lds = search_lds_bytes(p.g.dim, p.cap, p.hash_bits, false);
+    size_t lds = search_lds_bytes(p.g.dim, p.cap, p.hash_bits);
     int nb = 0;
     hipError_t e;
     if (filtered && p.g.dim == 128 && p.g.space == 0)

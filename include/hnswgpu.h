@@ -161,6 +161,14 @@ int hg_gpu_search_device_filtered(void* gpu_index, const float* d_queries, size_
                                   const int32_t* d_k_per_query, size_t ef, size_t k, size_t k_stride,
                                   uint64_t* d_labels, float* d_dists, int32_t* d_counters, void* filter,
                                   uint32_t* d_status, void* stream);
+/* The bare-bone device search stopped after d_hop_cap[q] level-0 expansions (device int32 [nq];
+ * <= 0 = no cap): searchBaseLayerST driven by a stop condition that ends the search after a given
+ * number of hops (hnswalg.h:343-355 with hnswlib.h:134-150).  It replays a learned search's
+ * stopping points without the network (to time the two hop for hop) and is the GPU side of the
+ * oracle's hop-capped mode.  Asynchronous on stream. */
+int hg_gpu_search_device_hopcap(void* gpu_index, const float* d_queries, size_t nq, const int32_t* d_hop_cap,
+                                size_t ef, size_t k, uint64_t* d_labels, float* d_dists, int32_t* d_counters,
+                                void* stream);
 /* hg_gpu_search_device_filtered (filter may be NULL; the bare-bone search when the index holds no
  * deleted elements and no filter is given) that allocates nothing: no hipMalloc, hipFree or other
  * allocating call.  No counterpart in the reference: it is for callers that cannot allocate --
@@ -329,16 +337,26 @@ int hg_stop_model_set_params(void* model, const float* params, size_t count);
 int hg_stop_model_get_params(void* model, float* params, size_t count);
 /* Feature normalisation (mean, 1/std per feature) applied before the network. */
 int hg_stop_model_set_norm(void* model, const float* mean, const float* inv_std, size_t count);
-/* Inference thresholds: every `period` level-0 expansions the network runs; a query stops
- * when sigmoid(logit) >= threshold, and after `max_ext` extensions of ef... (see DESIGN). */
+/* Inference settings: every `period` level-0 expansions (from min_hops on) the network runs on
+ * the 12 traversal features and a query stops when sigmoid(logit) >= threshold, i.e. when the
+ * network is confident its top-k has converged; otherwise the search runs to its own end (the
+ * plain fixed-ef rule, hnswalg.h:343-355). */
 int hg_stop_model_config(void* model, int period, float threshold, int min_hops);
+/* Extension: where a query's search ends by its own rule, the network runs once more; when
+ * sigmoid(logit) < ext_threshold (not confident the top-k has converged) the query is listed on
+ * the device and answered by a second launch at ext_ef -- the plain fixed-ef search at the larger
+ * ef, with the same stop checks.  ext_ef = 0 (default) turns it off; it applies when ext_ef >
+ * max(ef, k) of the search.  hg_gpu_search_status reports how many queries were extended.  No
+ * counterpart in the reference: a stop condition of this project (hnswlib.h:134-150 slot). */
+int hg_stop_model_config_extend(void* model, int ext_ef, float ext_threshold);
 /* Trace collection: run the plain search and write, per query, up to max_records feature
  * vectors (every `period` level-0 expansions) plus the current top-k internal ids at each
- * record, and the final top-k internal ids.  d_features [nq][max_records][12] f32,
- * d_topk [nq][max_records][k] u32, d_nrec [nq] int32, d_final [nq][k] u32. */
+ * record, and the final top-k internal ids; with end_record != 0 one more record where the
+ * search ends by its own rule.  d_features [nq][max_records][12] f32, d_topk
+ * [nq][max_records][k] u32, d_nrec [nq] int32, d_final [nq][k] u32. */
 int hg_gpu_collect_traces(void* gpu_index, const float* d_queries, size_t nq, size_t ef, size_t k, int period,
                           int max_records, float* d_features, uint32_t* d_topk, int32_t* d_nrec, uint32_t* d_final,
-                          void* stream);
+                          int end_record, void* stream);
 /* External labels of n internal ids (as in trace records and d_final), on the device; an empty
  * slot (0xFFFFFFFF) maps to all ones.  Counterpart of the reference's getExternalLabel
  * (hnswalg.h:185-189) for device arrays. */

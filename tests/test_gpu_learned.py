@@ -190,3 +190,117 @@ def test_learned_stop_matches_hop_capped_oracle(hg, orc):
     print(f"stop decisions checked against the oracle network: {checked} of {nq}", flush=True)
     assert checked >= nq * 0.9
     m.free()
+
+
+def test_learned_extension_matches_oracle(hg, orc):
+    """The extended learned search (base ef 40, extension to ef 160): a query is answered by the
+    search at ef 40 stopped where the network fired, or -- when at that search's own end the
+    network is not confident its top-10 has converged -- by the search at ef 160 (stopped where
+    the network fired there).  Every row equals the oracle's total-order search of the path the
+    oracle network's decisions give, hop-capped at the GPU's expansion count."""
+    from hnsw_rabitq_amd import DeviceArray, _lib, datasets, learned_stop
+    p, space, data, queries = built_index("c2s")
+    gpu = p.gpu()
+    nq, k, ef, ext, period, thr, thr_ext = queries.shape[0], 10, 40, 160, 8, 0.95, 0.6
+    print(f"case=c2s learned extension ef={ef} ext={ext}", flush=True)
+    tq = datasets.c2_gamma(n=4000, seed=7)
+    X, Y = learned_stop.extension_records(gpu, DeviceArray.from_numpy(tq), 4000, ef, ext, k, period)
+    m, mean, inv_std, _ = learned_stop.train_model(X, Y, steps=400, batch=2048)
+    m.config(period=period, threshold=thr, min_hops=period)
+    m.config_extend(ext, thr_ext)
+    dq = DeviceArray.from_numpy(queries)
+    dl, dd, dc = DeviceArray(nq * k * 8), DeviceArray(nq * k * 4), DeviceArray(nq * 16)
+    gpu.search_device(dq, nq, k, ef, dl, dd, d_counters=dc, stop_model=m.h)
+    lab = dl.to_numpy(np.uint64, (nq, k))
+    dis = dd.to_numpy(np.float32, (nq, k))
+    h0 = dc.to_numpy(np.int32, (nq, 4))[:, 0]
+    st, n_ext = ctypes.c_uint32(), ctypes.c_uint32()
+    _lib.call("hg_gpu_search_status", gpu.handle, ctypes.byref(st), ctypes.byref(n_ext))
+    print(f"extended {n_ext.value} of {nq}; mean hops {h0.mean():.1f}", flush=True)
+    assert 0 < n_ext.value < nq
+    # the oracle network's decisions on both searches' traces
+    eff = effective(m.get_params(), 12, 64)
+    lthr, lext = np.log(thr / (1 - thr)), np.log(thr_ext / (1 - thr_ext))
+
+    def decisions(e, end):
+        f, _, r, _ = learned_stop.collect_traces(gpu, dq, nq, e, k, period=period, max_records=512, end_record=end)
+        fn = ((f - mean[None, None, :]) * inv_std[None, None, :]).astype(np.float32)
+        lg = orc.mlp_forward(eff, 12, 64, fn.reshape(-1, 12)).reshape(nq, 512).astype(np.float64)
+        return lg, r
+
+    lb, rb = decisions(ef, True)   # periodic records, then the end record
+    le, re = decisions(ext, False)
+    o = orc.OracleHnsw(blob=p.serialize())
+    base = o.search_hopcap(0, queries, k, ef, h0, mode=orc.MODE_TOTAL, counters=True)
+    extd = o.search_hopcap(0, queries, k, ext, h0, mode=orc.MODE_TOTAL, counters=True)
+    checked = predicted_ext = 0
+    for q in range(nq):
+        per = lb[q, :rb[q] - 1]  # periodic records of the base search
+        fire = np.nonzero(per >= lthr)[0]
+        near = np.abs(per[:fire[0] + 1 if fire.size else per.size] - lthr).min(initial=1.0) < 0.01
+        if fire.size:
+            path, cap = base, (fire[0] + 1) * period
+        else:
+            end = lb[q, rb[q] - 1]
+            near |= abs(end - lext) < 0.01
+            if end >= lext:
+                path, cap = base, None
+            else:
+                predicted_ext += 1
+                pe = le[q, :re[q]]
+                fe = np.nonzero(pe >= lthr)[0]
+                near |= np.abs(pe[:fe[0] + 1 if fe.size else pe.size] - lthr).min(initial=1.0) < 0.01
+                path, cap = extd, ((fe[0] + 1) * period if fe.size else None)
+        if near:
+            continue
+        checked += 1
+        if cap is not None:
+            assert h0[q] == cap, (q, h0[q], cap)
+        np.testing.assert_array_equal(lab[q], path[0][q])
+        np.testing.assert_array_equal(dis[q].view(np.uint32), path[1][q].view(np.uint32))
+        assert h0[q] == path[2][q, 0]
+    print(f"checked {checked} of {nq}; extended by the oracle network {predicted_ext}", flush=True)
+    assert checked >= nq * 0.85
+    assert abs(predicted_ext - n_ext.value) <= nq - checked
+    m.free()
+
+
+def test_learned_stop_gain_over_plain(hg, orc):
+    """What the trained network gains over the method without it: on the reduced 1M x 128
+    workload (20,000 x 128 integer Gamma), the learned search at ef=96 (confidence 0.9) against
+    the plain fixed-ef search's recall-vs-expansions curve (ef 16..96): the level-0 expansions
+    the plain search needs for the learned search's recall, over the learned search's own.
+    Expansions are the search's unit of work (one link row and up to 2M vector reads each); the
+    time per expansion is measured by the benchmark."""
+    from hnsw_rabitq_amd import DeviceArray, datasets, learned_stop
+    p, space, data, _ = built_index("c2s")
+    gpu = p.gpu()
+    q = datasets.c2_gamma(n=2000, seed=9)
+    nq, k, ef, period = q.shape[0], 10, 96, 8
+    gt, _ = orc.bf_search(0, data, q, k)
+    dq = DeviceArray.from_numpy(q)
+    dl, dd, dc = DeviceArray(nq * k * 8), DeviceArray(nq * k * 4), DeviceArray(nq * 16)
+
+    def run(e, model=None):
+        gpu.search_device(dq, nq, k, e, dl, dd, d_counters=dc, stop_model=model)
+        lab = dl.to_numpy(np.uint64, (nq, k))
+        rec = np.mean([len(np.intersect1d(a, b)) for a, b in zip(lab, gt)]) / k
+        return float(rec), float(dc.to_numpy(np.int32, (nq, 4))[:, 0].mean())
+
+    curve = [run(e) for e in range(16, ef + 1, 4)]
+    m, _, _, _ = _trained_model(gpu, ef, k, period, n_train=6000)
+    m.config(period=period, threshold=0.9, min_hops=period)
+    rec_l, hops_l = run(ef, m.h)
+    assert hops_l < 0.95 * ef, "the network should stop a share of the search"
+    recs = np.array([c[0] for c in curve])
+    hops = np.array([c[1] for c in curve])
+    assert rec_l <= recs.max(), "the learned search stops a fixed-ef search early: it cannot exceed it"
+    i = int(np.searchsorted(recs, rec_l))
+    i = min(max(i, 1), len(recs) - 1)
+    w = (rec_l - recs[i - 1]) / max(recs[i] - recs[i - 1], 1e-12)
+    hops_plain = hops[i - 1] + w * (hops[i] - hops[i - 1])
+    gain = hops_plain / hops_l
+    print(f"learned ef={ef}: recall {rec_l:.4f} at {hops_l:.1f} expansions; plain needs {hops_plain:.1f} for that "
+          f"recall: gain {gain:.3f}x in expansions", flush=True)
+    assert gain > 1.0
+    m.free()
