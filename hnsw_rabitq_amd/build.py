@@ -61,6 +61,13 @@ def build(verbose=False):
     tmp = LIB + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp] + objs + ["-pthread"])
     os.replace(tmp, LIB)
+    # the C++ drop-in's test program (include/hnswgpu.hpp), linked against the library in-tree
+    root = os.path.dirname(HERE)
+    src = os.path.join(root, "tests", "cpp", "closer_first_test.cpp")
+    if os.path.exists(src):
+        _run(["g++", "-O2", "-std=c++17", "-Wall", "-I" + os.path.join(root, "include"), src, "-o",
+              os.path.join(root, "tests", "cpp", "closer_first_test"), "-L" + HERE, "-lhnswgpu",
+              "-Wl,-rpath,$ORIGIN/../../hnsw_rabitq_amd"])
     return LIB
 
 
