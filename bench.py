@@ -353,10 +353,13 @@ def main():
     tpath = os.path.join(ROOT, "profiles", "search_traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
-            rec = json.load(f)
-        if rec.get("workload") == workload and rec.get("ef") == ef_op:
-            traffic = rec.get("traffic_bytes_per_launch")
-            traffic_plan = rec.get("plan")
+            recs = json.load(f)
+        # one record per launch plan the counters were read at: report the one of this run's plan
+        for rec in recs if isinstance(recs, list) else [recs]:
+            rp = rec.get("plan") or {}
+            if rec.get("workload") == workload and rec.get("ef") == ef_op and \
+                    rp.get("workgroups") == plan["workgroups"] and rp.get("hash_slots") == plan["hash_slots"]:
+                traffic, traffic_plan = rec.get("traffic_bytes_per_launch"), rp
     line = {
         "metric": "QPS@recall10=0.95, 1M×128 L2: plain vs learned-stop GPU; brute-force TFLOPS",
         "value": value,
