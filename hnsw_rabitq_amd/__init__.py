@@ -587,10 +587,10 @@ class BFIndex:
 
     def lowp_report(self):
         """What the last bf16 / fp8 search did besides the re-rank (hg_bf_lowp_report)."""
-        out = np.zeros(8, np.float64)
+        out = np.zeros(9, np.float64)
         _lib.call("hg_bf_lowp_report", self._h, _ptr(out))
         keys = ("exact_overflow", "exact_uncertified", "near_ties", "near_tie_disagree", "boundary_near_ties",
-                "boundary_disagree", "max_bound", "certified")
+                "boundary_disagree", "max_bound", "certified", "reranked_rows")
         return {k: (float(v) if k == "max_bound" else int(v)) for k, v in zip(keys, out)}
 
     def init_index(self, max_elements):

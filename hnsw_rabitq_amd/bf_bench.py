@@ -75,14 +75,15 @@ def run(H, lib, n, nq, stream, timer, log, dim=768, k=100, c=4096, library=True)
                     f"bf_{name}_rerank_neartie_pairs": rep["near_ties"],
                     f"bf_{name}_rerank_neartie_fp64_disagree": rep["near_tie_disagree"],
                     f"bf_{name}_boundary_neartie_pairs": rep["boundary_near_ties"],
-                    f"bf_{name}_boundary_fp64_disagree": rep["boundary_disagree"]})
+                    f"bf_{name}_boundary_fp64_disagree": rep["boundary_disagree"],
+                    f"bf_{name}_reranked_rows_per_query": rep["reranked_rows"] / nq})
         log(f"  {name}: GEMM {stats[0]:.2f} ms = {gemm_tf:.0f} TFLOP/s ({100 * gemm_tf / peak:.1f}% of peak); "
             f"prep {stats[1]:.2f} ms, re-rank {stats[2]:.2f} ms, end-to-end {e2e:.2f} ms; recall@100 {rec:.4f}; "
             f"rows identical to exact {same:.4f}; fp64 near-tie pairs {chk[0]:.0f}, disagreements {chk[1]:.0f}; "
             f"certified {nq - rep['exact_overflow'] - rep['exact_uncertified']} of {nq} (bound <= "
             f"{rep['max_bound']:.4f}), exact path {rep['exact_overflow']} overflow + {rep['exact_uncertified']} "
             f"uncertified; k-th boundary near ties {rep['boundary_near_ties']}, FP64 disagreements "
-            f"{rep['boundary_disagree']}")
+            f"{rep['boundary_disagree']}; exact FP32 rows per query {rep['reranked_rows'] / nq:.1f}")
     if library:
         out.update(library_gemm(n, nq, dim, log))
     return out

@@ -777,10 +777,13 @@ __global__ __launch_bounds__(256, 2) void lowp_gemm_kernel(LowpGemmArgs g) {
             int idx = tid + 256 * i;
             int row = idx >> 3, ch = idx & 7;
             uint32_t qa = m0 + row, qb = n0 + row;
-            ra[i] = qa < g.nq ? *reinterpret_cast<const uint4*>(g.A + qa * row_bytes + kb + ch * 16)
-                              : make_uint4(0, 0, 0, 0);
-            rb[i] = qb < g.n ? *reinterpret_cast<const uint4*>(g.B + qb * row_bytes + kb + ch * 16)
-                             : make_uint4(0, 0, 0, 0);
+            // an FP8 row of dim % 128 == 64 ends halfway through the last stage: the chunks past
+            // its end are zero (a zero code adds nothing to the dot product)
+            const bool in_row = kb + ch * 16 < row_bytes;
+            ra[i] = qa < g.nq && in_row ? *reinterpret_cast<const uint4*>(g.A + qa * row_bytes + kb + ch * 16)
+                                        : make_uint4(0, 0, 0, 0);
+            rb[i] = qb < g.n && in_row ? *reinterpret_cast<const uint4*>(g.B + qb * row_bytes + kb + ch * 16)
+                                       : make_uint4(0, 0, 0, 0);
         }
     };
     auto lstore = [&](int buf) {
@@ -883,16 +886,19 @@ __global__ __launch_bounds__(256, 2) void lowp_gemm_wide_kernel(LowpGemmArgs g) 
             int idx = tid + 256 * i;
             int row = idx >> 3, ch = idx & 7;
             uint32_t qa = m0 + row;
-            ra[i] = qa < g.nq ? *reinterpret_cast<const uint4*>(g.A + qa * row_bytes + kb + ch * 16)
-                              : make_uint4(0, 0, 0, 0);
+            // chunks past the end of an FP8 row of dim % 128 == 64 are zero
+            ra[i] = qa < g.nq && kb + ch * 16 < row_bytes
+                        ? *reinterpret_cast<const uint4*>(g.A + qa * row_bytes + kb + ch * 16)
+                        : make_uint4(0, 0, 0, 0);
         }
 #pragma unroll
         for (int i = 0; i < LB; i++) {
             int idx = tid + 256 * i;
             int row = idx >> 3, ch = idx & 7;
             uint32_t qb = n0 + row;
-            rb[i] = qb < g.n ? *reinterpret_cast<const uint4*>(g.B + qb * row_bytes + kb + ch * 16)
-                             : make_uint4(0, 0, 0, 0);
+            rb[i] = qb < g.n && kb + ch * 16 < row_bytes
+                        ? *reinterpret_cast<const uint4*>(g.B + qb * row_bytes + kb + ch * 16)
+                        : make_uint4(0, 0, 0, 0);
         }
     };
     auto lstore = [&]() {
@@ -1313,6 +1319,43 @@ __global__ __launch_bounds__(256) void kth_kernel(const float* __restrict__ mat,
 constexpr int RR_THREADS = 512;
 constexpr int RR_CAP = 16384;  // candidate slots per query
 constexpr int RR_TIES = 2048;  // candidates tied at the c-th approximate distance one query can resolve
+constexpr int RR_PER = 4096 / RR_THREADS;  // chosen rows per thread (c <= 4096)
+
+// Key of the rank-th smallest approximate distance among buf[0, m) (radix select over the LDS
+// entries, 4 x 8 bits); every thread returns it.
+template <int NT>
+__device__ uint32_t lds_select_key(const SelEntry* buf, uint32_t m, uint32_t rank, uint32_t* hist, uint32_t& prefix,
+                                   uint32_t& want) {
+    if (threadIdx.x == 0) {
+        prefix = 0;
+        want = rank;
+    }
+    for (int shift = 24; shift >= 0; shift -= 8) {
+        for (int b = threadIdx.x; b < 256; b += NT) hist[b] = 0;
+        __syncthreads();
+        const uint32_t pre = prefix;
+        const uint32_t mask_hi = shift == 24 ? 0u : (0xFFFFFFFFu << (shift + 8));
+        for (uint32_t i = threadIdx.x; i < m; i += NT) {
+            const uint32_t key = f2key(buf[i].d);
+            if ((key & mask_hi) == pre) atomicAdd(&hist[(key >> shift) & 255u], 1u);
+        }
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            uint32_t w = want, acc = 0, bsel = 255;
+            for (uint32_t bkt = 0; bkt < 256; bkt++) {
+                if (acc + hist[bkt] >= w) {
+                    bsel = bkt;
+                    break;
+                }
+                acc += hist[bkt];
+            }
+            want = w - acc;
+            prefix = pre | (bsel << shift);
+        }
+        __syncthreads();
+    }
+    return prefix;
+}
 
 // Re-rank of one query per workgroup.  The c candidates with the smallest approximate distance
 // (ties by label) are chosen by a radix select that reads the candidate list from global memory,
@@ -1326,6 +1369,14 @@ constexpr int RR_TIES = 2048;  // candidates tied at the c-th approximate distan
 // adjacent pairs of the re-ranked order, up to the pair at the k-th boundary, whose FP32
 // distances differ by at most rel_tol (relative) are recomputed in FP64; counters [0] near-tie
 // pairs, [1] pairs whose FP64 order differs, [2] and [3] the same for the boundary pair.
+//
+// Bound pruning (with a bound): a chosen row whose approximate distance exceeds a1 + 2 bound[q],
+// a1 the (k+1)-th smallest approximate distance among the chosen rows, has exact distance
+// > a1 + bound[q] >= the (k+1)-th smallest exact distance among them (the k + 1 rows of smallest
+// approximate distance all lie at or below a1 + bound[q]), so it cannot take any of the first
+// k + 1 places of the exact order: only the other rows are read in FP32, and the k results, the
+// certification and the near-tie pairs (up to the one at the k-th boundary) are those of the
+// full re-rank.  rr_ctr[0] += rows re-ranked.
 template <int SPACE>
 __global__ __launch_bounds__(RR_THREADS) void rerank_kernel(const float* __restrict__ Q, const float* __restrict__ X,
                                                             const uint64_t* __restrict__ labels, uint32_t dim,
@@ -1337,7 +1388,8 @@ __global__ __launch_bounds__(RR_THREADS) void rerank_kernel(const float* __restr
                                                             uint32_t* __restrict__ overflow, uint32_t n_rows,
                                                             const float* __restrict__ tau,
                                                             const float* __restrict__ bound,
-                                                            unsigned long long* __restrict__ tie_ctr, double rel_tol) {
+                                                            unsigned long long* __restrict__ tie_ctr, double rel_tol,
+                                                            unsigned long long* __restrict__ rr_ctr) {
     extern __shared__ __align__(16) unsigned char smem[];
     uint32_t cpow = 2;  // >= 2 keeps the query copy 16-byte aligned
     while (cpow < c) cpow <<= 1;
@@ -1348,7 +1400,7 @@ __global__ __launch_bounds__(RR_THREADS) void rerank_kernel(const float* __restr
     const uint32_t slots = cpow > (uint32_t)RR_TIES ? cpow : (uint32_t)RR_TIES;
     float* qs = reinterpret_cast<float*>(buf + slots);     // dim floats
     __shared__ uint32_t hist[256];
-    __shared__ uint32_t prefix, want, nsel, ntie, tfill, bad, cut_on;
+    __shared__ uint32_t prefix, want, nsel, ntie, tfill, bad, cut_on, npr;
     __shared__ uint64_t cut_label;
     const uint32_t q = blockIdx.x;
     const uint32_t cnt = cand_cnt[q];
@@ -1372,6 +1424,7 @@ __global__ __launch_bounds__(RR_THREADS) void rerank_kernel(const float* __restr
         cut_label = 0;
         cut_on = 0;
         bad = 0;
+        npr = 0;
     }
     __syncthreads();
     if (!all) {
@@ -1447,8 +1500,10 @@ __global__ __launch_bounds__(RR_THREADS) void rerank_kernel(const float* __restr
             const uint32_t i = i0 + threadIdx.x;
             bool take = false;
             uint32_t r = 0;
+            float a = 0.f;
             if (i < cnt) {
-                const uint32_t key = f2key(cdq[i]);
+                a = cdq[i];
+                const uint32_t key = f2key(a);
                 r = crq[i];
                 take = key < kc || (key == kc && (!con || labels[r] <= cl));
             }
@@ -1460,7 +1515,7 @@ __global__ __launch_bounds__(RR_THREADS) void rerank_kernel(const float* __restr
             if (take) {
                 const uint32_t slot = base + (uint32_t)__popcll(bm & ((lane ? ~0ull >> (64 - lane) : 0ull)));
                 if (slot < m)
-                    buf[slot] = SelEntry{0.f, r};
+                    buf[slot] = SelEntry{a, r};  // approximate distance until the exact pass
                 else
                     bad = 1;
             }
@@ -1471,31 +1526,61 @@ __global__ __launch_bounds__(RR_THREADS) void rerank_kernel(const float* __restr
             return;
         }
     } else {
-        for (uint32_t i = threadIdx.x; i < cnt; i += RR_THREADS) buf[i] = SelEntry{0.f, crq[i]};
+        for (uint32_t i = threadIdx.x; i < cnt; i += RR_THREADS) buf[i] = SelEntry{cdq[i], crq[i]};
         __syncthreads();
     }
+    // bound pruning: the rows that can take one of the first k + 1 places, compacted in place
+    // (each thread holds its strided share of the chosen rows in registers across a barrier)
+    uint32_t mw = m;
+    if (bound && m > k + 1) {
+        const double a1 = key2f(lds_select_key<RR_THREADS>(buf, m, k + 1, hist, prefix, want));
+        const double e2 = 2.0 * (double)bound[q];
+        // (the double sum is exact to 2^-53; the margin keeps the test on the safe side)
+        const double t = a1 + e2 + (fabs(a1) + e2) * 0x1p-40;
+        SelEntry mine[RR_PER];
+        bool kp[RR_PER];
+#pragma unroll
+        for (int u = 0; u < RR_PER; u++) {
+            const uint32_t i = threadIdx.x + (uint32_t)u * RR_THREADS;
+            kp[u] = i < m && (double)buf[i].d <= t;
+            mine[u] = kp[u] ? buf[i] : SelEntry{0.f, 0u};
+        }
+        __syncthreads();
+        const int ln = threadIdx.x & 63;
+#pragma unroll
+        for (int u = 0; u < RR_PER; u++) {
+            const uint64_t bm = __ballot(kp[u]);
+            uint32_t base = 0;
+            if (ln == 0 && bm) base = atomicAdd(&npr, (uint32_t)__popcll(bm));
+            base = (uint32_t)__shfl((int)base, 0);
+            if (kp[u]) buf[base + (uint32_t)__popcll(bm & ((ln ? ~0ull >> (64 - ln) : 0ull)))] = mine[u];
+        }
+        __syncthreads();
+        mw = npr;
+    }
+    if (threadIdx.x == 0) atomicAdd(&rr_ctr[0], (unsigned long long)mw);
     // exact distances, four lanes per row (reference order)
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int grp = lane >> 2, sub = lane & 3;
     const int nch = (int)(dim >> 4);
     constexpr int NW = RR_THREADS / 64;
-    for (uint32_t base = wave * 16; base < m; base += NW * 16) {
+    for (uint32_t base = wave * 16; base < mw; base += NW * 16) {
         const uint32_t idx = base + grp;
-        const bool act = idx < m;
+        const bool act = idx < mw;
         const uint32_t r = act ? buf[idx].r : 0u;
         float res;
         dist_group16<SPACE, 0>(X + (size_t)r * dim, qs, nch, act, sub, res, lane);
         if (act && sub == 0) buf[idx].d = res;
     }
     __syncthreads();
-    sel_sort<RR_THREADS>(buf, (int)m, labels);
+    sel_sort<RR_THREADS>(buf, (int)mw, labels);
     // certification (uniform over the workgroup: every thread reads the same values)
     if (bound && !((double)acut - (double)bound[q] > (double)buf[k - 1].d)) {
         if (threadIdx.x == 0) overflow[q] = 2;
         return;
     }
     if (tie_ctr) {
-        for (uint32_t i = threadIdx.x; i < k && i + 1 < m; i += RR_THREADS) {
+        for (uint32_t i = threadIdx.x; i < k && i + 1 < mw; i += RR_THREADS) {
             const double a = buf[i].d, b = buf[i + 1].d;
             if (b - a > rel_tol * fabs(a)) continue;
             const float* xa = X + (size_t)buf[i].r * dim;
@@ -1769,22 +1854,22 @@ hipError_t launch_bf_lowp(const BfData& d, int precision, const float* q, uint32
         (void)hipFuncSetAttribute((const void*)rerank_kernel<0>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         hipLaunchKernelGGL(rerank_kernel<0>, dim3(nq), dim3(RR_THREADS), lds, s, q, d.vec, d.labels, dim, crow, cd,
                            ccnt, (uint32_t)RR_CAP, c, k, labels, dists, (size_t)k, ovf, n, (const float*)tau, bnd, tc,
-                           tie_rel);
+                           tie_rel, tie_ctr + 4);
     } else {
         (void)hipFuncSetAttribute((const void*)rerank_kernel<1>, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         hipLaunchKernelGGL(rerank_kernel<1>, dim3(nq), dim3(RR_THREADS), lds, s, q, d.vec, d.labels, dim, crow, cd,
                            ccnt, (uint32_t)RR_CAP, c, k, labels, dists, (size_t)k, ovf, n, (const float*)tau, bnd, tc,
-                           tie_rel);
+                           tie_rel, tie_ctr + 4);
     }
     (void)hipEventRecord(ev.ev[3], s);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return e;
     // queries the re-rank flagged take the exact path, all in one batch
     std::vector<uint32_t> flags(nq);
-    unsigned long long ties[4] = {0, 0, 0, 0};
+    unsigned long long ties[5] = {0, 0, 0, 0, 0};
     float bmax = 0.f;
     e = hipMemcpyAsync(flags.data(), ovf, (size_t)nq * 4, hipMemcpyDeviceToHost, s);
-    if (e == hipSuccess && tc) e = hipMemcpyAsync(ties, tie_ctr, sizeof(ties), hipMemcpyDeviceToHost, s);
+    if (e == hipSuccess && rep) e = hipMemcpyAsync(ties, tie_ctr, sizeof(ties), hipMemcpyDeviceToHost, s);
     std::vector<float> bounds(xstats && rep ? nq : 0);
     if (e == hipSuccess && !bounds.empty()) e = hipMemcpyAsync(bounds.data(), bound, (size_t)nq * 4, hipMemcpyDeviceToHost, s);
     if (e == hipSuccess) e = hipStreamSynchronize(s);
@@ -1844,6 +1929,7 @@ hipError_t launch_bf_lowp(const BfData& d, int precision, const float* q, uint32
         rep->boundary_disagree = ties[3];
         rep->max_bound = bmax;
         rep->certified = xstats != nullptr;
+        rep->reranked_rows = ties[4];
     }
     return hipGetLastError();
 }

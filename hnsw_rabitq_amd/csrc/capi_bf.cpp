@@ -326,6 +326,7 @@ HG_API int hg_bf_lowp_report(void* bv, double* out) {
         out[5] = (double)r.boundary_disagree;
         out[6] = r.max_bound;
         out[7] = r.certified ? 1.0 : 0.0;
+        out[8] = (double)r.reranked_rows;
         return HG_OK;
     });
 }

@@ -87,6 +87,7 @@ struct LowpReport {
     uint64_t boundary_near_ties = 0, boundary_disagree = 0;   // the same at the k-th boundary
     float max_bound = 0.f;           // largest per-query error bound
     bool certified = false;          // the bound was applied
+    uint64_t reranked_rows = 0;      // rows whose exact FP32 distance the re-rank computed
 };
 // Quantisation statistics of the stored rows (3 uint32: float bits of max |x|, max |x~|,
 // max |x - x~|), folded in by quant_stats_kernel into out (cleared by the caller).

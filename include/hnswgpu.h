@@ -279,7 +279,9 @@ int hg_bf_search_precision(void* bf, const float* queries, size_t nq, size_t k, 
  * certify them, out[2] near-tie pairs checked in FP64, out[3] those whose FP64 order differs from
  * the FP32 order (the reference's order is the FP32 one: the lists keep it), out[4] / out[5] the
  * same two counts for the pair at the k-th boundary, out[6] the largest per-query bound, out[7]
- * 1 when the bound was applied.  No counterpart in the reference. */
+ * 1 when the bound was applied, out[8] rows whose exact FP32 distance the re-rank computed (with
+ * the bound, only the rows whose approximate distance is within twice the bound of the (k+1)-th
+ * smallest among the candidates).  No counterpart in the reference. */
 int hg_bf_lowp_report(void* bf, double* out);
 /* FP64 report on final lists (any precision): every listed (query, label) distance is recomputed
  * in FP64; pairs of neighbouring results whose FP32 distances differ by at most rel_tol

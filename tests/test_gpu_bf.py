@@ -239,6 +239,11 @@ def test_bf_lowp_c3_workload_matches_oracle(hg, orc, prec):
     assert rep["exact_uncertified"] == stats[3]
     if prec == "bf16":
         assert rep["exact_uncertified"] == 0  # the BF16 bound (~3e-3) is far below the gap
+        # bound pruning: the exact pass reads only the rows within twice the bound of the 101st
+        # approximate distance, a small part of the 4,096 candidates
+        assert rep["reranked_rows"] < 0.25 * 4096 * q.shape[0]
+    # every certified query re-ranks at least its k + 1 best candidates
+    assert rep["reranked_rows"] >= (k + 1) * q.shape[0]
     # the bound is an upper bound on the approximate error: never below the BF16 / FP8
     # quantisation error of a unit vector
     assert rep["max_bound"] >= (2.0 ** -9 if prec == "bf16" else 2.0 ** -5)
@@ -346,7 +351,12 @@ def test_rerank_fp64_near_tie_check(hg, orc):
     assert want[0] > 50
     assert rep["near_ties"] - rep["boundary_near_ties"] == want[0]
     assert rep["near_tie_disagree"] - rep["boundary_disagree"] == want[1]
-    assert rep["boundary_near_ties"] <= q.shape[0]
+    # the pair at the k-th boundary is (k-th, (k+1)-th) of the exact order: the bound-pruned
+    # re-rank keeps the (k+1)-th row, so the boundary counts are the oracle's k + 1 list's
+    ol1, od1 = orc.bf_search(1, data, q, k + 1)
+    want1 = _expected_near_ties(orc, 1, data, q, ol1.astype(np.int64), od1, 1e-6)
+    assert rep["boundary_near_ties"] == want1[0] - want[0]
+    assert rep["boundary_disagree"] == want1[1] - want[1]
 
 
 def test_bf_add_over_capacity_keeps_placed_rows(hg, orc):
