@@ -451,6 +451,8 @@ __global__ __launch_bounds__(64) void hnsw_search_kernel(SearchParams P) {
                         // entry does not move writes to its own slot of the distance-staging area
                         // (sd/si, which hold no live data during the merge), so no lane branches.
                         const int trash_d = (int)(sd - bd) + lane, trash_i = (int)(si - bi) + lane;
+                        // the rank lanes of spos (the insertion points, non-decreasing in rank)
+                        const uint64_t rank_lanes = m >= 64 ? ~0ull : ((1ull << m) - 1ull);
                         for (int b0 = hi_chunk; b0 >= lo_chunk; b0 -= 8 * 64) {
                             float vd[8];
                             uint32_t vi[8];
@@ -462,12 +464,19 @@ __global__ __launch_bounds__(64) void hnsw_search_kernel(SearchParams P) {
                                 const int ic = i > 0 ? i : 0;
                                 vd[u] = bd[ic];
                                 vi[u] = bi[ic];
-                                sh[u] = 0;
                             }
-                            for (int rr = 0; rr < m; rr++) {
-                                const int sp = __builtin_amdgcn_readlane(spos, rr);
+                            // shift of entry i = insertion points <= i.  Per chunk [cb, cb + 64):
+                            // lo points lie at or below cb and move the whole chunk; only the
+                            // points inside the chunk (ranks lo..hi-1) are compared per lane
 #pragma unroll
-                                for (int u = 0; u < 8; u++) sh[u] += sp <= base - 64 * u ? 1 : 0;
+                            for (int u = 0; u < 8; u++) {
+                                const int cb = b0 - 64 * u;
+                                const int lo = __popcll(__builtin_amdgcn_ballot_w64(spos <= cb) & rank_lanes);
+                                const int hi = __popcll(__builtin_amdgcn_ballot_w64(spos <= cb + 63) & rank_lanes);
+                                int sv = lo;
+                                for (int rr = lo; rr < hi; rr++)
+                                    sv += __builtin_amdgcn_readlane(spos, rr) <= base - 64 * u ? 1 : 0;
+                                sh[u] = sv;
                             }
                             __asm__ volatile("" ::: "memory");
 #pragma unroll
