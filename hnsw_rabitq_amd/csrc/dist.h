@@ -131,6 +131,49 @@ __device__ __forceinline__ float dist_generic(int space, const float* q, const f
 //      _mm512_reduce_add_ps halving tree (i,i+8), (i,i+4), (i,i+2), (0,1); distance 1 - sum.
 // Results: out[idx] for each of the c ids in ids[].
 // ------------------------------------------------------------------------------------------
+// One chunk of four coordinates into the lane's four accumulators.
+template <int SPACE>
+__device__ __forceinline__ void group16_acc(const float4& x, const float4& q, float& a0, float& a1, float& a2,
+                                            float& a3) {
+    if (SPACE == 0) {
+        float d0 = q.x - x.x, d1 = q.y - x.y, d2 = q.z - x.z, d3 = q.w - x.w;
+        a0 = fma_(d0, d0, a0);
+        a1 = fma_(d1, d1, a1);
+        a2 = fma_(d2, d2, a2);
+        a3 = fma_(d3, d3, a3);
+    } else {
+        a0 = fma_(q.x, x.x, a0);
+        a1 = fma_(q.y, x.y, a1);
+        a2 = fma_(q.z, x.z, a2);
+        a3 = fma_(q.w, x.w, a3);
+    }
+}
+
+// The group's 16 accumulators (four per lane) to the distance, in the reference's order; every
+// lane of the group returns it.
+template <int SPACE>
+__device__ __forceinline__ float group16_reduce(float a0, float a1, float a2, float a3, int g, int lane) {
+    const int gbase = lane & ~3;
+    if (SPACE == 0) {
+        float r = ((a0 + a1) + a2) + a3;
+#pragma unroll
+        for (int s = 1; s < 4; s++) {
+            float p = __shfl(r, gbase + s - 1);
+            if (g == s) r = (((p + a0) + a1) + a2) + a3;
+        }
+        return __shfl(r, gbase + 3);
+    }
+    // (i, i+8): lanes g=0,1 pair with g=2,3
+    float o0 = __shfl(a0, lane + 2), o1 = __shfl(a1, lane + 2), o2 = __shfl(a2, lane + 2), o3 = __shfl(a3, lane + 2);
+    float h0 = a0 + o0, h1 = a1 + o1, h2 = a2 + o2, h3 = a3 + o3;
+    // (i, i+4): lane g=0 pairs with g=1
+    float p0 = __shfl(h0, lane + 1), p1 = __shfl(h1, lane + 1), p2 = __shfl(h2, lane + 1), p3 = __shfl(h3, lane + 1);
+    float u0 = h0 + p0, u1 = h1 + p1, u2 = h2 + p2, u3 = h3 + p3;
+    float s0 = u0 + u2, s1 = u1 + u3;
+    float ip = s0 + s1;
+    return __shfl(1.0f - ip, gbase);
+}
+
 template <int SPACE, int NCH>
 __device__ __forceinline__ void dist_group16(const float* __restrict__ row, const float* qs, int nch, bool act, int g,
                                              float& res, int lane) {
@@ -141,61 +184,26 @@ __device__ __forceinline__ void dist_group16(const float* __restrict__ row, cons
     for (int t = 0; t < (NCH > 0 ? NCH : 1); t++) {
         if (NCH == 0) break;
         float4 x = act ? r4[4 * t + g] : make_float4(0.f, 0.f, 0.f, 0.f);
-        float4 q = *reinterpret_cast<const float4*>(qs + 16 * t + 4 * g);
-        if (SPACE == 0) {
-            float d0 = q.x - x.x, d1 = q.y - x.y, d2 = q.z - x.z, d3 = q.w - x.w;
-            a0 = fma_(d0, d0, a0);
-            a1 = fma_(d1, d1, a1);
-            a2 = fma_(d2, d2, a2);
-            a3 = fma_(d3, d3, a3);
-        } else {
-            a0 = fma_(q.x, x.x, a0);
-            a1 = fma_(q.y, x.y, a1);
-            a2 = fma_(q.z, x.z, a2);
-            a3 = fma_(q.w, x.w, a3);
-        }
+        group16_acc<SPACE>(x, *reinterpret_cast<const float4*>(qs + 16 * t + 4 * g), a0, a1, a2, a3);
     }
     if (NCH == 0) {
 #pragma unroll 4
         for (int t = 0; t < n; t++) {
             float4 x = act ? r4[4 * t + g] : make_float4(0.f, 0.f, 0.f, 0.f);
-            float4 q = *reinterpret_cast<const float4*>(qs + 16 * t + 4 * g);
-            if (SPACE == 0) {
-                float d0 = q.x - x.x, d1 = q.y - x.y, d2 = q.z - x.z, d3 = q.w - x.w;
-                a0 = fma_(d0, d0, a0);
-                a1 = fma_(d1, d1, a1);
-                a2 = fma_(d2, d2, a2);
-                a3 = fma_(d3, d3, a3);
-            } else {
-                a0 = fma_(q.x, x.x, a0);
-                a1 = fma_(q.y, x.y, a1);
-                a2 = fma_(q.z, x.z, a2);
-                a3 = fma_(q.w, x.w, a3);
-            }
+            group16_acc<SPACE>(x, *reinterpret_cast<const float4*>(qs + 16 * t + 4 * g), a0, a1, a2, a3);
         }
     }
-    const int gbase = lane & ~3;
-    if (SPACE == 0) {
-        float r = ((a0 + a1) + a2) + a3;
+    res = group16_reduce<SPACE>(a0, a1, a2, a3, g, lane);
+}
+
+// The same with the row's NCH chunks already in registers (x[t] = coordinates 16t+4g..16t+4g+3).
+template <int SPACE, int NCH>
+__device__ __forceinline__ float dist_group16_regs(const float4 (&x)[NCH], const float* qs, int g, int lane) {
+    float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f;
 #pragma unroll
-        for (int s = 1; s < 4; s++) {
-            float p = __shfl(r, gbase + s - 1);
-            if (g == s) r = (((p + a0) + a1) + a2) + a3;
-        }
-        res = __shfl(r, gbase + 3);
-    } else {
-        // (i, i+8): lanes g=0,1 pair with g=2,3
-        float o0 = __shfl(a0, lane + 2), o1 = __shfl(a1, lane + 2), o2 = __shfl(a2, lane + 2),
-              o3 = __shfl(a3, lane + 2);
-        float h0 = a0 + o0, h1 = a1 + o1, h2 = a2 + o2, h3 = a3 + o3;
-        // (i, i+4): lane g=0 pairs with g=1
-        float p0 = __shfl(h0, lane + 1), p1 = __shfl(h1, lane + 1), p2 = __shfl(h2, lane + 1),
-              p3 = __shfl(h3, lane + 1);
-        float u0 = h0 + p0, u1 = h1 + p1, u2 = h2 + p2, u3 = h3 + p3;
-        float s0 = u0 + u2, s1 = u1 + u3;
-        float ip = s0 + s1;
-        res = __shfl(1.0f - ip, gbase);
-    }
+    for (int t = 0; t < NCH; t++)
+        group16_acc<SPACE>(x[t], *reinterpret_cast<const float4*>(qs + 16 * t + 4 * g), a0, a1, a2, a3);
+    return group16_reduce<SPACE>(a0, a1, a2, a3, g, lane);
 }
 
 }  // namespace hg

@@ -36,8 +36,23 @@ __device__ __forceinline__ void dist_batch(const DevGraph& g, const float* qs, c
         bool a0 = i0 < c, a1 = i1 < c;
         uint32_t id0 = a0 ? ids[i0] : 0u, id1 = a1 ? ids[i1] : 0u;
         float r0, r1;
-        dist_group16<SPACE, NCH>(g.vec + (size_t)id0 * g.stride, qs, nch, a0, sub, r0, lane);
-        dist_group16<SPACE, NCH>(g.vec + (size_t)id1 * g.stride, qs, nch, a1, sub, r1, lane);
+        if constexpr (NCH > 0) {
+            // both vectors' chunks are loaded before either is used, with no branch around a
+            // load: an inactive slot reads row 0 and its result is dropped
+            const float4* x0 = reinterpret_cast<const float4*>(g.vec + (size_t)id0 * g.stride);
+            const float4* x1 = reinterpret_cast<const float4*>(g.vec + (size_t)id1 * g.stride);
+            float4 v0[NCH], v1[NCH];
+#pragma unroll
+            for (int t = 0; t < NCH; t++) v0[t] = x0[4 * t + sub];
+#pragma unroll
+            for (int t = 0; t < NCH; t++) v1[t] = x1[4 * t + sub];
+            __builtin_amdgcn_sched_barrier(0);
+            r0 = dist_group16_regs<SPACE, NCH>(v0, qs, sub, lane);
+            r1 = dist_group16_regs<SPACE, NCH>(v1, qs, sub, lane);
+        } else {
+            dist_group16<SPACE, NCH>(g.vec + (size_t)id0 * g.stride, qs, nch, a0, sub, r0, lane);
+            dist_group16<SPACE, NCH>(g.vec + (size_t)id1 * g.stride, qs, nch, a1, sub, r1, lane);
+        }
         if (sub == 0) {
             if (a0) out[i0] = r0;
             if (a1) out[i1] = r1;
