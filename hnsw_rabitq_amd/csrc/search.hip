@@ -159,9 +159,8 @@ __global__ __launch_bounds__(64) void hnsw_search_kernel(SearchParams P) {
     const uint32_t hmask = hsize - 1u;
 
     float* qs = reinterpret_cast<float*>(smem);
-    float* bd = qs + dimpad;
-    uint32_t* bi = reinterpret_cast<uint32_t*>(bd + cap);
-    uint32_t* hash = bi + cap;
+    uint64_t* bk = reinterpret_cast<uint64_t*>(qs + dimpad);  // beam keys (bkey), 8-byte aligned
+    uint32_t* hash = reinterpret_cast<uint32_t*>(bk + cap);
     float* sd = reinterpret_cast<float*>(hash + hsize);
     uint32_t* si = reinterpret_cast<uint32_t*>(sd + 64);
     // STOP / TRACE: the features (16 floats) and the network's hidden activations (64 floats)
@@ -268,8 +267,7 @@ __global__ __launch_bounds__(64) void hnsw_search_kernel(SearchParams P) {
             nallowed = id_allowed(wdel, wal, cur) ? 1 : 0;
         }
         if (lane == 0) {
-            bd[0] = curdist;
-            bi[0] = cur | ((FILT && nallowed == 0) ? DFLAG : 0u);
+            bk[0] = bkey(curdist, cur) | ((FILT && nallowed == 0) ? BD : 0ull);
             visit(hash, hmask, cur);
         }
         E0 = 1;
@@ -293,21 +291,21 @@ __global__ __launch_bounds__(64) void hnsw_search_kernel(SearchParams P) {
         // the 12 traversal features of the current state into feat[] (STOP / TRACE)
         auto make_features = [&]() {
             int kk = kq < n ? kq : n;
-            float d1 = bd[0], dk = bd[kk - 1], dlast = bd[n - 1];
+            float d1 = bdist(bk[0]), dk = bdist(bk[kk - 1]), dlast = bdist(bk[n - 1]);
             // the next expansion: first unexpanded entry (the search's stop rule ends the loop
             // when there is none)
             float dnext = dlast;
             int pnext = n;
             for (int b = scan_lo; b < n; b += 64) {
                 int i = b + lane;
-                uint64_t m = ballot(i < n && (bi[i] & XFLAG) == 0);
+                uint64_t m = ballot(i < n && (bk[i] & BX) == 0);
                 if (m) {
                     pnext = b + __ffsll((unsigned long long)m) - 1;
-                    dnext = bd[pnext];
+                    dnext = bdist(bk[pnext]);
                     break;
                 }
             }
-            const int crowd = beam_lower_bound(bd, bi, n, 1.2f * dk, 0u);
+            const int crowd = beam_lower_bound64(bk, n, bkey(1.2f * dk, 0u));
             if (lane < kStopFeatures)
                 feat[lane] = stop_feature(lane, st, d1, dk, dlast, dnext, pnext, kq, H0, ef_eff, crowd);
         };
@@ -321,26 +319,27 @@ __global__ __launch_bounds__(64) void hnsw_search_kernel(SearchParams P) {
             uint32_t node2 = SENT;
             for (int b = scan_lo; b < n; b += 64) {
                 int i = b + lane;
-                bool un = i < n && (bi[i] & XFLAG) == 0;
+                const int ic = i < n ? i : n - 1;  // every lane reads; the ballot drops i >= n
+                const bool un = (i < n) & ((bk[ic] & BX) == 0);
                 uint64_t m = ballot(un);
                 if (m) {
                     p = b + __ffsll((unsigned long long)m) - 1;
                     const uint64_t m2 = m & (m - 1);
-                    if (m2) node2 = bi[b + __ffsll((unsigned long long)m2) - 1] & ID_MASK;
+                    if (m2) node2 = bid(bk[b + __ffsll((unsigned long long)m2) - 1]);
                     break;
                 }
             }
             if (p < 0 || H0 >= hcap) break;
-            const uint32_t bip = bi[p];
+            const uint64_t bkp = bk[p];
             if (FILT && P.eps_mode) {
                 // EpsilonSearchStopCondition::should_stop_search (stop_condition.h:232-243); a
                 // disallowed entry point is the reference's candidate at FLT_MAX
-                const float cd = (H0 == 0 && (bip & DFLAG)) ? __FLT_MAX__ : bd[p];
+                const float cd = (H0 == 0 && (bkp & BD)) ? __FLT_MAX__ : bdist(bkp);
                 if (cd > P.eps && nallowed >= P.eps_min) break;
             }
-            const uint32_t node = bip & ID_MASK;
+            const uint32_t node = bid(bkp);
             __syncthreads();
-            if (lane == 0) bi[p] = bip | XFLAG;
+            if (lane == 0) bk[p] = bkp | BX;
             H0++;
 
             // neighbours of node: keep those not visited, in list order
@@ -398,27 +397,23 @@ __global__ __launch_bounds__(64) void hnsw_search_kernel(SearchParams P) {
                 }
                 E0 += cnt;
                 // admission against the current worst (when full) and sort of the batch
-                float d = INFINITY;
-                uint32_t id = SENT, idf = SENT;  // idf: id with its DFLAG (FILT)
+                uint64_t key = ~0ull;            // the staged key (bkey), ~0 when not admitted
+                uint32_t idf = SENT;             // id with its DFLAG (FILT)
                 if (lane < cnt) {
-                    d = sd[lane];
                     idf = si[lane];
-                    id = idf & ID_MASK;
+                    key = bkey(sd[lane], idf & ID_MASK);
                     const bool full = FILT ? nallowed >= ef_eff : n >= ef_eff;
-                    if (full && !key_less(d, id, bd[n - 1], bi[n - 1] & ID_MASK)) {
-                        d = INFINITY;
-                        id = SENT;
-                    }
+                    if (full && !(key < (bk[n - 1] & ~BFLAGS))) key = ~0ull;
                 }
-                uint64_t live = ballot(id != SENT);
+                uint64_t live = ballot(key != ~0ull);
                 if (live) {
                     // Drop nodes already in the beam (re-evaluated after a hash overwrite).  pos =
                     // number of beam keys below this key: its insertion point.
                     int pos = 0;
-                    bool ok = id != SENT;
+                    bool ok = key != ~0ull;
                     if (ok) {
-                        pos = beam_lower_bound(bd, bi, n, d, id);
-                        if (pos < n && bd[pos] == d && (bi[pos] & ID_MASK) == id) ok = false;
+                        pos = beam_lower_bound64(bk, n, key);
+                        if (pos < n && (bk[pos] & ~BFLAGS) == key) ok = false;
                     }
                     const uint64_t km = ballot(ok);
                     const int m = __popcll(km);
@@ -426,11 +421,12 @@ __global__ __launch_bounds__(64) void hnsw_search_kernel(SearchParams P) {
                         // Rank of each staged key among the staged keys (keys are distinct):
                         // a scalar loop over the staged lanes, no cross-lane sort network.
                         int r = 0;
+                        const uint32_t klo = (uint32_t)key, khi = (uint32_t)(key >> 32);
                         for (uint64_t t = km; t; t &= t - 1) {
                             const int L = __ffsll((unsigned long long)t) - 1;
-                            const float dl = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(d), L));
-                            const uint32_t il = (uint32_t)__builtin_amdgcn_readlane((int)id, L);
-                            r += key_less(dl, il, d, id) ? 1 : 0;
+                            const uint64_t kl = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)khi, L) << 32) |
+                                                (uint32_t)__builtin_amdgcn_readlane((int)klo, L);
+                            r += kl < key ? 1 : 0;
                         }
                         // insertion points in rank order: lane r holds the point of rank r (the
                         // other lanes park their value in lanes m..63)
@@ -448,22 +444,20 @@ __global__ __launch_bounds__(64) void hnsw_search_kernel(SearchParams P) {
                         const int hi_chunk = (n - 1) & ~63;
                         const int lo_chunk = first_pos & ~63;
                         // Reads are unconditional (indices below cap, clamped at 0); a lane whose
-                        // entry does not move writes to its own slot of the distance-staging area
-                        // (sd/si, which hold no live data during the merge), so no lane branches.
-                        const int trash_d = (int)(sd - bd) + lane, trash_i = (int)(si - bi) + lane;
+                        // entry does not move writes to its own 8-byte slot of the distance-staging
+                        // area (sd and si, 512 contiguous bytes holding no live data during the
+                        // merge), so no lane branches.
+                        const int trash = (int)(reinterpret_cast<uint64_t*>(sd) - bk) + lane;
                         // the rank lanes of spos (the insertion points, non-decreasing in rank)
                         const uint64_t rank_lanes = m >= 64 ? ~0ull : ((1ull << m) - 1ull);
                         for (int b0 = hi_chunk; b0 >= lo_chunk; b0 -= 8 * 64) {
-                            float vd[8];
-                            uint32_t vi[8];
+                            uint64_t vk[8];
                             int sh[8];
                             const int base = b0 + lane;
 #pragma unroll
                             for (int u = 0; u < 8; u++) {
                                 const int i = base - 64 * u;
-                                const int ic = i > 0 ? i : 0;
-                                vd[u] = bd[ic];
-                                vi[u] = bi[ic];
+                                vk[u] = bk[i > 0 ? i : 0];
                             }
                             // shift of entry i = insertion points <= i.  Per chunk [cb, cb + 64):
                             // lo points lie at or below cb and move the whole chunk; only the
@@ -484,16 +478,12 @@ __global__ __launch_bounds__(64) void hnsw_search_kernel(SearchParams P) {
                                 const int i = base - 64 * u;
                                 const int np = i + sh[u];
                                 const bool w = (b0 - 64 * u >= lo_chunk) & (i >= first_pos) & (i < n) & (np < lim);
-                                bd[w ? np : trash_d] = vd[u];
-                                bi[w ? np : trash_i] = vi[u];
+                                bk[w ? np : trash] = vk[u];
                             }
                             __asm__ volatile("" ::: "memory");
                         }
                         const int fp = pos + r;
-                        if (ok && fp < lim) {
-                            bd[fp] = d;
-                            bi[fp] = FILT ? idf : id;
-                        }
+                        if (ok && fp < lim) bk[fp] = key | ((FILT && (idf & DFLAG)) ? BD : 0ull);
                         const int min_fp = first_pos;
                         // Every position below min(first insert, p) holds an old entry whose old
                         // index was below p, hence already expanded.
@@ -516,7 +506,7 @@ __global__ __launch_bounds__(64) void hnsw_search_kernel(SearchParams P) {
                                 int remaining = nallowed - ef_eff, newn = n;
                                 for (int b = (n - 1) & ~63; b >= 0; b -= 64) {
                                     const int i = b + lane;
-                                    const uint64_t am = ballot(i < n && (bi[i] & DFLAG) == 0u);
+                                    const uint64_t am = ballot(i < n && (bk[i] & BD) == 0u);
                                     const int c = __popcll(am);
                                     if (c > remaining) {
                                         uint64_t t = am;
@@ -553,7 +543,7 @@ __global__ __launch_bounds__(64) void hnsw_search_kernel(SearchParams P) {
                         float* fo = P.trace_feat + ((size_t)qi * P.trace_max_rec + nrec) * kStopFeatures;
                         if (lane < kStopFeatures) fo[lane] = feat[lane];
                         uint32_t* to = P.trace_topk + ((size_t)qi * P.trace_max_rec + nrec) * P.k_stride;
-                        for (int i = lane; i < kq; i += 64) to[i] = i < n ? (bi[i] & ID_MASK) : SENT;
+                        for (int i = lane; i < kq; i += 64) to[i] = i < n ? bid(bk[i]) : SENT;
                         nrec++;
                     }
                     if (STOP && H0 >= P.stop.min_hops) {
@@ -583,7 +573,7 @@ __global__ __launch_bounds__(64) void hnsw_search_kernel(SearchParams P) {
                     float* fo = P.trace_feat + ((size_t)qi * P.trace_max_rec + nrec) * kStopFeatures;
                     if (lane < kStopFeatures) fo[lane] = feat[lane];
                     uint32_t* to = P.trace_topk + ((size_t)qi * P.trace_max_rec + nrec) * P.k_stride;
-                    for (int i = lane; i < kq; i += 64) to[i] = i < n ? (bi[i] & ID_MASK) : SENT;
+                    for (int i = lane; i < kq; i += 64) to[i] = i < n ? bid(bk[i]) : SENT;
                     nrec++;
                 }
                 if (STOP) {
@@ -613,20 +603,12 @@ __global__ __launch_bounds__(64) void hnsw_search_kernel(SearchParams P) {
             int got = 0;
             for (int b = 0; b < n && got < keep; b += 64) {
                 const int i = b + lane;
-                float dv = 0.f;
-                uint32_t iv = 0u;
-                if (i < n) {
-                    dv = bd[i];
-                    iv = bi[i];
-                }
-                const bool al = i < n && (iv & DFLAG) == 0u;
+                const uint64_t kv = i < n ? bk[i] : BD;
+                const bool al = i < n && (kv & BD) == 0u;
                 const uint64_t am = ballot(al);
                 const int slot = got + lanes_below(am, lane);
                 __asm__ volatile("" ::: "memory");
-                if (al && slot < keep) {
-                    bd[slot] = dv;
-                    bi[slot] = iv;
-                }
+                if (al && slot < keep) bk[slot] = kv;
                 __asm__ volatile("" ::: "memory");
                 got += __popcll(am);
             }
@@ -644,7 +626,7 @@ __global__ __launch_bounds__(64) void hnsw_search_kernel(SearchParams P) {
             int neps = 0;
             for (int b = 0; b < n; b += 64) {
                 const int i = b + lane;
-                const uint64_t m = ballot(i < n && bd[i] <= P.eps);
+                const uint64_t m = ballot(i < n && bdist(bk[i]) <= P.eps);
                 neps += __popcll(m);
                 if (m != ~0ull) break;
             }
@@ -656,14 +638,16 @@ __global__ __launch_bounds__(64) void hnsw_search_kernel(SearchParams P) {
         uint64_t* ol = P.out_labels + (size_t)qi * P.k_stride;
         float* od = P.out_dists + (size_t)qi * P.k_stride;
         for (int i = lane; i < nout; i += 64) {
-            const float di = bd[i];
-            const uint64_t li = g.labels[bi[i] & ID_MASK];
+            const uint64_t ki = bk[i];
+            const float di = bdist(ki);
+            const uint32_t oi = (uint32_t)(ki >> 32);  // the distance's order bits: equal iff equal
+            const uint64_t li = g.labels[bid(ki)];
             int s0 = i;
-            while (s0 > 0 && bd[s0 - 1] == di) s0--;
+            while (s0 > 0 && (uint32_t)(bk[s0 - 1] >> 32) == oi) s0--;
             int r = s0;
-            for (int j = s0; j < nout && bd[j] == di; j++) {
+            for (int j = s0; j < nout && (uint32_t)(bk[j] >> 32) == oi; j++) {
                 if (j == i) continue;
-                const uint64_t lj = g.labels[bi[j] & ID_MASK];
+                const uint64_t lj = g.labels[bid(bk[j])];
                 r += (lj < li || (lj == li && j < i)) ? 1 : 0;
             }
             if (r < row_len) {
@@ -685,7 +669,7 @@ __global__ __launch_bounds__(64) void hnsw_search_kernel(SearchParams P) {
         if (TRACE) {
             if (lane == 0) P.trace_nrec[qi] = nrec;
             uint32_t* tf = P.trace_final + (size_t)qi * P.k_stride;
-            for (int i = lane; i < kq; i += 64) tf[i] = i < n ? (bi[i] & ID_MASK) : SENT;
+            for (int i = lane; i < kq; i += 64) tf[i] = i < n ? bid(bk[i]) : SENT;
         }
         __syncthreads();
     }

@@ -89,6 +89,38 @@ __device__ __forceinline__ bool hash_insert(uint32_t* hash, uint32_t mask, uint3
     return true;
 }
 
+// Beam entries of the query kernel: one 64-bit key per entry ordered as (distance, id): the
+// distance's order-preserving bits in the high word, id << 2 and two flag bits in the low word
+// (BX: expanded, BD: not in the result set).  Keys of distinct nodes compare as (distance, id)
+// whatever their flags (the flags sit below the id); the keys of one node differ only in the
+// flags.  One 8-byte LDS access and one 64-bit compare per entry.
+constexpr uint64_t BX = 1ull, BD = 2ull, BFLAGS = 3ull;
+// float -> uint32 with the same order (-0 taken as +0; the distances are never NaN)
+__device__ __forceinline__ uint32_t f2ord(float f) {
+    const uint32_t u = __float_as_uint(f + 0.0f);
+    return u ^ ((uint32_t)((int32_t)u >> 31) | 0x80000000u);
+}
+__device__ __forceinline__ float ord2f(uint32_t k) {
+    return __uint_as_float(k ^ ((k & 0x80000000u) ? 0x80000000u : 0xFFFFFFFFu));
+}
+__device__ __forceinline__ uint64_t bkey(float d, uint32_t id) { return ((uint64_t)f2ord(d) << 32) | ((uint64_t)id << 2); }
+__device__ __forceinline__ float bdist(uint64_t k) { return ord2f((uint32_t)(k >> 32)); }
+__device__ __forceinline__ uint32_t bid(uint64_t k) { return (uint32_t)k >> 2; }
+
+// Number of beam entries whose key (flags aside) is below `key` (a flag-free key), over
+// beam[0, n): a fixed-step lower bound (the probe index is clamped, so every lane runs the same
+// steps with no branch).
+__device__ __forceinline__ int beam_lower_bound64(const uint64_t* bk, int n, uint64_t key) {
+    int pos = 0;
+    for (int step = 1 << (31 - __clz(n > 0 ? n : 1)); step > 0; step >>= 1) {
+        const int probe = pos + step - 1;
+        const int pc = probe < n ? probe : (n > 0 ? n - 1 : 0);
+        const bool lt = (probe < n) & ((bk[pc] & ~BFLAGS) < key);
+        pos += lt ? step : 0;
+    }
+    return pos;
+}
+
 // Number of beam entries with key < (d, id), over beam[0, n): a fixed-step lower bound (the
 // probe index is clamped, so every lane runs the same steps with no branch).
 __device__ __forceinline__ int beam_lower_bound(const float* bd, const uint32_t* bi, int n, float d, uint32_t id) {
