@@ -63,12 +63,13 @@ class Timer:
         return ms.value
 
 
-def algorithmic_bytes(ctr, dim, k, maxM, maxM0):
+def algorithmic_bytes(ctr, dim, k, maxM, maxM0, vb=4):
     """Bytes the search must read per query in the GPU layout: the query, one upper link row
-    (4*maxM) per upper hop, one level-0 link row (4*maxM0) per expansion, one vector per distance
+    (4*maxM) per upper hop, one level-0 link row (4*maxM0) per expansion, one vector row
+    (vb*dim: 4 bytes per coordinate for the FP32 rows, 1 for the 8-bit copy) per distance
     evaluation, and the k results written (8-byte label + 4-byte distance)."""
     h0, e0, hu, eu = (ctr[:, i].astype(np.int64) for i in range(4))
-    return (4 * dim + 4 * maxM * hu + 4 * dim * eu + 4 * maxM0 * h0 + 4 * dim * e0 + 12 * k).astype(np.float64)
+    return (4 * dim + 4 * maxM * hu + vb * dim * eu + 4 * maxM0 * h0 + vb * dim * e0 + 12 * k).astype(np.float64)
 
 
 def main():
@@ -92,6 +93,9 @@ def main():
                     help="extension ef of the learned search (default 1.5x the plain operating point)")
     ap.add_argument("--ef", type=int, default=0, help="operating-point ef (skips the sweep)")
     ap.add_argument("--plan", default="", help="SLOTS,PER_CU launch plan (skips the autotuner)")
+    ap.add_argument("--vector-rows", choices=["auto", "f32"], default="auto",
+                    help="rows the graph search reads: the lossless 8-bit copy where the export made "
+                         "one (auto), or the FP32 rows throughout (f32)")
     ap.add_argument("--copy-probe", type=int, default=0,
                     help="only run the streaming copy kernel over this many MiB (counter calibration)")
     args = ap.parse_args()
@@ -142,7 +146,10 @@ def main():
     stream = ctypes.c_void_p()
     _lib.call("hg_stream_create", ctypes.byref(stream))
     gpu = idx.gpu()
-    log(f"GPU index resident: {gpu.nbytes() / 1e6:.0f} MB")
+    gpu.set_vector_code(args.vector_rows)
+    vrows = gpu.vector_code
+    VB = 1 if vrows == "u8" else 4
+    log(f"GPU index resident: {gpu.nbytes() / 1e6:.0f} MB; the search reads the {vrows} vector rows")
     dq = DeviceArray.from_numpy(queries)
     nq = args.nq
     dl = DeviceArray(nq * K * 8)
@@ -173,7 +180,7 @@ def main():
         lab = dl.to_numpy(np.uint64, (nq, K))
         ctr = dc.to_numpy(np.int32, (nq, 4))
         rec = recall_at(lab, gt, K)
-        byts = algorithmic_bytes(ctr, D, K, maxM, maxM0).sum()
+        byts = algorithmic_bytes(ctr, D, K, maxM, maxM0, VB).sum()
         log(f"  ef={ef:4d} recall@10={rec:.4f} {ms:8.2f} ms  QPS={nq / (ms * 1e-3):11.0f}  "
             f"{byts / (ms * 1e-3) / 1e9:7.1f} GB/s  hops={ctr[:, 0].mean():.0f} evals={ctr[:, 1].mean():.0f}")
         return dict(ef=ef, recall=rec, ms=ms, qps=nq / (ms * 1e-3), gbs=byts / (ms * 1e-3) / 1e9,
@@ -233,8 +240,9 @@ def main():
     # counters of the tuned plan (evaluations including re-evaluations) for comparison
     gpu.search_device(dq, nq, K, ef_op, dl, dd, d_counters=dc, stream=stream)
     ctr = dc.to_numpy(np.int32, (nq, 4))
-    bytes_launch = algorithmic_bytes(ctr_alg, D, K, maxM, maxM0).sum()
-    bytes_read_tuned = algorithmic_bytes(ctr, D, K, maxM, maxM0).sum()
+    lab_op = dl.to_numpy(np.uint64, (nq, K))
+    bytes_launch = algorithmic_bytes(ctr_alg, D, K, maxM, maxM0, VB).sum()
+    bytes_read_tuned = algorithmic_bytes(ctr, D, K, maxM, maxM0, VB).sum()
     achieved = bytes_launch / (ev_ms * 1e-3) / 1e9
     log(f"plain: {plain_qps:.0f} QPS at ef={ef_op}; kernel {ev_ms:.3f} ms; {achieved:.1f} GB/s algorithmic "
         f"(evaluations/query {ctr_alg[:, 1].mean():.0f}; tuned plan {ctr[:, 1].mean():.0f})")
@@ -250,7 +258,36 @@ def main():
     result = dict(plain_qps_at_095=plain_qps, plain_ef=ef_op, plain_recall=op["recall"], plan=plan,
                   plain_qps_host_buffers=nq / (host_ms * 1e-3), plain_ms_host_buffers=host_ms,
                   evals_per_query=float(ctr_alg[:, 1].mean()), evals_per_query_tuned=float(ctr[:, 1].mean()),
-                  bytes_per_launch_with_reevaluations=bytes_read_tuned)
+                  bytes_per_launch_with_reevaluations=bytes_read_tuned, vector_rows=vrows)
+
+    # the same operating point on the FP32 rows (what the search reads without the 8-bit copy):
+    # same results bit for bit, four times the bytes per evaluation
+    if vrows == "u8":
+        gpu.set_vector_code("f32")
+        if args.plan:
+            gpu.set_plan(ef_op, K, slots, per_cu)
+        else:
+            gpu.autotune(dq, nq, ef_op, K, stream=stream)
+        f_plan = gpu.plan(ef_op, K, nq)
+        gpu.search_device(dq, nq, K, ef_op, dl, dd, stream=stream)
+        f_ms = []
+        for _ in range(args.steps):
+            timer.start()
+            gpu.search_device(dq, nq, K, ef_op, dl, dd, stream=stream)
+            f_ms.append(timer.stop())
+        f_ms = float(np.mean(f_ms))
+        f_same = bool(np.array_equal(dl.to_numpy(np.uint64, (nq, K)), lab_op))
+        f_bytes = algorithmic_bytes(ctr_alg, D, K, maxM, maxM0, 4).sum()
+        gpu.set_vector_code("auto")
+        if args.plan:
+            gpu.set_plan(ef_op, K, slots, per_cu)
+        else:
+            gpu.autotune(dq, nq, ef_op, K, stream=stream)
+        result.update(f32_rows_qps=nq / (f_ms * 1e-3), f32_rows_kernel_ms=f_ms, f32_rows_plan=f_plan,
+                      f32_rows_bytes_per_launch=f_bytes,
+                      f32_rows_frac_hbm=f_bytes / (f_ms * 1e-3) / 1e9 / PEAK_HBM_GBS, f32_rows_same_labels=f_same)
+        log(f"FP32 rows at the same ef: {nq / (f_ms * 1e-3):.0f} QPS, kernel {f_ms:.3f} ms, "
+            f"{f_bytes / (f_ms * 1e-3) / 1e9:.1f} GB/s algorithmic; labels identical: {f_same}")
 
     # learned early termination
     learned = None
@@ -358,6 +395,7 @@ def main():
         for rec in recs if isinstance(recs, list) else [recs]:
             rp = rec.get("plan") or {}
             if rec.get("workload") == workload and rec.get("ef") == ef_op and \
+                    rec.get("vector_rows", "f32") == vrows and \
                     rp.get("workgroups") == plan["workgroups"] and rp.get("hash_slots") == plan["hash_slots"]:
                 traffic, traffic_plan = rec.get("traffic_bytes_per_launch"), rp
     line = {
@@ -374,11 +412,14 @@ def main():
         "dtype": "f32",
         "data": "synthetic",
         "config": {"workload": workload,
-                   "n": args.n, "dim": D, "queries": nq, "ef_sweep": EF_SWEEP, "k": K},
+                   "n": args.n, "dim": D, "queries": nq, "ef_sweep": EF_SWEEP, "k": K,
+                   "vector_rows": vrows + (" (lossless 8-bit copy of the FP32 rows; distances bit-identical)"
+                                           if vrows == "u8" else "")},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": PEAK_HBM_GBS, "unit": "GB/s",
                      "frac": achieved / PEAK_HBM_GBS, "traffic": traffic, "traffic_plan": traffic_plan,
                      "plan": {"hash_slots": plan["hash_slots"], "workgroups": plan["workgroups"]},
-                     "kernel": "hnsw_search_kernel", "bytes_per_launch": bytes_launch, "kernel_ms": ev_ms},
+                     "kernel": "hnsw_search_kernel", "vector_rows": vrows, "bytes_per_launch": bytes_launch,
+                     "kernel_ms": ev_ms},
         "cpu_baseline": cpu,
         "sweep": [{k2: (round(v, 5) if isinstance(v, float) else v) for k2, v in s.items()} for s in sweep],
         "exact_bf_gt_ms": gt_ms,

@@ -110,6 +110,20 @@ class GpuHnsw:
         """Write the versioned GPU-layout cache of this index."""
         _lib.call("hg_gpu_index_save", self.handle, str(path).encode())
 
+    @property
+    def vector_code(self):
+        """"u8" when the search reads the lossless 8-bit copy of the vectors, else "f32"."""
+        c = ctypes.c_int()
+        _lib.call("hg_gpu_vector_code", self.handle, ctypes.byref(c))
+        return "u8" if c.value == _lib.VEC_U8 else "f32"
+
+    def set_vector_code(self, code):
+        """"auto": read the 8-bit copy where the export found it lossless (default); "f32": read
+        the FP32 rows.  Distances are bit-identical either way."""
+        if code not in ("auto", "f32"):
+            raise ValueError("code must be 'auto' or 'f32'")
+        _lib.call("hg_gpu_set_vector_code", self.handle, _lib.VEC_AUTO if code == "auto" else _lib.VEC_F32)
+
     def search(self, queries, k, ef, counters=False, filter=None):
         """filter: None or a GpuFilter of this index (non-bare-bone search)."""
         q = np.ascontiguousarray(queries, dtype=np.float32)

@@ -12,6 +12,7 @@ LIB_PATH = os.path.join(HERE, "libhnswgpu.so")
 HG_OK, HG_ERR_INVALID, HG_ERR_RUNTIME, HG_ERR_HIP, HG_ERR_SHORT = 0, 1, 2, 3, 4
 SPACE_L2, SPACE_IP = 0, 1
 PREC_FP32, PREC_BF16, PREC_FP8 = 0, 1, 2
+VEC_F32, VEC_U8, VEC_AUTO = 0, 1, 1
 
 _vp = ctypes.c_void_p
 _sz = ctypes.c_size_t
@@ -35,6 +36,8 @@ SIGNATURES = {
     "hg_gpu_index_create": [_vp, _i32, _pvp],
     "hg_gpu_index_free": [_vp],
     "hg_gpu_index_bytes": [_vp, ctypes.POINTER(_sz)],
+    "hg_gpu_set_vector_code": [_vp, _i32],
+    "hg_gpu_vector_code": [_vp, ctypes.POINTER(_i32)],
     "hg_gpu_search": [_vp, _vp, _sz, _sz, _sz, _vp, _vp, _vp],
     "hg_gpu_search_device": [_vp, _vp, _sz, _vp, _vp, _sz, _sz, _sz, _vp, _vp, _vp, _vp, _vp],
     "hg_gpu_search_plan": [_vp, _sz, _sz, _sz, _vp],

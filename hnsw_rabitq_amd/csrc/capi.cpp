@@ -40,6 +40,10 @@ struct GpuIndex {
     int cus = 0;
     hg::DevGraph g{};
     DevBuf vec, links0, upper_off, upper, labels;
+    // 8-bit coded copy of the vectors (DevGraph::vec8), made at export when it is lossless;
+    // g.vec8 points at it unless the caller chose the FP32 rows (hg_gpu_set_vector_code)
+    DevBuf vec8;
+    bool has_u8 = false;
     // Per-launch device words (status, work counters, overflow-list counts; layout at kWords)
     // and ovf = two overflow lists of [nq] u32.  Launches take the next of kRing sets, so up to
     // kRing launches on different streams may be in flight on one index.
@@ -530,12 +534,33 @@ GpuIndex* finish_export(HostLayout&& L, int device) {
     upload(gi->labels, L.labels.data(), L.labels.size() * 8);
     upload(gi->delbits, L.del.data(), L.del.size() * 4);
     for (uint32_t w : L.del) gi->num_deleted += (size_t)__builtin_popcount(w);
+    if (hg::search_fixed_chunks((uint32_t)L.dim)) {
+        // the 8-bit copy, kept only when every coordinate is an integer in 0..255
+        DevBuf flag;
+        flag.reserve(4);
+        gi->vec8.reserve(std::max<size_t>(L.n * L.dim, 16));
+        hip_check(hipMemset(flag.p, 0, 4), "hipMemset");
+        hip_check(hg::launch_code_u8((const float*)gi->vec.p, (uint32_t)L.n, (uint32_t)L.dim, (uint32_t)L.stride,
+                                     (uint8_t*)gi->vec8.p, (uint32_t*)flag.p, nullptr),
+                  "code launch");
+        uint32_t bad = 1;
+        hip_check(hipMemcpy(&bad, flag.p, 4, hipMemcpyDeviceToHost), "hipMemcpy D2H");
+        if (bad == 0) {
+            gi->has_u8 = true;
+            gi->bytes += L.n * L.dim;
+        } else {
+            hip_check(hipFree(gi->vec8.p), "hipFree");
+            gi->vec8.p = nullptr;
+            gi->vec8.bytes = 0;
+        }
+    }
     for (auto& lw : gi->ring) lw.words.reserve(kWords * 4);
     gi->host_labels = std::move(L.labels);
     gi->urows = L.urows;
     hip_check(hipStreamCreateWithFlags(&gi->stream, hipStreamNonBlocking), "hipStreamCreate");
     hg::DevGraph& g = gi->g;
     g.vec = (const float*)gi->vec.p;
+    g.vec8 = gi->has_u8 ? (const uint8_t*)gi->vec8.p : nullptr;
     g.links0 = (const uint32_t*)gi->links0.p;
     g.upper_off = (const uint32_t*)gi->upper_off.p;
     g.upper = (const uint32_t*)gi->upper.p;
@@ -744,6 +769,29 @@ HG_API int hg_gpu_index_free(void* g) {
     delete gi;
     if (prev >= 0) (void)hipSetDevice(prev);
     return HG_OK;
+}
+
+HG_API int hg_gpu_set_vector_code(void* gv, int code) {
+    return guarded([&] {
+        require(gv, "null argument");
+        require(code == HG_VEC_AUTO || code == HG_VEC_F32, "code must be HG_VEC_AUTO or HG_VEC_F32");
+        auto* gi = static_cast<GpuIndex*>(gv);
+        std::lock_guard<std::mutex> lk(gi->mu);
+        const uint8_t* want = (code == HG_VEC_AUTO && gi->has_u8) ? (const uint8_t*)gi->vec8.p : nullptr;
+        if (want != gi->g.vec8) {
+            gi->g.vec8 = want;
+            gi->tuned.clear();  // plans were timed with the other kernel
+        }
+        return HG_OK;
+    });
+}
+
+HG_API int hg_gpu_vector_code(void* gv, int* code) {
+    return guarded([&] {
+        require(gv && code, "null argument");
+        *code = static_cast<GpuIndex*>(gv)->g.vec8 ? HG_VEC_U8 : HG_VEC_F32;
+        return HG_OK;
+    });
 }
 
 HG_API int hg_gpu_index_bytes(void* g, size_t* bytes) {

@@ -16,8 +16,14 @@ constexpr uint32_t ID_MASK = 0x3FFFFFFFu; // ids below 2^30
 //   upper_off[n] u32: row of level 1 in `upper` (levels l>1 follow at +l-1), SENT if none
 //   upper    [rows][maxM] u32 upper-level neighbours, SENT padded
 //   labels   [n] u64 external labels
+//   vec8     optional [n][dim] u8: the same vectors when every stored coordinate is an integer in
+//            0..255 (its FP32 bits those of the byte's value), dim % 16 == 0.  Row r holds, for
+//            lane g = 0..3 of a distance group, the words t = 0..dim/16-1 of coordinates
+//            16t+4g..16t+4g+3: byte (g * dim/16 + t) * 4 + j = coordinate 16t + 4g + j, so each
+//            lane reads one contiguous run.  The query kernel reads it instead of vec when set.
 struct DevGraph {
     const float* vec;
+    const uint8_t* vec8;
     const uint32_t* links0;
     const uint32_t* upper_off;
     const uint32_t* upper;

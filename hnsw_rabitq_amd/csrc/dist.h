@@ -206,4 +206,20 @@ __device__ __forceinline__ float dist_group16_regs(const float4 (&x)[NCH], const
     return group16_reduce<SPACE>(a0, a1, a2, a3, g, lane);
 }
 
+// The same from the 8-bit coded row: w[t] holds coordinates 16t+4g..16t+4g+3 as four bytes (low
+// byte first).  Every coded value is an integer in 0..255 whose FP32 form is the stored
+// coordinate itself, so the conversion is exact and the accumulation sees the FP32 values.
+template <int SPACE, int NCH>
+__device__ __forceinline__ float dist_group16_u8(const uint32_t (&w)[NCH], const float* qs, int g, int lane) {
+    float a0 = 0.f, a1 = 0.f, a2 = 0.f, a3 = 0.f;
+#pragma unroll
+    for (int t = 0; t < NCH; t++) {
+        const uint32_t v = w[t];
+        const float4 x = make_float4((float)(v & 0xFFu), (float)((v >> 8) & 0xFFu), (float)((v >> 16) & 0xFFu),
+                                     (float)(v >> 24));
+        group16_acc<SPACE>(x, *reinterpret_cast<const float4*>(qs + 16 * t + 4 * g), a0, a1, a2, a3);
+    }
+    return group16_reduce<SPACE>(a0, a1, a2, a3, g, lane);
+}
+
 }  // namespace hg

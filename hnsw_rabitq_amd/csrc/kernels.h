@@ -17,6 +17,14 @@ hipError_t launch_search(const SearchParams& p, int blocks, hipStream_t s);
 hipError_t launch_zero_u32(uint32_t* p, uint32_t n, hipStream_t s);  // a kernel (graph-safe), not a memset
 hipError_t launch_copy(void* dst, const void* src, size_t bytes, hipStream_t s);
 hipError_t launch_id_labels(const DevGraph& g, const uint32_t* ids, size_t n, uint64_t* out, hipStream_t s);
+// dim / 16 is one of the chunk counts the query kernel is compiled for (what the 8-bit coded rows
+// need).
+bool search_fixed_chunks(uint32_t dim);
+// The 8-bit coded copy of vec ([n][stride] f32) in the DevGraph::vec8 layout; *flag (cleared by
+// the caller) gets bit 0 when a coordinate is not an integer in 0..255 with that FP32 value's
+// bits (then out is not a lossless copy and must not be used).
+hipError_t launch_code_u8(const float* vec, uint32_t n, uint32_t dim, uint32_t stride, uint8_t* out, uint32_t* flag,
+                          hipStream_t s);
 
 // ---- multi-vector search (mvsearch.hip) ----
 size_t mv_lds_bytes(uint32_t dim, uint32_t cap, uint32_t hash_bits, uint32_t dcap);

@@ -148,7 +148,7 @@ __device__ __forceinline__ bool id_allowed(uint32_t wdel, uint32_t wal, uint32_t
 // the result set's worst once that set is full); before that every admitted key is kept, up to
 // the capacity `cap`; a query that needs more is abandoned and listed for a pass at a larger
 // capacity (ovf_list), or, in the last pass, sets status bit 0.
-template <int SPACE, int MODE, int NCH, bool STOP, bool TRACE, bool FILT>
+template <int SPACE, int MODE, int NCH, bool STOP, bool TRACE, bool FILT, bool U8>
 __global__ __launch_bounds__(64) void hnsw_search_kernel(SearchParams P) {
     extern __shared__ __align__(16) unsigned char smem[];
     const int lane = threadIdx.x;
@@ -206,7 +206,7 @@ __global__ __launch_bounds__(64) void hnsw_search_kernel(SearchParams P) {
         uint32_t cur = g.entry;
         if (lane == 0) si[0] = cur;
         __syncthreads();
-        dist_batch<SPACE, MODE, NCH>(g, qs, si, 1, sd, lane);
+        dist_batch<SPACE, MODE, NCH, U8>(g, qs, si, 1, sd, lane);
         __syncthreads();
         float curdist = sd[0];
         Eu = 1;
@@ -227,7 +227,7 @@ __global__ __launch_bounds__(64) void hnsw_search_kernel(SearchParams P) {
                 }
                 __syncthreads();
                 if (cnt == 0) continue;
-                dist_batch<SPACE, MODE, NCH>(g, qs, si, cnt, sd, lane);
+                dist_batch<SPACE, MODE, NCH, U8>(g, qs, si, cnt, sd, lane);
                 __syncthreads();
                 Eu += cnt;
                 // first minimum in list order
@@ -386,7 +386,7 @@ __global__ __launch_bounds__(64) void hnsw_search_kernel(SearchParams P) {
                     if (P.del_bits) fw_del = P.del_bits[fid >> 5];
                     if (P.allow_bits) fw_al = P.allow_bits[fid >> 5];
                 }
-                dist_batch<SPACE, MODE, NCH>(g, qs, si, cnt, sd, lane);
+                dist_batch<SPACE, MODE, NCH, U8>(g, qs, si, cnt, sd, lane);
                 __syncthreads();
                 if (FILT) {
                     if (lane < cnt) {
@@ -708,6 +708,42 @@ hipError_t launch_id_labels(const DevGraph& g, const uint32_t* ids, size_t n, ui
     return hipGetLastError();
 }
 
+// One thread per output word: the four coordinates 16t+4g..16t+4g+3 of row r as bytes, placed at
+// word g * (dim/16) + t of the coded row.  A coordinate whose FP32 bits are not those of an
+// integer in 0..255 (a fraction, out of range, -0, NaN) sets the flag.
+__global__ void code_u8_kernel(const float* __restrict__ vec, uint32_t n, uint32_t dim, uint32_t stride,
+                               uint32_t* __restrict__ out, uint32_t* __restrict__ flag) {
+    const uint32_t words = dim / 4, nch = dim / 16;
+    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= (size_t)n * words) return;
+    const uint32_t r = (uint32_t)(i / words), wi = (uint32_t)(i % words);
+    const uint32_t g = wi / nch, t = wi % nch;
+    const float4 x = *reinterpret_cast<const float4*>(vec + (size_t)r * stride + 16 * t + 4 * g);
+    const float xs[4] = {x.x, x.y, x.z, x.w};
+    uint32_t w = 0u;
+    bool ok = true;
+#pragma unroll
+    for (int j = 0; j < 4; j++) {
+        const float f = xs[j];
+        const bool in = f >= 0.f && f <= 255.f;  // false for NaN
+        const uint32_t b = in ? (uint32_t)f : 0u;
+        ok &= in && __float_as_uint((float)b) == __float_as_uint(f);
+        w |= b << (8 * j);
+    }
+    out[i] = w;
+    if (!ok) atomicOr(flag, 1u);
+}
+
+hipError_t launch_code_u8(const float* vec, uint32_t n, uint32_t dim, uint32_t stride, uint8_t* out, uint32_t* flag,
+                          hipStream_t s) {
+    if (!search_fixed_chunks(dim) || stride % 4 != 0) return hipErrorInvalidValue;
+    const size_t words = (size_t)n * (dim / 4);
+    if (words == 0) return hipSuccess;
+    hipLaunchKernelGGL(code_u8_kernel, dim3((unsigned)((words + 255) / 256)), dim3(256), 0, s, vec, n, dim, stride,
+                       reinterpret_cast<uint32_t*>(out), flag);
+    return hipGetLastError();
+}
+
 __global__ void zero_u32_kernel(uint32_t* p, uint32_t n) {
     uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0u;
@@ -727,9 +763,9 @@ size_t search_lds_bytes(uint32_t dim, uint32_t cap, uint32_t hash_bits) {
     return (size_t)dimpad * 4 + (size_t)cap * 8 + ((size_t)1 << hash_bits) * 4 + 64 * 8;
 }
 
-template <int SPACE, int MODE, int NCH, bool STOP, bool TRACE, bool FILT = false>
+template <int SPACE, int MODE, int NCH, bool STOP, bool TRACE, bool FILT, bool U8 = false>
 static hipError_t launch_t(const SearchParams& p, int blocks, size_t lds, hipStream_t s) {
-    auto k = hnsw_search_kernel<SPACE, MODE, NCH, STOP, TRACE, FILT>;
+    auto k = hnsw_search_kernel<SPACE, MODE, NCH, STOP, TRACE, FILT, U8>;
     if (lds > 64 * 1024) {
         hipError_t e = hipFuncSetAttribute((const void*)k, hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
         if (e != hipSuccess) return e;
@@ -738,15 +774,30 @@ static hipError_t launch_t(const SearchParams& p, int blocks, size_t lds, hipStr
     return hipGetLastError();
 }
 
+// Chunk counts compiled with a fixed unroll (dim = 16, 32, 64, 96, 128); the 8-bit coded rows
+// are read only by these.
+bool search_fixed_chunks(uint32_t dim) {
+    if (dim % 16 != 0) return false;
+    const uint32_t c = dim / 16;
+    return c == 1 || c == 2 || c == 4 || c == 6 || c == 8;
+}
+
+template <int SPACE, int NCH, bool STOP, bool TRACE, bool FILT>
+static hipError_t launch_code(const SearchParams& p, int blocks, size_t lds, hipStream_t s) {
+    return p.g.vec8 ? launch_t<SPACE, 1, NCH, STOP, TRACE, FILT, true>(p, blocks, lds, s)
+                    : launch_t<SPACE, 1, NCH, STOP, TRACE, FILT, false>(p, blocks, lds, s);
+}
+
 template <int SPACE, bool STOP, bool TRACE, bool FILT = false>
 static hipError_t launch_dim(const SearchParams& p, int blocks, size_t lds, hipStream_t s) {
+    if (p.g.vec8 && !search_fixed_chunks(p.g.dim)) return hipErrorInvalidValue;
     if (p.g.dim % 16 != 0) return launch_t<SPACE, 0, 0, STOP, TRACE, FILT>(p, blocks, lds, s);
     switch (p.g.dim / 16) {
-        case 1: return launch_t<SPACE, 1, 1, STOP, TRACE, FILT>(p, blocks, lds, s);
-        case 2: return launch_t<SPACE, 1, 2, STOP, TRACE, FILT>(p, blocks, lds, s);
-        case 4: return launch_t<SPACE, 1, 4, STOP, TRACE, FILT>(p, blocks, lds, s);
-        case 6: return launch_t<SPACE, 1, 6, STOP, TRACE, FILT>(p, blocks, lds, s);
-        case 8: return launch_t<SPACE, 1, 8, STOP, TRACE, FILT>(p, blocks, lds, s);
+        case 1: return launch_code<SPACE, 1, STOP, TRACE, FILT>(p, blocks, lds, s);
+        case 2: return launch_code<SPACE, 2, STOP, TRACE, FILT>(p, blocks, lds, s);
+        case 4: return launch_code<SPACE, 4, STOP, TRACE, FILT>(p, blocks, lds, s);
+        case 6: return launch_code<SPACE, 6, STOP, TRACE, FILT>(p, blocks, lds, s);
+        case 8: return launch_code<SPACE, 8, STOP, TRACE, FILT>(p, blocks, lds, s);
         default: return launch_t<SPACE, 1, 0, STOP, TRACE, FILT>(p, blocks, lds, s);
     }
 }
@@ -774,18 +825,21 @@ int search_blocks_per_cu(const SearchParams& p, bool filtered) {
     size_t lds = search_lds_bytes(p.g.dim, p.cap, p.hash_bits);
     int nb = 0;
     hipError_t e;
-    if (filtered && p.g.dim == 128 && p.g.space == 0)
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, hnsw_search_kernel<0, 1, 8, false, false, true>, 64, lds);
+    const bool d128 = p.g.dim == 128 && p.g.space == 0, u8 = p.g.vec8 != nullptr;
+    if (filtered && d128)
+        e = u8 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, hnsw_search_kernel<0, 1, 8, false, false, true, true>, 64, lds)
+               : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, hnsw_search_kernel<0, 1, 8, false, false, true, false>, 64, lds);
     else if (filtered)
         e = p.g.space == 0
-                ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, hnsw_search_kernel<0, 1, 0, false, false, true>, 64, lds)
-                : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, hnsw_search_kernel<1, 1, 0, false, false, true>, 64, lds);
-    else if (p.g.dim % 16 == 0 && p.g.dim / 16 == 8 && p.g.space == 0)
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, hnsw_search_kernel<0, 1, 8, false, false, false>, 64, lds);
+                ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, hnsw_search_kernel<0, 1, 0, false, false, true, false>, 64, lds)
+                : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, hnsw_search_kernel<1, 1, 0, false, false, true, false>, 64, lds);
+    else if (d128)
+        e = u8 ? hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, hnsw_search_kernel<0, 1, 8, false, false, false, true>, 64, lds)
+               : hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, hnsw_search_kernel<0, 1, 8, false, false, false, false>, 64, lds);
     else if (p.g.space == 0)
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, hnsw_search_kernel<0, 1, 0, false, false, false>, 64, lds);
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, hnsw_search_kernel<0, 1, 0, false, false, false, false>, 64, lds);
     else
-        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, hnsw_search_kernel<1, 1, 0, false, false, false>, 64, lds);
+        e = hipOccupancyMaxActiveBlocksPerMultiprocessor(&nb, hnsw_search_kernel<1, 1, 0, false, false, false, false>, 64, lds);
     if (e != hipSuccess || nb <= 0) nb = 1;
     return nb;
 }

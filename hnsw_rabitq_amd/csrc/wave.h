@@ -13,11 +13,39 @@ __device__ __forceinline__ bool key_less(float da, uint32_t ia, float db, uint32
     return (int)(da < db) | ((int)(da == db) & (int)(ia < ib));
 }
 
+// One lane's NCH words of an 8-bit coded row (DevGraph::vec8): one contiguous run of 4*NCH bytes,
+// read with the widest loads its alignment allows.
+template <int NCH>
+__device__ __forceinline__ void load_u8_words(const uint8_t* run, uint32_t (&w)[NCH]) {
+    if constexpr (NCH % 4 == 0) {
+#pragma unroll
+        for (int t = 0; t < NCH; t += 4) {
+            const uint4 v = reinterpret_cast<const uint4*>(run)[t / 4];
+            w[t] = v.x;
+            w[t + 1] = v.y;
+            w[t + 2] = v.z;
+            w[t + 3] = v.w;
+        }
+    } else if constexpr (NCH % 2 == 0) {
+#pragma unroll
+        for (int t = 0; t < NCH; t += 2) {
+            const uint2 v = reinterpret_cast<const uint2*>(run)[t / 2];
+            w[t] = v.x;
+            w[t + 1] = v.y;
+        }
+    } else {
+#pragma unroll
+        for (int t = 0; t < NCH; t++) w[t] = reinterpret_cast<const uint32_t*>(run)[t];
+    }
+}
+
 // Distances of the c vectors whose ids are ids[0..c) to the query in LDS; results to out[].
 // MODE 0 = generic (one lane per vector), otherwise dim % 16 == 0 with NCH chunks (0 = runtime).
-template <int SPACE, int MODE, int NCH>
+// U8 (NCH > 0 only): the vectors are read from the 8-bit coded copy g.vec8.
+template <int SPACE, int MODE, int NCH, bool U8 = false>
 __device__ __forceinline__ void dist_batch(const DevGraph& g, const float* qs, const uint32_t* ids, int c,
                                            float* out, int lane) {
+    static_assert(!U8 || (MODE == 1 && NCH > 0), "the 8-bit rows are read with a compile-time chunk count");
     if (MODE == 0) {
         for (int base = 0; base < c; base += 64) {
             int idx = base + lane;
@@ -36,7 +64,19 @@ __device__ __forceinline__ void dist_batch(const DevGraph& g, const float* qs, c
         bool a0 = i0 < c, a1 = i1 < c;
         uint32_t id0 = a0 ? ids[i0] : 0u, id1 = a1 ? ids[i1] : 0u;
         float r0, r1;
-        if constexpr (NCH > 0) {
+        if constexpr (U8) {
+            // a lane's run of the coded row: 4*NCH bytes at (sub * NCH) words; an inactive slot
+            // reads row 0 and its result is dropped
+            constexpr int RUN = 4 * NCH;
+            const uint8_t* x0 = g.vec8 + (size_t)id0 * (16 * NCH) + sub * RUN;
+            const uint8_t* x1 = g.vec8 + (size_t)id1 * (16 * NCH) + sub * RUN;
+            uint32_t w0[NCH], w1[NCH];
+            load_u8_words<NCH>(x0, w0);
+            load_u8_words<NCH>(x1, w1);
+            __builtin_amdgcn_sched_barrier(0);
+            r0 = dist_group16_u8<SPACE, NCH>(w0, qs, sub, lane);
+            r1 = dist_group16_u8<SPACE, NCH>(w1, qs, sub, lane);
+        } else if constexpr (NCH > 0) {
             // both vectors' chunks are loaded before either is used, with no branch around a
             // load: an inactive slot reads row 0 and its result is dropped
             const float4* x0 = reinterpret_cast<const float4*>(g.vec + (size_t)id0 * g.stride);

@@ -66,6 +66,11 @@ extern "C" {
 #define HG_PREC_BF16 1 /* BF16 MFMA distance GEMM, FP32 accumulate, top-c FP32 re-rank */
 #define HG_PREC_FP8 2  /* FP8 e4m3 (per-vector scale) MFMA GEMM, FP32 accumulate, re-rank */
 
+/* Vector rows the GPU graph search reads (hg_gpu_set_vector_code / hg_gpu_vector_code). */
+#define HG_VEC_F32 0  /* the FP32 rows as stored by the index */
+#define HG_VEC_U8 1   /* an 8-bit copy: every coordinate is an integer in 0..255 (lossless) */
+#define HG_VEC_AUTO 1 /* the 8-bit copy where the export found it lossless, else FP32 */
+
 const char* hg_last_error(void);
 
 /* ---- CPU-side HNSW index (construction prerequisite, persistence) ---- */
@@ -119,6 +124,17 @@ int hg_gpu_index_load(const char* path, int device, void** out_gpu_index);
 int hg_gpu_index_free(void* gpu_index);
 /* Bytes resident on the device for this index. */
 int hg_gpu_index_bytes(void* gpu_index, size_t* bytes);
+/* No counterpart in the reference (a storage choice of the GPU layout; the reference keeps FP32
+ * rows in data_level0_memory_, hnswalg.h:121-127).  At export, when every stored coordinate is an
+ * integer in 0..255 whose FP32 bits are the stored ones (descriptor data such as SIFT), dim is
+ * 16, 32, 64, 96 or 128 and the rows are exact, the export also keeps an 8-bit copy of the
+ * vectors, and the graph search reads it (a quarter of the bytes per distance evaluation); the
+ * kernel converts each byte back to that same FP32 value, so every distance is bit-identical to
+ * the FP32 rows'.  HG_VEC_AUTO (default) uses the copy where it exists; HG_VEC_F32 makes the
+ * search read the FP32 rows.  hg_gpu_vector_code reports which rows the search reads
+ * (HG_VEC_U8 or HG_VEC_F32).  Changing the choice drops the autotuned launch plans. */
+int hg_gpu_set_vector_code(void* gpu_index, int code);
+int hg_gpu_vector_code(void* gpu_index, int* code);
 
 /* knn_query on host buffers: queries [nq][dim] f32; labels/dists [nq][k], closest first.
  * counters (optional) receives 4 int32 per query: level-0 expansions, level-0 distance
