@@ -130,6 +130,10 @@ __device__ __noinline__ float mlp_logit(const DevStopModel& m, const float* feat
     return o + w3[H];
 }
 
+// Staged keys up to which the insertion points are located through the segment heads (a loop of
+// one ballot per key) rather than by a full-depth LDS search per lane.
+constexpr int kHeadKeys = 16;
+
 // ------------------------------------------------------------------------------------------
 // The search kernel.  Block = one wave (64 lanes) = one query at a time; workgroups loop over
 // queries taken from a global counter until none are left.
@@ -272,6 +276,11 @@ __global__ __launch_bounds__(64) void hnsw_search_kernel(SearchParams P) {
         }
         E0 = 1;
         __syncthreads();
+        // segments of 2^sh entries, at most 64 of them over the most entries the beam can hold
+        const int beam_max = FILT ? (int)cap : ef_eff;
+        int sh = 0;
+        while ((64 << sh) < beam_max) sh++;
+        uint64_t head = beam_head(bk, n, sh, lane);
         int scan_lo = 0;
         StopState st;
         st.d_entry = curdist;
@@ -311,12 +320,68 @@ __global__ __launch_bounds__(64) void hnsw_search_kernel(SearchParams P) {
         };
         const int hcap = (P.hop_cap && P.hop_cap[qi] > 0) ? P.hop_cap[qi] : 0x7FFFFFFF;
         bool ovf = false;  // FILT: the beam overflowed (the result may differ from the reference's)
+        // Link-row prefetch (maxM0 <= 32): lanes 32..63 load the row of the node expected to be
+        // expanded next, pf_node; the next expansion takes it from them when it is that node.
         uint32_t pf_node = SENT, pf_row = SENT;
+        auto prefetch_row = [&](uint32_t nd) {
+            uint32_t v = SENT;
+            if (lane >= 32 && nd != SENT && (lane & 31) < (int)g.maxM0) v = g.links0[(size_t)nd * g.maxM0 + (lane & 31)];
+            pf_row = v;
+            pf_node = nd;
+        };
+        // Pipelined expansions (the plain kernel on 8-bit rows, maxM0 <= 32): when the next
+        // expansion is known before the beam moves (node2, whose row lanes 32..63 loaded at the
+        // start of this expansion, with no new key before it), its neighbours are checked against
+        // the visited hash and their row loads issued then, and the next expansion starts from
+        // them (pre): the loads run while the beam moves.  The visited hash is not touched by the
+        // merge, so checking the neighbours early finds the same fresh set.  The fresh ids are
+        // staged again after the shift, which overwrites the stage.
+        constexpr bool PIPE = U8 && !STOP && !TRACE && !FILT;
+        const bool pipe = PIPE && g.maxM0 <= 32;
+        bool pre = false;
+        int pre_cnt = 0;
+        uint32_t pw0[NCH > 0 ? NCH : 1], pw1[NCH > 0 ? NCH : 1];
+        uint32_t pre_id = SENT;  // lanes 0..31: the staged fresh id of this lane (SENT if none)
+        int pre_pos = 0;         // ... and its stage slot
+        // stages the fresh ids of a row (lanes 0..31 hold its ids, SENT for none) into si
+        auto stage_row = [&](uint32_t id) {
+            const bool fresh = id != SENT && visit(hash, hmask, id);
+            const uint64_t m = ballot(fresh);
+            const int pos = lanes_below(m, lane);
+            if (fresh) si[pos] = id;
+            pre_id = fresh ? id : SENT;
+            pre_pos = pos;
+            return __popcll(m);
+        };
+        // at the decision point of an expansion: the next one is `next`; when it is node2 with its
+        // row at hand, stage it and issue its loads (pre), else prefetch its row
+        // a query of integers in 0..255 takes the exact integer form of the distances
+        bool qint = false;
+        uint32_t qw[NCH > 0 ? NCH : 1];
+        int qsq = 0;
+        if constexpr (PIPE) qint = query_u8_words<NCH>(qs, lane, qw, qsq);
+        auto plan_next = [&](uint32_t next, uint32_t node2v) {
+            const bool stage = pipe && next != SENT && next == node2v && pf_node == node2v;
+            const uint32_t row = (uint32_t)__shfl_xor((int)pf_row, 32);  // node2's row on lanes 0..31
+            // one load site per expansion (the row of the next expansion; when it is staged, its
+            // row again, unused), so that the load stays in flight in every case
+            prefetch_row(next);
+            if (stage) {
+                pre_cnt = stage_row((lane < 32 && lane < (int)g.maxM0) ? row : SENT);
+                __syncthreads();
+                if constexpr (PIPE) {
+                    if (pre_cnt > 0) dist_u8_issue<NCH>(g, si, pre_cnt, lane, pw0, pw1);
+                }
+                pre = true;
+                pf_node = SENT;
+            }
+        };
         for (;;) {
-            // first unexpanded entry at or after scan_lo, and the one after it (whose link row is
-            // prefetched: it is the next expansion unless a better neighbour lands before it)
+            // first unexpanded entry at or after scan_lo, and the one after it (node2, key k2):
+            // the next expansion unless a new key of this expansion lands before it
             int p = -1;
             uint32_t node2 = SENT;
+            uint64_t k2 = ~0ull;
             for (int b = scan_lo; b < n; b += 64) {
                 int i = b + lane;
                 const int ic = i < n ? i : n - 1;  // every lane reads; the ballot drops i >= n
@@ -325,7 +390,10 @@ __global__ __launch_bounds__(64) void hnsw_search_kernel(SearchParams P) {
                 if (m) {
                     p = b + __ffsll((unsigned long long)m) - 1;
                     const uint64_t m2 = m & (m - 1);
-                    if (m2) node2 = bid(bk[b + __ffsll((unsigned long long)m2) - 1]);
+                    if (m2) {
+                        k2 = bk[b + __ffsll((unsigned long long)m2) - 1] & ~BFLAGS;
+                        node2 = bid(k2);
+                    }
                     break;
                 }
             }
@@ -344,23 +412,32 @@ __global__ __launch_bounds__(64) void hnsw_search_kernel(SearchParams P) {
 
             // neighbours of node: keep those not visited, in list order
             int cnt = 0;
-            if (g.maxM0 <= 32) {
-                // lanes 0..31 take node's row (from the previous prefetch when it matches),
-                // lanes 32..63 fetch node2's row for the next expansion
+            const bool staged = pre;  // node's fresh neighbours staged and their loads in flight
+            pre = false;
+            if (pipe) {
+                // lanes 32..63 load node2's row (the next expansion unless a new key lands before
+                // it); lanes 0..31 load node's row unless it was staged or prefetched
                 const int lr = lane & 31;
                 const bool upper = lane >= 32;
                 const uint32_t prev = (uint32_t)__shfl_xor((int)pf_row, 32);
-                const bool hit = node == pf_node;
+                const bool need = upper ? node2 != SENT : (!staged && node != pf_node);
                 uint32_t v = SENT;
-                const bool need = upper ? node2 != SENT : !hit;
                 if (need && lr < (int)g.maxM0) v = g.links0[(size_t)(upper ? node2 : node) * g.maxM0 + lr];
-                uint32_t cur;
-                if (hit)
-                    cur = prev;
-                else
-                    cur = v;
+                if (staged) {
+                    cnt = pre_cnt;
+                } else {
+                    const uint32_t cur = node == pf_node ? prev : v;
+                    cnt = stage_row((!upper && lr < (int)g.maxM0) ? cur : SENT);
+                }
                 pf_row = v;
                 pf_node = node2;
+            } else if (g.maxM0 <= 32) {
+                // lanes 0..31 take node's row: from the prefetch when it was this node's, else
+                // loaded now
+                const int lr = lane & 31;
+                const bool upper = lane >= 32;
+                uint32_t cur = (uint32_t)__shfl_xor((int)pf_row, 32);
+                if (node != pf_node) cur = (!upper && lr < (int)g.maxM0) ? g.links0[(size_t)node * g.maxM0 + lr] : SENT;
                 const uint32_t id = (!upper && lr < (int)g.maxM0) ? cur : SENT;
                 const bool fresh = id != SENT && visit(hash, hmask, id);
                 const uint64_t m = ballot(fresh);
@@ -378,6 +455,12 @@ __global__ __launch_bounds__(64) void hnsw_search_kernel(SearchParams P) {
                 }
             }
             __syncthreads();
+            // the batch: distances, admission, insertion points and ranks; m > 0 keys to merge
+            int m = 0, r = 0, pos = 0, spos = 0, first_pos = 0;
+            bool ok = false;
+            uint64_t key = ~0ull;   // the staged key (bkey), ~0 when not admitted
+            uint32_t idf = SENT;    // id with its DFLAG (FILT)
+            uint64_t kmin = ~0ull;  // the smallest key to merge
             if (cnt > 0) {
                 // FILT: the membership words of the fresh ids load with the vectors
                 uint32_t fw_del = 0u, fw_al = ~0u;
@@ -386,7 +469,14 @@ __global__ __launch_bounds__(64) void hnsw_search_kernel(SearchParams P) {
                     if (P.del_bits) fw_del = P.del_bits[fid >> 5];
                     if (P.allow_bits) fw_al = P.allow_bits[fid >> 5];
                 }
-                dist_batch<SPACE, MODE, NCH, U8>(g, qs, si, cnt, sd, lane);
+                if (PIPE && pipe) {
+                    if constexpr (PIPE) {
+                        if (!staged) dist_u8_issue<NCH>(g, si, cnt, lane, pw0, pw1);
+                        dist_u8_finish<SPACE, NCH>(qs, cnt, pw0, pw1, sd, lane, qint, qw, qsq);
+                    }
+                } else {
+                    dist_batch<SPACE, MODE, NCH, U8>(g, qs, si, cnt, sd, lane);
+                }
                 __syncthreads();
                 if (FILT) {
                     if (lane < cnt) {
@@ -396,31 +486,28 @@ __global__ __launch_bounds__(64) void hnsw_search_kernel(SearchParams P) {
                     __syncthreads();
                 }
                 E0 += cnt;
-                // admission against the current worst (when full) and sort of the batch
-                uint64_t key = ~0ull;            // the staged key (bkey), ~0 when not admitted
-                uint32_t idf = SENT;             // id with its DFLAG (FILT)
+                // admission against the current worst (when full)
                 if (lane < cnt) {
                     idf = si[lane];
                     key = bkey(sd[lane], idf & ID_MASK);
                     const bool full = FILT ? nallowed >= ef_eff : n >= ef_eff;
                     if (full && !(key < (bk[n - 1] & ~BFLAGS))) key = ~0ull;
                 }
-                uint64_t live = ballot(key != ~0ull);
+                const uint64_t live = ballot(key != ~0ull);
                 if (live) {
                     // Drop nodes already in the beam (re-evaluated after a hash overwrite).  pos =
-                    // number of beam keys below this key: its insertion point.
-                    int pos = 0;
-                    bool ok = key != ~0ull;
-                    if (ok) {
-                        pos = beam_lower_bound64(bk, n, key);
-                        if (pos < n && (bk[pos] & ~BFLAGS) == key) ok = false;
-                    }
+                    // number of beam keys below this key: its insertion point, located through the
+                    // segment heads when few keys are staged (one ballot per key), else by a
+                    // full-depth search per lane.
+                    ok = key != ~0ull;
+                    pos = __popcll(live) <= kHeadKeys ? beam_lower_bound_heads(bk, n, sh, head, key, live, lane)
+                                                      : (ok ? beam_lower_bound64(bk, n, key) : 0);
+                    if (ok && pos < n && (bk[pos] & ~BFLAGS) == key) ok = false;
                     const uint64_t km = ballot(ok);
-                    const int m = __popcll(km);
+                    m = __popcll(km);
                     if (m > 0) {
                         // Rank of each staged key among the staged keys (keys are distinct):
                         // a scalar loop over the staged lanes, no cross-lane sort network.
-                        int r = 0;
                         const uint32_t klo = (uint32_t)key, khi = (uint32_t)(key >> 32);
                         for (uint64_t t = km; t; t &= t - 1) {
                             const int L = __ffsll((unsigned long long)t) - 1;
@@ -431,102 +518,109 @@ __global__ __launch_bounds__(64) void hnsw_search_kernel(SearchParams P) {
                         // insertion points in rank order: lane r holds the point of rank r (the
                         // other lanes park their value in lanes m..63)
                         const int dst = ok ? r : m + lanes_below(~km, lane);
-                        const int spos = __builtin_amdgcn_ds_permute(dst * 4, pos);
-                        const int first_pos = __builtin_amdgcn_readlane(spos, 0);
-                        // Move beam entries [first_pos, n) right, highest chunks first, eight
-                        // chunks per batch with every read of a batch issued before its writes.
-                        // Entry i moves by the number of staged insertion points <= i.  A batch
-                        // writes only at or above its lowest chunk base, which the next (lower)
-                        // batch never reads, and a wave's LDS operations execute in order.
-                        const int lim = FILT ? (int)cap : ef_eff;  // entries kept by position
-                        const int new_n = n + m < lim ? n + m : lim;
-                        ovf |= FILT && n + m > lim;
-                        const int hi_chunk = (n - 1) & ~63;
-                        const int lo_chunk = first_pos & ~63;
-                        // Reads are unconditional (indices below cap, clamped at 0); a lane whose
-                        // entry does not move writes to its own 8-byte slot of the distance-staging
-                        // area (sd and si, 512 contiguous bytes holding no live data during the
-                        // merge), so no lane branches.
-                        const int trash = (int)(reinterpret_cast<uint64_t*>(sd) - bk) + lane;
-                        // the rank lanes of spos (the insertion points, non-decreasing in rank)
-                        const uint64_t rank_lanes = m >= 64 ? ~0ull : ((1ull << m) - 1ull);
-                        for (int b0 = hi_chunk; b0 >= lo_chunk; b0 -= 8 * 64) {
-                            uint64_t vk[8];
-                            int sh[8];
-                            const int base = b0 + lane;
-#pragma unroll
-                            for (int u = 0; u < 8; u++) {
-                                const int i = base - 64 * u;
-                                vk[u] = bk[i > 0 ? i : 0];
-                            }
-                            // shift of entry i = insertion points <= i.  Per chunk [cb, cb + 64):
-                            // lo points lie at or below cb and move the whole chunk; only the
-                            // points inside the chunk (ranks lo..hi-1) are compared per lane
-#pragma unroll
-                            for (int u = 0; u < 8; u++) {
-                                const int cb = b0 - 64 * u;
-                                const int lo = __popcll(__builtin_amdgcn_ballot_w64(spos <= cb) & rank_lanes);
-                                const int hi = __popcll(__builtin_amdgcn_ballot_w64(spos <= cb + 63) & rank_lanes);
-                                int sv = lo;
-                                for (int rr = lo; rr < hi; rr++)
-                                    sv += __builtin_amdgcn_readlane(spos, rr) <= base - 64 * u ? 1 : 0;
-                                sh[u] = sv;
-                            }
-                            __asm__ volatile("" ::: "memory");
-#pragma unroll
-                            for (int u = 0; u < 8; u++) {
-                                const int i = base - 64 * u;
-                                const int np = i + sh[u];
-                                const bool w = (b0 - 64 * u >= lo_chunk) & (i >= first_pos) & (i < n) & (np < lim);
-                                bk[w ? np : trash] = vk[u];
-                            }
-                            __asm__ volatile("" ::: "memory");
-                        }
-                        const int fp = pos + r;
-                        if (ok && fp < lim) bk[fp] = key | ((FILT && (idf & DFLAG)) ? BD : 0ull);
-                        const int min_fp = first_pos;
-                        // Every position below min(first insert, p) holds an old entry whose old
-                        // index was below p, hence already expanded.
-                        scan_lo = min_fp < p ? min_fp : p;
-                        if (STOP || TRACE) {
-                            int kk = kq < new_n ? kq : new_n;
-                            if (min_fp < kk) {
-                                st.improved_recent |= 1;
-                                st.last_change = H0;
-                            }
-                            st.win_pos = min_fp < st.win_pos ? min_fp : st.win_pos;
-                        }
-                        n = new_n;
-                        __syncthreads();
-                        if (FILT) {
-                            nallowed += __popcll(ballot(ok && (idf & DFLAG) == 0u));
-                            if (nallowed >= ef_eff) {
-                                // end the beam right after the ef-th allowed entry: drop the last
-                                // (nallowed - ef) allowed entries and whatever lies beyond
-                                int remaining = nallowed - ef_eff, newn = n;
-                                for (int b = (n - 1) & ~63; b >= 0; b -= 64) {
-                                    const int i = b + lane;
-                                    const uint64_t am = ballot(i < n && (bk[i] & BD) == 0u);
-                                    const int c = __popcll(am);
-                                    if (c > remaining) {
-                                        uint64_t t = am;
-                                        for (int j = 1; j < c - remaining; j++) t &= t - 1;
-                                        newn = b + __ffsll((unsigned long long)t);
-                                        break;
-                                    }
-                                    remaining -= c;
-                                }
-                                n = newn;
-                                nallowed = ef_eff;
-                                scan_lo = scan_lo < n ? scan_lo : n;
-                            }
-                        }
-                    } else {
-                        scan_lo = p;
+                        spos = __builtin_amdgcn_ds_permute(dst * 4, pos);
+                        first_pos = __builtin_amdgcn_readlane(spos, 0);
+                        const int L0 = __ffsll((unsigned long long)ballot(ok && r == 0)) - 1;
+                        kmin = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)khi, L0) << 32) |
+                               (uint32_t)__builtin_amdgcn_readlane((int)klo, L0);
                     }
-                } else {
-                    scan_lo = p;
                 }
+            }
+            // the next expansion is node2 or the smallest new key, whichever is smaller: its row
+            // (or, pipelined, its neighbours' rows) loads while the beam moves
+            if (g.maxM0 <= 32) plan_next(kmin < k2 ? bid(kmin) : node2, node2);
+            if (m > 0) {
+                // Move beam entries [first_pos, n) right, highest chunks first, eight chunks per
+                // batch with every read of a batch issued before its writes.  Entry i moves by the
+                // number of staged insertion points <= i.  A batch writes only at or above its
+                // lowest chunk base, which the next (lower) batch never reads, and a wave's LDS
+                // operations execute in order.
+                const int lim = FILT ? (int)cap : ef_eff;  // entries kept by position
+                const int new_n = n + m < lim ? n + m : lim;
+                ovf |= FILT && n + m > lim;
+                const int hi_chunk = (n - 1) & ~63;
+                const int lo_chunk = first_pos & ~63;
+                // Reads are unconditional (indices below cap, clamped at 0); a lane whose entry
+                // does not move writes to its own 8-byte slot of the distance-staging area (sd and
+                // si, 512 contiguous bytes holding no live data during the merge), so no lane
+                // branches.
+                const int trash = (int)(reinterpret_cast<uint64_t*>(sd) - bk) + lane;
+                // the rank lanes of spos (the insertion points, non-decreasing in rank)
+                const uint64_t rank_lanes = m >= 64 ? ~0ull : ((1ull << m) - 1ull);
+                for (int b0 = hi_chunk; b0 >= lo_chunk; b0 -= 8 * 64) {
+                    uint64_t vk[8];
+                    int mv[8];
+                    const int base = b0 + lane;
+#pragma unroll
+                    for (int u = 0; u < 8; u++) {
+                        const int i = base - 64 * u;
+                        vk[u] = bk[i > 0 ? i : 0];
+                    }
+                    // shift of entry i = insertion points <= i.  Per chunk [cb, cb + 64): lo
+                    // points lie at or below cb and move the whole chunk; only the points inside
+                    // the chunk (ranks lo..hi-1) are compared per lane
+#pragma unroll
+                    for (int u = 0; u < 8; u++) {
+                        const int cb = b0 - 64 * u;
+                        const int lo = __popcll(__builtin_amdgcn_ballot_w64(spos <= cb) & rank_lanes);
+                        const int hi = __popcll(__builtin_amdgcn_ballot_w64(spos <= cb + 63) & rank_lanes);
+                        int sv = lo;
+                        for (int rr = lo; rr < hi; rr++)
+                            sv += __builtin_amdgcn_readlane(spos, rr) <= base - 64 * u ? 1 : 0;
+                        mv[u] = sv;
+                    }
+                    __asm__ volatile("" ::: "memory");
+#pragma unroll
+                    for (int u = 0; u < 8; u++) {
+                        const int i = base - 64 * u;
+                        const int np = i + mv[u];
+                        const bool w = (b0 - 64 * u >= lo_chunk) & (i >= first_pos) & (i < n) & (np < lim);
+                        bk[w ? np : trash] = vk[u];
+                    }
+                    __asm__ volatile("" ::: "memory");
+                }
+                const int fp = pos + r;
+                if (ok && fp < lim) bk[fp] = key | ((FILT && (idf & DFLAG)) ? BD : 0ull);
+                // the shift wrote over the stage: the next expansion's fresh ids again
+                if (pre && pre_id != SENT) si[pre_pos] = pre_id;
+                const int min_fp = first_pos;
+                // Every position below min(first insert, p) holds an old entry whose old index
+                // was below p, hence already expanded.
+                scan_lo = min_fp < p ? min_fp : p;
+                if (STOP || TRACE) {
+                    int kk = kq < new_n ? kq : new_n;
+                    if (min_fp < kk) {
+                        st.improved_recent |= 1;
+                        st.last_change = H0;
+                    }
+                    st.win_pos = min_fp < st.win_pos ? min_fp : st.win_pos;
+                }
+                n = new_n;
+                __syncthreads();
+                if (FILT) {
+                    nallowed += __popcll(ballot(ok && (idf & DFLAG) == 0u));
+                    if (nallowed >= ef_eff) {
+                        // end the beam right after the ef-th allowed entry: drop the last
+                        // (nallowed - ef) allowed entries and whatever lies beyond
+                        int remaining = nallowed - ef_eff, newn = n;
+                        for (int b = (n - 1) & ~63; b >= 0; b -= 64) {
+                            const int i = b + lane;
+                            const uint64_t am = ballot(i < n && (bk[i] & BD) == 0u);
+                            const int c = __popcll(am);
+                            if (c > remaining) {
+                                uint64_t t = am;
+                                for (int j = 1; j < c - remaining; j++) t &= t - 1;
+                                newn = b + __ffsll((unsigned long long)t);
+                                break;
+                            }
+                            remaining -= c;
+                        }
+                        n = newn;
+                        nallowed = ef_eff;
+                        scan_lo = scan_lo < n ? scan_lo : n;
+                    }
+                }
+                head = beam_head(bk, n, sh, lane);
             } else {
                 scan_lo = p;
             }

@@ -100,6 +100,92 @@ __device__ __forceinline__ void dist_batch(const DevGraph& g, const float* qs, c
     }
 }
 
+// dist_batch's 8-bit path split in two for c <= 32 ids, so that the row loads can be issued
+// well before the distances are needed: issue reads the ids of the lane's two slots (grp and
+// grp + 16) and starts the loads; finish computes the distances and writes out[] (an inactive slot
+// reads row 0 and its result is dropped).
+template <int NCH>
+__device__ __forceinline__ void dist_u8_issue(const DevGraph& g, const uint32_t* ids, int c, int lane,
+                                              uint32_t (&w0)[NCH], uint32_t (&w1)[NCH]) {
+    const int grp = lane >> 2, sub = lane & 3;
+    const uint32_t id0 = grp < c ? ids[grp] : 0u, id1 = grp + 16 < c ? ids[grp + 16] : 0u;
+    load_u8_words<NCH>(g.vec8 + (size_t)id0 * (16 * NCH) + sub * 4 * NCH, w0);
+    load_u8_words<NCH>(g.vec8 + (size_t)id1 * (16 * NCH) + sub * 4 * NCH, w1);
+}
+// Exact integer form of an 8-bit row's distance to a query whose coordinates are integers in
+// 0..255 as well (qw: the lane's query words in the row layout, sq = sum of the query's squares).
+// Every step of the FP32 computation is then exact: a difference is an integer of magnitude at
+// most 255, its square at most 65,025, and every partial sum of at most 16 * NCH <= 256 such
+// terms stays below 2^24, where FP32 holds integers exactly (the same for the products and sums
+// of the inner product).  So the FP32 distance equals the integer sum(q - x)^2 = sum q^2 +
+// sum x^2 - 2 sum q.x (L2), resp. 1 - sum q.x (IP) with that integer converted exactly, and four
+// byte products per instruction give it bit for bit.
+template <int SPACE, int NCH>
+__device__ __forceinline__ float dist_group16_u8int(const uint32_t (&w)[NCH], const uint32_t (&qw)[NCH], int sq) {
+    static_assert(16 * NCH <= 256, "partial sums must stay below 2^24");
+    uint32_t qx = 0u, xx = 0u;
+#pragma unroll
+    for (int t = 0; t < NCH; t++) {
+        qx = __builtin_amdgcn_udot4(qw[t], w[t], qx, false);
+        if (SPACE == 0) xx = __builtin_amdgcn_udot4(w[t], w[t], xx, false);
+    }
+    int v = SPACE == 0 ? (int)xx - 2 * (int)qx : (int)qx;
+    // the four lanes of the group (quad permutations [1,0,3,2] and [2,3,0,1])
+    v += __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xF, 0xF, false);
+    v += __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xF, 0xF, false);
+    return SPACE == 0 ? (float)(v + sq) : 1.0f - (float)v;
+}
+
+// The query's words in the 8-bit row layout for this lane (coordinates 16t+4g..16t+4g+3 of
+// lane g = lane & 3) and sum of its squares; returns whether every coordinate of the query is
+// an integer in 0..255 with exactly that FP32 value (uniform over the wave).
+template <int NCH>
+__device__ __forceinline__ bool query_u8_words(const float* qs, int lane, uint32_t (&qw)[NCH], int& sq) {
+    const int g = lane & 3;
+    bool ok = true;
+#pragma unroll
+    for (int t = 0; t < NCH; t++) {
+        const float4 q = *reinterpret_cast<const float4*>(qs + 16 * t + 4 * g);
+        const float qv[4] = {q.x, q.y, q.z, q.w};
+        uint32_t w = 0u;
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            const bool in = qv[j] >= 0.f && qv[j] <= 255.f;
+            const uint32_t b = in ? (uint32_t)qv[j] : 0u;
+            ok &= in && __float_as_uint((float)b) == __float_as_uint(qv[j]);
+            w |= b << (8 * j);
+        }
+        qw[t] = w;
+    }
+    uint32_t s = 0u;
+#pragma unroll
+    for (int t = 0; t < NCH; t++) s = __builtin_amdgcn_udot4(qw[t], qw[t], s, false);
+    int v = (int)s;
+    v += __builtin_amdgcn_update_dpp(0, v, 0xB1, 0xF, 0xF, false);
+    v += __builtin_amdgcn_update_dpp(0, v, 0x4E, 0xF, 0xF, false);
+    sq = v;
+    return __builtin_amdgcn_ballot_w64(!ok) == 0ull;
+}
+
+template <int SPACE, int NCH>
+__device__ __forceinline__ void dist_u8_finish(const float* qs, int c, const uint32_t (&w0)[NCH],
+                                               const uint32_t (&w1)[NCH], float* out, int lane, bool qint,
+                                               const uint32_t (&qw)[NCH], int sq) {
+    const int grp = lane >> 2, sub = lane & 3;
+    float r0, r1;
+    if (qint) {
+        r0 = dist_group16_u8int<SPACE, NCH>(w0, qw, sq);
+        r1 = dist_group16_u8int<SPACE, NCH>(w1, qw, sq);
+    } else {
+        r0 = dist_group16_u8<SPACE, NCH>(w0, qs, sub, lane);
+        r1 = dist_group16_u8<SPACE, NCH>(w1, qs, sub, lane);
+    }
+    if (sub == 0) {
+        if (grp < c) out[grp] = r0;
+        if (grp + 16 < c) out[grp + 16] = r1;
+    }
+}
+
 // ------------------------------------------------------------------------------------------
 // Wave helpers
 // ------------------------------------------------------------------------------------------
@@ -159,6 +245,42 @@ __device__ __forceinline__ int beam_lower_bound64(const uint64_t* bk, int n, uin
         pos += lt ? step : 0;
     }
     return pos;
+}
+
+// Segment heads of a beam: the beam is cut into segments of 2^sh entries (2^sh * 64 >= the most
+// entries it can hold), and lane c keeps the flag-free key of entry c << sh (~0 past the end), so
+// that locating a key among the segments takes register compares instead of LDS probes.
+__device__ __forceinline__ uint64_t beam_head(const uint64_t* bk, int n, int sh, int lane) {
+    const int i = lane << sh;
+    const uint64_t v = bk[i < n ? i : 0];
+    return i < n ? (v & ~BFLAGS) : ~0ull;
+}
+
+// Number of beam entries whose key is below the lane's key, for every lane of `lanes` (other
+// lanes return 0): the segment from the heads (one ballot per key of `lanes`, in registers), then
+// a fixed-step search of sh steps inside it.  Equals beam_lower_bound64.
+__device__ __forceinline__ int beam_lower_bound_heads(const uint64_t* bk, int n, int sh, uint64_t head,
+                                                      uint64_t key, uint64_t lanes, int lane) {
+    int seg = 0;  // heads below the key: entry (seg - 1) << sh is the last head below it
+    const uint32_t klo = (uint32_t)key, khi = (uint32_t)(key >> 32);
+    for (uint64_t t = lanes; t; t &= t - 1) {
+        const int L = __ffsll((unsigned long long)t) - 1;
+        const uint64_t kl = ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)khi, L) << 32) |
+                            (uint32_t)__builtin_amdgcn_readlane((int)klo, L);
+        const int c = __popcll(__builtin_amdgcn_ballot_w64(head < kl));
+        seg = lane == L ? c : seg;
+    }
+    // bk[base] < key; the count lies in (base, base + 2^sh]
+    const int base = (seg > 0 ? seg - 1 : 0) << sh;
+    const int lim = base + (1 << sh) < n ? base + (1 << sh) : n;
+    int q = base;
+    for (int step = (1 << sh) >> 1; step > 0; step >>= 1) {
+        const int probe = q + step;
+        const int pc = probe < lim ? probe : base;
+        const bool lt = (probe < lim) & ((bk[pc] & ~BFLAGS) < key);
+        q += lt ? step : 0;
+    }
+    return seg > 0 ? q + 1 : 0;
 }
 
 // Number of beam entries with key < (d, id), over beam[0, n): a fixed-step lower bound (the
