@@ -281,6 +281,7 @@ __global__ __launch_bounds__(64) void hnsw_search_kernel(SearchParams P) {
         int sh = 0;
         while ((64 << sh) < beam_max) sh++;
         uint64_t head = beam_head(bk, n, sh, lane);
+        uint64_t worst = bk[n - 1] & ~BFLAGS;  // the last beam entry's key (the admission bound)
         int scan_lo = 0;
         StopState st;
         st.d_entry = curdist;
@@ -334,23 +335,22 @@ __global__ __launch_bounds__(64) void hnsw_search_kernel(SearchParams P) {
         // start of this expansion, with no new key before it), its neighbours are checked against
         // the visited hash and their row loads issued then, and the next expansion starts from
         // them (pre): the loads run while the beam moves.  The visited hash is not touched by the
-        // merge, so checking the neighbours early finds the same fresh set.  The fresh ids are
-        // staged again after the shift, which overwrites the stage.
+        // merge, so checking the neighbours early finds the same fresh set.  In this kernel the
+        // batch moves between lanes by permutes, not through the LDS stage: lane c holds the id of
+        // fresh slot c (bid_c) and, after the distances, its distance.
         constexpr bool PIPE = U8 && !STOP && !TRACE && !FILT;
         const bool pipe = PIPE && g.maxM0 <= 32;
         bool pre = false;
         int pre_cnt = 0;
         uint32_t pw0[NCH > 0 ? NCH : 1], pw1[NCH > 0 ? NCH : 1];
-        uint32_t pre_id = SENT;  // lanes 0..31: the staged fresh id of this lane (SENT if none)
-        int pre_pos = 0;         // ... and its stage slot
-        // stages the fresh ids of a row (lanes 0..31 hold its ids, SENT for none) into si
+        uint32_t bid_c = SENT;
+        // the fresh ids of a row (lanes 0..31 hold its ids, SENT for none), compacted in list
+        // order onto lanes 0..cnt-1 (lanes that are not fresh send theirs to lanes 32..63)
         auto stage_row = [&](uint32_t id) {
             const bool fresh = id != SENT && visit(hash, hmask, id);
             const uint64_t m = ballot(fresh);
-            const int pos = lanes_below(m, lane);
-            if (fresh) si[pos] = id;
-            pre_id = fresh ? id : SENT;
-            pre_pos = pos;
+            const int dst = fresh ? lanes_below(m, lane) : 32 + (lane & 31);
+            bid_c = (uint32_t)__builtin_amdgcn_ds_permute(dst * 4, (int)id);
             return __popcll(m);
         };
         // at the decision point of an expansion: the next one is `next`; when it is node2 with its
@@ -368,9 +368,8 @@ __global__ __launch_bounds__(64) void hnsw_search_kernel(SearchParams P) {
             prefetch_row(next);
             if (stage) {
                 pre_cnt = stage_row((lane < 32 && lane < (int)g.maxM0) ? row : SENT);
-                __syncthreads();
                 if constexpr (PIPE) {
-                    if (pre_cnt > 0) dist_u8_issue<NCH>(g, si, pre_cnt, lane, pw0, pw1);
+                    if (pre_cnt > 0) dist_u8_issue_lanes<NCH>(g, bid_c, pre_cnt, lane, pw0, pw1);
                 }
                 pre = true;
                 pf_node = SENT;
@@ -381,24 +380,26 @@ __global__ __launch_bounds__(64) void hnsw_search_kernel(SearchParams P) {
             // the next expansion unless a new key of this expansion lands before it
             int p = -1;
             uint32_t node2 = SENT;
-            uint64_t k2 = ~0ull;
+            uint64_t k2 = ~0ull, bkp = 0;
             for (int b = scan_lo; b < n; b += 64) {
                 int i = b + lane;
                 const int ic = i < n ? i : n - 1;  // every lane reads; the ballot drops i >= n
-                const bool un = (i < n) & ((bk[ic] & BX) == 0);
+                const uint64_t e = bk[ic];
+                const bool un = (i < n) & ((e & BX) == 0);
                 uint64_t m = ballot(un);
                 if (m) {
-                    p = b + __ffsll((unsigned long long)m) - 1;
+                    const int l1 = __ffsll((unsigned long long)m) - 1;
+                    p = b + l1;
+                    bkp = readlane64(e, l1);
                     const uint64_t m2 = m & (m - 1);
                     if (m2) {
-                        k2 = bk[b + __ffsll((unsigned long long)m2) - 1] & ~BFLAGS;
+                        k2 = readlane64(e, __ffsll((unsigned long long)m2) - 1) & ~BFLAGS;
                         node2 = bid(k2);
                     }
                     break;
                 }
             }
             if (p < 0 || H0 >= hcap) break;
-            const uint64_t bkp = bk[p];
             if (FILT && P.eps_mode) {
                 // EpsilonSearchStopCondition::should_stop_search (stop_condition.h:232-243); a
                 // disallowed entry point is the reference's candidate at FLT_MAX
@@ -469,15 +470,16 @@ __global__ __launch_bounds__(64) void hnsw_search_kernel(SearchParams P) {
                     if (P.del_bits) fw_del = P.del_bits[fid >> 5];
                     if (P.allow_bits) fw_al = P.allow_bits[fid >> 5];
                 }
+                float dc = 0.f;  // pipe: the distance of fresh slot `lane`
                 if (PIPE && pipe) {
                     if constexpr (PIPE) {
-                        if (!staged) dist_u8_issue<NCH>(g, si, cnt, lane, pw0, pw1);
-                        dist_u8_finish<SPACE, NCH>(qs, cnt, pw0, pw1, sd, lane, qint, qw, qsq);
+                        if (!staged) dist_u8_issue_lanes<NCH>(g, bid_c, cnt, lane, pw0, pw1);
+                        dc = dist_u8_lanes<SPACE, NCH>(qs, pw0, pw1, lane, qint, qw, qsq);
                     }
                 } else {
                     dist_batch<SPACE, MODE, NCH, U8>(g, qs, si, cnt, sd, lane);
+                    __syncthreads();
                 }
-                __syncthreads();
                 if (FILT) {
                     if (lane < cnt) {
                         const uint32_t fid = si[lane];
@@ -488,10 +490,10 @@ __global__ __launch_bounds__(64) void hnsw_search_kernel(SearchParams P) {
                 E0 += cnt;
                 // admission against the current worst (when full)
                 if (lane < cnt) {
-                    idf = si[lane];
-                    key = bkey(sd[lane], idf & ID_MASK);
+                    idf = (PIPE && pipe) ? bid_c : si[lane];
+                    key = bkey((PIPE && pipe) ? dc : sd[lane], idf & ID_MASK);
                     const bool full = FILT ? nallowed >= ef_eff : n >= ef_eff;
-                    if (full && !(key < (bk[n - 1] & ~BFLAGS))) key = ~0ull;
+                    if (full && !(key < worst)) key = ~0ull;
                 }
                 const uint64_t live = ballot(key != ~0ull);
                 if (live) {
@@ -581,8 +583,6 @@ __global__ __launch_bounds__(64) void hnsw_search_kernel(SearchParams P) {
                 }
                 const int fp = pos + r;
                 if (ok && fp < lim) bk[fp] = key | ((FILT && (idf & DFLAG)) ? BD : 0ull);
-                // the shift wrote over the stage: the next expansion's fresh ids again
-                if (pre && pre_id != SENT) si[pre_pos] = pre_id;
                 const int min_fp = first_pos;
                 // Every position below min(first insert, p) holds an old entry whose old index
                 // was below p, hence already expanded.
@@ -621,6 +621,7 @@ __global__ __launch_bounds__(64) void hnsw_search_kernel(SearchParams P) {
                     }
                 }
                 head = beam_head(bk, n, sh, lane);
+                worst = bk[n - 1] & ~BFLAGS;
             } else {
                 scan_lo = p;
             }

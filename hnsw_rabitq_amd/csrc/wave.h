@@ -100,18 +100,21 @@ __device__ __forceinline__ void dist_batch(const DevGraph& g, const float* qs, c
     }
 }
 
-// dist_batch's 8-bit path split in two for c <= 32 ids, so that the row loads can be issued
-// well before the distances are needed: issue reads the ids of the lane's two slots (grp and
-// grp + 16) and starts the loads; finish computes the distances and writes out[] (an inactive slot
-// reads row 0 and its result is dropped).
+// dist_batch's 8-bit path split in two for c <= 32 ids, so that the row loads can be issued well
+// before the distances are needed: the issue step starts the loads of a lane's two slots (grp and
+// grp + 16; an inactive slot reads row 0 and its result is dropped), dist_u8_lanes computes.
+// The ids are on lanes (lane s holds the id of slot s, s < c): each lane fetches its two slots'
+// ids by permutes.
 template <int NCH>
-__device__ __forceinline__ void dist_u8_issue(const DevGraph& g, const uint32_t* ids, int c, int lane,
-                                              uint32_t (&w0)[NCH], uint32_t (&w1)[NCH]) {
+__device__ __forceinline__ void dist_u8_issue_lanes(const DevGraph& g, uint32_t ids_on_lanes, int c, int lane,
+                                                    uint32_t (&w0)[NCH], uint32_t (&w1)[NCH]) {
     const int grp = lane >> 2, sub = lane & 3;
-    const uint32_t id0 = grp < c ? ids[grp] : 0u, id1 = grp + 16 < c ? ids[grp + 16] : 0u;
+    const uint32_t a = (uint32_t)__shfl((int)ids_on_lanes, grp), b = (uint32_t)__shfl((int)ids_on_lanes, grp + 16);
+    const uint32_t id0 = grp < c ? a : 0u, id1 = grp + 16 < c ? b : 0u;
     load_u8_words<NCH>(g.vec8 + (size_t)id0 * (16 * NCH) + sub * 4 * NCH, w0);
     load_u8_words<NCH>(g.vec8 + (size_t)id1 * (16 * NCH) + sub * 4 * NCH, w1);
 }
+
 // Exact integer form of an 8-bit row's distance to a query whose coordinates are integers in
 // 0..255 as well (qw: the lane's query words in the row layout, sq = sum of the query's squares).
 // Every step of the FP32 computation is then exact: a difference is an integer of magnitude at
@@ -167,11 +170,12 @@ __device__ __forceinline__ bool query_u8_words(const float* qs, int lane, uint32
     return __builtin_amdgcn_ballot_w64(!ok) == 0ull;
 }
 
+// The distances of slots grp and grp + 16 (every lane of a group holds its group's two), moved so
+// that lane c returns the distance of slot c (c < 32; lanes 32..63 return slot c - 32's again).
 template <int SPACE, int NCH>
-__device__ __forceinline__ void dist_u8_finish(const float* qs, int c, const uint32_t (&w0)[NCH],
-                                               const uint32_t (&w1)[NCH], float* out, int lane, bool qint,
-                                               const uint32_t (&qw)[NCH], int sq) {
-    const int grp = lane >> 2, sub = lane & 3;
+__device__ __forceinline__ float dist_u8_lanes(const float* qs, const uint32_t (&w0)[NCH], const uint32_t (&w1)[NCH],
+                                               int lane, bool qint, const uint32_t (&qw)[NCH], int sq) {
+    const int sub = lane & 3;
     float r0, r1;
     if (qint) {
         r0 = dist_group16_u8int<SPACE, NCH>(w0, qw, sq);
@@ -180,10 +184,9 @@ __device__ __forceinline__ void dist_u8_finish(const float* qs, int c, const uin
         r0 = dist_group16_u8<SPACE, NCH>(w0, qs, sub, lane);
         r1 = dist_group16_u8<SPACE, NCH>(w1, qs, sub, lane);
     }
-    if (sub == 0) {
-        if (grp < c) out[grp] = r0;
-        if (grp + 16 < c) out[grp + 16] = r1;
-    }
+    const int src = 4 * (lane & 15);
+    const float a = __shfl(r0, src), b = __shfl(r1, src);
+    return (lane & 16) ? b : a;
 }
 
 // ------------------------------------------------------------------------------------------
@@ -192,6 +195,12 @@ __device__ __forceinline__ void dist_u8_finish(const float* qs, int c, const uin
 __device__ __forceinline__ uint64_t ballot(bool p) { return __ballot(p); }
 __device__ __forceinline__ int lanes_below(uint64_t m, int lane) {
     return __popcll(m & ((lane == 0) ? 0ull : (~0ull >> (64 - lane))));
+}
+
+// Lane L's 64-bit value (L uniform).
+__device__ __forceinline__ uint64_t readlane64(uint64_t v, int L) {
+    return ((uint64_t)(uint32_t)__builtin_amdgcn_readlane((int)(v >> 32), L) << 32) |
+           (uint32_t)__builtin_amdgcn_readlane((int)(uint32_t)v, L);
 }
 
 // Visited hash.  A forgotten node costs one re-evaluation (its key is then either in the beam,
