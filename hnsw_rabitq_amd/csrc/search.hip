@@ -130,6 +130,309 @@ __device__ __noinline__ float mlp_logit(const DevStopModel& m, const float* feat
     return o + w3[H];
 }
 
+// ------------------------------------------------------------------------------------------
+// Level-0 search of the plain kernel on 8-bit rows (maxM0 <= 32): lazy merge and pipelined
+// expansions.
+//
+// The beam is the sorted main list in LDS (bk[0, n)) together with up to 31 pending keys held in
+// registers, one per lane, sorted over lanes 0..P-1 (pk: key with its expanded flag; pp: its
+// insertion point in the main list).  A pending key's rank in the union is its lane plus its
+// insertion point, so the union's ef-th key -- the reference's lowerBound once top_candidates is
+// full -- is either a pending key of rank ef - 1 or the main entry at ef - 1 - (pending keys
+// below it), which lies in the main list's last 64 entries (kept in registers, `tail`).  Keys of
+// rank >= ef are those the reference evicted: pending ones are dropped at once, main ones are
+// never expanded (the search stops at the first unexpanded key above the bound) and fall off at
+// the next merge.  New keys join the pending list by one permute per 32-bit half; the main list is
+// rewritten (the shift of the eager merge) only when 32 keys are pending and at the end, instead
+// of after every expansion that admits a key.  Same admissions, same expansions, same result as
+// the eager merge: the union at every step holds exactly the eager beam plus keys of rank >= ef.
+//
+// Pipelined expansions: when the next expansion is known before this one ends (node2, the second
+// unexpanded key, whose row lanes 32..63 loaded at the start of this expansion, with no new key
+// before it), its neighbours are checked against the visited hash and their row loads issued at
+// once (pre); the next expansion starts from them.  The visited hash is not touched by a merge,
+// so checking them early finds the same fresh set.  The batch moves between lanes by permutes:
+// lane 32 + s holds fresh slot s's id and then its distance.  A query of integers in 0..255 takes
+// the exact integer form of the distances.
+// ------------------------------------------------------------------------------------------
+template <int SPACE, int NCH>
+__device__ __forceinline__ void level0_lazy(const DevGraph& g, const float* qs, uint64_t* bk, uint32_t* hash,
+                                            uint32_t hmask, bool probe, uint64_t* trash_base, int ef_eff, int hcap,
+                                            int lane, int& n_io, int& H0, int& E0) {
+    auto visit = [&](uint32_t id) {
+        return probe ? hash_insert<true>(hash, hmask, id) : hash_insert<false>(hash, hmask, id);
+    };
+    int n = n_io;
+    int sh = 0;  // segments of 2^sh main entries, at most 64 of them
+    while ((64 << sh) < ef_eff) sh++;
+    uint64_t head = 0, tail = 0;
+    int tb = 0;  // tail lane l holds main entry tb + l (flag-free; ~0 past the end)
+    auto load_main_regs = [&]() {
+        head = beam_head(bk, n, sh, lane);
+        tb = n > 64 ? n - 64 : 0;
+        const int i = tb + lane;
+        const uint64_t v = bk[i < n ? i : 0];
+        tail = i < n ? (v & ~BFLAGS) : ~0ull;
+    };
+    load_main_regs();
+    int P = 0;           // pending keys (lanes 0..P-1)
+    uint64_t pk = ~0ull;  // pending key with its expanded flag
+    int pp = 0;          // its insertion point in the main list
+    uint64_t T = ~0ull;  // the union's ef-th key once the union holds ef keys, else ~0
+    auto update_bound = [&]() {
+        if (n + P < ef_eff) {
+            T = ~0ull;
+            return;
+        }
+        const int t = ef_eff - 1, U = lane + pp;
+        const uint64_t eq = ballot(lane < P && U == t);
+        if (eq) {
+            T = readlane64(pk, __ffsll((unsigned long long)eq) - 1) & ~BFLAGS;
+        } else {
+            const int c = __popcll(ballot(lane < P && U < t));
+            T = readlane64(tail, t - c - tb);
+        }
+    };
+    update_bound();
+    int scan_lo = 0;
+    const int trash = (int)(trash_base - bk) + lane;
+    // the pending keys into the main list (the eager merge's shift with the pending list as its
+    // batch: lane i holds the key of rank i and its insertion point)
+    auto flush = [&]() {
+        if (P == 0) return;
+        const int m = P, lim = ef_eff;
+        const int new_n = n + m < lim ? n + m : lim;
+        const int first_pos = __builtin_amdgcn_readlane(pp, 0);
+        const int hi_chunk = (n - 1) & ~63;
+        const int lo_chunk = first_pos & ~63;
+        const uint64_t rank_lanes = (1ull << m) - 1ull;  // m <= 63
+        const int spos = lane < m ? pp : 0x7FFFFFFF;
+        for (int b0 = hi_chunk; b0 >= lo_chunk; b0 -= 8 * 64) {
+            uint64_t vk[8];
+            int mv[8];
+            const int base = b0 + lane;
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const int i = base - 64 * u;
+                vk[u] = bk[i > 0 ? i : 0];
+            }
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const int cb = b0 - 64 * u;
+                const int lo = __popcll(__builtin_amdgcn_ballot_w64(spos <= cb) & rank_lanes);
+                const int hi = __popcll(__builtin_amdgcn_ballot_w64(spos <= cb + 63) & rank_lanes);
+                int sv = lo;
+                for (int rr = lo; rr < hi; rr++) sv += __builtin_amdgcn_readlane(spos, rr) <= base - 64 * u ? 1 : 0;
+                mv[u] = sv;
+            }
+            __asm__ volatile("" ::: "memory");
+#pragma unroll
+            for (int u = 0; u < 8; u++) {
+                const int i = base - 64 * u;
+                const int np = i + mv[u];
+                const bool w = (b0 - 64 * u >= lo_chunk) & (i >= first_pos) & (i < n) & (np < lim);
+                bk[w ? np : trash] = vk[u];
+            }
+            __asm__ volatile("" ::: "memory");
+        }
+        const int fp = pp + lane;
+        if (lane < m && fp < lim) bk[fp] = pk;
+        n = new_n;
+        scan_lo = first_pos < scan_lo ? first_pos : scan_lo;
+        __syncthreads();
+        load_main_regs();
+        P = 0;
+        pk = ~0ull;
+        pp = 0;
+        update_bound();
+    };
+
+    // pipelined expansions
+    bool pre = false;
+    int pre_cnt = 0;
+    uint32_t pw0[NCH], pw1[NCH];
+    uint32_t bid_c = SENT;  // lane 32 + s: the id of fresh slot s
+    uint32_t pf_node = SENT, pf_row = SENT;  // lanes 32..63: the row of pf_node
+    uint32_t qw[NCH];
+    int qsq = 0;
+    const bool qint = query_u8_words<NCH>(qs, lane, qw, qsq);
+    // the fresh ids of a row (lanes 0..31 hold its ids, SENT for none) onto lanes 32.. in list order
+    auto stage_row = [&](uint32_t id) {
+        const bool fresh = id != SENT && visit(id);
+        const uint64_t fm = ballot(fresh);
+        const int dst = fresh ? 32 + lanes_below(fm, lane) : (lane & 31);
+        bid_c = (uint32_t)__builtin_amdgcn_ds_permute(dst * 4, (int)id);
+        return __popcll(fm);
+    };
+
+    for (;;) {
+        // the first unexpanded key of the main list (from scan_lo) and of the pending list, and the
+        // ones after them; node2 = the second unexpanded of the union when it is known
+        int pm = -1;
+        uint64_t xm = ~0ull, xm2 = ~0ull;
+        bool xm2_known = true;
+        for (int b = scan_lo; b < n; b += 64) {
+            const int i = b + lane;
+            const int ic = i < n ? i : n - 1;
+            const uint64_t e = bk[ic];
+            const uint64_t mu = ballot((i < n) & ((e & BX) == 0));
+            if (mu) {
+                const int l1 = __ffsll((unsigned long long)mu) - 1;
+                pm = b + l1;
+                xm = readlane64(e, l1) & ~BFLAGS;
+                const uint64_t m2 = mu & (mu - 1);
+                if (m2) xm2 = readlane64(e, __ffsll((unsigned long long)m2) - 1) & ~BFLAGS;
+                xm2_known = m2 != 0 || b + 64 >= n;
+                break;
+            }
+        }
+        const uint64_t up = ballot(lane < P && (pk & BX) == 0);
+        uint64_t xp = ~0ull, xp2 = ~0ull;
+        int lp = -1;
+        if (up) {
+            lp = __ffsll((unsigned long long)up) - 1;
+            xp = readlane64(pk, lp) & ~BFLAGS;
+            const uint64_t u2 = up & (up - 1);
+            if (u2) xp2 = readlane64(pk, __ffsll((unsigned long long)u2) - 1) & ~BFLAGS;
+        }
+        const bool from_main = xm < xp;
+        const uint64_t x = from_main ? xm : xp;
+        // no unexpanded key, or the first one is beyond the bound (evicted): the search ends
+        if (x == ~0ull || x > T || H0 >= hcap) break;
+        uint64_t k2;
+        if (from_main) {
+            if (lane == 0) bk[pm] = x | BX;
+            scan_lo = pm;
+            k2 = xm2_known ? (xm2 < xp ? xm2 : xp) : ~0ull;
+        } else {
+            if (lane == lp) pk |= BX;
+            k2 = xm < xp2 ? xm : xp2;
+        }
+        const uint32_t node = bid(x), node2 = k2 == ~0ull ? SENT : bid(k2);
+        H0++;
+
+        // neighbours: staged by the previous expansion, or from node's row (prefetched or loaded
+        // now); lanes 32..63 load node2's row
+        const bool staged = pre;
+        pre = false;
+        int cnt;
+        {
+            const int lr = lane & 31;
+            const bool upper = lane >= 32;
+            const uint32_t prev = (uint32_t)__shfl_xor((int)pf_row, 32);
+            const bool need = upper ? node2 != SENT : (!staged && node != pf_node);
+            uint32_t v = SENT;
+            if (need && lr < (int)g.maxM0) v = g.links0[(size_t)(upper ? node2 : node) * g.maxM0 + lr];
+            if (staged) {
+                cnt = pre_cnt;
+            } else {
+                const uint32_t cur = node == pf_node ? prev : v;
+                cnt = stage_row((!upper && lr < (int)g.maxM0) ? cur : SENT);
+            }
+            pf_row = v;
+            pf_node = node2;
+        }
+
+        // distances, admission against the bound, insertion points in the main list, duplicates
+        // (in the main list or pending), ranks; lanes 32..32+cnt-1 hold the batch
+        int m = 0, cntb = 0, cp = 0, pos = 0;
+        bool ok = false;
+        uint64_t key = ~0ull, kmin = ~0ull;
+        if (cnt > 0) {
+            if (!staged) dist_u8_issue_lanes<NCH>(g, bid_c, cnt, lane, pw0, pw1);
+            const float dc = dist_u8_lanes<SPACE, NCH>(qs, pw0, pw1, lane, qint, qw, qsq);
+            E0 += cnt;
+            if (lane >= 32 && lane - 32 < cnt) {
+                key = bkey(dc, bid_c);
+                if (!(key < T)) key = ~0ull;
+            }
+            const uint64_t live = ballot(key != ~0ull);
+            if (live) {
+                // per staged key: its segment of the main list (heads), whether it is pending,
+                // and how many pending keys lie below it
+                int seg = 0;
+                bool dup = false;
+                const uint64_t pkf = lane < P ? (pk & ~BFLAGS) : ~0ull;
+                for (uint64_t t = live; t; t &= t - 1) {
+                    const int L = __ffsll((unsigned long long)t) - 1;
+                    const uint64_t kl = readlane64(key, L);
+                    const int c1 = __popcll(__builtin_amdgcn_ballot_w64(head < kl));
+                    const bool d1 = __builtin_amdgcn_ballot_w64(pkf == kl) != 0ull;
+                    const int c2 = __popcll(__builtin_amdgcn_ballot_w64(pkf < kl));
+                    seg = lane == L ? c1 : seg;
+                    dup = lane == L ? d1 : dup;
+                    cp = lane == L ? c2 : cp;
+                }
+                // the count inside the segment: bk[base] < key, the count lies in (base, base + 2^sh]
+                const int base = (seg > 0 ? seg - 1 : 0) << sh;
+                const int lim = base + (1 << sh) < n ? base + (1 << sh) : n;
+                int q = base;
+                for (int step = (1 << sh) >> 1; step > 0; step >>= 1) {
+                    const int probe_i = q + step;
+                    const int pc = probe_i < lim ? probe_i : base;
+                    const bool lt = (probe_i < lim) & ((bk[pc] & ~BFLAGS) < key);
+                    q += lt ? step : 0;
+                }
+                pos = seg > 0 ? q + 1 : 0;
+                ok = key != ~0ull && !dup;
+                if (ok && pos < n && (bk[pos] & ~BFLAGS) == key) ok = false;
+                const uint64_t km = ballot(ok);
+                m = __popcll(km);
+                if (m > 0) {
+                    // new-key lanes: rank among the new keys; pending lanes: new keys below them
+                    const uint64_t vv = lane < 32 ? pkf : key;
+                    for (uint64_t t = km; t; t &= t - 1) {
+                        const int L = __ffsll((unsigned long long)t) - 1;
+                        cntb += readlane64(key, L) < vv ? 1 : 0;
+                    }
+                    kmin = readlane64(key, __ffsll((unsigned long long)ballot(ok && cntb == 0)) - 1);
+                }
+            }
+        }
+        // the next expansion is node2 or the smallest new key, whichever is smaller; when it is
+        // node2 (known, its row at hand) its neighbours are staged now, else its row is prefetched
+        {
+            const uint32_t next = kmin < k2 ? bid(kmin) : node2;
+            const bool stage = next != SENT && next == node2 && pf_node == node2;
+            const uint32_t row = (uint32_t)__shfl_xor((int)pf_row, 32);
+            uint32_t v = SENT;
+            if (lane >= 32 && !stage && next != SENT && (lane & 31) < (int)g.maxM0)
+                v = g.links0[(size_t)next * g.maxM0 + (lane & 31)];
+            pf_row = v;
+            pf_node = stage ? SENT : next;
+            if (stage) {
+                pre_cnt = stage_row((lane < 32 && lane < (int)g.maxM0) ? row : SENT);
+                if (pre_cnt > 0) dist_u8_issue_lanes<NCH>(g, bid_c, pre_cnt, lane, pw0, pw1);
+                pre = true;
+            }
+        }
+        if (m > 0) {
+            // the new keys join the pending list: pending lane i goes to i + (new keys below it),
+            // a new key to (its rank among the new keys) + (pending keys below it); other lanes
+            // park above
+            const bool src_p = lane < P, src_n = lane >= 32 && ok;
+            const uint64_t srcm = ballot(src_p || src_n);
+            const int dst = src_p ? lane + cntb : (src_n ? cntb + cp : P + m + lanes_below(~srcm, lane));
+            const uint64_t val = src_p ? pk : key;
+            const int vpos = src_p ? pp : pos;
+            const uint32_t lo = (uint32_t)__builtin_amdgcn_ds_permute(dst * 4, (int)(uint32_t)val);
+            const uint32_t hi = (uint32_t)__builtin_amdgcn_ds_permute(dst * 4, (int)(uint32_t)(val >> 32));
+            pp = __builtin_amdgcn_ds_permute(dst * 4, vpos);
+            pk = ((uint64_t)hi << 32) | lo;
+            P += m;
+            // keys of union rank >= ef are evicted (a prefix of the pending list stays)
+            P = __popcll(ballot(lane < P && lane + pp < ef_eff));
+            if (lane >= P) pk = ~0ull;
+            if (P >= 32)
+                flush();
+            else
+                update_bound();
+        }
+    }
+    flush();
+    n_io = n;
+}
+
 // Staged keys up to which the insertion points are located through the segment heads (a loop of
 // one ballot per key) rather than by a full-depth LDS search per lane.
 constexpr int kHeadKeys = 16;
@@ -330,52 +633,16 @@ __global__ __launch_bounds__(64) void hnsw_search_kernel(SearchParams P) {
             pf_row = v;
             pf_node = nd;
         };
-        // Pipelined expansions (the plain kernel on 8-bit rows, maxM0 <= 32): when the next
-        // expansion is known before the beam moves (node2, whose row lanes 32..63 loaded at the
-        // start of this expansion, with no new key before it), its neighbours are checked against
-        // the visited hash and their row loads issued then, and the next expansion starts from
-        // them (pre): the loads run while the beam moves.  The visited hash is not touched by the
-        // merge, so checking the neighbours early finds the same fresh set.  In this kernel the
-        // batch moves between lanes by permutes, not through the LDS stage: lane c holds the id of
-        // fresh slot c (bid_c) and, after the distances, its distance.
-        constexpr bool PIPE = U8 && !STOP && !TRACE && !FILT;
-        const bool pipe = PIPE && g.maxM0 <= 32;
-        bool pre = false;
-        int pre_cnt = 0;
-        uint32_t pw0[NCH > 0 ? NCH : 1], pw1[NCH > 0 ? NCH : 1];
-        uint32_t bid_c = SENT;
-        // the fresh ids of a row (lanes 0..31 hold its ids, SENT for none), compacted in list
-        // order onto lanes 0..cnt-1 (lanes that are not fresh send theirs to lanes 32..63)
-        auto stage_row = [&](uint32_t id) {
-            const bool fresh = id != SENT && visit(hash, hmask, id);
-            const uint64_t m = ballot(fresh);
-            const int dst = fresh ? lanes_below(m, lane) : 32 + (lane & 31);
-            bid_c = (uint32_t)__builtin_amdgcn_ds_permute(dst * 4, (int)id);
-            return __popcll(m);
-        };
-        // at the decision point of an expansion: the next one is `next`; when it is node2 with its
-        // row at hand, stage it and issue its loads (pre), else prefetch its row
-        // a query of integers in 0..255 takes the exact integer form of the distances
-        bool qint = false;
-        uint32_t qw[NCH > 0 ? NCH : 1];
-        int qsq = 0;
-        if constexpr (PIPE) qint = query_u8_words<NCH>(qs, lane, qw, qsq);
-        auto plan_next = [&](uint32_t next, uint32_t node2v) {
-            const bool stage = pipe && next != SENT && next == node2v && pf_node == node2v;
-            const uint32_t row = (uint32_t)__shfl_xor((int)pf_row, 32);  // node2's row on lanes 0..31
-            // one load site per expansion (the row of the next expansion; when it is staged, its
-            // row again, unused), so that the load stays in flight in every case
-            prefetch_row(next);
-            if (stage) {
-                pre_cnt = stage_row((lane < 32 && lane < (int)g.maxM0) ? row : SENT);
-                if constexpr (PIPE) {
-                    if (pre_cnt > 0) dist_u8_issue_lanes<NCH>(g, bid_c, pre_cnt, lane, pw0, pw1);
-                }
-                pre = true;
-                pf_node = SENT;
+        auto plan_next = [&](uint32_t next) { prefetch_row(next); };
+        bool lazy = false;  // the plain kernel on 8-bit rows runs its own level-0 loop
+        if constexpr (U8 && !STOP && !TRACE && !FILT) {
+            if (g.maxM0 <= 32) {
+                level0_lazy<SPACE, NCH>(g, qs, bk, hash, hmask, probe, reinterpret_cast<uint64_t*>(sd), ef_eff, hcap,
+                                        lane, n, H0, E0);
+                lazy = true;
             }
-        };
-        for (;;) {
+        }
+        if (!lazy) for (;;) {
             // first unexpanded entry at or after scan_lo, and the one after it (node2, key k2):
             // the next expansion unless a new key of this expansion lands before it
             int p = -1;
@@ -413,26 +680,7 @@ __global__ __launch_bounds__(64) void hnsw_search_kernel(SearchParams P) {
 
             // neighbours of node: keep those not visited, in list order
             int cnt = 0;
-            const bool staged = pre;  // node's fresh neighbours staged and their loads in flight
-            pre = false;
-            if (pipe) {
-                // lanes 32..63 load node2's row (the next expansion unless a new key lands before
-                // it); lanes 0..31 load node's row unless it was staged or prefetched
-                const int lr = lane & 31;
-                const bool upper = lane >= 32;
-                const uint32_t prev = (uint32_t)__shfl_xor((int)pf_row, 32);
-                const bool need = upper ? node2 != SENT : (!staged && node != pf_node);
-                uint32_t v = SENT;
-                if (need && lr < (int)g.maxM0) v = g.links0[(size_t)(upper ? node2 : node) * g.maxM0 + lr];
-                if (staged) {
-                    cnt = pre_cnt;
-                } else {
-                    const uint32_t cur = node == pf_node ? prev : v;
-                    cnt = stage_row((!upper && lr < (int)g.maxM0) ? cur : SENT);
-                }
-                pf_row = v;
-                pf_node = node2;
-            } else if (g.maxM0 <= 32) {
+            if (g.maxM0 <= 32) {
                 // lanes 0..31 take node's row: from the prefetch when it was this node's, else
                 // loaded now
                 const int lr = lane & 31;
@@ -470,16 +718,8 @@ __global__ __launch_bounds__(64) void hnsw_search_kernel(SearchParams P) {
                     if (P.del_bits) fw_del = P.del_bits[fid >> 5];
                     if (P.allow_bits) fw_al = P.allow_bits[fid >> 5];
                 }
-                float dc = 0.f;  // pipe: the distance of fresh slot `lane`
-                if (PIPE && pipe) {
-                    if constexpr (PIPE) {
-                        if (!staged) dist_u8_issue_lanes<NCH>(g, bid_c, cnt, lane, pw0, pw1);
-                        dc = dist_u8_lanes<SPACE, NCH>(qs, pw0, pw1, lane, qint, qw, qsq);
-                    }
-                } else {
-                    dist_batch<SPACE, MODE, NCH, U8>(g, qs, si, cnt, sd, lane);
-                    __syncthreads();
-                }
+                dist_batch<SPACE, MODE, NCH, U8>(g, qs, si, cnt, sd, lane);
+                __syncthreads();
                 if (FILT) {
                     if (lane < cnt) {
                         const uint32_t fid = si[lane];
@@ -490,8 +730,8 @@ __global__ __launch_bounds__(64) void hnsw_search_kernel(SearchParams P) {
                 E0 += cnt;
                 // admission against the current worst (when full)
                 if (lane < cnt) {
-                    idf = (PIPE && pipe) ? bid_c : si[lane];
-                    key = bkey((PIPE && pipe) ? dc : sd[lane], idf & ID_MASK);
+                    idf = si[lane];
+                    key = bkey(sd[lane], idf & ID_MASK);
                     const bool full = FILT ? nallowed >= ef_eff : n >= ef_eff;
                     if (full && !(key < worst)) key = ~0ull;
                 }
@@ -529,8 +769,8 @@ __global__ __launch_bounds__(64) void hnsw_search_kernel(SearchParams P) {
                 }
             }
             // the next expansion is node2 or the smallest new key, whichever is smaller: its row
-            // (or, pipelined, its neighbours' rows) loads while the beam moves
-            if (g.maxM0 <= 32) plan_next(kmin < k2 ? bid(kmin) : node2, node2);
+            // loads while the beam moves
+            if (g.maxM0 <= 32) plan_next(kmin < k2 ? bid(kmin) : node2);
             if (m > 0) {
                 // Move beam entries [first_pos, n) right, highest chunks first, eight chunks per
                 // batch with every read of a batch issued before its writes.  Entry i moves by the

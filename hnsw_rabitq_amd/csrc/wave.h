@@ -103,13 +103,14 @@ __device__ __forceinline__ void dist_batch(const DevGraph& g, const float* qs, c
 // dist_batch's 8-bit path split in two for c <= 32 ids, so that the row loads can be issued well
 // before the distances are needed: the issue step starts the loads of a lane's two slots (grp and
 // grp + 16; an inactive slot reads row 0 and its result is dropped), dist_u8_lanes computes.
-// The ids are on lanes (lane s holds the id of slot s, s < c): each lane fetches its two slots'
-// ids by permutes.
+// The ids are on lanes (lane 32 + s holds the id of slot s, s < c): each lane fetches its two
+// slots' ids by permutes.
 template <int NCH>
 __device__ __forceinline__ void dist_u8_issue_lanes(const DevGraph& g, uint32_t ids_on_lanes, int c, int lane,
                                                     uint32_t (&w0)[NCH], uint32_t (&w1)[NCH]) {
     const int grp = lane >> 2, sub = lane & 3;
-    const uint32_t a = (uint32_t)__shfl((int)ids_on_lanes, grp), b = (uint32_t)__shfl((int)ids_on_lanes, grp + 16);
+    const uint32_t a = (uint32_t)__shfl((int)ids_on_lanes, 32 + grp),
+                   b = (uint32_t)__shfl((int)ids_on_lanes, 48 + grp);
     const uint32_t id0 = grp < c ? a : 0u, id1 = grp + 16 < c ? b : 0u;
     load_u8_words<NCH>(g.vec8 + (size_t)id0 * (16 * NCH) + sub * 4 * NCH, w0);
     load_u8_words<NCH>(g.vec8 + (size_t)id1 * (16 * NCH) + sub * 4 * NCH, w1);
@@ -171,7 +172,7 @@ __device__ __forceinline__ bool query_u8_words(const float* qs, int lane, uint32
 }
 
 // The distances of slots grp and grp + 16 (every lane of a group holds its group's two), moved so
-// that lane c returns the distance of slot c (c < 32; lanes 32..63 return slot c - 32's again).
+// that lane c returns the distance of slot c & 31 (lanes c and 32 + c the same).
 template <int SPACE, int NCH>
 __device__ __forceinline__ float dist_u8_lanes(const float* qs, const uint32_t (&w0)[NCH], const uint32_t (&w1)[NCH],
                                                int lane, bool qint, const uint32_t (&qw)[NCH], int sq) {
