@@ -104,3 +104,80 @@ def test_u8_filtered_search_equals_f32(hg):
     np.testing.assert_array_equal(la, lb)
     np.testing.assert_array_equal(da.view(np.uint32), db.view(np.uint32))
     np.testing.assert_array_equal(ca[:, 0], cb[:, 0])
+
+
+# The plain kernel on 8-bit rows runs its own level-0 loop (lazy merge: new keys pending in
+# registers, the main list rewritten every 32 keys; pipelined expansions; exact integer
+# distances).  Its results must equal the oracle's at every beam size and k, per-query ef and k,
+# and with a hop cap; the FP32 rows (the eager merge) must give the same.
+@pytest.mark.parametrize("ef,k", [(1, 1), (5, 10), (64, 1), (100, 10), (300, 50), (700, 100)])
+def test_lazy_merge_matches_oracle(hg, orc, ef, k):
+    p, _, data, queries = built_index("c2s")
+    print(f"lazy merge: c2s ef={ef} k={k}", flush=True)
+    g = p.gpu()
+    assert g.vector_code == "u8"
+    (la, da, ca, sa), (lb, db, cb, sb) = search_both(g, queries, k, ef)
+    o = orc.OracleHnsw(blob=p.serialize())
+    lt, dt, ct, _, short = o.search(0, queries, k, ef, mode=orc.MODE_TOTAL, counters=True)
+    assert sa == sb == (short != 0)
+    np.testing.assert_array_equal(la, lt)
+    np.testing.assert_array_equal(da.view(np.uint32), dt.view(np.uint32))
+    np.testing.assert_array_equal(ca[:, 0], ct[:, 0])
+    np.testing.assert_array_equal(ca[:, 2:], ct[:, 2:])
+    np.testing.assert_array_equal(la, lb)
+    np.testing.assert_array_equal(ca[:, 0], cb[:, 0])
+
+
+def test_lazy_merge_fractional_queries(hg, orc):
+    """Queries that are not integers take the FP32 form of the distances on the same rows."""
+    p, _, data, queries = built_index("c2s")
+    q = queries + np.float32(0.25)
+    g = p.gpu()
+    la, da, ca, _ = g.search(q, 10, 80, counters=True)
+    o = orc.OracleHnsw(blob=p.serialize())
+    lt, dt, ct, _, short = o.search(0, q, 10, 80, mode=orc.MODE_TOTAL, counters=True)
+    assert short == 0
+    np.testing.assert_array_equal(la, lt)
+    np.testing.assert_array_equal(da.view(np.uint32), dt.view(np.uint32))
+    np.testing.assert_array_equal(ca[:, 0], ct[:, 0])
+
+
+def test_lazy_merge_per_query_ef_k(hg, orc):
+    from hnsw_rabitq_amd import DeviceArray, datasets
+    p, _, data, queries = built_index("c2s")
+    nq = queries.shape[0]
+    ef, k = datasets.c4_ef_k(nq, seed=11, ef_lo=1, ef_hi=800, ks=(1, 10, 100))
+    kmax = int(k.max())
+    print(f"lazy merge uneven: ef in [{ef.min()},{ef.max()}]", flush=True)
+    g = p.gpu()
+    dq = DeviceArray.from_numpy(queries)
+    def_, dk = DeviceArray.from_numpy(ef), DeviceArray.from_numpy(k)
+    dl, dd = DeviceArray(nq * kmax * 8), DeviceArray(nq * kmax * 4)
+    g.search_device(dq, nq, kmax, int(ef.max()), dl, dd, d_ef=def_, d_k=dk, k_stride=kmax)
+    lab = dl.to_numpy(np.uint64, (nq, kmax))
+    dis = dd.to_numpy(np.float32, (nq, kmax))
+    o = orc.OracleHnsw(blob=p.serialize())
+    ol, od, _, _, _ = o.search(0, queries, kmax, int(ef.max()), mode=orc.MODE_TOTAL, ef_per_query=ef,
+                               k_per_query=k)
+    for i in range(nq):
+        np.testing.assert_array_equal(lab[i, :k[i]], ol[i, :k[i]])
+        np.testing.assert_array_equal(dis[i, :k[i]].view(np.uint32), od[i, :k[i]].view(np.uint32))
+
+
+def test_lazy_merge_hop_cap(hg, orc):
+    from hnsw_rabitq_amd import DeviceArray, _lib
+    p, _, data, queries = built_index("c2s")
+    nq, k, ef = queries.shape[0], 10, 400
+    caps = np.random.default_rng(4).integers(-5, 450, nq).astype(np.int32)
+    g = p.gpu()
+    dq, dcap = DeviceArray.from_numpy(queries), DeviceArray.from_numpy(caps)
+    dl, dd, dc = DeviceArray(nq * k * 8), DeviceArray(nq * k * 4), DeviceArray(nq * 16)
+    _lib.call("hg_gpu_search_device_hopcap", g.handle, dq.ptr, nq, dcap.ptr, ef, k, dl.ptr, dd.ptr, dc.ptr, None)
+    _lib.call("hg_device_synchronize")
+    lab, dis = dl.to_numpy(np.uint64, (nq, k)), dd.to_numpy(np.float32, (nq, k))
+    ctr = dc.to_numpy(np.int32, (nq, 4))
+    o = orc.OracleHnsw(blob=p.serialize())
+    ol, od, oc, _ = o.search_hopcap(0, queries, k, ef, caps, mode=orc.MODE_TOTAL, counters=True)
+    np.testing.assert_array_equal(lab, ol)
+    np.testing.assert_array_equal(dis.view(np.uint32), od.view(np.uint32))
+    np.testing.assert_array_equal(ctr[:, 0], oc[:, 0])
