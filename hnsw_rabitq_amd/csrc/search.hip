@@ -76,6 +76,29 @@ __device__ __forceinline__ float stop_feature(int f, const StopState& st, float 
     }
 }
 
+// The 12 features of the current beam bk[0, n) (sorted, every entry below scan_lo expanded) into
+// feat[0..11].
+__device__ __forceinline__ void beam_features(const uint64_t* bk, int n, int scan_lo, int kq, int ef_eff, int hops,
+                                              const StopState& st, float* feat, int lane) {
+    const int kk = kq < n ? kq : n;
+    const float d1 = bdist(bk[0]), dk = bdist(bk[kk - 1]), dlast = bdist(bk[n - 1]);
+    // the next expansion: first unexpanded entry (the search's stop rule ends the loop when there
+    // is none)
+    float dnext = dlast;
+    int pnext = n;
+    for (int b = scan_lo; b < n; b += 64) {
+        const int i = b + lane;
+        const uint64_t m = ballot(i < n && (bk[i] & BX) == 0);
+        if (m) {
+            pnext = b + __ffsll((unsigned long long)m) - 1;
+            dnext = bdist(bk[pnext]);
+            break;
+        }
+    }
+    const int crowd = beam_lower_bound64(bk, n, bkey(1.2f * dk, 0u));
+    if (lane < kStopFeatures) feat[lane] = stop_feature(lane, st, d1, dk, dlast, dnext, pnext, kq, hops, ef_eff, crowd);
+}
+
 __device__ __forceinline__ float gelu_tanh(float x) {
     const float c = 0.7978845608028654f;
     return 0.5f * x * (1.0f + tanhf(c * (x + 0.044715f * x * x * x)));
@@ -154,11 +177,18 @@ __device__ __noinline__ float mlp_logit(const DevStopModel& m, const float* feat
 // so checking them early finds the same fresh set.  The batch moves between lanes by permutes:
 // lane 32 + s holds fresh slot s's id and then its distance.  A query of integers in 0..255 takes
 // the exact integer form of the distances.
+//
+// STOPV (learned stop): every `period` expansions the pending keys are merged, so that the
+// features are computed from the eager beam itself, and the network decides whether to stop.
+// The union rank of a batch's smallest new key is its eager insertion position, which the
+// near-miss and change features record.
 // ------------------------------------------------------------------------------------------
-template <int SPACE, int NCH>
+template <int SPACE, int NCH, bool STOPV>
 __device__ __forceinline__ void level0_lazy(const DevGraph& g, const float* qs, uint64_t* bk, uint32_t* hash,
                                             uint32_t hmask, bool probe, uint64_t* trash_base, int ef_eff, int hcap,
-                                            int lane, int& n_io, int& H0, int& E0) {
+                                            int lane, int& n_io, int& H0, int& E0, int& scan_lo_io,
+                                            const SearchParams& Pm, StopState& st, int kq, float* feat,
+                                            float* hbuf, bool& stopped) {
     auto visit = [&](uint32_t id) {
         return probe ? hash_insert<true>(hash, hmask, id) : hash_insert<false>(hash, hmask, id);
     };
@@ -194,7 +224,7 @@ __device__ __forceinline__ void level0_lazy(const DevGraph& g, const float* qs, 
         }
     };
     update_bound();
-    int scan_lo = 0;
+    int scan_lo = scan_lo_io;
     const int trash = (int)(trash_base - bk) + lane;
     // the pending keys into the main list (the eager merge's shift with the pending list as its
     // batch: lane i holds the key of rank i and its insertion point)
@@ -335,7 +365,7 @@ __device__ __forceinline__ void level0_lazy(const DevGraph& g, const float* qs, 
 
         // distances, admission against the bound, insertion points in the main list, duplicates
         // (in the main list or pending), ranks; lanes 32..32+cnt-1 hold the batch
-        int m = 0, cntb = 0, cp = 0, pos = 0;
+        int m = 0, cntb = 0, cp = 0, pos = 0, kmin_rank = 0;
         bool ok = false;
         uint64_t key = ~0ull, kmin = ~0ull;
         if (cnt > 0) {
@@ -385,7 +415,9 @@ __device__ __forceinline__ void level0_lazy(const DevGraph& g, const float* qs, 
                         const int L = __ffsll((unsigned long long)t) - 1;
                         cntb += readlane64(key, L) < vv ? 1 : 0;
                     }
-                    kmin = readlane64(key, __ffsll((unsigned long long)ballot(ok && cntb == 0)) - 1);
+                    const int L0 = __ffsll((unsigned long long)ballot(ok && cntb == 0)) - 1;
+                    kmin = readlane64(key, L0);
+                    kmin_rank = __builtin_amdgcn_readlane(pos + cp, L0);
                 }
             }
         }
@@ -423,14 +455,46 @@ __device__ __forceinline__ void level0_lazy(const DevGraph& g, const float* qs, 
             // keys of union rank >= ef are evicted (a prefix of the pending list stays)
             P = __popcll(ballot(lane < P && lane + pp < ef_eff));
             if (lane >= P) pk = ~0ull;
+            if constexpr (STOPV) {
+                // the eager merge's record: the smallest new key's position and whether it
+                // entered the top k of the beam it made
+                const int live_n = n + P < ef_eff ? n + P : ef_eff;
+                const int kk = kq < live_n ? kq : live_n;
+                if (kmin_rank < kk) {
+                    st.improved_recent |= 1;
+                    st.last_change = H0;
+                }
+                st.win_pos = kmin_rank < st.win_pos ? kmin_rank : st.win_pos;
+            }
             if (P >= 32)
                 flush();
             else
                 update_bound();
         }
+        if constexpr (STOPV) {
+            if (H0 % Pm.stop.period == 0) {
+                flush();
+                beam_features(bk, n, scan_lo, kq, ef_eff, H0, st, feat, lane);
+                st.prev_pos[2] = st.prev_pos[1];
+                st.prev_pos[1] = st.prev_pos[0];
+                st.prev_pos[0] = st.win_pos < ef_eff ? st.win_pos : ef_eff;
+                st.win_pos = 1 << 30;
+                __syncthreads();
+                if (H0 >= Pm.stop.min_hops) {
+                    if (lane < kStopFeatures) feat[lane] = (feat[lane] - Pm.stop.mean[lane]) * Pm.stop.inv_std[lane];
+                    __syncthreads();
+                    const float logit = mlp_logit(Pm.stop, feat, hbuf, lane);
+                    if (logit >= Pm.stop.logit_threshold) stopped = true;
+                }
+                __syncthreads();
+            }
+            st.improved_recent <<= 1;
+            if (stopped) break;
+        }
     }
     flush();
     n_io = n;
+    scan_lo_io = scan_lo;
 }
 
 // Staged keys up to which the insertion points are located through the segment heads (a loop of
@@ -602,26 +666,7 @@ __global__ __launch_bounds__(64) void hnsw_search_kernel(SearchParams P) {
         int nrec = 0;
         bool stopped = false;
         // the 12 traversal features of the current state into feat[] (STOP / TRACE)
-        auto make_features = [&]() {
-            int kk = kq < n ? kq : n;
-            float d1 = bdist(bk[0]), dk = bdist(bk[kk - 1]), dlast = bdist(bk[n - 1]);
-            // the next expansion: first unexpanded entry (the search's stop rule ends the loop
-            // when there is none)
-            float dnext = dlast;
-            int pnext = n;
-            for (int b = scan_lo; b < n; b += 64) {
-                int i = b + lane;
-                uint64_t m = ballot(i < n && (bk[i] & BX) == 0);
-                if (m) {
-                    pnext = b + __ffsll((unsigned long long)m) - 1;
-                    dnext = bdist(bk[pnext]);
-                    break;
-                }
-            }
-            const int crowd = beam_lower_bound64(bk, n, bkey(1.2f * dk, 0u));
-            if (lane < kStopFeatures)
-                feat[lane] = stop_feature(lane, st, d1, dk, dlast, dnext, pnext, kq, H0, ef_eff, crowd);
-        };
+        auto make_features = [&]() { beam_features(bk, n, scan_lo, kq, ef_eff, H0, st, feat, lane); };
         const int hcap = (P.hop_cap && P.hop_cap[qi] > 0) ? P.hop_cap[qi] : 0x7FFFFFFF;
         bool ovf = false;  // FILT: the beam overflowed (the result may differ from the reference's)
         // Link-row prefetch (maxM0 <= 32): lanes 32..63 load the row of the node expected to be
@@ -634,11 +679,11 @@ __global__ __launch_bounds__(64) void hnsw_search_kernel(SearchParams P) {
             pf_node = nd;
         };
         auto plan_next = [&](uint32_t next) { prefetch_row(next); };
-        bool lazy = false;  // the plain kernel on 8-bit rows runs its own level-0 loop
-        if constexpr (U8 && !STOP && !TRACE && !FILT) {
+        bool lazy = false;  // the plain and learned kernels on 8-bit rows run their own level-0 loop
+        if constexpr (U8 && !TRACE && !FILT) {
             if (g.maxM0 <= 32) {
-                level0_lazy<SPACE, NCH>(g, qs, bk, hash, hmask, probe, reinterpret_cast<uint64_t*>(sd), ef_eff, hcap,
-                                        lane, n, H0, E0);
+                level0_lazy<SPACE, NCH, STOP>(g, qs, bk, hash, hmask, probe, reinterpret_cast<uint64_t*>(sd), ef_eff,
+                                              hcap, lane, n, H0, E0, scan_lo, P, st, kq, feat, hbuf, stopped);
                 lazy = true;
             }
         }

@@ -75,11 +75,15 @@ def test_adam_step_decreases_loss():
     m.free()
 
 
-def test_stop_model_that_never_fires_is_plain_search(hg):
+@pytest.mark.parametrize("case,ef", [("c1", 50), ("c2s", 300)])
+def test_stop_model_that_never_fires_is_plain_search(hg, case, ef):
+    """FP32 rows (c1) and 8-bit rows (c2s, whose learned kernel merges lazily and merges its
+    pending keys at every evaluation)."""
     from hnsw_rabitq_amd import DeviceArray, learned_stop
-    p, space, data, queries = built_index("c1")
+    p, space, data, queries = built_index(case)
     gpu = p.gpu()
-    nq, k, ef = queries.shape[0], 10, 50
+    assert gpu.vector_code == ("u8" if case == "c2s" else "f32")
+    nq, k = queries.shape[0], 10
     dq = DeviceArray.from_numpy(queries)
     l0, d0 = DeviceArray(nq * k * 8), DeviceArray(nq * k * 4)
     l1, d1 = DeviceArray(nq * k * 8), DeviceArray(nq * k * 4)
