@@ -154,8 +154,8 @@ __device__ __noinline__ float mlp_logit(const DevStopModel& m, const float* feat
 }
 
 // ------------------------------------------------------------------------------------------
-// Level-0 search of the plain kernel on 8-bit rows (maxM0 <= 32): lazy merge and pipelined
-// expansions.
+// Level-0 search of the plain and learned kernels (dim % 16 == 0 with a compiled chunk count,
+// maxM0 <= 32): lazy merge, and on the 8-bit rows pipelined expansions.
 //
 // The beam is the sorted main list in LDS (bk[0, n)) together with up to 31 pending keys held in
 // registers, one per lane, sorted over lanes 0..P-1 (pk: key with its expanded flag; pp: its
@@ -170,7 +170,7 @@ __device__ __noinline__ float mlp_logit(const DevStopModel& m, const float* feat
 // of after every expansion that admits a key.  Same admissions, same expansions, same result as
 // the eager merge: the union at every step holds exactly the eager beam plus keys of rank >= ef.
 //
-// Pipelined expansions: when the next expansion is known before this one ends (node2, the second
+// Pipelined expansions (U8): when the next expansion is known before this one ends (node2, the second
 // unexpanded key, whose row lanes 32..63 loaded at the start of this expansion, with no new key
 // before it), its neighbours are checked against the visited hash and their row loads issued at
 // once (pre); the next expansion starts from them.  The visited hash is not touched by a merge,
@@ -183,7 +183,7 @@ __device__ __noinline__ float mlp_logit(const DevStopModel& m, const float* feat
 // The union rank of a batch's smallest new key is its eager insertion position, which the
 // near-miss and change features record.
 // ------------------------------------------------------------------------------------------
-template <int SPACE, int NCH, bool STOPV>
+template <int SPACE, int NCH, bool STOPV, bool U8>
 __device__ __forceinline__ void level0_lazy(const DevGraph& g, const float* qs, uint64_t* bk, uint32_t* hash,
                                             uint32_t hmask, bool probe, uint64_t* trash_base, int ef_eff, int hcap,
                                             int lane, int& n_io, int& H0, int& E0, int& scan_lo_io,
@@ -280,12 +280,13 @@ __device__ __forceinline__ void level0_lazy(const DevGraph& g, const float* qs, 
     // pipelined expansions
     bool pre = false;
     int pre_cnt = 0;
-    uint32_t pw0[NCH], pw1[NCH];
+    uint32_t pw0[U8 ? NCH : 1], pw1[U8 ? NCH : 1];
     uint32_t bid_c = SENT;  // lane 32 + s: the id of fresh slot s
     uint32_t pf_node = SENT, pf_row = SENT;  // lanes 32..63: the row of pf_node
     uint32_t qw[NCH];
     int qsq = 0;
-    const bool qint = query_u8_words<NCH>(qs, lane, qw, qsq);
+    bool qint = false;
+    if constexpr (U8) qint = query_u8_words<NCH>(qs, lane, qw, qsq);
     // the fresh ids of a row (lanes 0..31 hold its ids, SENT for none) onto lanes 32.. in list order
     auto stage_row = [&](uint32_t id) {
         const bool fresh = id != SENT && visit(id);
@@ -369,8 +370,13 @@ __device__ __forceinline__ void level0_lazy(const DevGraph& g, const float* qs, 
         bool ok = false;
         uint64_t key = ~0ull, kmin = ~0ull;
         if (cnt > 0) {
-            if (!staged) dist_u8_issue_lanes<NCH>(g, bid_c, cnt, lane, pw0, pw1);
-            const float dc = dist_u8_lanes<SPACE, NCH>(qs, pw0, pw1, lane, qint, qw, qsq);
+            float dc;
+            if constexpr (U8) {
+                if (!staged) dist_u8_issue_lanes<NCH>(g, bid_c, cnt, lane, pw0, pw1);
+                dc = dist_u8_lanes<SPACE, NCH>(qs, pw0, pw1, lane, qint, qw, qsq);
+            } else {
+                dc = dist_f32_lanes<SPACE, NCH>(g, qs, bid_c, cnt, lane);
+            }
             E0 += cnt;
             if (lane >= 32 && lane - 32 < cnt) {
                 key = bkey(dc, bid_c);
@@ -425,7 +431,8 @@ __device__ __forceinline__ void level0_lazy(const DevGraph& g, const float* qs, 
         // node2 (known, its row at hand) its neighbours are staged now, else its row is prefetched
         {
             const uint32_t next = kmin < k2 ? bid(kmin) : node2;
-            const bool stage = next != SENT && next == node2 && pf_node == node2;
+            // (the FP32 rows' loads would hold four times the registers across the merge: not staged)
+            const bool stage = U8 && next != SENT && next == node2 && pf_node == node2;
             const uint32_t row = (uint32_t)__shfl_xor((int)pf_row, 32);
             uint32_t v = SENT;
             if (lane >= 32 && !stage && next != SENT && (lane & 31) < (int)g.maxM0)
@@ -434,7 +441,9 @@ __device__ __forceinline__ void level0_lazy(const DevGraph& g, const float* qs, 
             pf_node = stage ? SENT : next;
             if (stage) {
                 pre_cnt = stage_row((lane < 32 && lane < (int)g.maxM0) ? row : SENT);
-                if (pre_cnt > 0) dist_u8_issue_lanes<NCH>(g, bid_c, pre_cnt, lane, pw0, pw1);
+                if constexpr (U8) {
+                    if (pre_cnt > 0) dist_u8_issue_lanes<NCH>(g, bid_c, pre_cnt, lane, pw0, pw1);
+                }
                 pre = true;
             }
         }
@@ -473,14 +482,18 @@ __device__ __forceinline__ void level0_lazy(const DevGraph& g, const float* qs, 
         }
         if constexpr (STOPV) {
             if (H0 % Pm.stop.period == 0) {
-                flush();
-                beam_features(bk, n, scan_lo, kq, ef_eff, H0, st, feat, lane);
+                // the network runs from min_hops on; before that only the windows advance
+                const bool eval = H0 >= Pm.stop.min_hops;
+                if (eval) {
+                    flush();
+                    beam_features(bk, n, scan_lo, kq, ef_eff, H0, st, feat, lane);
+                }
                 st.prev_pos[2] = st.prev_pos[1];
                 st.prev_pos[1] = st.prev_pos[0];
                 st.prev_pos[0] = st.win_pos < ef_eff ? st.win_pos : ef_eff;
                 st.win_pos = 1 << 30;
                 __syncthreads();
-                if (H0 >= Pm.stop.min_hops) {
+                if (eval) {
                     if (lane < kStopFeatures) feat[lane] = (feat[lane] - Pm.stop.mean[lane]) * Pm.stop.inv_std[lane];
                     __syncthreads();
                     const float logit = mlp_logit(Pm.stop, feat, hbuf, lane);
@@ -679,11 +692,11 @@ __global__ __launch_bounds__(64) void hnsw_search_kernel(SearchParams P) {
             pf_node = nd;
         };
         auto plan_next = [&](uint32_t next) { prefetch_row(next); };
-        bool lazy = false;  // the plain and learned kernels on 8-bit rows run their own level-0 loop
-        if constexpr (U8 && !TRACE && !FILT) {
+        bool lazy = false;  // the plain and learned kernels run their own level-0 loop (dim % 16 == 0)
+        if constexpr (MODE == 1 && NCH > 0 && !TRACE && !FILT) {
             if (g.maxM0 <= 32) {
-                level0_lazy<SPACE, NCH, STOP>(g, qs, bk, hash, hmask, probe, reinterpret_cast<uint64_t*>(sd), ef_eff,
-                                              hcap, lane, n, H0, E0, scan_lo, P, st, kq, feat, hbuf, stopped);
+                level0_lazy<SPACE, NCH, STOP, U8>(g, qs, bk, hash, hmask, probe, reinterpret_cast<uint64_t*>(sd), ef_eff,
+                                                  hcap, lane, n, H0, E0, scan_lo, P, st, kq, feat, hbuf, stopped);
                 lazy = true;
             }
         }

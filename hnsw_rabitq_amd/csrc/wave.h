@@ -190,6 +190,30 @@ __device__ __forceinline__ float dist_u8_lanes(const float* qs, const uint32_t (
     return (lane & 16) ? b : a;
 }
 
+// The FP32 rows' counterpart for c <= 32 ids on lanes (lane 32 + s holds the id of slot s): loads
+// and distances of the lane's two slots at once; lane c returns the distance of slot c & 31.
+template <int SPACE, int NCH>
+__device__ __forceinline__ float dist_f32_lanes(const DevGraph& g, const float* qs, uint32_t ids_on_lanes, int c,
+                                                int lane) {
+    const int grp = lane >> 2, sub = lane & 3;
+    const uint32_t a = (uint32_t)__shfl((int)ids_on_lanes, 32 + grp),
+                   b = (uint32_t)__shfl((int)ids_on_lanes, 48 + grp);
+    const uint32_t id0 = grp < c ? a : 0u, id1 = grp + 16 < c ? b : 0u;
+    const float4* x0 = reinterpret_cast<const float4*>(g.vec + (size_t)id0 * g.stride);
+    const float4* x1 = reinterpret_cast<const float4*>(g.vec + (size_t)id1 * g.stride);
+    float4 v0[NCH], v1[NCH];
+#pragma unroll
+    for (int t = 0; t < NCH; t++) v0[t] = x0[4 * t + sub];
+#pragma unroll
+    for (int t = 0; t < NCH; t++) v1[t] = x1[4 * t + sub];
+    __builtin_amdgcn_sched_barrier(0);
+    const float r0 = dist_group16_regs<SPACE, NCH>(v0, qs, sub, lane);
+    const float r1 = dist_group16_regs<SPACE, NCH>(v1, qs, sub, lane);
+    const int src = 4 * (lane & 15);
+    const float ra = __shfl(r0, src), rb = __shfl(r1, src);
+    return (lane & 16) ? rb : ra;
+}
+
 // ------------------------------------------------------------------------------------------
 // Wave helpers
 // ------------------------------------------------------------------------------------------
