@@ -23,6 +23,8 @@
 //
 // Distances follow the reference's FP32 order exactly (see dist_* below); all comparisons use
 // (distance, id), so ties between equal distances resolve by node id.
+#include <algorithm>
+
 #include "common.h"
 #include "dist.h"
 #include "kernels.h"
@@ -1101,30 +1103,34 @@ hipError_t launch_id_labels(const DevGraph& g, const uint32_t* ids, size_t n, ui
     return hipGetLastError();
 }
 
-// One thread per output word: the four coordinates 16t+4g..16t+4g+3 of row r as bytes, placed at
-// word g * (dim/16) + t of the coded row.  A coordinate whose FP32 bits are not those of an
-// integer in 0..255 (a fraction, out of range, -0, NaN) sets the flag.
-__global__ void code_u8_kernel(const float* __restrict__ vec, uint32_t n, uint32_t dim, uint32_t stride,
-                               uint32_t* __restrict__ out, uint32_t* __restrict__ flag) {
+// One output word per thread and step (grid-stride): the four coordinates 16t+4g..16t+4g+3 of row
+// r as bytes, placed at word g * (dim/16) + t of the coded row.  A coordinate whose FP32 bits are
+// not those of an integer in 0..255 (a fraction, out of range, -0, NaN) sets the flag, once per
+// wave that met one.
+__global__ __launch_bounds__(256) void code_u8_kernel(const float* __restrict__ vec, uint32_t n, uint32_t dim,
+                                                      uint32_t stride, uint32_t* __restrict__ out,
+                                                      uint32_t* __restrict__ flag) {
     const uint32_t words = dim / 4, nch = dim / 16;
-    const size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i >= (size_t)n * words) return;
-    const uint32_t r = (uint32_t)(i / words), wi = (uint32_t)(i % words);
-    const uint32_t g = wi / nch, t = wi % nch;
-    const float4 x = *reinterpret_cast<const float4*>(vec + (size_t)r * stride + 16 * t + 4 * g);
-    const float xs[4] = {x.x, x.y, x.z, x.w};
-    uint32_t w = 0u;
+    const size_t total = (size_t)n * words;
     bool ok = true;
+    for (size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (size_t)gridDim.x * blockDim.x) {
+        const uint32_t r = (uint32_t)(i / words), wi = (uint32_t)(i % words);
+        const uint32_t g = wi / nch, t = wi % nch;
+        const float4 x = *reinterpret_cast<const float4*>(vec + (size_t)r * stride + 16 * t + 4 * g);
+        const float xs[4] = {x.x, x.y, x.z, x.w};
+        uint32_t w = 0u;
 #pragma unroll
-    for (int j = 0; j < 4; j++) {
-        const float f = xs[j];
-        const bool in = f >= 0.f && f <= 255.f;  // false for NaN
-        const uint32_t b = in ? (uint32_t)f : 0u;
-        ok &= in && __float_as_uint((float)b) == __float_as_uint(f);
-        w |= b << (8 * j);
+        for (int j = 0; j < 4; j++) {
+            const float f = xs[j];
+            const bool in = f >= 0.f && f <= 255.f;  // false for NaN
+            const uint32_t b = in ? (uint32_t)f : 0u;
+            ok &= in && __float_as_uint((float)b) == __float_as_uint(f);
+            w |= b << (8 * j);
+        }
+        out[i] = w;
     }
-    out[i] = w;
-    if (!ok) atomicOr(flag, 1u);
+    const uint64_t bad = __ballot(!ok);
+    if (bad && (threadIdx.x & 63) == (unsigned)(__ffsll((unsigned long long)bad) - 1)) atomicOr(flag, 1u);
 }
 
 hipError_t launch_code_u8(const float* vec, uint32_t n, uint32_t dim, uint32_t stride, uint8_t* out, uint32_t* flag,
@@ -1132,7 +1138,8 @@ hipError_t launch_code_u8(const float* vec, uint32_t n, uint32_t dim, uint32_t s
     if (!search_fixed_chunks(dim) || stride % 4 != 0) return hipErrorInvalidValue;
     const size_t words = (size_t)n * (dim / 4);
     if (words == 0) return hipSuccess;
-    hipLaunchKernelGGL(code_u8_kernel, dim3((unsigned)((words + 255) / 256)), dim3(256), 0, s, vec, n, dim, stride,
+    const size_t blocks = std::min<size_t>((words + 255) / 256, 4096);
+    hipLaunchKernelGGL(code_u8_kernel, dim3((unsigned)blocks), dim3(256), 0, s, vec, n, dim, stride,
                        reinterpret_cast<uint32_t*>(out), flag);
     return hipGetLastError();
 }
