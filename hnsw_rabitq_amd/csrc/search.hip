@@ -227,6 +227,11 @@ __device__ __forceinline__ void level0_lazy(const DevGraph& g, const float* qs, 
     };
     update_bound();
     int scan_lo = scan_lo_io;
+    // The scan window: 64 main entries from wb held one per lane (entries past the end read as
+    // expanded), kept between merges -- the main list changes only in flush(), which drops it,
+    // and in the expanded flags, which are set in the window too.
+    int wb = -1;
+    uint64_t we = 0;
     const int trash = (int)(trash_base - bk) + lane;
     // the pending keys into the main list (the eager merge's shift with the pending list as its
     // batch: lane i holds the key of rank i and its insertion point)
@@ -271,6 +276,7 @@ __device__ __forceinline__ void level0_lazy(const DevGraph& g, const float* qs, 
         if (lane < m && fp < lim) bk[fp] = pk;
         n = new_n;
         scan_lo = first_pos < scan_lo ? first_pos : scan_lo;
+        wb = -1;
         __syncthreads();
         load_main_regs();
         P = 0;
@@ -304,17 +310,20 @@ __device__ __forceinline__ void level0_lazy(const DevGraph& g, const float* qs, 
         int pm = -1;
         uint64_t xm = ~0ull, xm2 = ~0ull;
         bool xm2_known = true;
-        for (int b = scan_lo; b < n; b += 64) {
-            const int i = b + lane;
-            const int ic = i < n ? i : n - 1;
-            const uint64_t e = bk[ic];
-            const uint64_t mu = ballot((i < n) & ((e & BX) == 0));
+        if (wb < 0 || scan_lo < wb || scan_lo >= wb + 64) wb = -1;
+        for (int b = wb < 0 ? scan_lo : wb; b < n; b += 64) {
+            if (b != wb) {
+                wb = b;
+                const int i = b + lane;
+                we = i < n ? bk[i] : BX;
+            }
+            const uint64_t mu = ballot((b + lane >= scan_lo) & ((we & BX) == 0));
             if (mu) {
                 const int l1 = __ffsll((unsigned long long)mu) - 1;
                 pm = b + l1;
-                xm = readlane64(e, l1) & ~BFLAGS;
+                xm = readlane64(we, l1) & ~BFLAGS;
                 const uint64_t m2 = mu & (mu - 1);
-                if (m2) xm2 = readlane64(e, __ffsll((unsigned long long)m2) - 1) & ~BFLAGS;
+                if (m2) xm2 = readlane64(we, __ffsll((unsigned long long)m2) - 1) & ~BFLAGS;
                 xm2_known = m2 != 0 || b + 64 >= n;
                 break;
             }
@@ -335,6 +344,7 @@ __device__ __forceinline__ void level0_lazy(const DevGraph& g, const float* qs, 
         uint64_t k2;
         if (from_main) {
             if (lane == 0) bk[pm] = x | BX;
+            if (lane == pm - wb) we |= BX;
             scan_lo = pm;
             k2 = xm2_known ? (xm2 < xp ? xm2 : xp) : ~0ull;
         } else {
@@ -401,15 +411,25 @@ __device__ __forceinline__ void level0_lazy(const DevGraph& g, const float* qs, 
                     dup = lane == L ? d1 : dup;
                     cp = lane == L ? c2 : cp;
                 }
-                // the count inside the segment: bk[base] < key, the count lies in (base, base + 2^sh]
+                // the count inside the segment: bk[base] < key, the count lies in (base, base + 2^sh];
+                // found four ways per step (three probes read together), then one binary step
+                // when sh is odd: 3 dependent LDS reads at 32-entry segments instead of 5
                 const int base = (seg > 0 ? seg - 1 : 0) << sh;
                 const int lim = base + (1 << sh) < n ? base + (1 << sh) : n;
                 int q = base;
-                for (int step = (1 << sh) >> 1; step > 0; step >>= 1) {
-                    const int probe_i = q + step;
-                    const int pc = probe_i < lim ? probe_i : base;
-                    const bool lt = (probe_i < lim) & ((bk[pc] & ~BFLAGS) < key);
-                    q += lt ? step : 0;
+                int step = 1 << sh;
+                while (step >= 4) {
+                    step >>= 2;
+                    const int p1 = q + step, p2 = q + 2 * step, p3 = q + 3 * step;
+                    const uint64_t v1 = bk[p1 < lim ? p1 : base], v2 = bk[p2 < lim ? p2 : base],
+                                   v3 = bk[p3 < lim ? p3 : base];
+                    const int c = ((p1 < lim) & ((v1 & ~BFLAGS) < key)) + ((p2 < lim) & ((v2 & ~BFLAGS) < key)) +
+                                  ((p3 < lim) & ((v3 & ~BFLAGS) < key));
+                    q += c * step;
+                }
+                if (step == 2) {
+                    const int p1 = q + 1;
+                    q += ((p1 < lim) & ((bk[p1 < lim ? p1 : base] & ~BFLAGS) < key)) ? 1 : 0;
                 }
                 pos = seg > 0 ? q + 1 : 0;
                 ok = key != ~0ull && !dup;
