@@ -195,12 +195,16 @@ __device__ __forceinline__ void level0_lazy(const DevGraph& g, const float* qs, 
         return probe ? hash_insert<true>(hash, hmask, id) : hash_insert<false>(hash, hmask, id);
     };
     int n = n_io;
-    int sh = 0;  // segments of 2^sh main entries, at most 64 of them
-    while ((64 << sh) < ef_eff) sh++;
-    uint64_t head = 0, tail = 0;
+    int sh = 0;  // segments of 2^sh main entries, at most 32 of them
+    while ((32 << sh) < ef_eff) sh++;
+    uint64_t head = 0, tail = 0;  // head: lanes 32..63, the key of main entry (lane - 32) << sh
     int tb = 0;  // tail lane l holds main entry tb + l (flag-free; ~0 past the end)
     auto load_main_regs = [&]() {
-        head = beam_head(bk, n, sh, lane);
+        {
+            const int i = (lane - 32) << sh;
+            const uint64_t v = bk[lane >= 32 && i < n ? i : 0];
+            head = lane >= 32 && i < n ? (v & ~BFLAGS) : ~0ull;
+        }
         tb = n > 64 ? n - 64 : 0;
         const int i = tb + lane;
         const uint64_t v = bk[i < n ? i : 0];
@@ -396,24 +400,26 @@ __device__ __forceinline__ void level0_lazy(const DevGraph& g, const float* qs, 
             }
             const uint64_t live = ballot(key != ~0ull);
             if (live) {
-                // per staged key: its segment of the main list (heads), whether it is pending,
-                // and how many pending keys lie below it
-                int seg = 0;
-                bool dup = false;
+                // per staged key, one ballot over the pending keys (lanes 0..31) and the segment
+                // heads (lanes 32..63): the pending keys below it and its segment of the main list
                 const uint64_t pkf = lane < P ? (pk & ~BFLAGS) : ~0ull;
+                const uint64_t below = lane < 32 ? pkf : head;
+                int sc = 0;  // (segment << 8) | pending keys below
                 for (uint64_t t = live; t; t &= t - 1) {
                     const int L = __ffsll((unsigned long long)t) - 1;
-                    const uint64_t kl = readlane64(key, L);
-                    const int c1 = __popcll(__builtin_amdgcn_ballot_w64(head < kl));
-                    const bool d1 = __builtin_amdgcn_ballot_w64(pkf == kl) != 0ull;
-                    const int c2 = __popcll(__builtin_amdgcn_ballot_w64(pkf < kl));
-                    seg = lane == L ? c1 : seg;
-                    dup = lane == L ? d1 : dup;
-                    cp = lane == L ? c2 : cp;
+                    const uint64_t b = __builtin_amdgcn_ballot_w64(below < readlane64(key, L));
+                    const int v = (__popc((uint32_t)(b >> 32)) << 8) | __popc((uint32_t)b);
+                    sc = lane == L ? v : sc;
                 }
+                const int seg = sc >> 8;
+                cp = sc & 0xFF;
+                // pending already: the pending key of rank cp equals it (lanes >= P hold ~0)
+                const uint32_t plo = (uint32_t)__builtin_amdgcn_ds_bpermute(cp * 4, (int)(uint32_t)pkf);
+                const uint32_t phi = (uint32_t)__builtin_amdgcn_ds_bpermute(cp * 4, (int)(uint32_t)(pkf >> 32));
+                const bool dup = (((uint64_t)phi << 32) | plo) == key;
                 // the count inside the segment: bk[base] < key, the count lies in (base, base + 2^sh];
                 // found four ways per step (three probes read together), then one binary step
-                // when sh is odd: 3 dependent LDS reads at 32-entry segments instead of 5
+                // when sh is odd: 3 dependent LDS reads for segments of 32 or 64 entries
                 const int base = (seg > 0 ? seg - 1 : 0) << sh;
                 const int lim = base + (1 << sh) < n ? base + (1 << sh) : n;
                 int q = base;
