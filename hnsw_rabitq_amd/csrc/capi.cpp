@@ -732,6 +732,17 @@ HG_API int hg_gpu_index_load(const char* path, int device, void** out) {
         L.space = hdr2[2];
         check_layout(L);
         require(L.urows <= L.n * 64 && (L.space == 0 || L.space == 1), "Index seems to be corrupted or unsupported");
+        {
+            // the arrays the header announces must be the file's remaining bytes (no allocation
+            // sized by a damaged header)
+            const std::streamoff here = f.tellg();
+            f.seekg(0, f.end);
+            const size_t left = (size_t)(f.tellg() - here);
+            f.seekg(here, f.beg);
+            const size_t need = (L.n * L.stride + L.n * L.maxM0 + L.n + std::max<size_t>(L.urows, 1) * L.maxM) * 4 +
+                                L.n * 8 + (L.n + 31) / 32 * 4 + 8;
+            require(left == need, "Index seems to be corrupted or unsupported");
+        }
         L.vec.resize(L.n * L.stride);
         L.l0.resize(L.n * L.maxM0);
         L.uoff.resize(L.n);

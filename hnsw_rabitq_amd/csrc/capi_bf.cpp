@@ -466,10 +466,15 @@ HG_API int hg_bf_load(int space, size_t dim, const char* path, int device, void*
         require(path && out, "null argument");
         std::ifstream in(path, std::ios::binary);
         require((bool)in, std::string("Cannot open file ") + path);
+        in.seekg(0, in.end);
+        const size_t file_bytes = (size_t)in.tellg();
+        in.seekg(0, in.beg);
         size_t hdr[3] = {0, 0, 0};
         in.read((char*)hdr, sizeof(hdr));
         const size_t per = dim * 4 + 8;
-        require((bool)in && hdr[2] <= hdr[0], "Index seems to be corrupted or unsupported");
+        // the stored count must fit the file (no allocation sized by a damaged header)
+        require((bool)in && hdr[2] <= hdr[0] && hdr[2] <= (file_bytes - sizeof(hdr)) / per,
+                "Index seems to be corrupted or unsupported");
         require(hdr[1] == per, "Index seems to be corrupted or unsupported");
         void* h = nullptr;
         int rc = hg_bf_create(space, dim, hdr[0], device, &h);

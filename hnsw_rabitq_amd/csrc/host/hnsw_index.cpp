@@ -130,10 +130,12 @@ HnswIndex::HnswIndex(int space, size_t dim, const std::string& path, size_t max_
     rd(&M, 8);
     rd(&mult, 8);
     rd(&efc, 8);
-    // Whole-file size check as in loadIndex (hnswalg.h:753-769).
+    // Whole-file size check as in loadIndex (hnswalg.h:753-769), with the products checked
+    // for overflow (a garbled count must not wrap past the check).
     {
         const char* q = p;
-        if ((size_t)(end - q) < cur_count * size_per) throw std::runtime_error("Index seems to be corrupted or unsupported");
+        if (size_per == 0 || cur_count > (size_t)(end - q) / size_per)
+            throw std::runtime_error("Index seems to be corrupted or unsupported");
         q += cur_count * size_per;
         for (size_t i = 0; i < cur_count; i++) {
             if (q >= end) throw std::runtime_error("Index seems to be corrupted or unsupported");
@@ -149,6 +151,10 @@ HnswIndex::HnswIndex(int space, size_t dim, const std::string& path, size_t max_
     if (label_off - off_data != dim * sizeof(float) || size_per != label_off + sizeof(labeltype))
         throw std::runtime_error("Wrong dimensionality of the vectors");
     max_elements_ = max_elements_i < cur_count ? max_elements : max_elements_i;
+    // a stored capacity below the stored count, or an entry point or level that the lists do not
+    // hold, is a damaged file (the reference would index past its arrays)
+    if (max_elements_ < cur_count || (cur_count > 0 && entry >= cur_count) || maxlevel < 0 || maxM0 == 0 || maxM == 0)
+        throw std::runtime_error("Index seems to be corrupted or unsupported");
     M_ = M;
     maxM_ = maxM;
     maxM0_ = maxM0;
@@ -160,24 +166,53 @@ HnswIndex::HnswIndex(int space, size_t dim, const std::string& path, size_t max_
     init_layout();
     if (size_per != size_data_per_element_ || offset_level0 != 0)
         throw std::runtime_error("Index seems to be corrupted or unsupported");
+    if (max_elements_ > ((size_t)1 << 40) / size_data_per_element_)
+        throw std::runtime_error("Index seems to be corrupted or unsupported");  // capacity beyond any host
     level0_ = (char*)malloc(std::max<size_t>(max_elements_, 1) * size_data_per_element_);
     if (!level0_) throw std::runtime_error("out of host memory for the level-0 records of the loaded index");
-    rd(level0_, cur_count * size_data_per_element_);
-    link_lists_.assign(max_elements_, nullptr);
-    element_levels_.assign(max_elements_, 0);
-    std::vector<std::mutex>(max_elements_).swap(link_list_locks_);
-    for (size_t i = 0; i < cur_count; i++) {
-        label_lookup_[label((tableint)i)] = (tableint)i;
-        uint32_t sz;
-        rd(&sz, 4);
-        if (sz == 0) {
-            element_levels_[i] = 0;
-        } else {
-            element_levels_[i] = (int)(sz / size_links_per_element_);
-            link_lists_[i] = (char*)malloc(sz);
-            if (!link_lists_[i]) throw std::runtime_error("out of host memory for an upper-level link list of the loaded index");
-            rd(link_lists_[i], sz);
+    // a constructor that throws runs no destructor: what was allocated is freed here
+    try {
+        rd(level0_, cur_count * size_data_per_element_);
+        link_lists_.assign(max_elements_, nullptr);
+        element_levels_.assign(max_elements_, 0);
+        std::vector<std::mutex>(max_elements_).swap(link_list_locks_);
+        for (size_t i = 0; i < cur_count; i++) {
+            label_lookup_[label((tableint)i)] = (tableint)i;
+            uint32_t sz;
+            rd(&sz, 4);
+            if (sz == 0) {
+                element_levels_[i] = 0;
+            } else {
+                if (sz % size_links_per_element_ != 0) throw std::runtime_error("Index seems to be corrupted or unsupported");
+                element_levels_[i] = (int)(sz / size_links_per_element_);
+                link_lists_[i] = (char*)malloc(sz);
+                if (!link_lists_[i]) throw std::runtime_error("out of host memory for an upper-level link list of the loaded index");
+                rd(link_lists_[i], sz);
+            }
         }
+        // every list within its capacity, every id an element of the index
+        auto bad = [&]() { return std::runtime_error("Index seems to be corrupted or unsupported"); };
+        for (size_t i = 0; i < cur_count; i++) {
+            const uint32_t* l0 = linklist0((tableint)i);
+            const size_t c0 = *(const uint16_t*)l0;
+            if (c0 > maxM0_) throw bad();
+            for (size_t j = 0; j < c0; j++)
+                if (l0[1 + j] >= cur_count) throw bad();
+            for (int lv = 1; lv <= element_levels_[i]; lv++) {
+                const uint32_t* l = linklist((tableint)i, lv);
+                const size_t c = *(const uint16_t*)l;
+                if (c > maxM_) throw bad();
+                for (size_t j = 0; j < c; j++)
+                    if (l[1 + j] >= cur_count) throw bad();
+            }
+        }
+        if (cur_count > 0 && element_levels_[entry] < maxlevel) throw bad();
+    } catch (...) {
+        for (char* l : link_lists_) free(l);
+        link_lists_.clear();
+        free(level0_);
+        level0_ = nullptr;
+        throw;
     }
     cur_element_count_ = cur_count;
     next_label_ = cur_count;

@@ -216,7 +216,14 @@ Hnsw* parse(const char* buf, size_t len) {
               rd(p, end, h->size_per_element) && rd(p, end, h->label_offset) && rd(p, end, h->offsetData) &&
               rd(p, end, h->maxlevel) && rd(p, end, h->entry) && rd(p, end, h->maxM) && rd(p, end, h->maxM0) &&
               rd(p, end, h->M) && rd(p, end, h->mult) && rd(p, end, h->efc);
-    if (!ok || (size_t)(end - p) < h->count * h->size_per_element) {
+    // the record layout of hnswalg.h:121-127 (links, vector, label) and sizes that fit the file,
+    // with the products checked for overflow
+    ok = ok && h->size_per_element > 0 && h->count <= (size_t)(end - p) / h->size_per_element &&
+         h->offsetLevel0 == 0 && h->maxM0 > 0 && h->maxM > 0 && h->maxM0 < (1u << 20) && h->maxM < (1u << 20) &&
+         h->offsetData == h->maxM0 * 4 + 4 && h->label_offset > h->offsetData &&
+         (h->label_offset - h->offsetData) % 4 == 0 && h->size_per_element == h->label_offset + 8 &&
+         h->max_elements >= h->count && (h->count == 0 || h->entry < h->count) && h->maxlevel >= 0;
+    if (!ok) {
         delete h;
         throw std::runtime_error("Index seems to be corrupted or unsupported");
     }
@@ -231,6 +238,10 @@ Hnsw* parse(const char* buf, size_t len) {
             delete h;
             throw std::runtime_error("Index seems to be corrupted or unsupported");
         }
+        if (sz % h->size_links_per_element != 0) {
+            delete h;
+            throw std::runtime_error("Index seems to be corrupted or unsupported");
+        }
         if (sz) {
             h->links[i].assign(p, p + sz);
             h->levels[i] = (int)(sz / h->size_links_per_element);
@@ -242,6 +253,34 @@ Hnsw* parse(const char* buf, size_t len) {
         throw std::runtime_error("Index seems to be corrupted or unsupported");
     }
     h->dim = (h->label_offset - h->offsetData) / 4;
+    // every list within its capacity and every id an element; the entry point holds the top level
+    bool lists_ok = h->count == 0 || h->levels[h->entry] >= h->maxlevel;
+    for (size_t i = 0; lists_ok && i < h->count; i++) {
+        const uint32_t* l0 = reinterpret_cast<const uint32_t*>(h->level0.data() + i * h->size_per_element);
+        uint16_t c0;
+        memcpy(&c0, l0, 2);
+        lists_ok = c0 <= h->maxM0;
+        for (size_t j = 0; lists_ok && j < c0; j++) {
+            uint32_t id;
+            memcpy(&id, l0 + 1 + j, 4);
+            lists_ok = id < h->count;
+        }
+        for (int lv = 1; lists_ok && lv <= h->levels[i]; lv++) {
+            const char* l = h->links[i].data() + (size_t)(lv - 1) * h->size_links_per_element;
+            uint16_t c;
+            memcpy(&c, l, 2);
+            lists_ok = c <= h->maxM;
+            for (size_t j = 0; lists_ok && j < c; j++) {
+                uint32_t id;
+                memcpy(&id, l + 4 + 4 * j, 4);
+                lists_ok = id < h->count;
+            }
+        }
+    }
+    if (!lists_ok) {
+        delete h;
+        throw std::runtime_error("Index seems to be corrupted or unsupported");
+    }
     for (size_t i = 0; i < h->count; i++) h->num_deleted += h->deleted((uint32_t)i) ? 1 : 0;
     return h;
 }
