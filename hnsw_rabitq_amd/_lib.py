@@ -12,6 +12,7 @@ LIB_PATH = os.path.join(HERE, "libhnswgpu.so")
 HG_OK, HG_ERR_INVALID, HG_ERR_RUNTIME, HG_ERR_HIP, HG_ERR_SHORT = 0, 1, 2, 3, 4
 SPACE_L2, SPACE_IP = 0, 1
 PREC_FP32, PREC_BF16, PREC_FP8 = 0, 1, 2
+GEMM_VARIANTS = 5  # HG_GEMM_VARIANTS
 VEC_F32, VEC_U8, VEC_AUTO = 0, 1, 1
 
 _vp = ctypes.c_void_p

@@ -32,12 +32,12 @@ def run(H, lib, n, nq, stream, timer, log, dim=768, k=100, c=4096, library=True)
     # for this index and precision
     for name, prec in (("bf16", lib.PREC_BF16), ("fp8", lib.PREC_FP8)):
         ll, ld = DeviceArray(nq * k * 8), DeviceArray(nq * k * 4)
-        times = np.zeros(5, np.float64)
+        times = np.zeros(lib.GEMM_VARIANTS, np.float64)
         chosen = ctypes.c_int()
         lib.call("hg_bf_autotune", bf._h, dq.ptr, nq, k, prec, c, ll.ptr, ld.ptr,
                  ctypes.c_void_p(times.ctypes.data), ctypes.byref(chosen), stream)
         got = ll.to_numpy(np.uint64, (nq, k))
-        for v in range(5):
+        for v in range(lib.GEMM_VARIANTS):
             log(f"  variant {v} {name}: GEMM {times[v]:.2f} ms = {flops / (times[v] * 1e-3) / 1e12:.0f} TFLOP/s")
             out[f"bf_{name}_gemm_ms_variant{v}"] = float(times[v])
         out[f"bf_{name}_gemm_variant"] = chosen.value

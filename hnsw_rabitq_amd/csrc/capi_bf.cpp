@@ -371,6 +371,8 @@ HG_API int hg_bf_fp64_check(void* bv, const float* dq, size_t nq, size_t k, cons
     });
 }
 
+static_assert(hg::kGemmVariants == HG_GEMM_VARIANTS, "variant count of the C ABI");
+
 // Autotuner of the distance GEMM's tile configuration: runs the reduced-precision search on the
 // caller's queries with every variant (each twice, the second timed), keeps the variant with the
 // shortest filter GEMM for this index and precision, and reports the times.

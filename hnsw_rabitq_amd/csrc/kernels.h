@@ -106,7 +106,7 @@ hipError_t launch_bf_lowp(const BfData& d, int precision, const float* q, uint32
                           uint64_t* labels, float* dists, void* scratch, hipStream_t s, double* stats,
                           int variant = -1, const uint32_t* xstats = nullptr, double tie_rel = 0.0,
                           LowpReport* rep = nullptr);
-constexpr int kGemmVariants = 5;
+constexpr int kGemmVariants = 5;  // HG_GEMM_VARIANTS of the C ABI
 void set_gemm_variant(int v);
 // FP64 report of final lists: labels are mapped to rows through (sorted_labels, sorted_rows), the
 // index's labels ascending and their rows; out_host[5]; scratch >= 64 bytes.

@@ -335,11 +335,13 @@ int hg_bf_search_device_filtered(void* bf, const float* d_queries, size_t nq, si
  * with 4x2 blocks per wave and a register-prefetched LDS stage; 2 (default) = 256x128 tiles
  * fed by direct-to-LDS loads through a ring of three k-slices; 3 = 256x256 tiles, eight waves,
  * the same direct-to-LDS ring; 4 = 256x256 tiles, four waves of 4x4 blocks, the same ring. */
+#define HG_GEMM_VARIANTS 5
 int hg_bf_gemm_variant(int variant);
 /* Autotuner of the GEMM tile configuration for this index and precision (BF16 or FP8): runs the
  * reduced-precision search on the device-resident queries with every variant, keeps the one
  * with the shortest filter GEMM for later hg_bf_search_device calls on this index (it overrides
- * hg_bf_gemm_variant there), writes the five GEMM times (ms) to times_ms[5] and the choice to
+ * hg_bf_gemm_variant there), writes the GEMM time (ms) of every variant to
+ * times_ms[HG_GEMM_VARIANTS] and the choice to
  * *chosen (either may be NULL).  dl/dd receive the last run's results ([nq][k]).  No counterpart
  * in the reference: the tile shape is a GPU concern. */
 int hg_bf_autotune(void* bf_index, const float* d_queries, size_t nq, size_t k, int precision, size_t rerank_c,

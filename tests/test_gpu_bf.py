@@ -188,13 +188,13 @@ def test_bf_gemm_autotune(hg):
     el, ed = b.knn_query(q, k=k)
     dq = DeviceArray.from_numpy(q)
     dl, dd = DeviceArray(q.shape[0] * k * 8), DeviceArray(q.shape[0] * k * 4)
-    times = np.zeros(5, np.float64)
+    times = np.zeros(_lib.GEMM_VARIANTS, np.float64)
     chosen = ctypes.c_int(-1)
     _lib.call("hg_bf_autotune", b._h, dq.ptr, q.shape[0], k, _lib.PREC_BF16, 512, dl.ptr, dd.ptr,
               ctypes.c_void_p(times.ctypes.data), ctypes.byref(chosen), None)
     _lib.call("hg_device_synchronize")
     print(f"autotune times {times} chosen {chosen.value}", flush=True)
-    assert 0 <= chosen.value < 5 and (times > 0).all()
+    assert 0 <= chosen.value < _lib.GEMM_VARIANTS and (times > 0).all()
     assert times[chosen.value] == times.min()
     np.testing.assert_array_equal(dl.to_numpy(np.uint64, (q.shape[0], k)), el)
     _lib.call("hg_bf_search_device", b._h, dq.ptr, q.shape[0], k, _lib.PREC_BF16, 512, dl.ptr, dd.ptr, None, None)
