@@ -86,3 +86,21 @@ def test_gpu_knn_query_shapes_and_errors(hg):
         small.init_index(5)
         small.add_items(data[:5])
         small.knn_query(data[:2], k=10)
+
+
+def test_gpu_ef_limit(hg, orc):
+    """The kernel holds a query's whole beam in LDS: max(ef, k) above 4096 is refused with a
+    clear error, and ef = 4096 itself (segments of 128 main entries in the lazy merge) equals
+    the oracle."""
+    from hnsw_rabitq_amd import _lib
+    p, _, data, queries = built_index("c2s")
+    g = p.gpu()
+    print("ef limit: c2s ef=4097 and ef=4096", flush=True)
+    with pytest.raises(_lib.HnswGpuError, match="above 4096"):
+        g.search(queries[:4], 10, 4097)
+    lab, dis, ctr, short = g.search(queries[:20], 10, 4096, counters=True)
+    o = orc.OracleHnsw(blob=p.serialize())
+    lt, dt, ct, _, _ = o.search(0, queries[:20], 10, 4096, mode=orc.MODE_TOTAL, counters=True)
+    np.testing.assert_array_equal(lab, lt)
+    np.testing.assert_array_equal(dis.view(np.uint32), dt.view(np.uint32))
+    np.testing.assert_array_equal(ctr[:, 0], ct[:, 0])

@@ -110,7 +110,8 @@ def test_u8_filtered_search_equals_f32(hg):
 # registers, the main list rewritten every 32 keys; pipelined expansions; exact integer
 # distances).  Its results must equal the oracle's at every beam size and k, per-query ef and k,
 # and with a hop cap; the FP32 rows (the eager merge) must give the same.
-@pytest.mark.parametrize("ef,k", [(1, 1), (5, 10), (64, 1), (100, 10), (300, 50), (700, 100)])
+@pytest.mark.parametrize("ef,k", [(1, 1), (5, 10), (64, 1), (100, 10), (300, 50), (700, 100), (2048, 10),
+                                  (3000, 10)])
 def test_lazy_merge_matches_oracle(hg, orc, ef, k):
     p, _, data, queries = built_index("c2s")
     print(f"lazy merge: c2s ef={ef} k={k}", flush=True)
