@@ -1,7 +1,9 @@
 #!/usr/bin/env python3
 """Benchmark: QPS at recall@10 = 0.95 on 1M x 128 L2 (Gamma(0.5, 40) integer-valued descriptors,
 M=16, ef_construction=200), plain fixed-ef GPU search vs learned early termination; brute-force
-MFMA TFLOPS on 1M x 768 inner product.
+MFMA TFLOPS on 1M x 768 inner product.  Beside them (outside the timed region): the filtered,
+range and multi-vector searches, the GPU construction, and the uneven batch (10M x 96 inner
+product, 100k queries with per-query ef and k, one hipGraph replay; --no-uneven skips it).
 
 One "step" = one batched GPU search of the 10,000 queries at the operating point (the smallest
 swept ef whose recall@10 >= 0.95).  Inputs are resident in HBM before timing starts; timing
@@ -86,6 +88,8 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-filter", action="store_true", help="skip the filtered / deleted-element search")
     ap.add_argument("--no-gpu-build", action="store_true", help="skip the GPU construction measurement")
+    ap.add_argument("--no-uneven", action="store_true",
+                    help="skip the uneven batch (10M x 96 IP, 100k queries, per-query ef and k, one hipGraph replay)")
     ap.add_argument("--bf-n", type=int, default=1_000_000)
     ap.add_argument("--bf-nq", type=int, default=10_000)
     ap.add_argument("--cpu-queries", type=int, default=2000)
@@ -316,6 +320,17 @@ def main():
     if not args.no_bf:
         from hnsw_rabitq_amd import bf_bench
         result.update(bf_bench.run(H, _lib, n=args.bf_n, nq=args.bf_nq, stream=stream, timer=timer, log=log))
+
+    # uneven batch: 10M x 96 IP GMM built on the GPU, 100k queries with per-query ef 16-512 and
+    # k in {1, 10, 100}, one captured launch replayed (a child process: its index is freed with it)
+    if not args.no_uneven:
+        from hnsw_rabitq_amd import c4_bench
+        log("uneven batch (10M x 96 IP, 100k queries, per-query ef and k): child process")
+        c4 = c4_bench.run_child(log)
+        if c4:
+            result["uneven_batch"] = c4
+            log(f"uneven batch: {c4['qps']:.0f} QPS ({c4['ms_per_batch']:.2f} ms per replay), recall by k "
+                f"{c4['recall_by_k']}, graph built on the GPU in {c4['build_s']:.1f} s")
 
     # CPU baseline: the oracle restatement of knn_query on all host cores, bounded sample
     cpu = None

@@ -109,5 +109,24 @@ def main():
     print(json.dumps(out), flush=True)
 
 
+def run_child(log, gpu_build=True, reps=5, timeout=900):
+    """The full uneven batch in a child process (its 10M-row index and brute-force copy are freed
+    with it); its progress lines go to this process's stderr as they come.  Returns the child's
+    JSON result, or {} when it fails."""
+    import subprocess
+    cmd = [sys.executable, "-m", "hnsw_rabitq_amd.c4_bench", "--reps", str(reps)]
+    if gpu_build:
+        cmd.append("--gpu-build")
+    try:
+        r = subprocess.run(cmd, stdout=subprocess.PIPE, text=True, timeout=timeout, cwd=ROOT)
+    except subprocess.TimeoutExpired:
+        log("  uneven batch: child process timed out")
+        return {}
+    if r.returncode != 0 or not r.stdout.strip():
+        log(f"  uneven batch: child process exited with {r.returncode}")
+        return {}
+    return json.loads(r.stdout.strip().splitlines()[-1])
+
+
 if __name__ == "__main__":
     main()
