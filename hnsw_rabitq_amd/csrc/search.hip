@@ -201,7 +201,7 @@ __device__ __forceinline__ void level0_lazy(const DevGraph& g, const float* qs, 
     int tb = 0;  // tail lane l holds main entry tb + l (flag-free; ~0 past the end)
     auto load_main_regs = [&]() {
         {
-            const int i = (lane - 32) << sh;
+            const int i = (lane & 31) << sh;  // used on lanes 32..63 only
             const uint64_t v = bk[lane >= 32 && i < n ? i : 0];
             head = lane >= 32 && i < n ? (v & ~BFLAGS) : ~0ull;
         }
