@@ -800,7 +800,9 @@ HG_API int hg_gpu_set_vector_code(void* gv, int code) {
 HG_API int hg_gpu_vector_code(void* gv, int* code) {
     return guarded([&] {
         require(gv && code, "null argument");
-        *code = static_cast<GpuIndex*>(gv)->g.vec8 ? HG_VEC_U8 : HG_VEC_F32;
+        auto* gi = static_cast<GpuIndex*>(gv);
+        std::lock_guard<std::mutex> lk(gi->mu);
+        *code = gi->g.vec8 ? HG_VEC_U8 : HG_VEC_F32;
         return HG_OK;
     });
 }

@@ -132,7 +132,9 @@ int hg_gpu_index_bytes(void* gpu_index, size_t* bytes);
  * kernel converts each byte back to that same FP32 value, so every distance is bit-identical to
  * the FP32 rows'.  HG_VEC_AUTO (default) uses the copy where it exists; HG_VEC_F32 makes the
  * search read the FP32 rows.  hg_gpu_vector_code reports which rows the search reads
- * (HG_VEC_U8 or HG_VEC_F32).  Changing the choice drops the autotuned launch plans. */
+ * (HG_VEC_U8 or HG_VEC_F32).  Changing the choice drops the autotuned launch plans; searches
+ * already launched and graphs captured before the change keep the rows they were launched with
+ * (both copies stay resident, and both give the same results). */
 int hg_gpu_set_vector_code(void* gpu_index, int code);
 int hg_gpu_vector_code(void* gpu_index, int* code);
 
