@@ -240,7 +240,10 @@ def run(gpu, H, lib, data_dim, queries, dq, gt, stream, timer, sweep, log, steps
     table = []
     best = None
     st, n_ext = ctypes.c_uint32(), ctypes.c_uint32()
-    for ef in (base, ext):
+    # base ef of the learned search: the plain operating point, the swept beam sizes between it
+    # and ext_ef (a stop rule that ends early needs a larger beam to keep recall 0.95), and ext_ef
+    efs = [base] + sorted({s["ef"] for s in sweep if base < s["ef"] < ext}) + [ext]
+    for ef in efs:
         if ef != base:
             gpu.autotune(dq, nq, ef, k, stream=stream)
         for thr in (0.8, threshold, 0.95):
