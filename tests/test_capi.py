@@ -50,6 +50,44 @@ def test_corrupted_file_is_rejected(hg, tmp_path):
         q.load_index(str(tmp_path / "missing.bin"))
 
 
+def test_damaged_headers_are_refused_before_allocation(hg, tmp_path):
+    """The GPU-layout cache and the brute-force index file announce array sizes in their headers;
+    a header that does not match the file's length is refused before anything is allocated or
+    uploaded (so these run without a GPU).  Sizes that would need tens of GB are among them."""
+    import struct
+    from hnsw_rabitq_amd import _lib
+    out = ctypes.c_void_p()
+
+    def layout(n, body, magic=0x4C474748, dim=16, maxM=16, maxM0=32, urows=5):
+        path = str(tmp_path / "layout.bin")
+        with open(path, "wb") as f:
+            f.write(struct.pack("<II", magic, 1))
+            f.write(struct.pack("<7Q", n, dim, dim, maxM, maxM0, urows, 0))
+            f.write(struct.pack("<3i", 1, 0, 0))
+            f.write(b"\0" * body)
+        return path.encode()
+
+    # n = 100: the arrays take (100*16 + 100*32 + 100 + 5*16) * 4 + 100*8 + 4*4 + 8 = 20,744 bytes
+    for n, body in ((100, 20_743), (100, 64), ((1 << 29) + 7, 4096)):
+        with pytest.raises(_lib.HnswGpuError, match="corrupted or unsupported"):
+            _lib.call("hg_gpu_index_load", layout(n, body), 0, ctypes.byref(out))
+    with pytest.raises(_lib.HnswGpuError, match="not a GPU layout file"):
+        _lib.call("hg_gpu_index_load", layout(100, 20_744, magic=0x12345678), 0, ctypes.byref(out))
+
+    def bf_file(hdr, records, dim=4):
+        path = str(tmp_path / "bf.bin")
+        with open(path, "wb") as f:
+            f.write(struct.pack("<3Q", *hdr))
+            f.write(b"\0" * (records * (dim * 4 + 8)))
+        return path.encode()
+
+    # [max_elements, size_per_element, count]; dim 4 -> 24 bytes per record
+    for hdr, records in (((10, 24, 5), 2), ((10, 24, 11), 11), ((1 << 40, 24, 1 << 40), 3), ((10, 20, 2), 2)):
+        with pytest.raises(_lib.HnswGpuError, match="corrupted or unsupported"):
+            _lib.call("hg_bf_load", 0, 4, bf_file(hdr, records), 0, ctypes.byref(out))
+    assert not out.value
+
+
 def test_capacity_and_argument_errors(hg):
     with pytest.raises(RuntimeError, match="Space name must be one of l2, ip, or cosine."):
         hg.Index("hamming", 4)
