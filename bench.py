@@ -330,7 +330,9 @@ def main():
         if c4:
             result["uneven_batch"] = c4
             log(f"uneven batch: {c4['qps']:.0f} QPS ({c4['ms_per_batch']:.2f} ms per replay), recall by k "
-                f"{c4['recall_by_k']}, graph built on the GPU in {c4['build_s']:.1f} s")
+                f"{c4['recall_by_k']}, graph built on the GPU in {c4['build_s']:.1f} s; "
+                f"{c4['evaluations_per_query']:.0f} evaluations/query, {c4['algorithmic_gbs']:.0f} GB/s algorithmic "
+                f"({c4['frac_hbm']:.2f} of HBM peak, {c4['vector_rows']} rows)")
 
     # CPU baseline: the oracle restatement of knn_query on all host cores, bounded sample
     cpu = None

@@ -87,6 +87,17 @@ def main():
         _lib.call("hg_timer_stop", timer, stream, ctypes.byref(t))
         ms.append(t.value)
     lab = dl.to_numpy(np.uint64, (nq, kmax))
+    # per-query counters of one more launch of the same batch (same rows as the replays), for the
+    # algorithmic bytes: query, one upper link row (4*M) per upper hop, one level-0 link row
+    # (4*2M) per expansion, one vector row per distance evaluation, the k results
+    dc = DeviceArray(nq * 16)
+    gpu.search_device(dq, nq, kmax, int(ef.max()), dl, dd, d_counters=dc, d_ef=def_, d_k=dk, stream=stream)
+    _lib.call("hg_stream_synchronize", stream)
+    ctr = dc.to_numpy(np.int32, (nq, 4)).astype(np.int64)
+    vb = 1 if gpu.vector_code == "u8" else 4
+    m = 16
+    abytes = float((4 * args.dim + 4 * m * ctr[:, 2] + vb * args.dim * ctr[:, 3] + 8 * m * ctr[:, 0]
+                    + vb * args.dim * ctr[:, 1] + 12 * k.astype(np.int64)).sum())
     # recall against exact brute force on a sample of queries
     # (every (nq/ns)-th query of the ef-sorted batch, so every ef range is represented)
     ns = min(args.recall_sample, nq)
@@ -104,7 +115,10 @@ def main():
     out = dict(workload=f"hnsw_ip_gmm_{args.n}x{args.dim}_q{nq}_ef16-512_k1-10-100", qps=nq / (t * 1e-3),
                ms_per_batch=t, graph_replays=args.reps, plan=plan, recall_by_k=rec, build_s=build_s,
                build="gpu" if args.gpu_build else "cpu",
-               ef_mean=float(ef.mean()), k_counts={str(kv): int((k == kv).sum()) for kv in sorted(set(k.tolist()))})
+               ef_mean=float(ef.mean()), k_counts={str(kv): int((k == kv).sum()) for kv in sorted(set(k.tolist()))},
+               expansions_per_query=float(ctr[:, 0].mean()), evaluations_per_query=float(ctr[:, 1].mean()),
+               vector_rows=gpu.vector_code, algorithmic_bytes_per_batch=abytes,
+               algorithmic_gbs=abytes / (t * 1e-3) / 1e9, frac_hbm=abytes / (t * 1e-3) / 8e12)
     _lib.call("hg_graph_free", graph)
     print(json.dumps(out), flush=True)
 
