@@ -18,6 +18,14 @@ def test_header_compiles():
     assert r.returncode == 0, r.stderr
 
 
+def test_abi_driver_compiles():
+    """tests/cpp/abi_drive.cpp (the C ABI driver of the host-sanitizer check,
+    tests/cpp/host_asan_gpu.sh) stays in step with the header."""
+    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall", "-Werror", "-I" + os.path.join(ROOT, "include"),
+                        os.path.join(ROOT, "tests", "cpp", "abi_drive.cpp")], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+
+
 @pytest.mark.gpu
 def test_cpp_closer_first_invariant():
     assert os.path.exists(PROG), "build the library first (hnsw_rabitq_amd/build.py builds the test program)"
