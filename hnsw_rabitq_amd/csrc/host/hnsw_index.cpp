@@ -423,8 +423,10 @@ tableint HnswIndex::insert(const float* point, labeltype lab, int level) {
 
     std::unique_lock<std::mutex> templock(global_);
     int maxlevelcopy = maxlevel_;
-    if (curlevel <= maxlevelcopy) templock.unlock();
+    // the entry point is read under the lock that its writers hold (an inserter that raises the
+    // top level keeps the lock to the end and writes it there)
     tableint curr_obj = enterpoint_node_;
+    if (curlevel <= maxlevelcopy) templock.unlock();
 
     if ((int32_t)curr_obj != -1 && !(skip_level0_ && curlevel == 0)) {
         if (curlevel < maxlevelcopy) {
