@@ -403,7 +403,7 @@ def main():
     value = max(plain_qps, learned_qps)
     result["value_method"] = "learned" if learned_qps > plain_qps else "plain"
     workload = f"hnsw_l2_gamma_{args.n}x{D}_q{nq}_M16_efC200_k10"
-    traffic = traffic_plan = None
+    traffic = traffic_plan = issue = None
     tpath = os.path.join(ROOT, "profiles", "search_traffic.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
@@ -415,6 +415,7 @@ def main():
                     rec.get("vector_rows", "f32") == vrows and \
                     rp.get("workgroups") == plan["workgroups"] and rp.get("hash_slots") == plan["hash_slots"]:
                 traffic, traffic_plan = rec.get("traffic_bytes_per_launch"), rp
+                issue = rec.get("issue")  # the kernel's issue counters at that plan, where measured
     line = {
         "metric": "QPS@recall10=0.95, 1M×128 L2: plain vs learned-stop GPU; brute-force TFLOPS",
         "value": value,
@@ -436,7 +437,7 @@ def main():
                      "frac": achieved / PEAK_HBM_GBS, "traffic": traffic, "traffic_plan": traffic_plan,
                      "plan": {"hash_slots": plan["hash_slots"], "workgroups": plan["workgroups"]},
                      "kernel": "hnsw_search_kernel", "vector_rows": vrows, "bytes_per_launch": bytes_launch,
-                     "kernel_ms": ev_ms},
+                     "kernel_ms": ev_ms, "issue": issue},
         "cpu_baseline": cpu,
         "sweep": [{k2: (round(v, 5) if isinstance(v, float) else v) for k2, v in s.items()} for s in sweep],
         "exact_bf_gt_ms": gt_ms,
