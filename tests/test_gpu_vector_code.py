@@ -61,7 +61,7 @@ def search_both(g, q, k, ef):
 
 
 @pytest.mark.parametrize("space,dim,ef,k", [("l2", 128, 40, 10), ("ip", 96, 64, 100), ("l2", 16, 50, 10),
-                                            ("l2", 32, 20, 1)])
+                                            ("l2", 32, 20, 1), ("l2", 64, 80, 10), ("ip", 128, 40, 10)])
 def test_u8_rows_equal_f32_rows_and_oracle(hg, orc, space, dim, ef, k):
     print(f"u8 vs f32 rows: space={space} dim={dim} ef={ef} k={k}", flush=True)
     if space == "l2" and dim == 128:
