@@ -43,9 +43,11 @@ def test_per_query_ef_k_match_oracle(hg, orc):
         np.testing.assert_array_equal(dis[i, :k[i]].view(np.uint32), od[i, :k[i]].view(np.uint32))
 
 
-def test_graph_capture_replays_equal_eager(hg):
+@pytest.mark.parametrize("case", ["c4s", "c2s"])  # FP32 rows; the 8-bit rows of integer data
+def test_graph_capture_replays_equal_eager(hg, case):
     from hnsw_rabitq_amd import DeviceArray, _lib
-    p, space, data, queries = built_index("c4s")
+    print(f"graph capture: {case}", flush=True)
+    p, space, data, queries = built_index(case)
     nq = queries.shape[0]
     ef, k = _arrays(nq, 6)
     kmax = int(k.max())
