@@ -125,11 +125,13 @@ def test_traces_record_features_and_topk(hg):
     np.testing.assert_allclose(f[:, 0, 0], np.linalg.norm(queries[:nq], axis=1), rtol=1e-5)
 
 
-def _trained_model(gpu, ef, k, period, seed=7, n_train=4000):
+def _trained_model(gpu, ef, k, period, seed=7, n_train=4000, train_queries=None):
     """A network trained (GPU forward/backward/Adam) on traces of the plain search over training
-    queries of the reduced 1M x 128 generator (20,000 x 128 integer Gamma, separate seed)."""
+    queries (by default of the reduced 1M x 128 generator: 20,000 x 128 integer Gamma, separate
+    seed)."""
     from hnsw_rabitq_amd import DeviceArray, datasets, learned_stop
-    tq = datasets.c2_gamma(n=n_train, seed=seed)
+    tq = train_queries if train_queries is not None else datasets.c2_gamma(n=n_train, seed=seed)
+    n_train = tq.shape[0]
     dtq = DeviceArray.from_numpy(tq)
     f, t, r, fin = learned_stop.collect_traces(gpu, dtq, n_train, ef, k, period=period, max_records=128)
     eq, valid = learned_stop.label_records(t, r, fin)
@@ -142,18 +144,22 @@ def _trained_model(gpu, ef, k, period, seed=7, n_train=4000):
     return m, mean.astype(np.float32), (1.0 / std).astype(np.float32), float(Y.mean())
 
 
-def test_learned_stop_matches_hop_capped_oracle(hg, orc):
+@pytest.mark.parametrize("case", ["c2s", "c4s"])  # 8-bit rows (L2, dim 128); FP32 rows (IP, dim 96)
+def test_learned_stop_matches_hop_capped_oracle(hg, orc, case):
     """The learned search's rows equal the oracle's total-order search stopped after the same
     number of level-0 expansions (the stop-condition slot, hnswlib.h:134-150, hnswalg.h:343-355),
     and the in-kernel network's stop decisions equal those of the oracle's FP64 network on the
     same traversal features (queries whose logit comes within 0.01 of the threshold excepted: the
     kernel's FP32 arithmetic differs from the oracle's FP64 by ~1e-5 there)."""
-    from hnsw_rabitq_amd import DeviceArray, learned_stop
-    p, space, data, queries = built_index("c2s")
+    from hnsw_rabitq_amd import DeviceArray, datasets, learned_stop
+    p, space, data, queries = built_index(case)
     gpu = p.gpu()
     nq, k, ef, period, thr = queries.shape[0], 10, 160, 8, 0.9
-    print(f"case=c2s learned stop ef={ef} period={period}", flush=True)
-    m, mean, inv_std, pos = _trained_model(gpu, ef, k, period)
+    print(f"case={case} learned stop ef={ef} period={period} rows={gpu.vector_code}", flush=True)
+    tq = None
+    if case == "c4s":  # training queries from the same mixture (64 centres, seed 0), another seed
+        tq = datasets.gmm_points(datasets.gmm_centres(64, 96, 0), 4000, 0.5, 7)
+    m, mean, inv_std, pos = _trained_model(gpu, ef, k, period, train_queries=tq)
     m.config(period=period, threshold=thr, min_hops=period)
     dq = DeviceArray.from_numpy(queries)
     dl, dd, dc = DeviceArray(nq * k * 8), DeviceArray(nq * k * 4), DeviceArray(nq * 16)
@@ -168,7 +174,8 @@ def test_learned_stop_matches_hop_capped_oracle(hg, orc):
           f"{ctr[:, 0].mean():.1f} vs {plain_h0.mean():.1f}", flush=True)
     assert stopped.sum() >= nq // 10, "the trained network should stop a share of the queries early"
     o = orc.OracleHnsw(blob=p.serialize())
-    ol, od, oc, short = o.search_hopcap(0, queries, k, ef, ctr[:, 0], mode=orc.MODE_TOTAL, counters=True)
+    ol, od, oc, short = o.search_hopcap(0 if space == "l2" else 1, queries, k, ef, ctr[:, 0], mode=orc.MODE_TOTAL,
+                                        counters=True)
     assert short == 0
     np.testing.assert_array_equal(lab, ol)
     np.testing.assert_array_equal(dis.view(np.uint32), od.view(np.uint32))
